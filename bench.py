@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X kNN-graph builder on its headline workload.
+
+Workload (default): n = 6,350,000 rows x d = 1024 float32, a 4096-cluster
+Gaussian mixture generated directly in HBM, k = 15, the reference's build
+constants (K_BUILD = 50, 16 trees, leaf 64, sketch prefilter x8, its stop
+rule).  One step is the whole hot path from resident float32 rows to the
+final lists: quantise + sketch, RP forest, leaf-tile init, join iterations,
+plain export and the FP32-rerank export.  The value is seconds per step
+(lower is better).  Recall@15 of both exports is measured afterwards against
+exact FP32 brute force on 10,000 sampled rows.
+
+Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: this build is
+single-GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "sisap-2026-task1_amd")
+ORACLE = os.path.join(REPO, "oracle")
+for _p in (PKG, ORACLE):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "graph build s at recall@15≥0.85 (6.35M×1024) vs plain; query p50/p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def exact_topk(torch, x, rows, k, chunk=262144):
+    """Exact FP32 dot-product top-k of x[rows] over all rows, self excluded."""
+    q = x[rows]
+    R = q.shape[0]
+    best_s = torch.full((R, k + 1), float("-inf"), device=x.device)
+    best_i = torch.zeros((R, k + 1), dtype=torch.int64, device=x.device)
+    ar = torch.arange(R, device=x.device)
+    for a in range(0, x.shape[0], chunk):
+        b = min(x.shape[0], a + chunk)
+        s = q @ x[a:b].T
+        inside = (rows >= a) & (rows < b)
+        s[ar[inside], rows[inside] - a] = float("-inf")
+        ts, ti = s.topk(k + 1, dim=1)
+        cs = torch.cat([best_s, ts], 1)
+        ci = torch.cat([best_i, ti + a], 1)
+        best_s, o = cs.topk(k + 1, dim=1)
+        best_i = ci.gather(1, o)
+    return best_i[:, :k]
+
+
+def recall(knns_rows_1b, gt_0b, k):
+    import numpy as np
+    hits = 0
+    for p, g in zip(knns_rows_1b, gt_0b):
+        hits += len(np.intersect1d(p - 1, g[:k]))
+    return hits / (len(gt_0b) * k)
+
+
+def cpu_baseline(args, nproc):
+    """The compiled reference engine (oracle/_ref/apex_main, built from
+    src/main.cpp with its own flags) on a bounded sample of the workload, at
+    the reference's own 8 OpenMP threads; its buildtime attribute is scaled
+    linearly to the full row count.  Falls back to the single-thread oracle
+    port when the reference binary cannot run."""
+    import numpy as np
+
+    import knng
+    import oracle
+    import synth
+
+    ns = args.cpu_sample
+    x = synth.gaussian_mixture(args.seed, ns, args.dim, args.clusters)
+    threads = min(8, nproc)
+    if os.path.exists(oracle.REF_BIN) and args.dim == 1024:
+        with tempfile.TemporaryDirectory() as td:
+            h5p = os.path.join(td, "sample.h5")
+            knng.h5_write_train(h5p, x)
+            t0 = time.time()
+            try:
+                r = oracle.run_reference_binary(h5p, td, threads=threads, timeout=args.cpu_timeout)
+                wall = time.time() - t0
+                if r["returncode"] == 0:
+                    bt = knng.h5_attr_f32(r["out"], "buildtime")
+                    return {"value": round(bt * args.n / ns, 2), "unit": "s (extrapolated)",
+                            "cores": threads, "kind": "reference",
+                            "sample": f"{ns} rows of the same generator (seed {args.seed}, "
+                                      f"{args.clusters} clusters); reference buildtime {bt:.2f} s "
+                                      f"(wall {wall:.2f} s incl. HDF5 read) x {args.n}/{ns}",
+                            "measured_s": round(bt, 3)}
+                log("reference binary failed:", r["stdout"][-400:])
+            except subprocess.TimeoutExpired:
+                log("reference binary timed out")
+    ns = min(ns, 10000)
+    x = x[:ns]
+    q, sk = oracle.quantize(x)
+    t0 = time.time()
+    oracle.build_reference(q, sk)
+    bt = time.time() - t0
+    return {"value": round(bt * args.n / ns, 2), "unit": "s (extrapolated)", "cores": 1,
+            "kind": "port", "sample": f"{ns} rows, oracle reference mode, 1 thread, x {args.n}/{ns}",
+            "measured_s": round(bt, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=6_350_000)
+    ap.add_argument("--dim", type=int, default=1024)
+    ap.add_argument("--clusters", type=int, default=4096)
+    ap.add_argument("--trees", type=int, default=16)
+    ap.add_argument("--k-build", type=int, default=50)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--recall-rows", type=int, default=10000)
+    ap.add_argument("--cpu-sample", type=int, default=50000)
+    ap.add_argument("--cpu-timeout", type=float, default=300.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this kNN-graph build runs on one GPU only; --gpus must be 1 "
+              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import knng
+    import synth
+
+    torch.backends.cuda.matmul.allow_tf32 = False
+    n, d = args.n, args.dim
+    log(f"generating {n} x {d} ({args.clusters} clusters) in HBM")
+    x = synth.gaussian_mixture_torch(args.seed, n, d, args.clusters, device="cuda")
+    torch.cuda.synchronize()
+    params = knng.default_params(dim=d, num_trees=args.trees, k_build=args.k_build, keep_fp32=0)
+    b = knng.Builder(n, params)
+    b.attach_fp32(x.data_ptr())
+    kp = torch.empty((n, 15), dtype=torch.int32, device="cuda")
+    dp = torch.empty((n, 15), dtype=torch.float32, device="cuda")
+    kr = torch.empty_like(kp)
+    dr = torch.empty_like(dp)
+    stage = {}
+
+    def step(tag):
+        t0 = time.time()
+        b.ingest_device(x.data_ptr(), 0, n)
+        t1 = time.time()
+        b.build()
+        b.synchronize()
+        t2 = time.time()
+        b.export_device(kp.data_ptr(), dp.data_ptr(), 15, False)
+        t3 = time.time()
+        b.export_device(kr.data_ptr(), dr.data_ptr(), 15, True)
+        b.synchronize()
+        t4 = time.time()
+        st = b.stats()
+        stage.update(ingest_s=t1 - t0, build_s=t2 - t1, export_plain_s=t3 - t2,
+                     export_rerank_s=t4 - t3, stats=st)
+        log(f"{tag}: ingest {t1 - t0:.3f}s build {t2 - t1:.3f}s (forest {st['ms_forest']:.0f} ms, "
+            f"init {st['ms_init']:.0f} ms, join {st['ms_join']:.0f} ms / {st['iterations']} it) "
+            f"rerank export {t4 - t3:.3f}s")
+
+    for w in range(args.warmup):
+        step(f"warmup {w + 1}")
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for s in range(args.steps):
+        step(f"step {s + 1}")
+    torch.cuda.synchronize()
+    elapsed = time.time() - t0
+    per_step = elapsed / max(1, args.steps)
+    st = stage["stats"]
+
+    # quality: exact FP32 brute force on sampled rows
+    rng = np.random.default_rng(12345)
+    rows_np = np.sort(rng.choice(n, min(args.recall_rows, n), replace=False))
+    rows = torch.from_numpy(rows_np).cuda()
+    log(f"exact top-15 of {len(rows_np)} sampled rows")
+    gt = exact_topk(torch, x, rows, 15).cpu().numpy()
+    rec_plain = recall(kp[rows].cpu().numpy(), gt, 15)
+    rec_rerank = recall(kr[rows].cpu().numpy(), gt, 15)
+    # emitted FP32 dists against FP64 on the sampled rows
+    xs = x[rows].double()
+    nb = x[kr[rows].long().reshape(-1) - 1].double().view(len(rows_np), 15, d)
+    exact = (xs[:, None, :] * nb).sum(-1)
+    rel = ((dr[rows].double() - exact).abs() / exact.abs().clamp_min(1e-30)).cpu().numpy()
+    log(f"recall@15 plain {rec_plain:.4f} rerank {rec_rerank:.4f}; rerank dist rel err max {rel.max():.2e}")
+
+    # roofline of the dominant kernel (the join): algorithmic bytes / HIP-event time
+    it = st["iterations"]
+    jb = sum(st["join_bytes"][:it])
+    jms = sum(st["ms_join_iter"][:it])
+    achieved = jb / (jms / 1e3) / 1e9 if jms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_join", "launches": it,
+                "algorithmic_bytes_per_launch": round(jb / max(it, 1)),
+                "avg_launch_ms": round(jms / max(it, 1), 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline on a {args.cpu_sample}-row sample")
+        cpu = cpu_baseline(args, os.cpu_count() or 1)
+
+    out = {
+        "metric": METRIC, "value": round(per_step, 4), "unit": "s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per_step * 1e3, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "int8",
+        "data": "synthetic",
+        "config": {"workload": f"kNN-graph build {n}x{d} float32, {args.clusters}-cluster Gaussian "
+                               f"mixture, k=15, int8 build + FP32 rerank",
+                   "n": n, "dim": d, "clusters": args.clusters, "k": 15, "k_build": args.k_build,
+                   "trees": args.trees, "leaf": 64, "parallelism": "single-gpu"},
+        "recall_at_15": {"plain": round(rec_plain, 4), "rerank": round(rec_rerank, 4),
+                         "rows": int(len(rows_np)), "rerank_rel_err_max": float(rel.max())},
+        "plain_s": round(stage["ingest_s"] + stage["build_s"] + stage["export_plain_s"], 4),
+        "rerank_extra_s": round(stage["export_rerank_s"], 4),
+        "stages_ms": {"ingest": round(stage["ingest_s"] * 1e3, 1), "forest": round(st["ms_forest"], 1),
+                      "init": round(st["ms_init"], 1), "join": round(st["ms_join"], 1),
+                      "join_iter": [round(v, 1) for v in st["ms_join_iter"]],
+                      "export_rerank": round(st["ms_export"], 1)},
+        "iterations": it, "update_rates": [round(r, 5) for r in st["rates"]],
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

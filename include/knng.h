@@ -1,0 +1,137 @@
+/* knng.h -- C-ABI of the MI355X kNN-graph builder (libknng.so).
+ *
+ * The reference engine (src/main.cpp of Aftab5550/SISAP-2026-task1) has no
+ * library API: its boundary is the process plus the HDF5 file contract
+ * (`main --task <name> --dataset <file.h5>`, src/main.cpp:114-141, and
+ * results/<task>/apex_simd.h5, src/main.cpp:459-480).  knng_main reproduces
+ * that boundary on top of the entry points below; each entry point names
+ * the part of src/main.cpp it replaces.
+ *
+ * Conventions: every function returns 0 on success and a negative code on
+ * error (knng_last_error() then describes it); no exception crosses the ABI.
+ * Host buffers are caller-owned, device buffers context-owned unless said
+ * otherwise.  One context drives one GPU on one stream and is not
+ * thread-safe.  Calls are stream-ordered; functions that return host data
+ * synchronise.
+ */
+#ifndef KNNG_H
+#define KNNG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KNNG_OK 0
+#define KNNG_EINVAL -1   /* bad argument */
+#define KNNG_EHIP -2     /* HIP runtime error */
+#define KNNG_ENOMEM -3   /* device or pinned allocation failed */
+#define KNNG_ESTATE -4   /* call out of order (e.g. join before init) */
+#define KNNG_ERANGE -5   /* a size exceeds a kernel limit */
+
+/* Build parameters.  Defaults (knng_default_params) are the reference's
+ * compile-time constants, src/main.cpp:19-29. */
+typedef struct knng_params {
+  int32_t dim;             /* DIM = 1024 (any multiple of 64, <= 4096) */
+  int32_t k_eval;          /* K_EVAL = 15 */
+  int32_t k_build;         /* K_BUILD = 50 (<= 64) */
+  int32_t num_trees;       /* NUM_TREES = 16 (<= 64) */
+  int32_t rerank_mult;     /* RERANK_MULTIPLIER = 8: sketch prefilter keeps k_build*8 */
+  int32_t leaf_size;       /* LEAF_SIZE = 64 (2..64) */
+  int32_t max_iters;       /* MAX_ITERATIONS = 30 */
+  float early_stop_rate;   /* EARLY_STOP_RATE = 0.04 */
+  float rate_diff_tol;     /* RATE_DIFF_TOLERANCE = 0.01 */
+  uint64_t seed;           /* pivot-hash seed (1337, the reference's tree seed base) */
+  int32_t keep_fp32;       /* 1: keep the float32 rows in HBM for the FP32 rerank */
+  int32_t rerank_width;    /* list entries re-scored in FP32 (k_eval..k_build); 0 = k_build */
+} knng_params;
+
+/* Per-build statistics.  Times are HIP-event milliseconds on the context's
+ * stream; bytes are algorithmic (counted from the work done, no cache credit). */
+typedef struct knng_stats {
+  int64_t n;
+  int32_t dim;
+  int32_t iterations;          /* join iterations run */
+  int32_t forest_levels;
+  int32_t pad0;
+  int64_t num_leaves;
+  int64_t updates[32];         /* per-iteration id changes (src/main.cpp:421-428 rule) */
+  float rates[32];             /* updates / (n * k_build) */
+  float ms_ingest;             /* K1 quantise + sketch (device time) */
+  float ms_forest;
+  float ms_init;
+  float ms_join;               /* all iterations */
+  float ms_join_iter[32];
+  float ms_export;
+  float ms_build;              /* forest + init + join, the reference's buildtime span */
+  double join_bytes[32];       /* algorithmic HBM bytes per join iteration */
+  int64_t join_candidates[32]; /* unique 2-hop candidates scored by sketch */
+  int64_t join_rescored[32];   /* candidates re-scored with int8 dots */
+} knng_stats;
+
+typedef struct knng_ctx knng_ctx;
+
+/* Fill p with the reference's constants (src/main.cpp:19-29). */
+int knng_default_params(knng_params* p);
+
+/* Allocate a context for n rows on HIP device `device`.  Replaces the
+ * buffer set-up of src/main.cpp:143-165.  n >= 2. */
+int knng_create(knng_ctx** out, int32_t device, const knng_params* p, int64_t n);
+int knng_destroy(knng_ctx* ctx);
+const char* knng_last_error(const knng_ctx* ctx);
+
+/* Ingest rows [row0, row0+nrows) from HOST memory: pinned double-buffered
+ * H2D copy + fused quantise/sketch kernel.  Replaces the per-chunk loop body
+ * of src/main.cpp:168-192. */
+int knng_ingest_chunk(knng_ctx* ctx, const float* rows, int64_t row0, int64_t nrows);
+/* Same, rows already in device memory (not retained). */
+int knng_ingest_device(knng_ctx* ctx, const float* dev_rows, int64_t row0, int64_t nrows);
+/* Borrow a resident float32 copy of all n rows (device pointer) for the FP32
+ * rerank instead of keeping an owned copy.  The caller keeps it alive. */
+int knng_attach_fp32_device(knng_ctx* ctx, const float* dev_x);
+
+/* Whole build: forest + init + join loop with the reference's stop rule.
+ * Replaces src/main.cpp:203-442. */
+int knng_build(knng_ctx* ctx);
+/* Stages, for tests and tuning. */
+int knng_build_forest(knng_ctx* ctx);                      /* src/main.cpp:205-232 */
+int knng_build_init(knng_ctx* ctx);                        /* src/main.cpp:234-314 */
+int knng_join_iteration(knng_ctx* ctx, int64_t* updates);  /* src/main.cpp:321-430, one pass */
+
+/* Export the best k per row: knns = id+1 (int32), dists as float32, each row
+ * ascending with the best neighbour last.  rerank=0: dists = float(int8 dot),
+ * exactly src/main.cpp:449-457.  rerank=1: the first rerank_width list
+ * entries are re-scored with FP32 dot products of the original rows and the
+ * best k of those are emitted with their FP32 scores.  Host buffers n*k. */
+int knng_export(knng_ctx* ctx, int32_t* knns, float* dists, int32_t k, int32_t rerank);
+/* Same into device buffers (no host sync). */
+int knng_export_device(knng_ctx* ctx, int32_t* dev_knns, float* dev_dists, int32_t k,
+                       int32_t rerank);
+
+int knng_get_stats(knng_ctx* ctx, knng_stats* st);
+/* Block until all work queued on the context's stream is done. */
+int knng_synchronize(knng_ctx* ctx);
+
+/* Introspection for parity tests (host buffers, synchronising). */
+int knng_copy_quantized(knng_ctx* ctx, int8_t* q, uint64_t* sketch);   /* n*dim, n*dim/64 */
+/* perm: num_trees*n leaf-ordered point ids; leaves: (tree,start,size) triples
+ * sorted by (tree,start); *nleaf receives the count (<= cap). */
+int knng_copy_forest(knng_ctx* ctx, int32_t* perm, int64_t* leaves, int64_t cap, int64_t* nleaf);
+int knng_copy_graph(knng_ctx* ctx, int32_t* ids, int32_t* scores);      /* n*k_build each */
+int knng_load_graph(knng_ctx* ctx, const int32_t* ids, const int32_t* scores);
+
+/* Stand-alone primitives on device buffers (synchronous, device 0 stream):
+ * K1 quantise + sketch of n rows of dimension d (d % 64 == 0), src/main.cpp:179-192. */
+int knng_quantize_device(const float* dev_x, int64_t n, int32_t d, int8_t* dev_q,
+                         uint64_t* dev_sketch);
+/* Gathered pair scores: dot[j] = <q[a[j]], q[b[j]]> (int32, src/main.cpp:403-409) and
+ * sim[j] = d - popcount(sk[a[j]] ^ sk[b[j]]) (src/main.cpp:354-370). */
+int knng_score_pairs_device(const int8_t* dev_q, const uint64_t* dev_sketch, int32_t d,
+                            const int32_t* dev_a, const int32_t* dev_b, int64_t npairs,
+                            int32_t* dev_dot, int32_t* dev_sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KNNG_H */

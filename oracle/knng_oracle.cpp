@@ -1,0 +1,548 @@
+// knng_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the reference kNN-graph engine (src/main.cpp of
+// Aftab5550/SISAP-2026-task1) used as the checker for the MI355X build.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// this library; the product path (libknng.so, knng_main) never links it.
+//
+// Two modes live here:
+//
+//  * reference mode  (orc_build_reference): the reference pipeline with its
+//    own RNG (std::mt19937 seeded 1337+t, uniform_int_distribution pivots),
+//    its in-place (Gauss-Seidel) refinement with nth_element / partial_sort
+//    tie orders, and its stop rule.  At one thread the reference is
+//    deterministic, so this mode is pinned byte-for-byte against the compiled
+//    reference's golden outputs (tests/golden/).
+//
+//  * device mode (orc_forest_device / orc_init_device / orc_join_device /
+//    orc_rerank_device): the deterministic semantics the GPU kernels
+//    implement (hash-chosen pivots, Jacobi refinement, (score desc, id asc)
+//    order, dedup before the sketch prefilter).  The GPU is checked
+//    bit-exactly against this mode stage by stage.
+//
+// Shared primitives (quantise, sketch, popcount similarity, int8 dot,
+// xorshift32 padding) are the same functions in both modes.
+//
+// Pinning (what checks each function independently of this file):
+//   orc_quantize         closed form + edge rows + reference golden dists
+//   orc_dot_i8           closed form (exact int32) + golden dists
+//   orc_sketch_sim       closed form; golden knns (top-400 choice uses it)
+//   orc_build_reference  golden knns/dists from the compiled reference
+//   orc_forest_device    invariants (partition, stable order, leaf bound) +
+//                        brute-force recursion in tests/test_oracle.py
+//   orc_init_device      brute-force restatement in numpy (tests)
+//   orc_join_device      brute-force restatement in numpy (tests)
+//   orc_rerank_device    FP64 numpy dot products
+//   orc_export           reference golden outputs (through reference mode)
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <unordered_set>
+#include <utility>
+#include <vector>
+
+namespace {
+
+constexpr int32_t kScoreMin = std::numeric_limits<int32_t>::min();
+constexpr float kFltMin = 1.17549435e-38f;
+
+struct Edge {              // one graph slot: src/main.cpp:31-37
+  int32_t id;
+  int32_t score;
+};
+
+// xorshift32 step used for init padding: src/main.cpp:273-275
+inline uint32_t xorshift32(uint32_t s) {
+  s ^= s << 13;
+  s ^= s >> 17;
+  s ^= s << 5;
+  return s;
+}
+
+// Quantise one element: src/main.cpp:184-186 (multiply, min(127), max(-128),
+// truncating cast).  The reference is built with -ffast-math, so x86 runs it
+// with DAZ/FTZ set: a denormal input reads as zero.
+inline void quantize_elem(float x, int8_t* q, bool* bit) {
+  if (std::fabs(x) < kFltMin) x = 0.0f;   // DAZ
+  float scaled = x * 127.0f;
+  float hi = (scaled < 127.0f) ? scaled : 127.0f;
+  float c = (hi > -128.0f) ? hi : -128.0f;
+  *q = static_cast<int8_t>(c);
+  *bit = c > 0.0f;                        // src/main.cpp:188-190
+}
+
+inline int32_t dot_i8(const int8_t* a, const int8_t* b, int d) {
+  int32_t s = 0;
+  for (int k = 0; k < d; ++k) s += int32_t(a[k]) * int32_t(b[k]);
+  return s;
+}
+
+// 1024 - sum of popcount(a ^ b): src/main.cpp:354-370, generalised to d bits.
+inline int32_t sketch_sim(const uint64_t* a, const uint64_t* b, int words) {
+  int32_t h = words * 64;
+  for (int w = 0; w < words; ++w) h -= __builtin_popcountll(a[w] ^ b[w]);
+  return h;
+}
+
+// splitmix64 finaliser (Steele, Lea, Flood, "Fast splittable pseudorandom
+// number generators", OOPSLA 2014).  Device-mode pivot choice only.
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+struct orc_params {
+  int32_t k_build;        // src/main.cpp:22
+  int32_t num_trees;      // :23
+  int32_t rerank_mult;    // :24
+  int32_t leaf_size;      // :25
+  int32_t max_iters;      // :27
+  float early_stop_rate;  // :28
+  float rate_diff_tol;    // :29
+};
+
+struct orc_stats {
+  int32_t iterations;
+  int32_t num_leaves;
+  int64_t updates[64];
+  float rates[64];
+};
+
+// Device-mode pivot positions of a node: p1 in [0,size), p2 in [0,size)\{p1}.
+void orc_pivots(uint64_t seed, int32_t tree, int32_t depth, int64_t start,
+                int64_t size, int64_t* p1pos, int64_t* p2pos) {
+  uint64_t key = (uint64_t(uint32_t(tree)) << 48) ^ (uint64_t(uint32_t(depth)) << 40) ^
+                 uint64_t(start);
+  uint64_t h1 = mix64(seed ^ mix64(key));
+  uint64_t h2 = mix64(h1);
+  int64_t a = int64_t(h1 % uint64_t(size));
+  int64_t b = int64_t(h2 % uint64_t(size - 1));
+  if (b >= a) ++b;
+  *p1pos = a;
+  *p2pos = b;
+}
+
+// Quantise + sketch rows: src/main.cpp:179-192.  d % 64 == 0.
+int orc_quantize(const float* x, int64_t n, int32_t d, int8_t* q, uint64_t* sk) {
+  if (d <= 0 || d % 64) return -1;
+  const int words = d / 64;
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t* s = sk + i * words;
+    for (int w = 0; w < words; ++w) s[w] = 0;
+    for (int k = 0; k < d; ++k) {
+      bool bit;
+      quantize_elem(x[i * d + k], &q[i * d + k], &bit);
+      if (bit) s[k / 64] |= 1ull << (k % 64);
+    }
+  }
+  return 0;
+}
+
+int32_t orc_dot_i8(const int8_t* a, const int8_t* b, int32_t d) { return dot_i8(a, b, d); }
+
+int32_t orc_sketch_sim(const uint64_t* a, const uint64_t* b, int32_t words) {
+  return sketch_sim(a, b, words);
+}
+
+// ---------------------------------------------------------------------------
+// Reference mode
+// ---------------------------------------------------------------------------
+
+// Two-pivot RP tree, src/main.cpp:47-99.  Explicit stack, left subtree first,
+// so leaves come out in the reference's completion order and the shared
+// mt19937 is advanced in the same depth-first order.
+static void ref_tree(std::vector<int>& root, std::vector<std::vector<int>>& leaves,
+                     std::mt19937& rng, const int8_t* q, int d, int leaf_size) {
+  std::vector<std::vector<int>> stack;
+  stack.push_back(std::move(root));
+  while (!stack.empty()) {
+    std::vector<int> set = std::move(stack.back());
+    stack.pop_back();
+    if (set.size() <= size_t(leaf_size)) {
+      leaves.push_back(std::move(set));
+      continue;
+    }
+    std::uniform_int_distribution<int> pick(0, int(set.size()) - 1);
+    const int a = set[pick(rng)];
+    int b = set[pick(rng)];
+    while (b == a) b = set[pick(rng)];
+    std::vector<int> lo, hi;
+    lo.reserve(set.size());
+    hi.reserve(set.size());
+    const int8_t* pa = q + size_t(a) * d;
+    const int8_t* pb = q + size_t(b) * d;
+    for (int id : set) {
+      if (id == a) { lo.push_back(id); continue; }
+      if (id == b) { hi.push_back(id); continue; }
+      const int8_t* v = q + size_t(id) * d;
+      (dot_i8(v, pa, d) > dot_i8(v, pb, d) ? lo : hi).push_back(id);
+    }
+    // push right first so the left child is processed (and consumes the RNG) first
+    stack.push_back(std::move(hi));
+    stack.push_back(std::move(lo));
+  }
+}
+
+int orc_build_reference(const int8_t* q, const uint64_t* sk, int64_t n64, int32_t d,
+                        const orc_params* p, int32_t* out_ids, int32_t* out_scores,
+                        orc_stats* st) {
+  if (n64 < 2 || d <= 0 || d % 64) return -1;
+  const size_t n = size_t(n64);
+  const int K = p->k_build, T = p->num_trees, words = d / 64;
+  const int M = K * p->rerank_mult;
+
+  // forest: src/main.cpp:205-216 (trees in order at one thread)
+  std::vector<std::vector<int>> leaves;
+  for (int t = 0; t < T; ++t) {
+    std::mt19937 rng(1337 + t);
+    std::vector<int> all(n);
+    for (size_t i = 0; i < n; ++i) all[i] = int(i);
+    ref_tree(all, leaves, rng, q, d, p->leaf_size);
+  }
+
+  // leaf pointers: src/main.cpp:222-232
+  std::vector<int> leaf_of(n * T, -1);
+  for (size_t l = 0; l < leaves.size(); ++l)
+    for (int id : leaves[l]) {
+      int* slot = &leaf_of[size_t(id) * T];
+      int t = 0;
+      while (t < T && slot[t] != -1) ++t;
+      if (t < T) slot[t] = int(l);
+    }
+
+  std::vector<Edge> g(n * K, Edge{-1, kScoreMin});
+  auto by_score = [](const Edge& x, const Edge& y) { return x.score > y.score; };
+
+  // init: src/main.cpp:234-314
+  std::vector<int> cands;
+  std::vector<Edge> pool;
+  for (size_t i = 0; i < n; ++i) {
+    cands.clear();
+    pool.clear();
+    for (int t = 0; t < T; ++t) {
+      const int l = leaf_of[i * T + t];
+      if (l < 0) continue;
+      for (int m : leaves[l])
+        if (size_t(m) != i) cands.push_back(m);
+    }
+    std::sort(cands.begin(), cands.end());
+    cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
+    const int8_t* qi = q + i * d;
+    for (int c : cands) pool.push_back(Edge{c, dot_i8(qi, q + size_t(c) * d, d)});
+    uint32_t s = uint32_t(1337 + i);
+    while (pool.size() < size_t(K)) {
+      s = xorshift32(s);
+      const int r = int(s % n);
+      if (size_t(r) == i) continue;
+      pool.push_back(Edge{r, dot_i8(qi, q + size_t(r) * d, d)});
+    }
+    if (size_t(K) < pool.size())
+      std::partial_sort(pool.begin(), pool.begin() + K, pool.end(), by_score);
+    else
+      std::sort(pool.begin(), pool.end(), by_score);
+    int kept = 0, prev = -1;
+    for (size_t j = 0; j < pool.size() && kept < K; ++j)
+      if (pool[j].id != prev) {
+        g[i * K + kept++] = pool[j];
+        prev = pool[j].id;
+      }
+    while (kept < K) {
+      s = xorshift32(s);
+      const int r = int(s % n);
+      if (size_t(r) != i) g[i * K + kept++] = Edge{r, kScoreMin};
+    }
+  }
+
+  // refinement: src/main.cpp:320-440 (in place, row order, one thread)
+  std::vector<std::pair<int, int32_t>> sk_c(size_t(K) * K);
+  std::vector<Edge> live(K);
+  float prev_rate = 1.0f;
+  int iters = 0;
+  for (int it = 0; it < p->max_iters; ++it) {
+    int64_t updates = 0;
+    for (size_t i = 0; i < n; ++i) {
+      Edge* row = &g[i * K];
+      for (int k = 0; k < K; ++k) live[k] = row[k];
+      const uint64_t* si = sk + i * words;
+      int cnt = 0;
+      for (int k = 0; k < K; ++k) {
+        const int n1 = row[k].id;
+        for (int k2 = 0; k2 < K; ++k2) {
+          const int n2 = g[size_t(n1) * K + k2].id;
+          if (size_t(n2) == i) continue;
+          sk_c[cnt++] = {n2, sketch_sim(si, sk + size_t(n2) * words, words)};
+        }
+      }
+      const int top = cnt < M ? cnt : M;
+      if (top > 0 && top < cnt)
+        std::nth_element(sk_c.begin(), sk_c.begin() + top, sk_c.begin() + cnt,
+                         [](const std::pair<int, int32_t>& x, const std::pair<int, int32_t>& y) {
+                           return x.second > y.second;
+                         });
+      const int8_t* qi = q + i * d;
+      for (int m = 0; m < top; ++m) {
+        const int c = sk_c[m].first;
+        bool present = false;
+        for (int k = 0; k < K && !present; ++k) present = live[k].id == c;
+        if (present) continue;
+        const int32_t sc = dot_i8(qi, q + size_t(c) * d, d);
+        if (sc <= live[K - 1].score) continue;
+        int pos = K - 1;                       // insert after every score >= sc
+        for (; pos > 0 && live[pos - 1].score < sc; --pos) live[pos] = live[pos - 1];
+        live[pos] = Edge{c, sc};
+      }
+      int changed = 0;
+      for (int k = 0; k < K; ++k)
+        if (row[k].id != live[k].id) {
+          row[k] = live[k];
+          ++changed;
+        }
+      updates += changed;
+    }
+    const float rate = float(updates) / float(n * size_t(K));
+    if (st && it < 64) {
+      st->updates[it] = updates;
+      st->rates[it] = rate;
+    }
+    iters = it + 1;
+    if (rate < p->early_stop_rate || (it > 0 && (prev_rate - rate) < p->rate_diff_tol)) break;
+    prev_rate = rate;
+  }
+  if (st) {
+    st->iterations = iters;
+    st->num_leaves = int32_t(leaves.size());
+  }
+  for (size_t j = 0; j < n * K; ++j) {
+    out_ids[j] = g[j].id;
+    out_scores[j] = g[j].score;
+  }
+  return 0;
+}
+
+// Export best k of each K-list: id+1, float(score), row reversed.
+// src/main.cpp:449-457.
+int orc_export(const int32_t* ids, const int32_t* scores, int64_t n, int32_t K,
+               int32_t k_eval, int32_t* knns, float* dists) {
+  if (k_eval > K) return -1;
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < k_eval; ++k) {
+      knns[i * k_eval + (k_eval - 1 - k)] = ids[i * K + k] + 1;
+      dists[i * k_eval + (k_eval - 1 - k)] = float(scores[i * K + k]);
+    }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Device mode
+// ---------------------------------------------------------------------------
+
+// Forest with hash-chosen pivots, all trees.  perm is T x n; leaves are
+// written as (tree, start, size) triples in (tree, start) order.
+// Returns the number of leaves, or -1.
+int64_t orc_forest_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t leaf_size,
+                          uint64_t seed, int32_t* perm, int64_t* leaf_tri, int64_t leaf_cap) {
+  if (n < 2 || leaf_size < 2) return -1;
+  int64_t nleaf = 0;
+  std::vector<int32_t> lo, hi;
+  struct Node { int64_t start, size; int32_t depth; };
+  for (int t = 0; t < T; ++t) {
+    int32_t* P = perm + int64_t(t) * n;
+    for (int64_t i = 0; i < n; ++i) P[i] = int32_t(i);
+    std::vector<Node> stack{{0, n, 0}};
+    while (!stack.empty()) {
+      Node nd = stack.back();
+      stack.pop_back();
+      if (nd.size <= leaf_size) {
+        if (leaf_tri) {
+          if (nleaf >= leaf_cap) return -1;
+          leaf_tri[3 * nleaf] = t;
+          leaf_tri[3 * nleaf + 1] = nd.start;
+          leaf_tri[3 * nleaf + 2] = nd.size;
+        }
+        ++nleaf;
+        continue;
+      }
+      int64_t a, b;
+      orc_pivots(seed, t, nd.depth, nd.start, nd.size, &a, &b);
+      const int32_t pa = P[nd.start + a], pb = P[nd.start + b];
+      lo.clear();
+      hi.clear();
+      for (int64_t j = 0; j < nd.size; ++j) {
+        const int32_t id = P[nd.start + j];
+        if (j == a) { lo.push_back(id); continue; }
+        if (j == b) { hi.push_back(id); continue; }
+        const int8_t* v = q + int64_t(id) * d;
+        const int32_t diff = dot_i8(v, q + int64_t(pa) * d, d) - dot_i8(v, q + int64_t(pb) * d, d);
+        (diff > 0 ? lo : hi).push_back(id);
+      }
+      std::copy(lo.begin(), lo.end(), P + nd.start);
+      std::copy(hi.begin(), hi.end(), P + nd.start + int64_t(lo.size()));
+      const int64_t nl = int64_t(lo.size());
+      stack.push_back(Node{nd.start + nl, nd.size - nl, nd.depth + 1});
+      stack.push_back(Node{nd.start, nl, nd.depth + 1});
+    }
+  }
+  return nleaf;
+}
+
+static inline bool edge_before(const Edge& x, const Edge& y) {
+  return x.score != y.score ? x.score > y.score : x.id < y.id;
+}
+
+// Init from the forest: union of leaf-mates, int8 dots, top-K by
+// (score desc, id asc) with duplicate ids dropped, xorshift32 padding as the
+// reference (src/main.cpp:271-312) when fewer than K candidates exist.
+int orc_init_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t K,
+                    const int32_t* perm, const int64_t* leaf_tri, int64_t nleaf,
+                    int32_t* ids, int32_t* scores) {
+  std::vector<int64_t> leaf_of(size_t(n) * T, -1);   // point -> leaf index per tree
+  for (int64_t l = 0; l < nleaf; ++l) {
+    const int64_t t = leaf_tri[3 * l], s = leaf_tri[3 * l + 1], z = leaf_tri[3 * l + 2];
+    for (int64_t j = 0; j < z; ++j) leaf_of[size_t(perm[t * n + s + j]) * T + t] = l;
+  }
+  std::vector<int32_t> cands;
+  std::vector<Edge> pool;
+  for (int64_t i = 0; i < n; ++i) {
+    cands.clear();
+    pool.clear();
+    for (int t = 0; t < T; ++t) {
+      const int64_t l = leaf_of[size_t(i) * T + t];
+      if (l < 0) return -1;
+      const int64_t s = leaf_tri[3 * l + 1], z = leaf_tri[3 * l + 2];
+      for (int64_t j = 0; j < z; ++j) {
+        const int32_t m = perm[int64_t(t) * n + s + j];
+        if (m != i) cands.push_back(m);
+      }
+    }
+    std::sort(cands.begin(), cands.end());
+    cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
+    const int8_t* qi = q + i * d;
+    for (int32_t c : cands) pool.push_back(Edge{c, dot_i8(qi, q + int64_t(c) * d, d)});
+    uint32_t s = uint32_t(1337 + i);
+    while (pool.size() < size_t(K)) {
+      s = xorshift32(s);
+      const int32_t r = int32_t(s % uint64_t(n));
+      if (r == i) continue;
+      pool.push_back(Edge{r, dot_i8(qi, q + int64_t(r) * d, d)});
+    }
+    std::sort(pool.begin(), pool.end(), edge_before);
+    int kept = 0, prev = -1;
+    for (size_t j = 0; j < pool.size() && kept < K; ++j)
+      if (pool[j].id != prev) {
+        ids[i * K + kept] = pool[j].id;
+        scores[i * K + kept] = pool[j].score;
+        prev = pool[j].id;
+        ++kept;
+      }
+    while (kept < K) {
+      s = xorshift32(s);
+      const int32_t r = int32_t(s % uint64_t(n));
+      if (r != i) {
+        ids[i * K + kept] = r;
+        scores[i * K + kept] = kScoreMin;
+        ++kept;
+      }
+    }
+  }
+  return 0;
+}
+
+// Sketch-prefilter threshold: the largest t with |{h >= t}| >= M (all ties
+// at t are kept).  Returns 0 when fewer than M candidates exist.
+int32_t orc_sketch_threshold(const int32_t* h, int64_t cnt, int32_t M, int32_t bits) {
+  if (cnt <= M) return 0;
+  std::vector<int64_t> hist(size_t(bits) + 1, 0);
+  for (int64_t j = 0; j < cnt; ++j) hist[size_t(h[j])]++;
+  int64_t acc = 0;
+  for (int32_t t = bits; t >= 0; --t) {
+    acc += hist[size_t(t)];
+    if (acc >= M) return t;
+  }
+  return 0;
+}
+
+// One Jacobi refinement iteration (device semantics).  Returns the number of
+// slots whose id changed (src/main.cpp:421-427 counting rule), or -1.
+int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t d, int32_t K,
+                        int32_t M, const int32_t* ids_in, const int32_t* sc_in,
+                        int32_t* ids_out, int32_t* sc_out) {
+  const int words = d / 64;
+  int64_t updates = 0;
+  std::unordered_set<int32_t> seen;
+  std::vector<int32_t> cand, hval;
+  struct Slot { Edge e; int32_t slot; };
+  std::vector<Slot> all;
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t* own = ids_in + i * K;
+    seen.clear();
+    cand.clear();
+    hval.clear();
+    seen.insert(int32_t(i));
+    for (int k = 0; k < K; ++k) seen.insert(own[k]);
+    for (int k = 0; k < K; ++k) {
+      const int32_t* nb = ids_in + int64_t(own[k]) * K;
+      for (int k2 = 0; k2 < K; ++k2)
+        if (seen.insert(nb[k2]).second) cand.push_back(nb[k2]);
+    }
+    const uint64_t* si = sk + i * words;
+    for (int32_t c : cand) hval.push_back(sketch_sim(si, sk + int64_t(c) * words, words));
+    const int32_t thr = orc_sketch_threshold(hval.data(), int64_t(cand.size()), M, d);
+    all.clear();
+    for (int k = 0; k < K; ++k) all.push_back(Slot{Edge{own[k], sc_in[i * K + k]}, k});
+    const int8_t* qi = q + i * d;
+    for (size_t j = 0; j < cand.size(); ++j)
+      if (hval[j] >= thr)
+        all.push_back(Slot{Edge{cand[j], dot_i8(qi, q + int64_t(cand[j]) * d, d)}, K});
+    std::sort(all.begin(), all.end(), [](const Slot& x, const Slot& y) {
+      if (x.e.score != y.e.score) return x.e.score > y.e.score;
+      if (x.e.id != y.e.id) return x.e.id < y.e.id;
+      return x.slot < y.slot;
+    });
+    for (int k = 0; k < K; ++k) {
+      if (all[k].e.id != own[k]) ++updates;
+      ids_out[i * K + k] = all[k].e.id;
+      sc_out[i * K + k] = all[k].e.score;
+    }
+  }
+  return updates;
+}
+
+// FP32-rerank export (device semantics): the first R list entries are
+// re-scored with exact (FP64) dot products of the original rows, the best
+// k_eval by (score desc, id asc) are emitted reversed, as orc_export.
+int orc_rerank_device(const float* x, int64_t n, int32_t d, int32_t K, int32_t R,
+                      int32_t k_eval, const int32_t* ids, int32_t* knns, double* dists) {
+  if (R > K || k_eval > R) return -1;
+  struct Cand { double s; int32_t id; };
+  std::vector<Cand> c(R);
+  for (int64_t i = 0; i < n; ++i) {
+    const float* xi = x + i * d;
+    for (int r = 0; r < R; ++r) {
+      const int32_t id = ids[i * K + r];
+      const float* xc = x + int64_t(id) * d;
+      double acc = 0.0;
+      for (int k = 0; k < d; ++k) acc += double(xi[k]) * double(xc[k]);
+      c[r] = Cand{acc, id};
+    }
+    std::sort(c.begin(), c.end(), [](const Cand& a, const Cand& b) {
+      return a.s != b.s ? a.s > b.s : a.id < b.id;
+    });
+    for (int k = 0; k < k_eval; ++k) {
+      knns[i * k_eval + (k_eval - 1 - k)] = c[k].id + 1;
+      dists[i * k_eval + (k_eval - 1 - k)] = c[k].s;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,205 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of oracle/liboracle.so, the CPU
+restatement of the reference engine (src/main.cpp).  Imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the product.
+
+See knng_oracle.cpp for what each function restates and what pins it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboracle.so")
+REF_BIN = os.path.join(HERE, "_ref", "apex_main")
+REF_BIN_NATIVE = os.path.join(HERE, "_ref", "apex_main_native")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("k_build", C.c_int32), ("num_trees", C.c_int32), ("rerank_mult", C.c_int32),
+                ("leaf_size", C.c_int32), ("max_iters", C.c_int32),
+                ("early_stop_rate", C.c_float), ("rate_diff_tol", C.c_float)]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("num_leaves", C.c_int32),
+                ("updates", C.c_int64 * 64), ("rates", C.c_float * 64)]
+
+
+_L = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _L
+    if _L is None:
+        if not os.path.exists(LIBPATH):
+            build()
+        L = C.CDLL(LIBPATH)
+        V, i32, i64, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64
+        sig = {
+            "orc_quantize": ([V, i64, i32, V, V], i32),
+            "orc_dot_i8": ([V, V, i32], i32),
+            "orc_sketch_sim": ([V, V, i32], i32),
+            "orc_pivots": ([u64, i32, i32, i64, i64, C.POINTER(i64), C.POINTER(i64)], None),
+            "orc_build_reference": ([V, V, i64, i32, C.POINTER(OrcParams), V, V, C.POINTER(OrcStats)], i32),
+            "orc_export": ([V, V, i64, i32, i32, V, V], i32),
+            "orc_forest_device": ([V, i64, i32, i32, i32, u64, V, V, i64], i64),
+            "orc_init_device": ([V, i64, i32, i32, i32, V, V, i64, V, V], i32),
+            "orc_sketch_threshold": ([V, i64, i32, i32], i32),
+            "orc_join_device": ([V, V, i64, i32, i32, i32, V, V, V, V], i64),
+            "orc_rerank_device": ([V, i64, i32, i32, i32, i32, V, V, V], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes, f.restype = args, res
+        _L = L
+    return _L
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def ref_params(**over) -> OrcParams:
+    """src/main.cpp:22-29 constants."""
+    p = OrcParams(50, 16, 8, 64, 30, 0.04, 0.01)
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+def quantize(x: np.ndarray):
+    x = np.ascontiguousarray(x, np.float32)
+    n, d = x.shape
+    q = np.empty((n, d), np.int8)
+    sk = np.empty((n, d // 64), np.uint64)
+    assert lib().orc_quantize(_p(x), n, d, _p(q), _p(sk)) == 0
+    return q, sk
+
+
+def dot_i8(a: np.ndarray, b: np.ndarray) -> int:
+    a = np.ascontiguousarray(a, np.int8)
+    b = np.ascontiguousarray(b, np.int8)
+    return int(lib().orc_dot_i8(_p(a), _p(b), a.shape[0]))
+
+
+def sketch_sim(a: np.ndarray, b: np.ndarray) -> int:
+    a = np.ascontiguousarray(a, np.uint64)
+    b = np.ascontiguousarray(b, np.uint64)
+    return int(lib().orc_sketch_sim(_p(a), _p(b), a.shape[0]))
+
+
+def pivots(seed: int, tree: int, depth: int, start: int, size: int):
+    a, b = C.c_int64(), C.c_int64()
+    lib().orc_pivots(seed, tree, depth, start, size, C.byref(a), C.byref(b))
+    return int(a.value), int(b.value)
+
+
+def build_reference(q: np.ndarray, sk: np.ndarray, params: OrcParams | None = None):
+    """Reference-mode build (bit-exact to src/main.cpp at one thread)."""
+    p = params or ref_params()
+    n, d = q.shape
+    ids = np.empty((n, p.k_build), np.int32)
+    sc = np.empty((n, p.k_build), np.int32)
+    st = OrcStats()
+    rc = lib().orc_build_reference(_p(q), _p(sk), n, d, C.byref(p), _p(ids), _p(sc), C.byref(st))
+    assert rc == 0
+    return ids, sc, st
+
+
+def export(ids: np.ndarray, sc: np.ndarray, k_eval: int = 15):
+    n, K = ids.shape
+    knns = np.empty((n, k_eval), np.int32)
+    dists = np.empty((n, k_eval), np.float32)
+    assert lib().orc_export(_p(np.ascontiguousarray(ids)), _p(np.ascontiguousarray(sc)), n, K,
+                            k_eval, _p(knns), _p(dists)) == 0
+    return knns, dists
+
+
+def forest_device(q: np.ndarray, trees: int, leaf: int, seed: int = 1337):
+    n, d = q.shape
+    perm = np.empty((trees, n), np.int32)
+    cap = trees * n + 16
+    leaves = np.empty((cap, 3), np.int64)
+    nl = lib().orc_forest_device(_p(q), n, d, trees, leaf, seed, _p(perm), _p(leaves), cap)
+    assert nl >= 0
+    return perm, leaves[:nl].copy()
+
+
+def init_device(q: np.ndarray, perm: np.ndarray, leaves: np.ndarray, K: int):
+    n, d = q.shape
+    T = perm.shape[0]
+    ids = np.empty((n, K), np.int32)
+    sc = np.empty((n, K), np.int32)
+    leaves = np.ascontiguousarray(leaves, np.int64)
+    assert lib().orc_init_device(_p(q), n, d, T, K, _p(np.ascontiguousarray(perm)), _p(leaves),
+                                 leaves.shape[0], _p(ids), _p(sc)) == 0
+    return ids, sc
+
+
+def sketch_threshold(h: np.ndarray, M: int, bits: int) -> int:
+    h = np.ascontiguousarray(h, np.int32)
+    return int(lib().orc_sketch_threshold(_p(h), h.shape[0], M, bits))
+
+
+def join_device(q, sk, ids, sc, M: int):
+    n, d = q.shape
+    K = ids.shape[1]
+    oi = np.empty_like(ids)
+    os_ = np.empty_like(sc)
+    upd = lib().orc_join_device(_p(q), _p(sk), n, d, K, M, _p(np.ascontiguousarray(ids)),
+                                _p(np.ascontiguousarray(sc)), _p(oi), _p(os_))
+    assert upd >= 0
+    return oi, os_, int(upd)
+
+
+def rerank_device(x, ids, R: int, k_eval: int = 15):
+    x = np.ascontiguousarray(x, np.float32)
+    n, d = x.shape
+    K = ids.shape[1]
+    knns = np.empty((n, k_eval), np.int32)
+    dists = np.empty((n, k_eval), np.float64)
+    assert lib().orc_rerank_device(_p(x), n, d, K, R, k_eval, _p(np.ascontiguousarray(ids)),
+                                   _p(knns), _p(dists)) == 0
+    return knns, dists
+
+
+def build_device_semantics(q, sk, trees=16, leaf=64, K=50, mult=8, max_iters=30,
+                           stop=0.04, tol=0.01, seed=1337):
+    """Full device-mode build with the reference's stop rule (src/main.cpp:432-439)."""
+    perm, leaves = forest_device(q, trees, leaf, seed)
+    ids, sc = init_device(q, perm, leaves, K)
+    n = q.shape[0]
+    prev = np.float32(1.0)
+    rates = []
+    for it in range(max_iters):
+        ids, sc, upd = join_device(q, sk, ids, sc, K * mult)
+        rate = np.float32(upd) / np.float32(n * K)
+        rates.append(float(rate))
+        if rate < np.float32(stop) or (it > 0 and (prev - rate) < np.float32(tol)):
+            break
+        prev = rate
+    return ids, sc, rates
+
+
+def run_reference_binary(h5_path: str, workdir: str, threads: int = 1, native: bool = False,
+                         task: str = "task1", timeout: float = 3600.0) -> dict:
+    """Run the compiled reference engine on an HDF5 /train file; returns the
+    path of its output file and its stdout."""
+    exe = REF_BIN_NATIVE if native else REF_BIN
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([exe, "--task", task, "--dataset", os.path.abspath(h5_path)], cwd=workdir,
+                       env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=timeout)
+    return {"returncode": r.returncode, "stdout": r.stdout,
+            "out": os.path.join(workdir, "results", task, "apex_simd.h5")}

@@ -1,0 +1,1642 @@
+// knng.hip -- MI355X (gfx950) kNN-graph builder: HIP kernels + the C-ABI of
+// include/knng.h.
+//
+// Pipeline (one context = one GPU, one stream), replacing src/main.cpp:143-457
+// of the reference engine:
+//   K1 k_quantize        fused float32 -> int8 (truncating) + 1-bit sign sketch
+//   K2 k_forest_*        level-synchronous two-pivot RP forest, all trees at once
+//   K3 k_leaf_gram       leaf brute force as int8 MFMA tiles (exact int32)
+//   K4 k_init_merge      per-point top-K of all leaf-mates, (score desc, id asc)
+//   K5-7 k_join          pull-style 2-hop join: LDS hash dedup, sketch popcount
+//                        prefilter, gathered int8 dots, rank merge, change count
+//   K8 k_export_*        plain export or FP32 rerank of each row's list
+//
+// Data layout in HBM: q int8 [n][d] (row = d bytes, d % 64 == 0), sketch u64
+// [n][d/64] (bit b of word w <=> dim 64w+b), graph as SoA int32 ids[n][K] and
+// scores[n][K], double-buffered (Jacobi iterations).  Forest state is
+// perm[T][n] (point ids, each tree's leaves contiguous) with a per-position
+// node index.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "knng.h"
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+namespace {
+
+constexpr int kMaxDim = 1024;
+constexpr int kMaxK = 64;
+constexpr int kMaxTrees = 64;
+constexpr int kLeafSlots = 64;          // candidate slots per (point, tree)
+constexpr int kNumCounters = 1024;      // spread of the per-block atomic counters
+constexpr float kFltMin = 1.17549435e-38f;
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+__device__ __forceinline__ int dot4_i8(int a, int b, int c) {
+  return __builtin_amdgcn_sdot4(a, b, c, false);
+}
+
+__device__ __forceinline__ int dot16_i8(const v4i& a, const v4i& b, int c) {
+  c = dot4_i8(a.x, b.x, c);
+  c = dot4_i8(a.y, b.y, c);
+  c = dot4_i8(a.z, b.z, c);
+  return dot4_i8(a.w, b.w, c);
+}
+
+// (score desc, id asc) as a descending u64 key; 0 is "no candidate".
+__device__ __host__ __forceinline__ uint64_t pack_key(int32_t score, int32_t id) {
+  return (uint64_t(uint32_t(score) ^ 0x80000000u) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(id));
+}
+__device__ __host__ __forceinline__ int32_t key_score(uint64_t k) {
+  return int32_t(uint32_t(k >> 32) ^ 0x80000000u);
+}
+__device__ __host__ __forceinline__ int32_t key_id(uint64_t k) {
+  return int32_t(0xFFFFFFFFu - uint32_t(k));
+}
+
+// xorshift32 step of the reference's init padding (src/main.cpp:273-275).
+__device__ __host__ __forceinline__ uint32_t xorshift32(uint32_t s) {
+  s ^= s << 13;
+  s ^= s >> 17;
+  s ^= s << 5;
+  return s;
+}
+
+// splitmix64 finaliser (Steele, Lea, Flood, OOPSLA 2014): node pivot hash.
+__device__ __host__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Pivot positions of node (tree, depth, start, size): p1 in [0,size),
+// p2 in [0,size)\{p1}.
+__device__ __host__ __forceinline__ void node_pivots(uint64_t seed, int tree, int depth,
+                                                     int64_t start, int64_t size, int32_t* p1,
+                                                     int32_t* p2) {
+  const uint64_t key =
+      (uint64_t(uint32_t(tree)) << 48) ^ (uint64_t(uint32_t(depth)) << 40) ^ uint64_t(start);
+  const uint64_t h1 = mix64(seed ^ mix64(key));
+  const uint64_t h2 = mix64(h1);
+  const int64_t a = int64_t(h1 % uint64_t(size));
+  int64_t b = int64_t(h2 % uint64_t(size - 1));
+  if (b >= a) ++b;
+  *p1 = int32_t(a);
+  *p2 = int32_t(b);
+}
+
+// ---------------------------------------------------------------------------
+// K1: quantise + sketch.  One wave per row; lane l owns dims 256g+4l..+3, so a
+// wave reads 1 KB of float32 per step and the four ballots of one step hold
+// sketch words 4g..4g+3, interleaved by 4 (bit 4i+j <- lane i, element j).
+// ---------------------------------------------------------------------------
+
+// bit i of v (16 bits) -> bit 4i
+__device__ __forceinline__ uint64_t spread4(uint64_t x) {
+  x &= 0xFFFFull;
+  x = (x | (x << 24)) & 0x000000FF000000FFull;
+  x = (x | (x << 12)) & 0x000F000F000F000Full;
+  x = (x | (x << 6)) & 0x0303030303030303ull;
+  x = (x | (x << 3)) & 0x1111111111111111ull;
+  return x;
+}
+
+// src/main.cpp:184-186: multiply, min(127), max(-128); x86 DAZ (the reference
+// is built with -ffast-math) makes a denormal input read as zero.
+__device__ __forceinline__ float q_clamp(float v) {
+  v = (fabsf(v) < kFltMin) ? 0.0f : v;
+  const float s = v * 127.0f;
+  const float hi = (s < 127.0f) ? s : 127.0f;
+  return (hi > -128.0f) ? hi : -128.0f;
+}
+
+__global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ x, int64_t nrows,
+                                                  int d, int8_t* __restrict__ q,
+                                                  uint64_t* __restrict__ sk) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  const int words = d >> 6;
+  for (int64_t r = wave; r < nrows; r += nwaves) {
+    const float* xr = x + r * d;
+    int8_t* qr = q + r * d;
+    uint64_t mine = 0;
+    for (int g = 0; g * 256 < d; ++g) {
+      const int col = g * 256 + lane * 4;
+      const bool ok = col < d;
+      float4 v = ok ? *reinterpret_cast<const float4*>(xr + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float c0 = q_clamp(v.x), c1 = q_clamp(v.y), c2 = q_clamp(v.z), c3 = q_clamp(v.w);
+      const uint32_t packed = uint32_t(uint8_t(int8_t(int(c0)))) |
+                              (uint32_t(uint8_t(int8_t(int(c1)))) << 8) |
+                              (uint32_t(uint8_t(int8_t(int(c2)))) << 16) |
+                              (uint32_t(uint8_t(int8_t(int(c3)))) << 24);
+      if (ok) *reinterpret_cast<uint32_t*>(qr + col) = packed;
+      const uint64_t b0 = __ballot(ok && c0 > 0.0f);
+      const uint64_t b1 = __ballot(ok && c1 > 0.0f);
+      const uint64_t b2 = __ballot(ok && c2 > 0.0f);
+      const uint64_t b3 = __ballot(ok && c3 > 0.0f);
+#pragma unroll
+      for (int qd = 0; qd < 4; ++qd) {
+        const int sh = 16 * qd;
+        const uint64_t w = spread4(b0 >> sh) | (spread4(b1 >> sh) << 1) |
+                           (spread4(b2 >> sh) << 2) | (spread4(b3 >> sh) << 3);
+        if (lane == 4 * g + qd) mine = w;
+      }
+    }
+    if (lane < words) sk[r * words + lane] = mine;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: RP forest, one level of every tree per pass.
+// ---------------------------------------------------------------------------
+
+struct Node {
+  int32_t tree, start, size, p1, p2;   // p1/p2: pivot positions inside the segment
+  int32_t left, right, nleft;          // children (index in next level, -1 = leaf)
+};
+
+__global__ void k_forest_seed(int64_t n, int T, int leaf, uint64_t seed, int32_t* perm,
+                              int32_t* nodepos, Node* nodes) {
+  const int64_t total = int64_t(T) * n;
+  for (int64_t gp = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; gp < total;
+       gp += int64_t(gridDim.x) * blockDim.x) {
+    const int t = int(gp / n);
+    perm[gp] = int32_t(gp - int64_t(t) * n);
+    nodepos[gp] = t;
+    if (gp - int64_t(t) * n == 0) {
+      Node nd;
+      nd.tree = t;
+      nd.start = 0;
+      nd.size = int32_t(n);
+      node_pivots(seed, t, 0, 0, n, &nd.p1, &nd.p2);
+      nd.left = nd.right = -1;
+      nd.nleft = 0;
+      nodes[t] = nd;
+    }
+  }
+}
+
+// side flag of every position in an active node: 1 = left.  One wave per
+// position (a 1 KB row per step); pivot rows stay in registers while
+// consecutive positions share a node.
+template <int CPL>   // 16-byte chunks per lane (d <= 1024*CPL)
+__global__ __launch_bounds__(256) void k_forest_flags(const int8_t* __restrict__ q, int d,
+                                                      int64_t n, int64_t total,
+                                                      const int32_t* __restrict__ perm,
+                                                      const int32_t* __restrict__ nodepos,
+                                                      const Node* __restrict__ nodes,
+                                                      int32_t* __restrict__ flags, int run) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  const int nch = d >> 4;
+  v4i pa[CPL], pb[CPL];
+  for (int64_t base = wave * run; base < total; base += nwaves * run) {
+    int cur = -1;
+    int64_t seg0 = 0;
+    int32_t p1 = 0, p2 = 0;
+    const int64_t end = base + run < total ? base + run : total;
+    for (int64_t gp = base; gp < end; ++gp) {
+      const int nd = nodepos[gp];
+      if (nd < 0) {
+        if (lane == 0) flags[gp] = 0;
+        continue;
+      }
+      if (nd != cur) {
+        const Node node = nodes[nd];
+        seg0 = int64_t(node.tree) * n + node.start;
+        p1 = node.p1;
+        p2 = node.p2;
+        const int8_t* ra = q + int64_t(perm[seg0 + p1]) * d;
+        const int8_t* rb = q + int64_t(perm[seg0 + p2]) * d;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          const int ch = lane + 64 * c;
+          pa[c] = ch < nch ? *reinterpret_cast<const v4i*>(ra + 16 * ch) : v4i{0, 0, 0, 0};
+          pb[c] = ch < nch ? *reinterpret_cast<const v4i*>(rb + 16 * ch) : v4i{0, 0, 0, 0};
+        }
+        cur = nd;
+      }
+      const int64_t rel = gp - seg0;
+      int f;
+      if (rel == p1) {
+        f = 1;
+      } else if (rel == p2) {
+        f = 0;
+      } else {
+        const int8_t* rx = q + int64_t(perm[gp]) * d;
+        int s = 0;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          const int ch = lane + 64 * c;
+          if (ch < nch) {
+            const v4i v = *reinterpret_cast<const v4i*>(rx + 16 * ch);
+            s = dot16_i8(v, pa[c], s);
+            s -= dot16_i8(v, pb[c], 0);
+          }
+        }
+        f = wave_sum_i(s) > 0 ? 1 : 0;   // <x,p1> > <x,p2>, ties go right
+      }
+      if (lane == 0) flags[gp] = f;
+    }
+  }
+}
+
+// exclusive scan over int32 (tile = 256 threads x 16 elements)
+constexpr int kScanTile = 4096;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) lds[w] = inc;
+  __syncthreads();
+  int off = 0;
+  const int nw = blockDim.x >> 6;
+  for (int j = 0; j < w; ++j) off += lds[j];
+  int tot = 0;
+  for (int j = 0; j < nw; ++j) tot += lds[j];
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
+                                                     int32_t* __restrict__ sums) {
+  __shared__ int lds[8];
+  const int64_t base = int64_t(blockIdx.x) * kScanTile + threadIdx.x * 16;
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (base + j < n) s += in[base + j];
+  int tot;
+  block_exclusive_scan(s, lds, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(int32_t* sums, int64_t nb, int32_t* out_total) {
+  __shared__ int lds[16];
+  int carry = 0;
+  for (int64_t b0 = 0; b0 < nb; b0 += blockDim.x) {
+    const int64_t j = b0 + threadIdx.x;
+    const int v = j < nb ? sums[j] : 0;
+    int tot;
+    const int ex = block_exclusive_scan(v, lds, &tot);
+    if (j < nb) sums[j] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(const int32_t* __restrict__ in, int64_t n,
+                                                   const int32_t* __restrict__ sums,
+                                                   int32_t* __restrict__ out) {
+  __shared__ int lds[8];
+  const int64_t base = int64_t(blockIdx.x) * kScanTile + threadIdx.x * 16;
+  int v[16];
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    v[j] = base + j < n ? in[base + j] : 0;
+    s += v[j];
+  }
+  int tot;
+  int run = block_exclusive_scan(s, lds, &tot) + sums[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    if (base + j < n) out[base + j] = run;
+    run += v[j];
+  }
+}
+
+// Children of every active node; children larger than a leaf become the next
+// level's nodes, the rest are appended to the leaf table.
+__global__ void k_forest_nodes(int64_t n, int leaf, uint64_t seed, int depth, Node* nodes,
+                               int nnodes, const int32_t* __restrict__ S, Node* next,
+                               int32_t* next_count, int4* leaves, int32_t* leaf_count,
+                               int64_t leaf_cap, int32_t* overflow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  Node nd = nodes[i];
+  const int64_t gs = int64_t(nd.tree) * n + nd.start;
+  const int32_t nl = S[gs + nd.size] - S[gs];
+  const int32_t cs[2] = {nd.start, nd.start + nl};
+  const int32_t cz[2] = {nl, nd.size - nl};
+  int32_t child[2];
+  for (int c = 0; c < 2; ++c) {
+    if (cz[c] > leaf) {
+      const int id = atomicAdd(next_count, 1);
+      Node ch;
+      ch.tree = nd.tree;
+      ch.start = cs[c];
+      ch.size = cz[c];
+      node_pivots(seed, nd.tree, depth + 1, cs[c], cz[c], &ch.p1, &ch.p2);
+      ch.left = ch.right = -1;
+      ch.nleft = 0;
+      next[id] = ch;
+      child[c] = id;
+    } else {
+      const int id = atomicAdd(leaf_count, 1);
+      if (id < leaf_cap)
+        leaves[id] = make_int4(nd.tree, cs[c], cz[c], 0);
+      else
+        *overflow = 1;
+      child[c] = -1;
+    }
+  }
+  nodes[i].left = child[0];
+  nodes[i].right = child[1];
+  nodes[i].nleft = nl;
+}
+
+// stable partition of each active node by its flags
+__global__ __launch_bounds__(256) void k_forest_partition(
+    int64_t n, int64_t total, const int32_t* __restrict__ perm, const int32_t* __restrict__ nodepos,
+    const Node* __restrict__ nodes, const int32_t* __restrict__ flags,
+    const int32_t* __restrict__ S, int32_t* __restrict__ perm2, int32_t* __restrict__ nodepos2) {
+  for (int64_t gp = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; gp < total;
+       gp += int64_t(gridDim.x) * blockDim.x) {
+    const int nd = nodepos[gp];
+    if (nd < 0) {
+      perm2[gp] = perm[gp];
+      nodepos2[gp] = -1;
+      continue;
+    }
+    const Node node = nodes[nd];
+    const int64_t gs = int64_t(node.tree) * n + node.start;
+    const int32_t lr = S[gp] - S[gs];
+    int64_t np;
+    int32_t child;
+    if (flags[gp]) {
+      np = gs + lr;
+      child = node.left;
+    } else {
+      np = gs + node.nleft + (gp - gs - lr);
+      child = node.right;
+    }
+    perm2[np] = perm[gp];
+    nodepos2[np] = child;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K3: leaf Gram tiles on int8 MFMA.  One block per leaf (<= 64 members), four
+// waves, wave w computes the 32x32 quadrant (w>>1, w&1) with
+// v_mfma_i32_32x32x32_i8 over K-chunks staged in LDS.  A and B fragments are
+// read with the same lane->k mapping, so the k order inside a fragment does
+// not change the exact int32 sums.  Every member's 64 slots for this tree are
+// written: key(score, mate id), or 0 for self / beyond the leaf.
+// ---------------------------------------------------------------------------
+
+constexpr int kGramKC = 512;
+
+__global__ __launch_bounds__(256) void k_leaf_gram(const int8_t* __restrict__ q, int d, int64_t n,
+                                                   int T, const int32_t* __restrict__ perm,
+                                                   const int4* __restrict__ leaves, int64_t nleaves,
+                                                   uint64_t* __restrict__ cand) {
+  __shared__ int32_t mid[64];
+  __shared__ __attribute__((aligned(16))) int8_t tile[64 * (kGramKC + 16)];
+  constexpr int kStride = kGramKC + 16;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int mq = w >> 1, nq = w & 1;
+  for (int64_t L = blockIdx.x; L < nleaves; L += gridDim.x) {
+    const int4 lf = leaves[L];
+    const int t = lf.x, start = lf.y, size = lf.z;
+    if (tid < 64) mid[tid] = tid < size ? perm[int64_t(t) * n + start + tid] : -1;
+    __syncthreads();
+    const bool rows_ok = mq * 32 < size, cols_ok = nq * 32 < size;
+    v16i acc;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0;
+    for (int k0 = 0; k0 < d; k0 += kGramKC) {
+      const int kc = (d - k0) < kGramKC ? (d - k0) : kGramKC;
+      const int upr = kc >> 4;   // 16-byte units per row
+      for (int u = tid; u < size * upr; u += 256) {
+        const int r = u / upr, c = u - r * upr;
+        *reinterpret_cast<v4i*>(tile + r * kStride + 16 * c) =
+            *reinterpret_cast<const v4i*>(q + int64_t(mid[r]) * d + k0 + 16 * c);
+      }
+      __syncthreads();
+      if (rows_ok && cols_ok) {
+        const int8_t* ra = tile + (mq * 32 + (lane & 31)) * kStride + 16 * (lane >> 5);
+        const int8_t* rb = tile + (nq * 32 + (lane & 31)) * kStride + 16 * (lane >> 5);
+        for (int ks = 0; ks < kc; ks += 32) {
+          const v4i a = *reinterpret_cast<const v4i*>(ra + ks);
+          const v4i b = *reinterpret_cast<const v4i*>(rb + ks);
+          acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc, 0, 0, 0);
+        }
+      }
+      __syncthreads();
+    }
+    if (rows_ok) {
+      const int ncol = nq * 32 + (lane & 31);
+      const int32_t mate = (cols_ok && ncol < size) ? mid[ncol] : -1;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int m = mq * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        if (m < size) {
+          const uint64_t key = (mate >= 0 && ncol != m) ? pack_key(acc[j], mate) : 0ull;
+          cand[int64_t(mid[m]) * (T * kLeafSlots) + t * kLeafSlots + ncol] = key;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K4: per-point top-K of the leaf candidates.  Bitonic sort (descending) of
+// the point's T*64 keys in LDS; duplicates (same mate from several trees)
+// carry identical keys and become adjacent; the first K distinct keys are the
+// list.  Fewer than K distinct candidates: the reference's xorshift32 padding
+// (src/main.cpp:271-312), done by one thread.
+// ---------------------------------------------------------------------------
+
+template <int S2>
+__global__ __launch_bounds__(256) void k_init_merge(const int8_t* __restrict__ q, int d, int64_t n,
+                                                    int T, int K, const uint64_t* __restrict__ cand,
+                                                    int32_t* __restrict__ gid,
+                                                    int32_t* __restrict__ gsc) {
+  __shared__ uint64_t keys[S2];
+  __shared__ int lds[8];
+  __shared__ int s_unique;
+  __shared__ uint64_t pool[2 * kMaxK];
+  __shared__ int32_t pad_id[kMaxK];
+  __shared__ int32_t pad_sc[kMaxK];
+  __shared__ int s_npad;
+  const int tid = threadIdx.x;
+  const int slots = T * kLeafSlots;
+  constexpr int PER = S2 / 256;   // elements per thread in the compaction step
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint64_t* src = cand + i * slots;
+    for (int j = tid; j < S2; j += 256) keys[j] = j < slots ? src[j] : 0ull;
+    __syncthreads();
+    for (int k = 2; k <= S2; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int p = tid; p < S2 / 2; p += 256) {
+          const int a = 2 * p - (p & (j - 1));
+          const int b = a + j;
+          const uint64_t x = keys[a], y = keys[b];
+          const bool desc = (a & k) == 0;
+          if ((x < y) == desc) {
+            keys[a] = y;
+            keys[b] = x;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // distinct valid keys, in order
+    int flag[PER];
+    int cnt = 0;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int j = tid * PER + e;
+      const uint64_t kj = keys[j];
+      flag[e] = (kj != 0ull && (j == 0 || keys[j - 1] != kj)) ? 1 : 0;
+      cnt += flag[e];
+    }
+    int tot;
+    int pos = block_exclusive_scan(cnt, lds, &tot);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      if (flag[e]) {
+        if (pos < K) {
+          const uint64_t kj = keys[tid * PER + e];
+          gid[i * K + pos] = key_id(kj);
+          gsc[i * K + pos] = key_score(kj);
+          pool[pos] = kj;
+        }
+        ++pos;
+      }
+    }
+    if (tid == 0) s_unique = tot;
+    __syncthreads();
+    const int U0 = s_unique;
+    if (U0 < K) {   // uniform across the block
+      // padding ids: src/main.cpp:271-286
+      if (tid == 0) {
+        uint32_t s = uint32_t(1337 + i);
+        int np = 0;
+        while (U0 + np < K) {
+          s = xorshift32(s);
+          const int32_t r = int32_t(s % uint64_t(n));
+          if (r == i) continue;
+          pad_id[np++] = r;
+        }
+        pad_sc[0] = int32_t(s);   // carry the xorshift state
+        s_npad = np;
+      }
+      __syncthreads();
+      const int np = s_npad;
+      const uint32_t state = uint32_t(pad_sc[0]);
+      __syncthreads();
+      // dots of the padded ids: one wave each
+      const int lane = tid & 63, w = tid >> 6;
+      for (int p = w; p < np; p += 4) {
+        const int8_t* a = q + i * d;
+        const int8_t* b = q + int64_t(pad_id[p]) * d;
+        int s = 0;
+        for (int c = lane; c < (d >> 4); c += 64)
+          s = dot16_i8(*reinterpret_cast<const v4i*>(a + 16 * c),
+                       *reinterpret_cast<const v4i*>(b + 16 * c), s);
+        s = wave_sum_i(s);
+        if (lane == 0) pad_sc[p] = s;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int m = U0;
+        for (int p = 0; p < np; ++p) pool[m++] = pack_key(pad_sc[p], pad_id[p]);
+        for (int a = 1; a < m; ++a) {   // insertion sort, descending
+          const uint64_t v = pool[a];
+          int b = a;
+          while (b > 0 && pool[b - 1] < v) {
+            pool[b] = pool[b - 1];
+            --b;
+          }
+          pool[b] = v;
+        }
+        int kept = 0;
+        int32_t prev = -1;
+        for (int a = 0; a < m && kept < K; ++a) {
+          const int32_t id = key_id(pool[a]);
+          if (id != prev) {
+            gid[i * K + kept] = id;
+            gsc[i * K + kept] = key_score(pool[a]);
+            prev = id;
+            ++kept;
+          }
+        }
+        uint32_t s = state;
+        while (kept < K) {   // src/main.cpp:303-312
+          s = xorshift32(s);
+          const int32_t r = int32_t(s % uint64_t(n));
+          if (r != i) {
+            gid[i * K + kept] = r;
+            gsc[i * K + kept] = INT32_MIN;
+            ++kept;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K5-K7: one Jacobi join iteration, one block (4 waves) per point.
+//  1. own list + self into an LDS hash set (excluded from candidates)
+//  2. 2-hop ids ids_in[own[k]][k2] -> hash set; first insert appends to cid[]
+//  3. sketch similarity, 8 lanes per candidate (128 B gather), LDS histogram
+//  4. threshold T = max t with |{h >= t}| >= M (ties at T kept)
+//  5. int8 dots of the selected, 16 lanes per candidate (1 KB gather)
+//  6. rank merge of own list and new candidates by (score desc, id asc, slot)
+//  7. write the new list, count changed slots (src/main.cpp:421-427)
+// ---------------------------------------------------------------------------
+
+struct JoinScratch {
+  int32_t own_id[kMaxK];
+  int32_t own_sc[kMaxK];
+  int32_t new_id[kMaxK];
+  int32_t new_sc[kMaxK];
+  uint64_t own_sk[kMaxDim / 64];
+  int32_t hist[kMaxDim + 1];
+  int32_t cnt[4];   // 0 unique candidates, 1 selected, 2 survivors, 3 threshold
+  int lds[8];
+};
+
+__device__ __forceinline__ bool hs_insert(int32_t* hs, int bits, int32_t key) {
+  const uint32_t mask = (1u << bits) - 1u;
+  uint32_t h = (uint32_t(key) * 2654435761u) >> (32 - bits);
+  while (true) {
+    const int32_t prev = atomicCAS(&hs[h], -1, key);
+    if (prev == -1) return true;
+    if (prev == key) return false;
+    h = (h + 1u) & mask;
+  }
+}
+
+// a before b in (score desc, id asc, slot asc)
+__device__ __forceinline__ bool before(int32_t sa, int32_t ia, int32_t la, int32_t sb, int32_t ib,
+                                       int32_t lb) {
+  if (sa != sb) return sa > sb;
+  if (ia != ib) return ia < ib;
+  return la < lb;
+}
+
+template <int HSBITS, int CAP>
+__global__ __launch_bounds__(256) void k_join(
+    const int8_t* __restrict__ q, const uint64_t* __restrict__ sk, int d, int64_t n, int K, int M,
+    const int32_t* __restrict__ ids_in, const int32_t* __restrict__ sc_in,
+    int32_t* __restrict__ ids_out, int32_t* __restrict__ sc_out,
+    unsigned long long* __restrict__ counters) {
+  constexpr int HS = 1 << HSBITS;
+  static_assert(HS * 4 >= CAP * 2 + CAP * 4, "selection arrays must fit in the hash area");
+  __shared__ __attribute__((aligned(16))) int32_t hs[HS];
+  __shared__ int32_t cid[CAP];
+  __shared__ uint16_t ch[CAP];
+  __shared__ __attribute__((aligned(16))) int8_t own_row[kMaxDim];
+  __shared__ JoinScratch S;
+  uint16_t* sel_idx = reinterpret_cast<uint16_t*>(hs);            // [CAP]
+  int32_t* sel_sc = hs + CAP / 2;                                 // [CAP]
+  uint16_t* surv = ch;                                            // [CAP] after selection
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int words = d >> 6, nch = d >> 4;
+  unsigned long long my_upd = 0, my_cand = 0, my_sel = 0;
+
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    // ---- 0/1: reset, own list
+    for (int j = tid; j < HS; j += 256) hs[j] = -1;
+    for (int j = tid; j <= d; j += 256) S.hist[j] = 0;
+    if (tid < 4) S.cnt[tid] = 0;
+    for (int c = tid; c < nch; c += 256)
+      *reinterpret_cast<v4i*>(own_row + 16 * c) = *reinterpret_cast<const v4i*>(q + i * d + 16 * c);
+    if (tid < words) S.own_sk[tid] = sk[i * words + tid];
+    if (tid < K) {
+      S.own_id[tid] = ids_in[i * K + tid];
+      S.own_sc[tid] = sc_in[i * K + tid];
+    }
+    __syncthreads();
+    if (tid < K) hs_insert(hs, HSBITS, S.own_id[tid]);
+    if (tid == K) hs_insert(hs, HSBITS, int32_t(i));
+    __syncthreads();
+    // ---- 2: 2-hop candidates, one neighbour row per wave step
+    for (int k = w; k < K; k += 4) {
+      const int32_t n1 = S.own_id[k];
+      if (lane < K) {
+        const int32_t c = ids_in[int64_t(n1) * K + lane];
+        if (hs_insert(hs, HSBITS, c)) {
+          const int slot = atomicAdd(&S.cnt[0], 1);
+          cid[slot] = c;
+        }
+      }
+    }
+    __syncthreads();
+    const int U = S.cnt[0];
+    // ---- 3: sketch similarity, 8 lanes per candidate
+    {
+      const int sub = tid & 7;
+      uint64_t o0 = 0, o1 = 0;
+      if (2 * sub < words) o0 = S.own_sk[2 * sub];
+      if (2 * sub + 1 < words) o1 = S.own_sk[2 * sub + 1];
+      for (int u0 = 0; u0 < U; u0 += 32) {
+        const int u = u0 + (tid >> 3);
+        int pc = 0;
+        if (u < U) {
+          const uint64_t* cs = sk + int64_t(cid[u]) * words;
+          if (2 * sub < words) pc = __popcll(cs[2 * sub] ^ o0);
+          if (2 * sub + 1 < words) pc += __popcll(cs[2 * sub + 1] ^ o1);
+        }
+        pc += __shfl_xor(pc, 1);
+        pc += __shfl_xor(pc, 2);
+        pc += __shfl_xor(pc, 4);
+        if (u < U && sub == 0) {
+          const int h = d - pc;
+          ch[u] = uint16_t(h);
+          atomicAdd(&S.hist[h], 1);
+        }
+      }
+    }
+    __syncthreads();
+    // ---- 4: threshold (wave 0): largest t with count(h >= t) >= M
+    if (w == 0) {
+      int thr = 0;
+      if (U > M) {
+        const int per = (d + 1 + 63) / 64;
+        const int hi = d - lane * per;            // this lane's bins: hi, hi-1, ..., hi-per+1
+        int local = 0;
+        for (int b = 0; b < per; ++b)
+          if (hi - b >= 0) local += S.hist[hi - b];
+        int inc = local;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int t = __shfl_up(inc, o);
+          if (lane >= o) inc += t;
+        }
+        const int excl = inc - local;
+        const unsigned long long hit = __ballot(inc >= M && excl < M);
+        const int src = __ffsll(hit) - 1;
+        int t_lane = 0;
+        if (lane == src) {
+          int acc = excl;
+          for (int b = 0; b < per; ++b) {
+            const int bin = hi - b;
+            if (bin < 0) break;
+            acc += S.hist[bin];
+            if (acc >= M) {
+              t_lane = bin;
+              break;
+            }
+          }
+        }
+        thr = __shfl(t_lane, src);
+      }
+      if (lane == 0) S.cnt[3] = thr;
+    }
+    __syncthreads();
+    // ---- selection
+    {
+      const int thr = S.cnt[3];
+      for (int u = tid; u < U; u += 256)
+        if (ch[u] >= thr) {
+          const int s = atomicAdd(&S.cnt[1], 1);
+          sel_idx[s] = uint16_t(u);
+        }
+    }
+    __syncthreads();
+    const int NS = S.cnt[1];
+    // ---- 5: int8 dots, 16 lanes per candidate, 4 candidates per wave step
+    {
+      const int g = lane >> 4, s16 = lane & 15;
+      v4i own[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int chn = s16 + 16 * c;
+        own[c] = chn < nch ? *reinterpret_cast<const v4i*>(own_row + 16 * chn) : v4i{0, 0, 0, 0};
+      }
+      for (int s0 = 0; s0 < NS; s0 += 32) {
+        const int sa = s0 + w * 4 + g;        // first candidate of this group
+        const int sb = sa + 16;               // second, issued together
+        const int8_t* ra = sa < NS ? q + int64_t(cid[sel_idx[sa]]) * d : nullptr;
+        const int8_t* rb = sb < NS ? q + int64_t(cid[sel_idx[sb]]) * d : nullptr;
+        v4i va[4], vb[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int chn = s16 + 16 * c;
+          va[c] = (ra && chn < nch) ? *reinterpret_cast<const v4i*>(ra + 16 * chn) : v4i{0, 0, 0, 0};
+          vb[c] = (rb && chn < nch) ? *reinterpret_cast<const v4i*>(rb + 16 * chn) : v4i{0, 0, 0, 0};
+        }
+        int da = 0, db = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          da = dot16_i8(va[c], own[c], da);
+          db = dot16_i8(vb[c], own[c], db);
+        }
+#pragma unroll
+        for (int m = 8; m >= 1; m >>= 1) {
+          da += __shfl_xor(da, m);
+          db += __shfl_xor(db, m);
+        }
+        if (s16 == 0) {
+          if (sa < NS) sel_sc[sa] = da;
+          if (sb < NS) sel_sc[sb] = db;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- 6a: survivors (fewer than K own entries precede them)
+    for (int s = tid; s < NS; s += 256) {
+      const int32_t sc = sel_sc[s], id = cid[sel_idx[s]];
+      int c = 0;
+      for (int k = 0; k < K; ++k) c += before(S.own_sc[k], S.own_id[k], k, sc, id, K) ? 1 : 0;
+      if (c < K) {
+        const int p = atomicAdd(&S.cnt[2], 1);
+        surv[p] = uint16_t(s);
+      }
+    }
+    __syncthreads();
+    const int P = S.cnt[2];
+    // ---- 6b: ranks
+    for (int e = tid; e < K + P; e += 256) {
+      int32_t sc, id, sl;
+      if (e < K) {
+        sc = S.own_sc[e];
+        id = S.own_id[e];
+        sl = e;
+      } else {
+        const int s = surv[e - K];
+        sc = sel_sc[s];
+        id = cid[sel_idx[s]];
+        sl = K;
+      }
+      int r = 0;
+      for (int k = 0; k < K; ++k) r += before(S.own_sc[k], S.own_id[k], k, sc, id, sl) ? 1 : 0;
+      for (int p = 0; p < P && r < K; ++p) {
+        const int s = surv[p];
+        r += before(sel_sc[s], cid[sel_idx[s]], K, sc, id, sl) ? 1 : 0;
+      }
+      if (r < K) {
+        S.new_id[r] = id;
+        S.new_sc[r] = sc;
+      }
+    }
+    __syncthreads();
+    // ---- 7: write back, count changes
+    if (w == 0) {
+      bool changed = false;
+      if (lane < K) {
+        ids_out[i * K + lane] = S.new_id[lane];
+        sc_out[i * K + lane] = S.new_sc[lane];
+        changed = S.new_id[lane] != S.own_id[lane];
+      }
+      const unsigned long long m = __ballot(changed);
+      if (lane == 0) {
+        my_upd += __popcll(m);
+        my_cand += U;
+        my_sel += NS;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    unsigned long long* c = counters + 4 * (blockIdx.x % kNumCounters);
+    atomicAdd(c + 0, my_upd);
+    atomicAdd(c + 1, my_cand);
+    atomicAdd(c + 2, my_sel);
+  }
+}
+
+__global__ void k_sum_counters(const unsigned long long* __restrict__ c,
+                               unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long part[3][256];
+  unsigned long long a = 0, b = 0, e = 0;
+  for (int j = threadIdx.x; j < kNumCounters; j += 256) {
+    a += c[4 * j];
+    b += c[4 * j + 1];
+    e += c[4 * j + 2];
+  }
+  part[0][threadIdx.x] = a;
+  part[1][threadIdx.x] = b;
+  part[2][threadIdx.x] = e;
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    unsigned long long s = 0;
+    for (int j = 0; j < 256; ++j) s += part[threadIdx.x][j];
+    out[threadIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K8: export.  Plain: src/main.cpp:449-457.  Rerank: FP32 dots of the first R
+// list entries, one wave per point, best k by (score desc, id asc, slot).
+// ---------------------------------------------------------------------------
+
+__global__ void k_export_plain(const int32_t* __restrict__ gid, const int32_t* __restrict__ gsc,
+                               int64_t n, int K, int k, int32_t* __restrict__ knns,
+                               float* __restrict__ dists) {
+  const int64_t total = n * k;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = e / k;
+    const int j = int(e - i * k);
+    knns[i * k + (k - 1 - j)] = gid[i * K + j] + 1;
+    dists[i * k + (k - 1 - j)] = float(gsc[i * K + j]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_export_rerank(const float* __restrict__ x, int d,
+                                                       int64_t n, int K, int R, int k,
+                                                       const int32_t* __restrict__ gid,
+                                                       int32_t* __restrict__ knns,
+                                                       float* __restrict__ dists) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nch = d >> 2;   // float4 chunks
+  for (int64_t i = int64_t(blockIdx.x) * 4 + w; i < n; i += int64_t(gridDim.x) * 4) {
+    float4 own[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int chn = lane + 64 * c;
+      own[c] = chn < nch ? reinterpret_cast<const float4*>(x + i * d)[chn]
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    // lane r ends up holding candidate r's id and FP32 score
+    const int32_t my_id = lane < R ? gid[i * K + lane] : -1;
+    float my_s = 0.f;
+    for (int r = 0; r < R; r += 2) {
+      const int32_t ia = __shfl(my_id, r);
+      const int32_t ib = __shfl(my_id, r + 1 < R ? r + 1 : r);
+      const float4* pa = reinterpret_cast<const float4*>(x + int64_t(ia) * d);
+      const float4* pb = reinterpret_cast<const float4*>(x + int64_t(ib) * d);
+      float sa = 0.f, sb = 0.f;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int chn = lane + 64 * c;
+        if (chn < nch) {
+          const float4 a = pa[chn], b = pb[chn];
+          sa = fmaf(own[c].x, a.x, sa);
+          sa = fmaf(own[c].y, a.y, sa);
+          sa = fmaf(own[c].z, a.z, sa);
+          sa = fmaf(own[c].w, a.w, sa);
+          sb = fmaf(own[c].x, b.x, sb);
+          sb = fmaf(own[c].y, b.y, sb);
+          sb = fmaf(own[c].z, b.z, sb);
+          sb = fmaf(own[c].w, b.w, sb);
+        }
+      }
+      sa = wave_sum_f(sa);
+      sb = wave_sum_f(sb);
+      if (lane == r) my_s = sa;
+      if (lane == r + 1) my_s = sb;
+    }
+    // rank among the R by (score desc, id asc, slot asc)
+    int rank = 0;
+    for (int r = 0; r < R; ++r) {
+      const float t = __shfl(my_s, r);
+      const int32_t tid = __shfl(my_id, r);
+      rank += (t > my_s || (t == my_s && (tid < my_id || (tid == my_id && r < lane)))) ? 1 : 0;
+    }
+    if (lane < R && rank < k) {
+      knns[i * k + (k - 1 - rank)] = my_id + 1;
+      dists[i * k + (k - 1 - rank)] = my_s;
+    }
+  }
+}
+
+// stand-alone gathered pair scores (wave per pair)
+__global__ __launch_bounds__(256) void k_score_pairs(const int8_t* __restrict__ q,
+                                                     const uint64_t* __restrict__ sk, int d,
+                                                     const int32_t* __restrict__ a,
+                                                     const int32_t* __restrict__ b, int64_t np,
+                                                     int32_t* __restrict__ dot,
+                                                     int32_t* __restrict__ sim) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  const int words = d >> 6, nch = d >> 4;
+  for (int64_t p = wave; p < np; p += nwaves) {
+    const int8_t* ra = q + int64_t(a[p]) * d;
+    const int8_t* rb = q + int64_t(b[p]) * d;
+    int s = 0;
+    for (int c = lane; c < nch; c += 64)
+      s = dot16_i8(*reinterpret_cast<const v4i*>(ra + 16 * c),
+                   *reinterpret_cast<const v4i*>(rb + 16 * c), s);
+    s = wave_sum_i(s);
+    int pc = lane < words ? __popcll(sk[int64_t(a[p]) * words + lane] ^
+                                     sk[int64_t(b[p]) * words + lane])
+                          : 0;
+    pc = wave_sum_i(pc);
+    if (lane == 0) {
+      dot[p] = s;
+      sim[p] = d - pc;
+    }
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+
+struct knng_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  knng_params p{};
+  int64_t n = 0;
+  int32_t d = 0, words = 0;
+  int num_cu = 256;
+  std::string err;
+
+  float* x = nullptr;          // owned fp32 rows (keep_fp32)
+  const float* x_ext = nullptr;
+  int8_t* q = nullptr;
+  uint64_t* sk = nullptr;
+  int64_t ingested = 0;
+
+  // forest
+  int32_t *perm = nullptr, *perm2 = nullptr, *nodepos = nullptr, *nodepos2 = nullptr;
+  int32_t *flags = nullptr, *scan = nullptr, *blocksums = nullptr;
+  Node *nodes = nullptr, *nodes2 = nullptr;
+  int4* leaves = nullptr;
+  int32_t* dcount = nullptr;   // [0] next nodes, [1] leaves, [2] overflow, [3] scan total
+  int64_t node_cap = 0, leaf_cap = 0, nleaves = 0;
+  bool have_forest = false;
+
+  // graph
+  int32_t *gid[2] = {nullptr, nullptr}, *gsc[2] = {nullptr, nullptr};
+  int cur = 0;
+  bool have_graph = false;
+  unsigned long long *counters = nullptr, *csum = nullptr;
+
+  // ingest staging
+  void* pin[2] = {nullptr, nullptr};
+  float* dstage[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  size_t stage_bytes = 64ull << 20;
+
+  // export scratch
+  int32_t* ex_knns = nullptr;
+  float* ex_dists = nullptr;
+  int64_t ex_cap = 0;
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  knng_stats st{};
+};
+
+namespace {
+
+int fail(knng_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                 \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(ctx, KNNG_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+template <typename T>
+int dalloc(knng_ctx* c, T** ptr, size_t count, const char* what) {
+  if (*ptr) return 0;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T) + 256);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return fail(c, KNNG_ENOMEM, std::string("hipMalloc failed for ") + what + " (" +
+                                     std::to_string(count * sizeof(T)) + " B): " +
+                                     hipGetErrorString(e));
+  }
+  return 0;
+}
+
+template <typename T>
+void dfree(T** ptr) {
+  if (*ptr) hipFree(reinterpret_cast<void*>(const_cast<typename std::remove_const<T>::type*>(*ptr)));
+  *ptr = nullptr;
+}
+
+int grid_for(const knng_ctx* c, int64_t work, int per_block, int blocks_per_cu) {
+  int64_t g = (work + per_block - 1) / per_block;
+  const int64_t cap = int64_t(c->num_cu) * blocks_per_cu;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return int(g);
+}
+
+int quantize_rows(knng_ctx* c, const float* dx, int64_t row0, int64_t nrows) {
+  if (nrows <= 0) return 0;
+  const int grid = grid_for(c, nrows, 4, 64);
+  hipLaunchKernelGGL(k_quantize, dim3(grid), dim3(256), 0, c->stream, dx, nrows, c->d,
+                     c->q + row0 * c->d, c->sk + row0 * c->words);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int check_params(const knng_params* p, int64_t n, std::string* why) {
+  if (n < 2) return *why = "n must be >= 2", KNNG_EINVAL;
+  if (p->dim <= 0 || p->dim % 64 || p->dim > kMaxDim) return *why = "dim must be a multiple of 64 in [64, 1024]", KNNG_ERANGE;
+  if (p->k_build < 1 || p->k_build > kMaxK) return *why = "k_build must be in [1, 64]", KNNG_ERANGE;
+  if (p->k_eval < 1 || p->k_eval > p->k_build) return *why = "k_eval must be in [1, k_build]", KNNG_EINVAL;
+  if (p->num_trees < 1 || p->num_trees > kMaxTrees) return *why = "num_trees must be in [1, 64]", KNNG_ERANGE;
+  if (p->leaf_size < 2 || p->leaf_size > kLeafSlots) return *why = "leaf_size must be in [2, 64]", KNNG_ERANGE;
+  if (p->rerank_mult < 1) return *why = "rerank_mult must be >= 1", KNNG_EINVAL;
+  if (p->max_iters < 0 || p->max_iters > 32) return *why = "max_iters must be in [0, 32]", KNNG_ERANGE;
+  if (p->rerank_width < 0 || p->rerank_width > p->k_build) return *why = "rerank_width must be in [0, k_build]", KNNG_EINVAL;
+  if (p->rerank_width && p->rerank_width < p->k_eval) return *why = "rerank_width must be >= k_eval", KNNG_EINVAL;
+  if (int64_t(p->num_trees) * n >= (int64_t(1) << 31) - 1) return *why = "num_trees * n must fit in int32", KNNG_ERANGE;
+  if (n >= (int64_t(1) << 31) - 1) return *why = "n must fit in int32", KNNG_ERANGE;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int knng_default_params(knng_params* p) {
+  if (!p) return KNNG_EINVAL;
+  p->dim = 1024;
+  p->k_eval = 15;
+  p->k_build = 50;
+  p->num_trees = 16;
+  p->rerank_mult = 8;
+  p->leaf_size = 64;
+  p->max_iters = 30;
+  p->early_stop_rate = 0.04f;
+  p->rate_diff_tol = 0.01f;
+  p->seed = 1337;
+  p->keep_fp32 = 1;
+  p->rerank_width = 0;
+  return 0;
+}
+
+const char* knng_last_error(const knng_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int knng_create(knng_ctx** out, int32_t device, const knng_params* p, int64_t n) {
+  if (!out || !p) return KNNG_EINVAL;
+  *out = nullptr;
+  std::string why;
+  if (int rc = check_params(p, n, &why)) {
+    fprintf(stderr, "[knng] knng_create: %s\n", why.c_str());
+    return rc;
+  }
+  knng_ctx* c = new knng_ctx();
+  c->device = device;
+  c->p = *p;
+  c->n = n;
+  c->d = p->dim;
+  c->words = p->dim / 64;
+  auto bail = [&](int rc) {
+    fprintf(stderr, "[knng] knng_create: %s\n", c->err.c_str());
+    knng_destroy(c);
+    return rc;
+  };
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return bail(fail(c, KNNG_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(c, KNNG_EHIP, "hipStreamCreate failed"));
+  hipEventCreate(&c->ev0);
+  hipEventCreate(&c->ev1);
+  int rc;
+  if ((rc = dalloc(c, &c->q, size_t(n) * c->d, "int8 rows"))) return bail(rc);
+  if ((rc = dalloc(c, &c->sk, size_t(n) * c->words, "sketches"))) return bail(rc);
+  if (p->keep_fp32 && (rc = dalloc(c, &c->x, size_t(n) * c->d, "float32 rows"))) return bail(rc);
+  if ((rc = dalloc(c, &c->counters, 4 * kNumCounters, "counters"))) return bail(rc);
+  if ((rc = dalloc(c, &c->csum, 4, "counter sums"))) return bail(rc);
+  c->st.n = n;
+  c->st.dim = c->d;
+  *out = c;
+  return 0;
+}
+
+int knng_destroy(knng_ctx* c) {
+  if (!c) return 0;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  dfree(&c->x);
+  dfree(&c->q);
+  dfree(&c->sk);
+  dfree(&c->perm);
+  dfree(&c->perm2);
+  dfree(&c->nodepos);
+  dfree(&c->nodepos2);
+  dfree(&c->flags);
+  dfree(&c->scan);
+  dfree(&c->blocksums);
+  dfree(&c->nodes);
+  dfree(&c->nodes2);
+  dfree(&c->leaves);
+  dfree(&c->dcount);
+  for (int b = 0; b < 2; ++b) {
+    dfree(&c->gid[b]);
+    dfree(&c->gsc[b]);
+    dfree(&c->dstage[b]);
+    if (c->pin[b]) hipHostFree(c->pin[b]);
+    if (c->pin_ev[b]) hipEventDestroy(c->pin_ev[b]);
+  }
+  dfree(&c->counters);
+  dfree(&c->csum);
+  dfree(&c->ex_knns);
+  dfree(&c->ex_dists);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int knng_synchronize(knng_ctx* c) {
+  if (!c) return KNNG_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int knng_attach_fp32_device(knng_ctx* c, const float* dev_x) {
+  if (!c || !dev_x) return KNNG_EINVAL;
+  c->x_ext = dev_x;
+  return 0;
+}
+
+int knng_ingest_device(knng_ctx* c, const float* dev_rows, int64_t row0, int64_t nrows) {
+  if (!c || !dev_rows || row0 < 0 || nrows < 0 || row0 + nrows > c->n)
+    return fail(c, KNNG_EINVAL, "knng_ingest_device: bad row range");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  if (c->x)
+    HIPCHK(c, hipMemcpyAsync(c->x + row0 * c->d, dev_rows, size_t(nrows) * c->d * sizeof(float),
+                             hipMemcpyDeviceToDevice, c->stream));
+  if (int rc = quantize_rows(c, dev_rows, row0, nrows)) return rc;
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->st.ms_ingest += ms;
+  c->ingested += nrows;
+  return 0;
+}
+
+int knng_ingest_chunk(knng_ctx* c, const float* rows, int64_t row0, int64_t nrows) {
+  if (!c || !rows || row0 < 0 || nrows < 0 || row0 + nrows > c->n)
+    return fail(c, KNNG_EINVAL, "knng_ingest_chunk: bad row range");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t row_bytes = size_t(c->d) * sizeof(float);
+  const int64_t slice_rows = std::max<int64_t>(1, int64_t(c->stage_bytes / row_bytes));
+  for (int b = 0; b < 2; ++b) {
+    if (!c->pin[b]) {
+      HIPCHK(c, hipHostMalloc(&c->pin[b], c->stage_bytes, hipHostMallocDefault));
+      HIPCHK(c, hipEventCreateWithFlags(&c->pin_ev[b], hipEventDisableTiming));
+      HIPCHK(c, hipEventRecord(c->pin_ev[b], c->stream));
+    }
+    if (!c->x && !c->dstage[b])
+      if (int rc = dalloc(c, &c->dstage[b], c->stage_bytes / sizeof(float), "ingest staging")) return rc;
+  }
+  // a caller buffer that is already pinned is copied directly
+  bool pinned = false;
+  {
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, rows) == hipSuccess && attr.type == hipMemoryTypeHost)
+      pinned = true;
+    (void)hipGetLastError();
+  }
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  int buf = 0;
+  for (int64_t r = 0; r < nrows; r += slice_rows) {
+    const int64_t m = std::min(slice_rows, nrows - r);
+    const float* src = rows + r * c->d;
+    if (!pinned) {
+      HIPCHK(c, hipEventSynchronize(c->pin_ev[buf]));   // staging buffer free again
+      memcpy(c->pin[buf], src, size_t(m) * row_bytes);
+      src = static_cast<const float*>(c->pin[buf]);
+    }
+    float* dst = c->x ? c->x + (row0 + r) * c->d : c->dstage[buf];
+    HIPCHK(c, hipMemcpyAsync(dst, src, size_t(m) * row_bytes, hipMemcpyHostToDevice, c->stream));
+    if (int rc = quantize_rows(c, dst, row0 + r, m)) return rc;
+    HIPCHK(c, hipEventRecord(c->pin_ev[buf], c->stream));
+    buf ^= 1;
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->st.ms_ingest += ms;
+  c->ingested += nrows;
+  return 0;
+}
+
+static int scan_flags(knng_ctx* c, int64_t total) {
+  const int64_t nb = (total + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, c->flags, total, c->blocksums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream, c->blocksums, nb, c->dcount + 3);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, c->stream, c->flags, total, c->blocksums,
+                     c->scan);
+  // scan[total] = grand total
+  HIPCHK(c, hipMemcpyAsync(c->scan + total, c->dcount + 3, sizeof(int32_t), hipMemcpyDeviceToDevice,
+                           c->stream));
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int knng_build_forest(knng_ctx* c) {
+  if (!c) return KNNG_EINVAL;
+  if (c->ingested < c->n) return fail(c, KNNG_ESTATE, "knng_build_forest: not all rows ingested");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t n = c->n, T = c->p.num_trees, total = T * n;
+  const int leaf = c->p.leaf_size;
+  c->node_cap = T * (n / (leaf + 1) + 1) + 16;
+  c->leaf_cap = total / 4 + T + 1024;
+  int rc;
+  if ((rc = dalloc(c, &c->perm, total, "perm")) || (rc = dalloc(c, &c->perm2, total, "perm2")) ||
+      (rc = dalloc(c, &c->nodepos, total, "nodepos")) ||
+      (rc = dalloc(c, &c->nodepos2, total, "nodepos2")) ||
+      (rc = dalloc(c, &c->flags, total, "flags")) || (rc = dalloc(c, &c->scan, total + 1, "scan")) ||
+      (rc = dalloc(c, &c->blocksums, total / kScanTile + 2, "scan blocks")) ||
+      (rc = dalloc(c, &c->nodes, c->node_cap, "nodes")) ||
+      (rc = dalloc(c, &c->nodes2, c->node_cap, "nodes2")) ||
+      (rc = dalloc(c, &c->leaves, c->leaf_cap, "leaves")) || (rc = dalloc(c, &c->dcount, 8, "counts")))
+    return rc;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->dcount, 0, 8 * sizeof(int32_t), c->stream));
+  hipLaunchKernelGGL(k_forest_seed, dim3(grid_for(c, total, 256, 16)), dim3(256), 0, c->stream, n,
+                     int(T), leaf, c->p.seed, c->perm, c->nodepos, c->nodes);
+  HIPCHK(c, hipGetLastError());
+  int nnodes = int(T);
+  int levels = 0;
+  if (n <= leaf) {
+    // every tree is one leaf
+    std::vector<int4> lv(T);
+    for (int t = 0; t < T; ++t) lv[t] = make_int4(t, 0, int(n), 0);
+    HIPCHK(c, hipMemcpyAsync(c->leaves, lv.data(), T * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    int32_t cnt[2] = {0, int32_t(T)};
+    HIPCHK(c, hipMemcpyAsync(c->dcount, cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    nnodes = 0;
+  }
+  const int nch = c->d / 16;
+  while (nnodes > 0) {
+    const int run = 16;
+    const int fgrid = grid_for(c, (total + run - 1) / run, 4, 32);
+    if (nch <= 64)
+      hipLaunchKernelGGL(k_forest_flags<1>, dim3(fgrid), dim3(256), 0, c->stream, c->q, c->d, n, total,
+                         c->perm, c->nodepos, c->nodes, c->flags, run);
+    else
+      return fail(c, KNNG_ERANGE, "dim too large for the forest kernel");
+    if ((rc = scan_flags(c, total))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->dcount, 0, sizeof(int32_t), c->stream));   // next-node count
+    hipLaunchKernelGGL(k_forest_nodes, dim3((nnodes + 255) / 256), dim3(256), 0, c->stream, n, leaf,
+                       c->p.seed, levels, c->nodes, nnodes, c->scan, c->nodes2, c->dcount,
+                       c->leaves, c->dcount + 1, c->leaf_cap, c->dcount + 2);
+    hipLaunchKernelGGL(k_forest_partition, dim3(grid_for(c, total, 256, 16)), dim3(256), 0, c->stream,
+                       n, total, c->perm, c->nodepos, c->nodes, c->flags, c->scan, c->perm2,
+                       c->nodepos2);
+    HIPCHK(c, hipGetLastError());
+    int32_t cnt[3];
+    HIPCHK(c, hipMemcpyAsync(cnt, c->dcount, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (cnt[2]) return fail(c, KNNG_ERANGE, "leaf table overflow");
+    std::swap(c->perm, c->perm2);
+    std::swap(c->nodepos, c->nodepos2);
+    std::swap(c->nodes, c->nodes2);
+    nnodes = cnt[0];
+    ++levels;
+    if (levels > 4096) return fail(c, KNNG_ERANGE, "forest did not terminate");
+  }
+  int32_t nl = 0;
+  HIPCHK(c, hipMemcpyAsync(&nl, c->dcount + 1, sizeof(nl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->st.ms_forest = ms;
+  c->st.forest_levels = levels;
+  c->st.num_leaves = nl;
+  c->nleaves = nl;
+  c->have_forest = true;
+  c->have_graph = false;
+  return 0;
+}
+
+int knng_build_init(knng_ctx* c) {
+  if (!c) return KNNG_EINVAL;
+  if (!c->have_forest) return fail(c, KNNG_ESTATE, "knng_build_init: no forest");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t n = c->n;
+  const int T = c->p.num_trees, K = c->p.k_build;
+  int rc;
+  for (int b = 0; b < 2; ++b)
+    if ((rc = dalloc(c, &c->gid[b], size_t(n) * K, "graph ids")) ||
+        (rc = dalloc(c, &c->gsc[b], size_t(n) * K, "graph scores")))
+      return rc;
+  uint64_t* cand = nullptr;
+  if ((rc = dalloc(c, &cand, size_t(n) * T * kLeafSlots, "init candidates"))) return rc;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(k_leaf_gram, dim3(grid_for(c, c->nleaves, 1, 64)), dim3(256), 0, c->stream, c->q,
+                     c->d, n, T, c->perm, c->leaves, c->nleaves, cand);
+  const int slots = T * kLeafSlots;
+  const int mgrid = grid_for(c, n, 1, 32);
+  c->cur = 0;
+  if (slots <= 256)
+    hipLaunchKernelGGL(k_init_merge<256>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
+                       cand, c->gid[0], c->gsc[0]);
+  else if (slots <= 512)
+    hipLaunchKernelGGL(k_init_merge<512>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
+                       cand, c->gid[0], c->gsc[0]);
+  else if (slots <= 1024)
+    hipLaunchKernelGGL(k_init_merge<1024>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
+                       cand, c->gid[0], c->gsc[0]);
+  else if (slots <= 2048)
+    hipLaunchKernelGGL(k_init_merge<2048>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
+                       cand, c->gid[0], c->gsc[0]);
+  else
+    hipLaunchKernelGGL(k_init_merge<4096>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
+                       cand, c->gid[0], c->gsc[0]);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->st.ms_init = ms;
+  dfree(&cand);
+  c->have_graph = true;
+  return 0;
+}
+
+int knng_join_iteration(knng_ctx* c, int64_t* updates) {
+  if (!c) return KNNG_EINVAL;
+  if (!c->have_graph) return fail(c, KNNG_ESTATE, "knng_join_iteration: no graph");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t n = c->n;
+  const int K = c->p.k_build, M = K * c->p.rerank_mult;
+  HIPCHK(c, hipMemsetAsync(c->counters, 0, 4 * kNumCounters * sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  const int a = c->cur, b = c->cur ^ 1;
+  const int grid = grid_for(c, n, 1, 4);
+  if (K <= 50)
+    hipLaunchKernelGGL((k_join<12, 2560>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n, K,
+                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters);
+  else
+    hipLaunchKernelGGL((k_join<13, 4096>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n, K,
+                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(256), 0, c->stream, c->counters, c->csum);
+  unsigned long long sums[3];
+  HIPCHK(c, hipMemcpyAsync(sums, c->csum, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->cur = b;
+  const int it = c->st.iterations;
+  if (it < 32) {
+    c->st.updates[it] = int64_t(sums[0]);
+    c->st.rates[it] = float(sums[0]) / float(size_t(n) * size_t(K));
+    c->st.ms_join_iter[it] = ms;
+    c->st.join_candidates[it] = int64_t(sums[1]);
+    c->st.join_rescored[it] = int64_t(sums[2]);
+    // algorithmic bytes: own list + own row/sketch, K neighbour id rows,
+    // one sketch per unique candidate, one row per re-scored candidate, writes
+    c->st.join_bytes[it] = double(n) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) +
+                           double(sums[1]) * 8.0 * c->words + double(sums[2]) * c->d;
+  }
+  c->st.iterations = it + 1;
+  c->st.ms_join += ms;
+  if (updates) *updates = int64_t(sums[0]);
+  return 0;
+}
+
+int knng_build(knng_ctx* c) {
+  if (!c) return KNNG_EINVAL;
+  memset(c->st.updates, 0, sizeof(c->st.updates));
+  c->st.iterations = 0;
+  c->st.ms_join = 0;
+  int rc;
+  if ((rc = knng_build_forest(c))) return rc;
+  if ((rc = knng_build_init(c))) return rc;
+  float prev = 1.0f;
+  for (int it = 0; it < c->p.max_iters; ++it) {
+    int64_t upd = 0;
+    if ((rc = knng_join_iteration(c, &upd))) return rc;
+    // stop rule: src/main.cpp:432-439
+    const float rate = float(upd) / float(size_t(c->n) * size_t(c->p.k_build));
+    if (rate < c->p.early_stop_rate || (it > 0 && (prev - rate) < c->p.rate_diff_tol)) break;
+    prev = rate;
+  }
+  c->st.ms_build = c->st.ms_forest + c->st.ms_init + c->st.ms_join;
+  return 0;
+}
+
+int knng_export_device(knng_ctx* c, int32_t* dk, float* dd, int32_t k, int32_t rerank) {
+  if (!c || !dk || !dd) return KNNG_EINVAL;
+  if (!c->have_graph) return fail(c, KNNG_ESTATE, "knng_export: no graph");
+  if (k < 1 || k > c->p.k_build) return fail(c, KNNG_EINVAL, "knng_export: k must be in [1, k_build]");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t n = c->n;
+  const int K = c->p.k_build;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  if (!rerank) {
+    hipLaunchKernelGGL(k_export_plain, dim3(grid_for(c, n * k, 256, 16)), dim3(256), 0, c->stream,
+                       c->gid[c->cur], c->gsc[c->cur], n, K, k, dk, dd);
+  } else {
+    const float* x = c->x_ext ? c->x_ext : c->x;
+    if (!x) return fail(c, KNNG_ESTATE, "knng_export: rerank needs float32 rows (keep_fp32 or attach)");
+    int R = c->p.rerank_width ? c->p.rerank_width : K;
+    if (R < k) R = k;
+    hipLaunchKernelGGL(k_export_rerank, dim3(grid_for(c, n, 4, 16)), dim3(256), 0, c->stream, x, c->d,
+                       n, K, R, k, c->gid[c->cur], dk, dd);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->st.ms_export = ms;
+  return 0;
+}
+
+int knng_export(knng_ctx* c, int32_t* knns, float* dists, int32_t k, int32_t rerank) {
+  if (!c || !knns || !dists) return KNNG_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t need = c->n * k;
+  if (c->ex_cap < need) {
+    dfree(&c->ex_knns);
+    dfree(&c->ex_dists);
+    int rc;
+    if ((rc = dalloc(c, &c->ex_knns, need, "export ids")) || (rc = dalloc(c, &c->ex_dists, need, "export dists")))
+      return rc;
+    c->ex_cap = need;
+  }
+  if (int rc = knng_export_device(c, c->ex_knns, c->ex_dists, k, rerank)) return rc;
+  HIPCHK(c, hipMemcpyAsync(knns, c->ex_knns, need * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dists, c->ex_dists, need * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int knng_get_stats(knng_ctx* c, knng_stats* st) {
+  if (!c || !st) return KNNG_EINVAL;
+  *st = c->st;
+  return 0;
+}
+
+int knng_copy_quantized(knng_ctx* c, int8_t* q, uint64_t* sk) {
+  if (!c) return KNNG_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (q) HIPCHK(c, hipMemcpyAsync(q, c->q, size_t(c->n) * c->d, hipMemcpyDeviceToHost, c->stream));
+  if (sk)
+    HIPCHK(c, hipMemcpyAsync(sk, c->sk, size_t(c->n) * c->words * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int knng_copy_forest(knng_ctx* c, int32_t* perm, int64_t* leaves, int64_t cap, int64_t* nleaf) {
+  if (!c) return KNNG_EINVAL;
+  if (!c->have_forest) return fail(c, KNNG_ESTATE, "knng_copy_forest: no forest");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t total = int64_t(c->p.num_trees) * c->n;
+  if (perm) HIPCHK(c, hipMemcpyAsync(perm, c->perm, total * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  std::vector<int4> lv(c->nleaves);
+  if (c->nleaves)
+    HIPCHK(c, hipMemcpyAsync(lv.data(), c->leaves, c->nleaves * sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::sort(lv.begin(), lv.end(), [](const int4& a, const int4& b) {
+    return a.x != b.x ? a.x < b.x : a.y < b.y;
+  });
+  if (nleaf) *nleaf = c->nleaves;
+  if (leaves) {
+    if (cap < c->nleaves) return fail(c, KNNG_EINVAL, "knng_copy_forest: leaf buffer too small");
+    for (int64_t l = 0; l < c->nleaves; ++l) {
+      leaves[3 * l] = lv[l].x;
+      leaves[3 * l + 1] = lv[l].y;
+      leaves[3 * l + 2] = lv[l].z;
+    }
+  }
+  return 0;
+}
+
+int knng_copy_graph(knng_ctx* c, int32_t* ids, int32_t* scores) {
+  if (!c) return KNNG_EINVAL;
+  if (!c->have_graph) return fail(c, KNNG_ESTATE, "knng_copy_graph: no graph");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t bytes = size_t(c->n) * c->p.k_build * sizeof(int32_t);
+  if (ids) HIPCHK(c, hipMemcpyAsync(ids, c->gid[c->cur], bytes, hipMemcpyDeviceToHost, c->stream));
+  if (scores) HIPCHK(c, hipMemcpyAsync(scores, c->gsc[c->cur], bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int knng_load_graph(knng_ctx* c, const int32_t* ids, const int32_t* scores) {
+  if (!c || !ids || !scores) return KNNG_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t n = c->n;
+  const int K = c->p.k_build;
+  for (int64_t j = 0; j < n * K; ++j)
+    if (ids[j] < 0 || ids[j] >= n) return fail(c, KNNG_EINVAL, "knng_load_graph: id out of range");
+  int rc;
+  for (int b = 0; b < 2; ++b)
+    if ((rc = dalloc(c, &c->gid[b], size_t(n) * K, "graph ids")) ||
+        (rc = dalloc(c, &c->gsc[b], size_t(n) * K, "graph scores")))
+      return rc;
+  c->cur = 0;
+  const size_t bytes = size_t(n) * K * sizeof(int32_t);
+  HIPCHK(c, hipMemcpyAsync(c->gid[0], ids, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->gsc[0], scores, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_graph = true;
+  return 0;
+}
+
+int knng_quantize_device(const float* dx, int64_t n, int32_t d, int8_t* dq, uint64_t* dsk) {
+  if (!dx || !dq || !dsk || n < 0 || d <= 0 || d % 64 || d > kMaxDim) return KNNG_EINVAL;
+  if (n == 0) return 0;
+  int64_t g = (n + 3) / 4;
+  if (g > 16384) g = 16384;
+  hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, 0, dx, n, d, dq, dsk);
+  if (hipGetLastError() != hipSuccess) return KNNG_EHIP;
+  return hipDeviceSynchronize() == hipSuccess ? 0 : KNNG_EHIP;
+}
+
+int knng_score_pairs_device(const int8_t* dq, const uint64_t* dsk, int32_t d, const int32_t* da,
+                            const int32_t* db, int64_t np, int32_t* ddot, int32_t* dsim) {
+  if (!dq || !dsk || !da || !db || !ddot || !dsim || np < 0 || d <= 0 || d % 64 || d > kMaxDim)
+    return KNNG_EINVAL;
+  if (np == 0) return 0;
+  int64_t g = (np + 3) / 4;
+  if (g > 16384) g = 16384;
+  hipLaunchKernelGGL(k_score_pairs, dim3(g), dim3(256), 0, 0, dq, dsk, d, da, db, np, ddot, dsim);
+  if (hipGetLastError() != hipSuccess) return KNNG_EHIP;
+  return hipDeviceSynchronize() == hipSuccess ? 0 : KNNG_EHIP;
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+// knng_main.cpp -- drop-in command line of the MI355X kNN-graph builder.
+//
+// Same invocation, exit codes and output file as the reference engine
+// (src/main.cpp:114-498):
+//   knng_main --task task1 --dataset X.h5      (or the dataset as the first
+//                                               non "--" argument)
+//   -> results/<task>/apex_simd.h5 with int32 knns (1-based), float32 dists
+//      (rows ascending, best neighbour last) and root attributes algo, task,
+//      buildtime (forest..refinement seconds), querytime (0) and params.
+// Exit status: 0 on success, -1 (255) on a missing filename, an HDF5 error,
+// an invalid shape or a device error.
+//
+// Extra flags (all optional; the reference has none of them):
+//   --rerank            FP32 rerank of each list before export (dists become
+//                       FP32 dot products; attribute dists_kind = fp32_dot)
+//   --trees N  --k-build N  --leaf N  --iters N  --rerank-mult N  --device N
+//   --chunk-rows N      rows per HDF5 hyperslab (default 2,200,000)
+
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "knng.h"
+#include "knng_h5.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double secs(clk::time_point a, clk::time_point b) {
+  return std::chrono::duration<double>(b - a).count();
+}
+
+bool parse_int(const std::string& s, long long* out) {
+  char* end = nullptr;
+  const long long v = strtoll(s.c_str(), &end, 10);
+  if (!end || *end) return false;
+  *out = v;
+  return true;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const auto t_start = clk::now();
+  std::cout << "==================================================\n"
+            << "   kNN-graph builder (MI355X / HIP)\n"
+            << "==================================================\n";
+
+  std::string filename, task = "task1";
+  knng_params p;
+  knng_default_params(&p);
+  int device = 0, rerank = 0;
+  long long chunk_rows = 2200000;   // CHUNK_SIZE, src/main.cpp:26
+
+  // argument rules of src/main.cpp:124-133, plus the optional tuning flags
+  for (int i = 1; i < argc; ++i) {
+    const std::string a = argv[i];
+    long long v = 0;
+    auto num = [&](long long* dst) {
+      if (i + 1 < argc && parse_int(argv[i + 1], &v)) {
+        *dst = v;
+        ++i;
+        return true;
+      }
+      return false;
+    };
+    long long tmp = 0;
+    if (a == "--dataset" && i + 1 < argc) {
+      filename = argv[++i];
+    } else if (a == "--task" && i + 1 < argc) {
+      task = argv[++i];
+    } else if (a == "--rerank") {
+      rerank = 1;
+    } else if (a == "--trees" && num(&tmp)) {
+      p.num_trees = int32_t(tmp);
+    } else if (a == "--k-build" && num(&tmp)) {
+      p.k_build = int32_t(tmp);
+    } else if (a == "--leaf" && num(&tmp)) {
+      p.leaf_size = int32_t(tmp);
+    } else if (a == "--iters" && num(&tmp)) {
+      p.max_iters = int32_t(tmp);
+    } else if (a == "--rerank-mult" && num(&tmp)) {
+      p.rerank_mult = int32_t(tmp);
+    } else if (a == "--device" && num(&tmp)) {
+      device = int(tmp);
+    } else if (a == "--chunk-rows" && num(&tmp)) {
+      chunk_rows = tmp > 0 ? tmp : chunk_rows;
+    } else if (filename.empty() && a.rfind("--", 0) != 0) {
+      filename = a;
+    }
+  }
+  std::cout << "[INFO] Target Dataset: " << filename << "\n";
+  std::cout << "[INFO] Target Task: " << task << "\n";
+  if (filename.empty()) {
+    std::cerr << "[FATAL] No filename provided.\n";
+    return -1;
+  }
+
+  knng_h5_reader* rd = nullptr;
+  int rc = knng_h5_open(filename.c_str(), "/train", &rd);
+  if (rc) {
+    std::cerr << "[FATAL] HDF5 Error: cannot open /train in " << filename << " (code " << rc << ")\n";
+    return -1;
+  }
+  int64_t n = 0, dcols = 0;
+  knng_h5_shape_of(rd, &n, &dcols);
+  std::cout << "[OK] Detected " << n << " vectors of dimension " << dcols << " in /train.\n";
+  if (n < 2 || dcols <= 0 || dcols % 64 || dcols > 1024) {
+    std::cerr << "[FATAL] /train must hold n >= 2 rows of dimension 64..1024 (multiple of 64)\n";
+    knng_h5_close(rd);
+    return -1;
+  }
+  p.dim = int32_t(dcols);
+  p.keep_fp32 = rerank;
+
+  knng_ctx* ctx = nullptr;
+  if ((rc = knng_create(&ctx, device, &p, n))) {
+    std::cerr << "[FATAL] Device set-up failed (code " << rc << ")\n";
+    knng_h5_close(rd);
+    return -1;
+  }
+  auto die = [&](const char* what) {
+    std::cerr << "[FATAL] " << what << ": " << knng_last_error(ctx) << "\n";
+    knng_destroy(ctx);
+    if (rd) knng_h5_close(rd);
+    return -1;
+  };
+
+  // chunked ingest: hyperslab -> host chunk -> pinned H2D + quantise kernel
+  const int64_t chunk = std::min<int64_t>(chunk_rows, n);
+  std::vector<float> buf(size_t(chunk) * size_t(dcols));
+  std::cout << "[INFO] Hyperslab chunk size initialized: " << chunk << " vectors.\n";
+  double t_read = 0;
+  for (int64_t off = 0; off < n; off += chunk) {
+    const int64_t m = std::min(chunk, n - off);
+    const auto a = clk::now();
+    if ((rc = knng_h5_read_rows(rd, off, m, dcols, buf.data()))) {
+      std::cerr << "[FATAL] HDF5 Error: hyperslab read failed (code " << rc << ")\n";
+      knng_destroy(ctx);
+      knng_h5_close(rd);
+      return -1;
+    }
+    t_read += secs(a, clk::now());
+    if ((rc = knng_ingest_chunk(ctx, buf.data(), off, m))) return die("ingest");
+  }
+  knng_h5_close(rd);
+  rd = nullptr;
+  std::vector<float>().swap(buf);
+  std::cout << "[OK] Dataset loaded and quantized completely.\n";
+
+  const auto t_build0 = clk::now();   // src/main.cpp:203
+  if ((rc = knng_build(ctx))) return die("build");
+  if ((rc = knng_synchronize(ctx))) return die("build");
+  const auto t_build1 = clk::now();   // src/main.cpp:442
+  const double buildtime = secs(t_build0, t_build1);
+
+  knng_stats st;
+  knng_get_stats(ctx, &st);
+  for (int it = 0; it < st.iterations && it < 32; ++it)
+    std::cout << "  -> Iteration " << it + 1 << " Updates: " << st.updates[it]
+              << " (Rate: " << st.rates[it] * 100.0f << "%)\n";
+
+  std::cout << "[INFO] Exporting Results...\n";
+  std::vector<int32_t> knns(size_t(n) * p.k_eval);
+  std::vector<float> dists(size_t(n) * p.k_eval);
+  if ((rc = knng_export(ctx, knns.data(), dists.data(), p.k_eval, rerank))) return die("export");
+  knng_destroy(ctx);
+
+  mkdir("results", 0777);   // src/main.cpp:459-462
+  const std::string dir = "results/" + task;
+  mkdir(dir.c_str(), 0777);
+  const std::string out = dir + "/apex_simd.h5";
+  char params[256];
+  snprintf(params, sizeof(params),
+           "K_BUILD=%d, RERANK=%d, TREES=%d, LEAF=%d, STOP=%.2f, DIFF=%.2f, DIM=%d, FP32_RERANK=%d",
+           p.k_build, p.rerank_mult, p.num_trees, p.leaf_size, double(p.early_stop_rate),
+           double(p.rate_diff_tol), p.dim, rerank);
+  rc = knng_h5_write_result(out.c_str(), knns.data(), dists.data(), n, p.k_eval,
+                            "Apex Hardware SIMD (MI355X HIP)", task.c_str(), float(buildtime), 0.0f,
+                            params, "dists_kind", rerank ? "fp32_dot" : "int8_dot");
+  if (rc) {
+    std::cerr << "[FATAL] File Write Error: cannot write " << out << " (code " << rc << ")\n";
+    return -1;
+  }
+  const double total = secs(t_start, clk::now());
+  std::cout << "==================================================\n"
+            << "   EXECUTION COMPLETE\n"
+            << "==================================================\n"
+            << "Build time (forest..refinement): " << buildtime << "s\n"
+            << "  forest " << st.ms_forest / 1000.0 << "s, init " << st.ms_init / 1000.0
+            << "s, join " << st.ms_join / 1000.0 << "s (" << st.iterations << " iterations)\n"
+            << "HDF5 read: " << t_read << "s, ingest (device): " << st.ms_ingest / 1000.0 << "s\n"
+            << "Total Execution Time: " << total << "s\n";
+  return 0;
+}

@@ -1,0 +1,124 @@
+"""CPU tests of the drop-in boundary: the C-ABI libraries export every
+declared symbol, the HDF5 contract round-trips, and the CLI's host-side
+failure modes match the reference's (src/main.cpp:124-141, 195-201)."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import knng
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(REPO, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(knng_\w+)\s*\(", text)))
+
+
+def test_knng_exports_every_declared_symbol():
+    names = _declared("knng.h")
+    assert sorted(knng.KNNG_SYMBOLS) == names
+    L = ctypes.CDLL(os.path.join(knng.LIBDIR, "libknng.so"))
+    for s in names:
+        assert hasattr(L, s), s
+
+
+def test_h5_exports_every_declared_symbol():
+    names = _declared("knng_h5.h")
+    assert sorted(knng.H5_SYMBOLS) == names
+    L = ctypes.CDLL(os.path.join(knng.LIBDIR, "libknng_h5.so"))
+    for s in names:
+        assert hasattr(L, s), s
+
+
+def test_default_params_are_reference_constants():
+    p = knng.default_params()   # src/main.cpp:19-29
+    assert (p.dim, p.k_eval, p.k_build, p.num_trees, p.rerank_mult, p.leaf_size, p.max_iters) == \
+        (1024, 15, 50, 16, 8, 64, 30)
+    assert np.float32(p.early_stop_rate) == np.float32(0.04)
+    assert np.float32(p.rate_diff_tol) == np.float32(0.01)
+
+
+def test_h5_train_roundtrip_and_hyperslab():
+    x = np.random.default_rng(0).standard_normal((1000, 128)).astype(np.float32)
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "t.h5")
+        knng.h5_write_train(p, x, chunk_rows=300)
+        assert knng.h5_shape(p, "/train") == (1000, 128)
+        assert np.array_equal(knng.h5_read(p, "/train"), x)
+        r = ctypes.c_void_p()
+        assert knng.h5().knng_h5_open(p.encode(), b"/train", ctypes.byref(r)) == 0
+        part = np.empty((100, 64), np.float32)
+        assert knng.h5().knng_h5_read_rows(r, 450, 100, 64, part.ctypes.data) == 0
+        assert np.array_equal(part, x[450:550, :64])
+        assert knng.h5().knng_h5_read_rows(r, 950, 100, 64, part.ctypes.data) < 0   # past the end
+        knng.h5().knng_h5_close(r)
+
+
+def test_h5_result_file_layout():
+    n, k = 7, 15
+    knns = np.arange(n * k, dtype=np.int32).reshape(n, k) + 1
+    dists = np.linspace(-1, 1, n * k, dtype=np.float32).reshape(n, k)
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "r.h5")
+        rc = knng.h5().knng_h5_write_result(p.encode(), knns.ctypes.data, dists.ctypes.data, n, k,
+                                            b"algo-x", b"task1", 1.5, 0.0, b"K_BUILD=50",
+                                            b"dists_kind", b"int8_dot")
+        assert rc == 0
+        assert np.array_equal(knng.h5_read(p, "knns", np.int32), knns)
+        assert np.array_equal(knng.h5_read(p, "dists", np.float32), dists)
+        assert knng.h5_attr_str(p, "algo") == "algo-x"
+        assert knng.h5_attr_str(p, "task") == "task1"
+        assert knng.h5_attr_str(p, "params") == "K_BUILD=50"
+        assert knng.h5_attr_f32(p, "buildtime") == 1.5
+        assert knng.h5_attr_f32(p, "querytime") == 0.0
+
+
+def _cli(args, cwd):
+    return subprocess.run([knng.CLI, *args], cwd=cwd, stdout=subprocess.PIPE,
+                          stderr=subprocess.PIPE, text=True, timeout=120)
+
+
+def test_cli_no_filename_exits_minus_one():
+    with tempfile.TemporaryDirectory() as td:
+        r = _cli(["--task", "task1"], td)
+        assert r.returncode == 255 and "No filename provided" in r.stderr
+
+
+def test_cli_missing_file_and_dataset():
+    with tempfile.TemporaryDirectory() as td:
+        r = _cli(["--dataset", os.path.join(td, "nope.h5")], td)
+        assert r.returncode == 255 and "HDF5 Error" in r.stderr
+        p = os.path.join(td, "other.h5")
+        knng.h5().knng_h5_write_f32(p.encode(), b"test", np.zeros((4, 64), np.float32).ctypes.data,
+                                    4, 64, 0, 0)
+        r = _cli([p], td)   # positional dataset, no /train inside
+        assert r.returncode == 255 and "HDF5 Error" in r.stderr
+        assert not os.path.exists(os.path.join(td, "results"))
+
+
+@pytest.mark.parametrize("shape", [(10, 100), (1, 64), (5, 2048)])
+def test_cli_rejects_bad_shapes(shape):
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "bad.h5")
+        knng.h5_write_train(p, np.zeros(shape, np.float32))
+        r = _cli(["--dataset", p], td)
+        assert r.returncode == 255
+        assert "must hold" in r.stderr
+
+
+def test_cli_without_device_fails_cleanly():
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "ok.h5")
+        knng.h5_write_train(p, np.ones((100, 64), np.float32))
+        r = _cli(["--dataset", p], td)
+        assert r.returncode == 255 and "Device set-up failed" in r.stderr

@@ -1,0 +1,272 @@
+"""GPU parity tests: every stage of the HIP path against the oracle on the
+same inputs, through the C-ABI (libknng.so).
+
+Integer stages (quantise, sketch, int8 dots, forest, init, join, plain
+export) must be bit-exact against the oracle's device mode; the FP32 rerank
+must be within 1e-5 relative of FP64 dot products.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import knng
+import make_golden
+import oracle
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _params(**kw):
+    return knng.default_params(**kw)
+
+
+def _builder(x, **kw):
+    n, d = x.shape
+    b = knng.Builder(n, _params(dim=d, **kw))
+    b.ingest(x)
+    return b
+
+
+# ------------------------------------------------------------------ K1
+
+@pytest.mark.parametrize("d", [1024, 768, 384, 64])
+def test_quantize_bitexact(torch_cuda, d):
+    say("quantize", d)
+    torch = torch_cuda
+    rng = np.random.default_rng(d)
+    x = (rng.standard_normal((2000, d)) * 0.2).astype(np.float32)
+    edge = make_golden.EDGE_VALUES
+    for r in range(16):
+        x[r] = np.tile(np.roll(edge, r), d // len(edge))
+    x[16, :4] = [np.inf, -np.inf, 1e-45, -1e-45]
+    q_ref, sk_ref = oracle.quantize(x)
+    dx = _dev(torch, x)
+    dq = torch.empty((2000, d), dtype=torch.int8, device="cuda")
+    dsk = torch.empty((2000, d // 64), dtype=torch.int64, device="cuda")
+    knng.quantize_device(dx.data_ptr(), 2000, d, dq.data_ptr(), dsk.data_ptr())
+    assert np.array_equal(dq.cpu().numpy(), q_ref)
+    assert np.array_equal(dsk.cpu().numpy().view(np.uint64), sk_ref)
+
+
+def test_ingest_host_and_device_paths(torch_cuda):
+    say("ingest paths")
+    torch = torch_cuda
+    x = synth.gaussian_mixture(5, 20000, 1024, 64)     # 80 MB: two pinned slices
+    q_ref, sk_ref = oracle.quantize(x)
+    b = knng.Builder(20000, _params())
+    b.ingest(x[:7000], 0)
+    b.ingest(x[7000:], 7000)
+    q, sk = b.quantized()
+    assert np.array_equal(q, q_ref) and np.array_equal(sk, sk_ref)
+    b.close()
+    b = knng.Builder(20000, _params(keep_fp32=0))
+    dx = _dev(torch, x)
+    b.ingest_device(dx.data_ptr(), 0, 20000)
+    q, sk = b.quantized()
+    assert np.array_equal(q, q_ref) and np.array_equal(sk, sk_ref)
+
+
+def test_score_pairs_bitexact(torch_cuda):
+    say("score pairs")
+    torch = torch_cuda
+    x = synth.gaussian_mixture(6, 3000, 1024, 16)
+    q, sk = oracle.quantize(x)
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, 3000, 5000).astype(np.int32)
+    b = rng.integers(0, 3000, 5000).astype(np.int32)
+    dq, dsk, da, db = _dev(torch, q), _dev(torch, sk.view(np.int64)), _dev(torch, a), _dev(torch, b)
+    dot = torch.empty(5000, dtype=torch.int32, device="cuda")
+    sim = torch.empty(5000, dtype=torch.int32, device="cuda")
+    knng.score_pairs_device(dq.data_ptr(), dsk.data_ptr(), 1024, da.data_ptr(), db.data_ptr(), 5000,
+                            dot.data_ptr(), sim.data_ptr())
+    Q = q.astype(np.int64)
+    assert np.array_equal(dot.cpu().numpy(), (Q[a] * Q[b]).sum(1))
+    exp = np.array([oracle.sketch_sim(sk[i], sk[j]) for i, j in zip(a, b)])
+    assert np.array_equal(sim.cpu().numpy(), exp)
+
+
+# ------------------------------------------------------------- K2-K4
+
+@pytest.mark.parametrize("n,d,T,leaf", [(3000, 1024, 16, 64), (5000, 128, 3, 8), (700, 384, 5, 33)])
+def test_forest_and_init_bitexact(torch_cuda, n, d, T, leaf):
+    say("forest+init", n, d, T, leaf)
+    x = synth.gaussian_mixture(n + d, n, d, 24)
+    q, sk = oracle.quantize(x)
+    K = 50 if n >= 1000 else 20
+    b = _builder(x, num_trees=T, leaf_size=leaf, k_build=K, k_eval=15)
+    b.build_forest()
+    perm, leaves = b.forest()
+    o_perm, o_leaves = oracle.forest_device(q, T, leaf, 1337)
+    assert np.array_equal(perm, o_perm)
+    assert np.array_equal(leaves, o_leaves)
+    b.build_init()
+    ids, sc = b.graph()
+    o_ids, o_sc = oracle.init_device(q, o_perm, o_leaves, K)
+    assert np.array_equal(ids, o_ids)
+    assert np.array_equal(sc, o_sc)
+
+
+# ------------------------------------------------------------- K5-K7
+
+def test_join_iterations_bitexact(torch_cuda):
+    say("join")
+    x = synth.gaussian_mixture(7, 6000, 1024, 32)
+    q, sk = oracle.quantize(x)
+    perm, leaves = oracle.forest_device(q, 4, 64, 1337)   # a weak forest leaves work to the join
+    ids, sc = oracle.init_device(q, perm, leaves, 50)
+    b = _builder(x)
+    b.load_graph(ids, sc)
+    for it in range(3):
+        upd = b.join_iteration()
+        ids, sc, o_upd = oracle.join_device(q, sk, ids, sc, 400)
+        g_ids, g_sc = b.graph()
+        assert upd == o_upd, it
+        assert np.array_equal(g_ids, ids), it
+        assert np.array_equal(g_sc, sc), it
+
+
+@pytest.mark.parametrize("K,mult", [(64, 8), (10, 3)])
+def test_join_other_shapes_bitexact(torch_cuda, K, mult):
+    say("join shapes", K, mult)
+    x = synth.gaussian_mixture(8, 3000, 768, 16)
+    q, sk = oracle.quantize(x)
+    perm, leaves = oracle.forest_device(q, 3, 64, 1337)
+    ids, sc = oracle.init_device(q, perm, leaves, K)
+    b = _builder(x, k_build=K, rerank_mult=mult, k_eval=min(15, K))
+    b.load_graph(ids, sc)
+    upd = b.join_iteration()
+    o_ids, o_sc, o_upd = oracle.join_device(q, sk, ids, sc, K * mult)
+    g_ids, g_sc = b.graph()
+    assert upd == o_upd
+    assert np.array_equal(g_ids, o_ids) and np.array_equal(g_sc, o_sc)
+
+
+# ---------------------------------------------------- whole build + export
+
+@pytest.mark.parametrize("n", [3000, 40, 2])
+def test_full_build_bitexact(torch_cuda, n):
+    say("full build", n)
+    x = synth.gaussian_mixture(9 + n, n, 1024, 8) if n > 100 else make_golden.case_input(15, n, 2, False)
+    q, sk = oracle.quantize(x)
+    o_ids, o_sc, o_rates = oracle.build_device_semantics(q, sk)
+    b = _builder(x)
+    b.build()
+    ids, sc = b.graph()
+    st = b.stats()
+    assert np.array_equal(ids, o_ids) and np.array_equal(sc, o_sc)
+    assert st["iterations"] == len(o_rates)
+    knns, dists = b.export(15, rerank=False)
+    e_knns, e_dists = oracle.export(o_ids, o_sc)
+    assert np.array_equal(knns, e_knns) and np.array_equal(dists, e_dists)
+
+
+def test_rerank_within_fp64_tolerance(torch_cuda):
+    say("rerank")
+    x = synth.gaussian_mixture(10, 4000, 1024, 16)
+    b = _builder(x)
+    b.build()
+    ids, _ = b.graph()
+    knns, dists = b.export(15, rerank=True)
+    o_knns, o_dists = oracle.rerank_device(x, ids, 50, 15)
+    # |fp32 - fp64| / |fp64| <= 1e-5 for every emitted dist
+    X = x.astype(np.float64)
+    i = np.repeat(np.arange(4000), 15)
+    exact = (X[i] * X[knns.reshape(-1) - 1]).sum(1).reshape(4000, 15)
+    rel = np.abs(dists - exact) / np.maximum(np.abs(exact), 1e-30)
+    assert rel.max() <= 1e-5, rel.max()
+    # same selection as the FP64 oracle except at FP32-indistinguishable ties
+    same = (knns == o_knns).all(1)
+    if not same.all():
+        bad = np.where(~same)[0]
+        for r in bad:
+            assert np.allclose(np.sort(dists[r]), np.sort(o_dists[r]), rtol=1e-5, atol=0)
+    assert same.mean() > 0.99
+
+
+def test_recall_plain_and_rerank(torch_cuda):
+    say("recall n=20000")
+    n = 20000
+    x = synth.gaussian_mixture(1, n, 1024, 256)
+    b = _builder(x, num_trees=8)
+    b.build()
+    kp, _ = b.export(15, rerank=False)
+    kr, _ = b.export(15, rerank=True)
+    rows = np.random.default_rng(0).choice(n, 2000, replace=False)
+    gt = synth.exact_knn(x, rows, 15)
+    rp = synth.recall_at_k(kp, gt, rows, 15)
+    rr = synth.recall_at_k(kr, gt, rows, 15)
+    say(f"recall plain {rp:.4f} rerank {rr:.4f} stats {b.stats()['iterations']}")
+    assert rp > 0.70
+    assert rr > 0.95
+    assert rr >= rp + 0.01
+
+
+def test_cli_end_to_end(torch_cuda):
+    say("cli")
+    x = synth.gaussian_mixture(11, 3000, 1024, 8)
+    q, sk = oracle.quantize(x)
+    o_ids, o_sc, _ = oracle.build_device_semantics(q, sk)
+    e_knns, e_dists = oracle.export(o_ids, o_sc)
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "in.h5")
+        knng.h5_write_train(p, x, chunk_rows=1000)
+        r = subprocess.run([knng.CLI, "--task", "task1", "--dataset", p, "--chunk-rows", "1100"],
+                           cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout
+        out = os.path.join(td, "results", "task1", "apex_simd.h5")
+        knns = knng.h5_read(out, "knns", np.int32)
+        dists = knng.h5_read(out, "dists", np.float32)
+        assert np.array_equal(knns, e_knns) and np.array_equal(dists, e_dists)
+        assert knng.h5_attr_str(out, "task") == "task1"
+        assert knng.h5_attr_str(out, "dists_kind") == "int8_dot"
+        assert knng.h5_attr_f32(out, "buildtime") > 0
+        assert knng.h5_attr_f32(out, "querytime") == 0.0
+        r = subprocess.run([knng.CLI, p, "--rerank", "--task", "t2"], cwd=td,
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout
+        out2 = os.path.join(td, "results", "t2", "apex_simd.h5")
+        assert knng.h5_attr_str(out2, "dists_kind") == "fp32_dot"
+        k2 = knng.h5_read(out2, "knns", np.int32)
+        assert k2.shape == (3000, 15) and k2.min() >= 1 and k2.max() <= 3000
+
+
+def test_large_build_properties(torch_cuda):
+    """n = 200,000 x 1024 generated in HBM: size-independent properties."""
+    torch = torch_cuda
+    n = 200000
+    say("large", n)
+    xt = synth.gaussian_mixture_torch(2, n, 1024, 1024)
+    b = knng.Builder(n, _params(keep_fp32=0))
+    b.ingest_device(xt.data_ptr(), 0, n)
+    b.attach_fp32(xt.data_ptr())
+    b.build()
+    ids, sc = b.graph()
+    assert ids.min() >= 0 and ids.max() < n
+    assert not (ids == np.arange(n)[:, None]).any()                    # no self loops
+    assert np.all(np.diff(sc, axis=1) <= 0)                            # sorted by score
+    srt = np.sort(ids, axis=1)
+    assert not (srt[:, 1:] == srt[:, :-1]).any()                       # no duplicate ids
+    rows = np.random.default_rng(1).choice(n, 500, replace=False)
+    q, _ = b.quantized()
+    Q = q.astype(np.int64)
+    assert np.array_equal((Q[rows][:, None, :] * Q[ids[rows]]).sum(-1), sc[rows])
+    kr, dr = b.export(15, rerank=True)
+    X = xt[torch.from_numpy(rows).cuda()].double()
+    nb = xt[torch.from_numpy(kr[rows].reshape(-1) - 1).cuda()].double().view(500, 15, 1024)
+    exact = (X[:, None, :] * nb).sum(-1).cpu().numpy()
+    assert np.abs(dr[rows] - exact).max() / np.abs(exact).min() <= 1e-5

@@ -1,0 +1,314 @@
+"""CPU tests that pin the oracle (oracle/knng_oracle.cpp).
+
+Reference mode is pinned byte-for-byte by the compiled reference's golden
+outputs (tests/golden/, written by make_golden.py at OMP_NUM_THREADS=1).
+The primitives are pinned by closed forms and known-answer rows, the
+device-mode stages by brute-force restatements of their definitions on small
+inputs and by invariants.
+"""
+import json
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import make_golden
+import oracle
+import synth
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+META = json.load(open(os.path.join(GOLDEN, "golden.json")))
+
+
+# ---------------------------------------------------------------- primitives
+
+def test_quantize_known_answers():
+    # src/main.cpp:184-190: multiply by 127, clamp to [-128,127], truncate; bit = clamped > 0
+    vals = np.array([0.0, -0.0, 1e-40, -1e-40, 0.5 / 127, -0.5 / 127, 1.0, -1.0, 1.5, -1.5,
+                     126.9 / 127, -128.5 / 127, 127.0 / 127, -128.0 / 127, np.inf, -np.inf,
+                     0.99 / 127, 3.3 / 127, -3.3 / 127, 2.0 / 127] + [0.0] * 44, np.float32)
+    q, sk = oracle.quantize(vals[None, :])
+    exp_q = [0, 0, 0, 0, 0, 0, 127, -127, 127, -128, 126, -128, 127, -128, 127, -128, 0, 3, -3, 2]
+    assert q[0, :20].tolist() == exp_q
+    bits = [(int(sk[0, 0]) >> b) & 1 for b in range(20)]
+    # 1e-40 is denormal: DAZ reads it as 0 -> bit 0; 0.5/127 -> q 0 but bit 1 (clamped > 0)
+    assert bits == [0, 0, 0, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 1, 0, 1]
+
+
+def test_fast_math_sets_daz():
+    """The reference is linked with -ffast-math (CMakeLists.txt:6); GCC then
+    sets FTZ/DAZ at start-up, which is why the oracle reads denormals as 0."""
+    src = ("#include <xmmintrin.h>\n#include <stdio.h>\n"
+           "int main(){unsigned c=_mm_getcsr();printf(\"%d %d\\n\",(c>>6)&1,(c>>15)&1);return 0;}\n")
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, "p.c")
+        open(c, "w").write(src)
+        exe = os.path.join(td, "p")
+        subprocess.run(["gcc", "-O3", "-ffast-math", c, "-o", exe], check=True)
+        daz, ftz = subprocess.run([exe], stdout=subprocess.PIPE, text=True).stdout.split()
+    assert (daz, ftz) == ("1", "1")
+
+
+def test_quantize_matches_closed_form_random():
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal((37, 256)) * 0.3).astype(np.float32)
+    x[0, :8] = [1e-39, -1e-39, 5e-41, 0.0, -0.0, 2.0, -2.0, 1e-45]
+    q, sk = oracle.quantize(x)
+    xd = np.where(np.abs(x) < np.finfo(np.float32).tiny, np.float32(0), x)
+    c = np.clip(xd * np.float32(127), -128, 127)
+    assert np.array_equal(q, np.trunc(c).astype(np.int8))
+    bits = np.packbits((c > 0).reshape(37, 4, 64)[:, :, ::-1], axis=2).view(">u8")[:, :, 0]
+    assert np.array_equal(sk, bits.astype(np.uint64))
+
+
+def test_dot_and_popcount_closed_form():
+    rng = np.random.default_rng(1)
+    a = rng.integers(-128, 128, 1024).astype(np.int8)
+    b = rng.integers(-128, 128, 1024).astype(np.int8)
+    assert oracle.dot_i8(a, b) == int(a.astype(np.int64) @ b.astype(np.int64))
+    extreme = np.full(1024, -128, np.int8)
+    assert oracle.dot_i8(extreme, extreme) == 1024 * 128 * 128
+    sa = rng.integers(0, 2 ** 63, 16, dtype=np.uint64)
+    sb = rng.integers(0, 2 ** 63, 16, dtype=np.uint64)
+    pc = sum(bin(int(u) ^ int(v)).count("1") for u, v in zip(sa, sb))
+    assert oracle.sketch_sim(sa, sb) == 1024 - pc
+    assert oracle.sketch_sim(sa, sa) == 1024
+
+
+# ---------------------------------------------------------- reference mode
+
+@pytest.mark.parametrize("case", make_golden.CASES, ids=[c[0] for c in make_golden.CASES])
+def test_reference_mode_matches_golden(case):
+    name, seed, n, clusters, edge, keep = case
+    x = make_golden.case_input(seed, n, clusters, edge)
+    assert synth.sha256(x) == META[name]["input_sha256"]   # the generator is pinned
+    q, sk = oracle.quantize(x)
+    ids, sc, st = oracle.build_reference(q, sk)
+    knns, dists = oracle.export(ids, sc)
+    f = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    rows = f["rows"] if "rows" in f.files else np.arange(n)
+    assert np.array_equal(knns[rows], f["knns"])
+    assert np.array_equal(dists[rows], f["dists"])
+    if keep is None:
+        assert synth.sha256(knns) == META[name]["knns_sha256"]
+        assert synth.sha256(dists) == META[name]["dists_sha256"]
+    rates = [round(float(st.rates[i]) * 100, 4) for i in range(st.iterations)]
+    assert rates == pytest.approx(META[name]["rates_percent"], abs=1e-3)
+
+
+def test_golden_dists_are_int8_dots():
+    """Every emitted dist is the int8 dot of the quantised rows (closed form)."""
+    x = make_golden.case_input(12, 2000, 32, False)
+    q, _ = oracle.quantize(x)
+    f = np.load(os.path.join(GOLDEN, "ref_n2000.npz"))
+    Q = q.astype(np.int64)
+    i = np.repeat(np.arange(2000), 15)
+    j = f["knns"].reshape(-1) - 1
+    assert np.array_equal((Q[i] * Q[j]).sum(1).astype(np.float32), f["dists"].reshape(-1))
+    # rows ascending by score, best neighbour last (src/main.cpp:455-456)
+    assert np.all(np.diff(f["dists"], axis=1) >= 0)
+
+
+# -------------------------------------------------------------- device mode
+
+M64 = (1 << 64) - 1
+
+
+def _mix64(z):
+    z = (z + 0x9E3779B97F4A7C15) & M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def _pivots(seed, t, depth, start, size):
+    h1 = _mix64(seed ^ _mix64((t << 48) ^ (depth << 40) ^ start))
+    h2 = _mix64(h1)
+    a, b = h1 % size, h2 % (size - 1)
+    return a, b + (b >= a)
+
+
+def test_pivot_hash():
+    for args in [(1337, 0, 0, 0, 1000), (1337, 5, 3, 12345, 77), (7, 63, 30, 2 ** 30, 65)]:
+        assert oracle.pivots(*args) == _pivots(*args)
+
+
+def _forest_bruteforce(q, T, leaf, seed):
+    Q = q.astype(np.int64)
+    n = len(q)
+    perms, leaves = [], []
+    for t in range(T):
+        perm = list(range(n))
+
+        def rec(start, size, depth):
+            if size <= leaf:
+                leaves.append((t, start, size))
+                return
+            a, b = _pivots(seed, t, depth, start, size)
+            seg = perm[start:start + size]
+            diff = Q[seg] @ (Q[seg[a]] - Q[seg[b]])
+            lo = [seg[j] for j in range(size) if j == a or (j != b and diff[j] > 0)]
+            hi = [seg[j] for j in range(size) if not (j == a or (j != b and diff[j] > 0))]
+            perm[start:start + size] = lo + hi
+            rec(start, len(lo), depth + 1)
+            rec(start + len(lo), size - len(lo), depth + 1)
+
+        rec(0, n, 0)
+        perms.append(perm)
+    return np.array(perms, np.int32), np.array(sorted(leaves), np.int64)
+
+
+def _small(seed=3, n=300, d=128, clusters=6):
+    x = synth.gaussian_mixture(seed, n, d, clusters)
+    return x, *oracle.quantize(x)
+
+
+def test_forest_device_bruteforce():
+    x, q, sk = _small()
+    perm, leaves = oracle.forest_device(q, 3, 8, seed=1337)
+    bp, bl = _forest_bruteforce(q, 3, 8, 1337)
+    assert np.array_equal(perm, bp)
+    assert np.array_equal(leaves, bl)
+    # invariants: permutations, leaves tile each tree, leaf bound
+    for t in range(3):
+        assert sorted(perm[t].tolist()) == list(range(300))
+        lt = leaves[leaves[:, 0] == t]
+        assert lt[0, 1] == 0 and np.all(lt[1:, 1] == lt[:-1, 1] + lt[:-1, 2])
+        assert lt[:, 2].sum() == 300 and lt[:, 2].max() <= 8 and lt[:, 2].min() >= 1
+
+
+def _topk_unique(cands, K):
+    cands = sorted(cands, key=lambda e: (-e[1], e[0]))
+    out, prev = [], None
+    for e in cands:
+        if e[0] != prev:
+            out.append(e)
+            prev = e[0]
+    return out[:K]
+
+
+def test_init_device_bruteforce():
+    x, q, sk = _small(n=300)
+    T, K = 3, 12
+    perm, leaves = oracle.forest_device(q, T, 8, seed=9)
+    ids, sc = oracle.init_device(q, perm, leaves, K)
+    Q = q.astype(np.int64)
+    mates = [set() for _ in range(300)]
+    for t, s, z in leaves:
+        mem = perm[t, s:s + z]
+        for m in mem:
+            mates[m].update(int(v) for v in mem if v != m)
+    for i in range(300):
+        cands = [(c, int(Q[i] @ Q[c])) for c in mates[i]]
+        exp = _topk_unique(cands, K)
+        if len(exp) == K:
+            assert ids[i].tolist() == [e[0] for e in exp]
+            assert sc[i].tolist() == [e[1] for e in exp]
+        else:   # padding rows: the real candidates still come first
+            assert set(e[0] for e in exp) <= set(ids[i].tolist())
+
+
+def _xorshift_ids(i, n, count):
+    """src/main.cpp:271-286: xorshift32 from 1337+i, id = state % n, self skipped."""
+    s, out = (1337 + i) & 0xFFFFFFFF, []
+    while len(out) < count:
+        s ^= (s << 13) & 0xFFFFFFFF
+        s ^= s >> 17
+        s ^= (s << 5) & 0xFFFFFFFF
+        if s % n != i:
+            out.append(s % n)
+    return out
+
+
+def test_init_padding_follows_reference_rule():
+    """n < K: every other row is a candidate; the pool is padded to K with
+    xorshift32 ids (duplicates), de-duplicated, and the free slots filled by
+    the same xorshift32 stream at INT32_MIN (src/main.cpp:271-312)."""
+    n, K = 40, 50
+    x = make_golden.case_input(15, n, 4, False)
+    q, sk = oracle.quantize(x)
+    perm, leaves = oracle.forest_device(q, 16, 64)
+    ids, sc = oracle.init_device(q, perm, leaves, K)
+    Q = q.astype(np.int64)
+    for i in range(n):
+        others = [c for c in range(n) if c != i]
+        exp = sorted(((c, int(Q[i] @ Q[c])) for c in others), key=lambda e: (-e[1], e[0]))
+        assert list(zip(ids[i, :n - 1].tolist(), sc[i, :n - 1].tolist())) == exp
+        stream = _xorshift_ids(i, n, K - (n - 1) + K)
+        assert ids[i, n - 1:].tolist() == stream[K - (n - 1):][:K - (n - 1)]
+        assert np.all(sc[i, n - 1:] == np.iinfo(np.int32).min)
+
+
+def test_sketch_threshold_bruteforce():
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        cnt = int(rng.integers(1, 3000))
+        h = rng.integers(400, 700, cnt).astype(np.int32)
+        M = int(rng.integers(1, 600))
+        t = oracle.sketch_threshold(h, M, 1024)
+        if cnt <= M:
+            assert t == 0
+        else:
+            assert (h >= t).sum() >= M and (h >= t + 1).sum() < M
+
+
+def test_join_device_bruteforce():
+    x, q, sk = _small(n=250, d=128)
+    K, M = 10, 30
+    perm, leaves = oracle.forest_device(q, 2, 16)
+    ids, sc = oracle.init_device(q, perm, leaves, K)
+    oi, osc, upd = oracle.join_device(q, sk, ids, sc, M)
+    Q = q.astype(np.int64)
+    bits = np.unpackbits(sk.view(np.uint8), axis=1, bitorder="little")
+    changes = 0
+    for i in range(250):
+        excl = set(ids[i].tolist()) | {i}
+        cands = []
+        for n1 in ids[i]:
+            for c in ids[n1]:
+                c = int(c)
+                if c not in excl and c not in cands:
+                    cands.append(c)
+        h = np.array([128 - int((bits[i] ^ bits[c]).sum()) for c in cands], np.int32)
+        if len(cands) > M:
+            thr = max(t for t in range(129) if (h >= t).sum() >= M)
+            sel = [c for c, hv in zip(cands, h) if hv >= thr]
+        else:
+            sel = cands
+        pool = [(int(ids[i, k]), int(sc[i, k]), k) for k in range(K)]
+        pool += [(c, int(Q[i] @ Q[c]), K) for c in sel]
+        pool.sort(key=lambda e: (-e[1], e[0], e[2]))
+        assert oi[i].tolist() == [e[0] for e in pool[:K]]
+        assert osc[i].tolist() == [e[1] for e in pool[:K]]
+        changes += sum(int(a != b) for a, b in zip(oi[i], ids[i]))
+    assert upd == changes
+
+
+def test_rerank_device_fp64():
+    x, q, sk = _small(n=200, d=128)
+    perm, leaves = oracle.forest_device(q, 4, 16)
+    ids, sc = oracle.init_device(q, perm, leaves, 20)
+    knns, dists = oracle.rerank_device(x, ids, 20, 15)
+    X = x.astype(np.float64)
+    for i in range(200):
+        s = [(float(X[i] @ X[c]), int(c)) for c in ids[i]]
+        s.sort(key=lambda e: (-e[0], e[1]))
+        assert knns[i].tolist() == [c + 1 for _, c in s[:15]][::-1]
+        assert np.allclose(dists[i], [v for v, _ in s[:15]][::-1], rtol=0, atol=1e-12)
+
+
+def test_device_semantics_recall_tracks_reference():
+    """Device mode (hash pivots, Jacobi, dedup) reaches the reference's recall
+    on the same input."""
+    x = synth.gaussian_mixture(21, 4000, 1024, 16)
+    q, sk = oracle.quantize(x)
+    gt = synth.exact_knn(x, np.arange(4000), 15)
+    r_ids, r_sc, _ = oracle.build_reference(q, sk)
+    d_ids, d_sc, _ = oracle.build_device_semantics(q, sk)
+    rk, _ = oracle.export(r_ids, r_sc)
+    dk, _ = oracle.export(d_ids, d_sc)
+    rec_ref = synth.recall_at_k(rk, gt, np.arange(4000), 15)
+    rec_dev = synth.recall_at_k(dk, gt, np.arange(4000), 15)
+    assert rec_dev >= rec_ref - 0.01, (rec_dev, rec_ref)
