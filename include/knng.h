@@ -33,7 +33,7 @@ extern "C" {
 /* Build parameters.  Defaults (knng_default_params) are the reference's
  * compile-time constants, src/main.cpp:19-29. */
 typedef struct knng_params {
-  int32_t dim;             /* DIM = 1024 (any multiple of 64, <= 4096) */
+  int32_t dim;             /* DIM = 1024 (any multiple of 64, <= 1024) */
   int32_t k_eval;          /* K_EVAL = 15 */
   int32_t k_build;         /* K_BUILD = 50 (<= 64) */
   int32_t num_trees;       /* NUM_TREES = 16 (<= 64) */
@@ -85,6 +85,12 @@ const char* knng_last_error(const knng_ctx* ctx);
  * H2D copy + fused quantise/sketch kernel.  Replaces the per-chunk loop body
  * of src/main.cpp:168-192. */
 int knng_ingest_chunk(knng_ctx* ctx, const float* rows, int64_t row0, int64_t nrows);
+/* Page-locked host buffer for zero-copy ingest: knng_ingest_chunk DMAs
+ * straight from it instead of staging through its own pinned buffers.  Lets a
+ * reader thread fill one buffer while the previous one is copied (the
+ * reference reads and quantises serially, src/main.cpp:168-193). */
+int knng_host_alloc(knng_ctx* ctx, void** ptr, int64_t bytes);
+int knng_host_free(knng_ctx* ctx, void* ptr);
 /* Same, rows already in device memory (not retained). */
 int knng_ingest_device(knng_ctx* ctx, const float* dev_rows, int64_t row0, int64_t nrows);
 /* Borrow a resident float32 copy of all n rows (device pointer) for the FP32

@@ -49,6 +49,7 @@ namespace {
 
 constexpr int32_t kScoreMin = std::numeric_limits<int32_t>::min();
 constexpr float kFltMin = 1.17549435e-38f;
+constexpr int32_t kMaxDepth = 64;   // device-mode forest depth cap
 
 struct Edge {              // one graph slot: src/main.cpp:31-37
   int32_t id;
@@ -362,14 +363,18 @@ int64_t orc_forest_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int3
     while (!stack.empty()) {
       Node nd = stack.back();
       stack.pop_back();
-      if (nd.size <= leaf_size) {
-        if (leaf_tri) {
-          if (nleaf >= leaf_cap) return -1;
-          leaf_tri[3 * nleaf] = t;
-          leaf_tri[3 * nleaf + 1] = nd.start;
-          leaf_tri[3 * nleaf + 2] = nd.size;
+      if (nd.size <= leaf_size || nd.depth >= kMaxDepth) {
+        // a node still too large at depth kMaxDepth (degenerate input, e.g.
+        // many identical rows) is cut into consecutive leaves
+        for (int64_t s0 = 0; s0 < nd.size; s0 += leaf_size) {
+          if (leaf_tri) {
+            if (nleaf >= leaf_cap) return -1;
+            leaf_tri[3 * nleaf] = t;
+            leaf_tri[3 * nleaf + 1] = nd.start + s0;
+            leaf_tri[3 * nleaf + 2] = std::min<int64_t>(leaf_size, nd.size - s0);
+          }
+          ++nleaf;
         }
-        ++nleaf;
         continue;
       }
       int64_t a, b;

@@ -28,113 +28,19 @@
 #include <vector>
 
 #include "knng.h"
+#include "knng_device.h"
+#include "knng_internal.h"
 
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
+using namespace knng_dev;
+
 
 namespace {
-
-constexpr int kMaxDim = 1024;
-constexpr int kMaxK = 64;
-constexpr int kMaxTrees = 64;
-constexpr int kLeafSlots = 64;          // candidate slots per (point, tree)
-constexpr int kNumCounters = 1024;      // spread of the per-block atomic counters
-constexpr float kFltMin = 1.17549435e-38f;
-
-// ---------------------------------------------------------------------------
-// device helpers
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-  return v;
-}
-
-__device__ __forceinline__ float wave_sum_f(float v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-  return v;
-}
-
-__device__ __forceinline__ int dot4_i8(int a, int b, int c) {
-  return __builtin_amdgcn_sdot4(a, b, c, false);
-}
-
-__device__ __forceinline__ int dot16_i8(const v4i& a, const v4i& b, int c) {
-  c = dot4_i8(a.x, b.x, c);
-  c = dot4_i8(a.y, b.y, c);
-  c = dot4_i8(a.z, b.z, c);
-  return dot4_i8(a.w, b.w, c);
-}
-
-// (score desc, id asc) as a descending u64 key; 0 is "no candidate".
-__device__ __host__ __forceinline__ uint64_t pack_key(int32_t score, int32_t id) {
-  return (uint64_t(uint32_t(score) ^ 0x80000000u) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(id));
-}
-__device__ __host__ __forceinline__ int32_t key_score(uint64_t k) {
-  return int32_t(uint32_t(k >> 32) ^ 0x80000000u);
-}
-__device__ __host__ __forceinline__ int32_t key_id(uint64_t k) {
-  return int32_t(0xFFFFFFFFu - uint32_t(k));
-}
-
-// xorshift32 step of the reference's init padding (src/main.cpp:273-275).
-__device__ __host__ __forceinline__ uint32_t xorshift32(uint32_t s) {
-  s ^= s << 13;
-  s ^= s >> 17;
-  s ^= s << 5;
-  return s;
-}
-
-// splitmix64 finaliser (Steele, Lea, Flood, OOPSLA 2014): node pivot hash.
-__device__ __host__ __forceinline__ uint64_t mix64(uint64_t z) {
-  z += 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-// Pivot positions of node (tree, depth, start, size): p1 in [0,size),
-// p2 in [0,size)\{p1}.
-__device__ __host__ __forceinline__ void node_pivots(uint64_t seed, int tree, int depth,
-                                                     int64_t start, int64_t size, int32_t* p1,
-                                                     int32_t* p2) {
-  const uint64_t key =
-      (uint64_t(uint32_t(tree)) << 48) ^ (uint64_t(uint32_t(depth)) << 40) ^ uint64_t(start);
-  const uint64_t h1 = mix64(seed ^ mix64(key));
-  const uint64_t h2 = mix64(h1);
-  const int64_t a = int64_t(h1 % uint64_t(size));
-  int64_t b = int64_t(h2 % uint64_t(size - 1));
-  if (b >= a) ++b;
-  *p1 = int32_t(a);
-  *p2 = int32_t(b);
-}
 
 // ---------------------------------------------------------------------------
 // K1: quantise + sketch.  One wave per row; lane l owns dims 256g+4l..+3, so a
 // wave reads 1 KB of float32 per step and the four ballots of one step hold
 // sketch words 4g..4g+3, interleaved by 4 (bit 4i+j <- lane i, element j).
 // ---------------------------------------------------------------------------
-
-// bit i of v (16 bits) -> bit 4i
-__device__ __forceinline__ uint64_t spread4(uint64_t x) {
-  x &= 0xFFFFull;
-  x = (x | (x << 24)) & 0x000000FF000000FFull;
-  x = (x | (x << 12)) & 0x000F000F000F000Full;
-  x = (x | (x << 6)) & 0x0303030303030303ull;
-  x = (x | (x << 3)) & 0x1111111111111111ull;
-  return x;
-}
-
-// src/main.cpp:184-186: multiply, min(127), max(-128); x86 DAZ (the reference
-// is built with -ffast-math) makes a denormal input read as zero.
-__device__ __forceinline__ float q_clamp(float v) {
-  v = (fabsf(v) < kFltMin) ? 0.0f : v;
-  const float s = v * 127.0f;
-  const float hi = (s < 127.0f) ? s : 127.0f;
-  return (hi > -128.0f) ? hi : -128.0f;
-}
 
 __global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ x, int64_t nrows,
                                                   int d, int8_t* __restrict__ q,
@@ -177,10 +83,6 @@ __global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ x, i
 // K2: RP forest, one level of every tree per pass.
 // ---------------------------------------------------------------------------
 
-struct Node {
-  int32_t tree, start, size, p1, p2;   // p1/p2: pivot positions inside the segment
-  int32_t left, right, nleft;          // children (index in next level, -1 = leaf)
-};
 
 __global__ void k_forest_seed(int64_t n, int T, int leaf, uint64_t seed, int32_t* perm,
                               int32_t* nodepos, Node* nodes) {
@@ -272,25 +174,6 @@ __global__ __launch_bounds__(256) void k_forest_flags(const int8_t* __restrict__
 // exclusive scan over int32 (tile = 256 threads x 16 elements)
 constexpr int kScanTile = 4096;
 
-__device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int* total) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int inc = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int t = __shfl_up(inc, o);
-    if (lane >= o) inc += t;
-  }
-  if (lane == 63) lds[w] = inc;
-  __syncthreads();
-  int off = 0;
-  const int nw = blockDim.x >> 6;
-  for (int j = 0; j < w; ++j) off += lds[j];
-  int tot = 0;
-  for (int j = 0; j < nw; ++j) tot += lds[j];
-  __syncthreads();
-  *total = tot;
-  return off + inc - v;
-}
 
 __global__ __launch_bounds__(256) void k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
                                                      int32_t* __restrict__ sums) {
@@ -343,19 +226,36 @@ __global__ __launch_bounds__(256) void k_scan_down(const int32_t* __restrict__ i
 // Children of every active node; children larger than a leaf become the next
 // level's nodes, the rest are appended to the leaf table.
 __global__ void k_forest_nodes(int64_t n, int leaf, uint64_t seed, int depth, Node* nodes,
-                               int nnodes, const int32_t* __restrict__ S, Node* next,
+                               int nnodes, const int32_t* __restrict__ S,
+                               const int32_t* __restrict__ perm, Node* next,
                                int32_t* next_count, int4* leaves, int32_t* leaf_count,
                                int64_t leaf_cap, int32_t* overflow) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnodes) return;
   Node nd = nodes[i];
   const int64_t gs = int64_t(nd.tree) * n + nd.start;
+  nodes[i].p1id = perm[gs + nd.p1];
+  nodes[i].p2id = perm[gs + nd.p2];
   const int32_t nl = S[gs + nd.size] - S[gs];
   const int32_t cs[2] = {nd.start, nd.start + nl};
   const int32_t cz[2] = {nl, nd.size - nl};
   int32_t child[2];
   for (int c = 0; c < 2; ++c) {
-    if (cz[c] > leaf) {
+    if (cz[c] > leaf && depth + 1 >= kMaxDepth) {
+      // degenerate input (e.g. many identical rows, whose ties always go
+      // right): cut the child into consecutive leaves of <= leaf rows
+      const int nch = (cz[c] + leaf - 1) / leaf;
+      const int id = atomicAdd(leaf_count, nch);
+      for (int k = 0; k < nch; ++k) {
+        const int32_t s0 = cs[c] + k * leaf;
+        const int32_t z = (cz[c] - k * leaf) < leaf ? (cz[c] - k * leaf) : leaf;
+        if (id + k < leaf_cap)
+          leaves[id + k] = make_int4(nd.tree, s0, z, 0);
+        else
+          *overflow = 1;
+      }
+      child[c] = -(id + 1);
+    } else if (cz[c] > leaf) {
       const int id = atomicAdd(next_count, 1);
       Node ch;
       ch.tree = nd.tree;
@@ -372,7 +272,7 @@ __global__ void k_forest_nodes(int64_t n, int leaf, uint64_t seed, int depth, No
         leaves[id] = make_int4(nd.tree, cs[c], cz[c], 0);
       else
         *overflow = 1;
-      child[c] = -1;
+      child[c] = -(id + 1);
     }
   }
   nodes[i].left = child[0];
@@ -408,6 +308,18 @@ __global__ __launch_bounds__(256) void k_forest_partition(
     perm2[np] = perm[gp];
     nodepos2[np] = child;
   }
+}
+
+// Append one level's nodes to the persistent hierarchy, turning next-level
+// child indices into global ones (the next level is appended right after).
+__global__ void k_hier_append(const Node* __restrict__ nodes, int nnodes, Node* __restrict__ hier,
+                              int64_t next_base) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  Node nd = nodes[i];
+  if (nd.left >= 0) nd.left = int32_t(next_base + nd.left);
+  if (nd.right >= 0) nd.right = int32_t(next_base + nd.right);
+  hier[i] = nd;
 }
 
 // ---------------------------------------------------------------------------
@@ -636,31 +548,14 @@ struct JoinScratch {
   int lds[8];
 };
 
-__device__ __forceinline__ bool hs_insert(int32_t* hs, int bits, int32_t key) {
-  const uint32_t mask = (1u << bits) - 1u;
-  uint32_t h = (uint32_t(key) * 2654435761u) >> (32 - bits);
-  while (true) {
-    const int32_t prev = atomicCAS(&hs[h], -1, key);
-    if (prev == -1) return true;
-    if (prev == key) return false;
-    h = (h + 1u) & mask;
-  }
-}
 
-// a before b in (score desc, id asc, slot asc)
-__device__ __forceinline__ bool before(int32_t sa, int32_t ia, int32_t la, int32_t sb, int32_t ib,
-                                       int32_t lb) {
-  if (sa != sb) return sa > sb;
-  if (ia != ib) return ia < ib;
-  return la < lb;
-}
 
 template <int HSBITS, int CAP>
 __global__ __launch_bounds__(256) void k_join(
     const int8_t* __restrict__ q, const uint64_t* __restrict__ sk, int d, int64_t n, int K, int M,
     const int32_t* __restrict__ ids_in, const int32_t* __restrict__ sc_in,
     int32_t* __restrict__ ids_out, int32_t* __restrict__ sc_out,
-    unsigned long long* __restrict__ counters) {
+    unsigned long long* __restrict__ counters, const int32_t* __restrict__ order) {
   constexpr int HS = 1 << HSBITS;
   static_assert(HS * 4 >= CAP * 2 + CAP * 4, "selection arrays must fit in the hash area");
   __shared__ __attribute__((aligned(16))) int32_t hs[HS];
@@ -676,7 +571,15 @@ __global__ __launch_bounds__(256) void k_join(
   const int words = d >> 6, nch = d >> 4;
   unsigned long long my_upd = 0, my_cand = 0, my_sel = 0;
 
-  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+  // Schedule: blocks are dealt to the 8 XCDs round-robin, so XCD x = block % 8
+  // takes the contiguous slice [x*n/8, (x+1)*n/8) of the processing order and
+  // its blocks walk it together.  With `order` = tree 0's leaf order,
+  // concurrently processed rows share most neighbours, which then hit that
+  // XCD's L2.  The result does not depend on the order (Jacobi iteration).
+  const int xcd = blockIdx.x & 7, per_xcd = gridDim.x >> 3;
+  const int64_t lo = n * xcd / 8, hi = n * (xcd + 1) / 8;
+  for (int64_t j = lo + (blockIdx.x >> 3); j < hi; j += per_xcd) {
+    const int64_t i = order ? int64_t(order[j]) : j;
     // ---- 0/1: reset, own list
     for (int j = tid; j < HS; j += 256) hs[j] = -1;
     for (int j = tid; j <= d; j += 256) S.hist[j] = 0;
@@ -692,40 +595,54 @@ __global__ __launch_bounds__(256) void k_join(
     if (tid < K) hs_insert(hs, HSBITS, S.own_id[tid]);
     if (tid == K) hs_insert(hs, HSBITS, int32_t(i));
     __syncthreads();
-    // ---- 2: 2-hop candidates, one neighbour row per wave step
-    for (int k = w; k < K; k += 4) {
-      const int32_t n1 = S.own_id[k];
-      if (lane < K) {
-        const int32_t c = ids_in[int64_t(n1) * K + lane];
-        if (hs_insert(hs, HSBITS, c)) {
-          const int slot = atomicAdd(&S.cnt[0], 1);
-          cid[slot] = c;
-        }
+    // ---- 2: 2-hop candidates; each wave loads all its neighbour rows first
+    {
+      constexpr int kRowsPerWave = kMaxK / 4;
+      int32_t c[kRowsPerWave];
+#pragma unroll
+      for (int r = 0; r < kRowsPerWave; ++r) {
+        const int k = w + 4 * r;
+        c[r] = (k < K && lane < K) ? ids_in[int64_t(S.own_id[k]) * K + lane] : -1;
       }
+#pragma unroll
+      for (int r = 0; r < kRowsPerWave; ++r)
+        if (c[r] >= 0 && hs_insert(hs, HSBITS, c[r])) {
+          const int slot = atomicAdd(&S.cnt[0], 1);
+          cid[slot] = c[r];
+        }
     }
     __syncthreads();
     const int U = S.cnt[0];
-    // ---- 3: sketch similarity, 8 lanes per candidate
+    // ---- 3: sketch similarity, 8 lanes per candidate, 4 candidates per group
     {
-      const int sub = tid & 7;
+      const int sub = tid & 7, grp = tid >> 3;
       uint64_t o0 = 0, o1 = 0;
       if (2 * sub < words) o0 = S.own_sk[2 * sub];
       if (2 * sub + 1 < words) o1 = S.own_sk[2 * sub + 1];
-      for (int u0 = 0; u0 < U; u0 += 32) {
-        const int u = u0 + (tid >> 3);
-        int pc = 0;
-        if (u < U) {
-          const uint64_t* cs = sk + int64_t(cid[u]) * words;
-          if (2 * sub < words) pc = __popcll(cs[2 * sub] ^ o0);
-          if (2 * sub + 1 < words) pc += __popcll(cs[2 * sub + 1] ^ o1);
+      for (int u0 = 0; u0 < U; u0 += 128) {
+        uint64_t a[4], b[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int u = u0 + grp + 32 * j;
+          a[j] = b[j] = 0;
+          if (u < U) {
+            const uint64_t* cs = sk + int64_t(cid[u]) * words;
+            if (2 * sub < words) a[j] = cs[2 * sub] ^ o0;
+            if (2 * sub + 1 < words) b[j] = cs[2 * sub + 1] ^ o1;
+          }
         }
-        pc += __shfl_xor(pc, 1);
-        pc += __shfl_xor(pc, 2);
-        pc += __shfl_xor(pc, 4);
-        if (u < U && sub == 0) {
-          const int h = d - pc;
-          ch[u] = uint16_t(h);
-          atomicAdd(&S.hist[h], 1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int u = u0 + grp + 32 * j;
+          int pc = __popcll(a[j]) + __popcll(b[j]);
+          pc += __shfl_xor(pc, 1);
+          pc += __shfl_xor(pc, 2);
+          pc += __shfl_xor(pc, 4);
+          if (u < U && sub == 0) {
+            const int h = d - pc;
+            ch[u] = uint16_t(h);
+            atomicAdd(&S.hist[h], 1);
+          }
         }
       }
     }
@@ -1005,102 +922,29 @@ __global__ __launch_bounds__(256) void k_score_pairs(const int8_t* __restrict__ 
 
 }  // namespace
 
-// ===========================================================================
-// host side
-// ===========================================================================
+namespace knng_host {
 
-struct knng_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  knng_params p{};
-  int64_t n = 0;
-  int32_t d = 0, words = 0;
-  int num_cu = 256;
-  std::string err;
+int launch_quantize(hipStream_t stream, const float* dx, int64_t nrows, int d, int8_t* q,
+                    uint64_t* sk, int num_cu) {
+  if (nrows <= 0) return 0;
+  int64_t g = (nrows + 3) / 4;
+  const int64_t cap = int64_t(num_cu) * 64;
+  if (g > cap) g = cap;
+  hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, stream, dx, nrows, d, q, sk);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
-  float* x = nullptr;          // owned fp32 rows (keep_fp32)
-  const float* x_ext = nullptr;
-  int8_t* q = nullptr;
-  uint64_t* sk = nullptr;
-  int64_t ingested = 0;
-
-  // forest
-  int32_t *perm = nullptr, *perm2 = nullptr, *nodepos = nullptr, *nodepos2 = nullptr;
-  int32_t *flags = nullptr, *scan = nullptr, *blocksums = nullptr;
-  Node *nodes = nullptr, *nodes2 = nullptr;
-  int4* leaves = nullptr;
-  int32_t* dcount = nullptr;   // [0] next nodes, [1] leaves, [2] overflow, [3] scan total
-  int64_t node_cap = 0, leaf_cap = 0, nleaves = 0;
-  bool have_forest = false;
-
-  // graph
-  int32_t *gid[2] = {nullptr, nullptr}, *gsc[2] = {nullptr, nullptr};
-  int cur = 0;
-  bool have_graph = false;
-  unsigned long long *counters = nullptr, *csum = nullptr;
-
-  // ingest staging
-  void* pin[2] = {nullptr, nullptr};
-  float* dstage[2] = {nullptr, nullptr};
-  hipEvent_t pin_ev[2] = {nullptr, nullptr};
-  size_t stage_bytes = 64ull << 20;
-
-  // export scratch
-  int32_t* ex_knns = nullptr;
-  float* ex_dists = nullptr;
-  int64_t ex_cap = 0;
-
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  knng_stats st{};
-};
+}  // namespace knng_host
 
 namespace {
 
-int fail(knng_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
-  return code;
-}
-
-#define HIPCHK(ctx, expr)                                                                 \
-  do {                                                                                    \
-    hipError_t e_ = (expr);                                                               \
-    if (e_ != hipSuccess)                                                                 \
-      return fail(ctx, KNNG_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
-  } while (0)
-
-template <typename T>
-int dalloc(knng_ctx* c, T** ptr, size_t count, const char* what) {
-  if (*ptr) return 0;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T) + 256);
-  if (e != hipSuccess) {
-    *ptr = nullptr;
-    return fail(c, KNNG_ENOMEM, std::string("hipMalloc failed for ") + what + " (" +
-                                     std::to_string(count * sizeof(T)) + " B): " +
-                                     hipGetErrorString(e));
-  }
-  return 0;
-}
-
-template <typename T>
-void dfree(T** ptr) {
-  if (*ptr) hipFree(reinterpret_cast<void*>(const_cast<typename std::remove_const<T>::type*>(*ptr)));
-  *ptr = nullptr;
-}
-
-int grid_for(const knng_ctx* c, int64_t work, int per_block, int blocks_per_cu) {
-  int64_t g = (work + per_block - 1) / per_block;
-  const int64_t cap = int64_t(c->num_cu) * blocks_per_cu;
-  if (g > cap) g = cap;
-  if (g < 1) g = 1;
-  return int(g);
-}
+using namespace knng_host;
 
 int quantize_rows(knng_ctx* c, const float* dx, int64_t row0, int64_t nrows) {
   if (nrows <= 0) return 0;
-  const int grid = grid_for(c, nrows, 4, 64);
-  hipLaunchKernelGGL(k_quantize, dim3(grid), dim3(256), 0, c->stream, dx, nrows, c->d,
-                     c->q + row0 * c->d, c->sk + row0 * c->words);
-  HIPCHK(c, hipGetLastError());
+  if (launch_quantize(c->stream, dx, nrows, c->d, c->q + row0 * c->d, c->sk + row0 * c->words,
+                      c->num_cu))
+    return fail(c, KNNG_EHIP, std::string("k_quantize launch: ") + hipGetErrorString(hipGetLastError()));
   return 0;
 }
 
@@ -1166,7 +1010,9 @@ int knng_create(knng_ctx** out, int32_t device, const knng_params* p, int64_t n)
   if (e != hipSuccess) return bail(fail(c, KNNG_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+  // a blocking stream: work the caller queued on the null stream (e.g. the
+  // kernels that produced a device input) completes before ours starts
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamDefault) != hipSuccess)
     return bail(fail(c, KNNG_EHIP, "hipStreamCreate failed"));
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
@@ -1199,6 +1045,7 @@ int knng_destroy(knng_ctx* c) {
   dfree(&c->nodes);
   dfree(&c->nodes2);
   dfree(&c->leaves);
+  dfree(&c->hier);
   dfree(&c->dcount);
   for (int b = 0; b < 2; ++b) {
     dfree(&c->gid[b]);
@@ -1227,6 +1074,23 @@ int knng_synchronize(knng_ctx* c) {
 int knng_attach_fp32_device(knng_ctx* c, const float* dev_x) {
   if (!c || !dev_x) return KNNG_EINVAL;
   c->x_ext = dev_x;
+  return 0;
+}
+
+int knng_host_alloc(knng_ctx* c, void** ptr, int64_t bytes) {
+  if (!c || !ptr || bytes <= 0) return KNNG_EINVAL;
+  *ptr = nullptr;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (hipHostMalloc(ptr, size_t(bytes), hipHostMallocDefault) != hipSuccess) {
+    *ptr = nullptr;
+    return fail(c, KNNG_ENOMEM, "hipHostMalloc failed (" + std::to_string(bytes) + " B)");
+  }
+  return 0;
+}
+
+int knng_host_free(knng_ctx* c, void* ptr) {
+  if (!c) return KNNG_EINVAL;
+  if (ptr) HIPCHK(c, hipHostFree(ptr));
   return 0;
 }
 
@@ -1325,8 +1189,10 @@ int knng_build_forest(knng_ctx* c) {
       (rc = dalloc(c, &c->blocksums, total / kScanTile + 2, "scan blocks")) ||
       (rc = dalloc(c, &c->nodes, c->node_cap, "nodes")) ||
       (rc = dalloc(c, &c->nodes2, c->node_cap, "nodes2")) ||
-      (rc = dalloc(c, &c->leaves, c->leaf_cap, "leaves")) || (rc = dalloc(c, &c->dcount, 8, "counts")))
+      (rc = dalloc(c, &c->leaves, c->leaf_cap, "leaves")) || (rc = dalloc(c, &c->dcount, 8, "counts")) ||
+      (rc = dalloc(c, &c->hier, c->leaf_cap, "forest hierarchy")))
     return rc;
+  c->hier_count = 0;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   HIPCHK(c, hipMemsetAsync(c->dcount, 0, 8 * sizeof(int32_t), c->stream));
   hipLaunchKernelGGL(k_forest_seed, dim3(grid_for(c, total, 256, 16)), dim3(256), 0, c->stream, n,
@@ -1356,8 +1222,12 @@ int knng_build_forest(knng_ctx* c) {
     if ((rc = scan_flags(c, total))) return rc;
     HIPCHK(c, hipMemsetAsync(c->dcount, 0, sizeof(int32_t), c->stream));   // next-node count
     hipLaunchKernelGGL(k_forest_nodes, dim3((nnodes + 255) / 256), dim3(256), 0, c->stream, n, leaf,
-                       c->p.seed, levels, c->nodes, nnodes, c->scan, c->nodes2, c->dcount,
+                       c->p.seed, levels, c->nodes, nnodes, c->scan, c->perm, c->nodes2, c->dcount,
                        c->leaves, c->dcount + 1, c->leaf_cap, c->dcount + 2);
+    if (c->hier_count + nnodes > c->leaf_cap) return fail(c, KNNG_ERANGE, "forest hierarchy overflow");
+    hipLaunchKernelGGL(k_hier_append, dim3((nnodes + 255) / 256), dim3(256), 0, c->stream, c->nodes,
+                       nnodes, c->hier + c->hier_count, c->hier_count + nnodes);
+    c->hier_count += nnodes;
     hipLaunchKernelGGL(k_forest_partition, dim3(grid_for(c, total, 256, 16)), dim3(256), 0, c->stream,
                        n, total, c->perm, c->nodepos, c->nodes, c->flags, c->scan, c->perm2,
                        c->nodepos2);
@@ -1442,13 +1312,15 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   HIPCHK(c, hipMemsetAsync(c->counters, 0, 4 * kNumCounters * sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   const int a = c->cur, b = c->cur ^ 1;
-  const int grid = grid_for(c, n, 1, 4);
+  // grid: a multiple of the 8 XCDs (the kernel's schedule needs it)
+  const int grid = std::max(8, (c->num_cu * 4) / 8 * 8);
+  const int32_t* order = c->have_forest ? c->perm : nullptr;   // tree 0's leaf order
   if (K <= 50)
     hipLaunchKernelGGL((k_join<12, 2560>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n, K,
-                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters);
+                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, order);
   else
     hipLaunchKernelGGL((k_join<13, 4096>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n, K,
-                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters);
+                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, order);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(256), 0, c->stream, c->counters, c->csum);

@@ -1,0 +1,109 @@
+// knng_internal.h -- host-side context of libknng.so shared by the build
+// (knng.hip) and serving (knng_serve.hip) translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "knng.h"
+#include "knng_device.h"
+
+using knng_dev::Node;
+
+
+struct knng_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  knng_params p{};
+  int64_t n = 0;
+  int32_t d = 0, words = 0;
+  int num_cu = 256;
+  std::string err;
+
+  float* x = nullptr;          // owned fp32 rows (keep_fp32)
+  const float* x_ext = nullptr;
+  int8_t* q = nullptr;
+  uint64_t* sk = nullptr;
+  int64_t ingested = 0;
+
+  // forest
+  int32_t *perm = nullptr, *perm2 = nullptr, *nodepos = nullptr, *nodepos2 = nullptr;
+  int32_t *flags = nullptr, *scan = nullptr, *blocksums = nullptr;
+  Node *nodes = nullptr, *nodes2 = nullptr;
+  int4* leaves = nullptr;
+  Node* hier = nullptr;        // every level's nodes, global child links (roots 0..T-1)
+  int64_t hier_count = 0;
+  int32_t* dcount = nullptr;   // [0] next nodes, [1] leaves, [2] overflow, [3] scan total
+  int64_t node_cap = 0, leaf_cap = 0, nleaves = 0;
+  bool have_forest = false;
+
+  // graph
+  int32_t *gid[2] = {nullptr, nullptr}, *gsc[2] = {nullptr, nullptr};
+  int cur = 0;
+  bool have_graph = false;
+  unsigned long long *counters = nullptr, *csum = nullptr;
+
+  // ingest staging
+  void* pin[2] = {nullptr, nullptr};
+  float* dstage[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  size_t stage_bytes = 64ull << 20;
+
+  // export scratch
+  int32_t* ex_knns = nullptr;
+  float* ex_dists = nullptr;
+  int64_t ex_cap = 0;
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  knng_stats st{};
+};
+
+namespace knng_host {
+
+inline int fail(knng_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                 \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(ctx, KNNG_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+template <typename T>
+inline int dalloc(knng_ctx* c, T** ptr, size_t count, const char* what) {
+  if (*ptr) return 0;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T) + 256);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return fail(c, KNNG_ENOMEM, std::string("hipMalloc failed for ") + what + " (" +
+                                     std::to_string(count * sizeof(T)) + " B): " +
+                                     hipGetErrorString(e));
+  }
+  return 0;
+}
+
+template <typename T>
+void dfree(T** ptr) {
+  if (*ptr) hipFree(reinterpret_cast<void*>(const_cast<typename std::remove_const<T>::type*>(*ptr)));
+  *ptr = nullptr;
+}
+
+inline int grid_for(const knng_ctx* c, int64_t work, int per_block, int blocks_per_cu) {
+  int64_t g = (work + per_block - 1) / per_block;
+  const int64_t cap = int64_t(c->num_cu) * blocks_per_cu;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return int(g);
+}
+
+// K1 on an arbitrary stream (defined in knng.hip)
+int launch_quantize(hipStream_t stream, const float* dx, int64_t nrows, int d, int8_t* q,
+                    uint64_t* sk, int num_cu);
+
+}  // namespace knng_host

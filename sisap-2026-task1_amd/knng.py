@@ -20,6 +20,7 @@ CLI = os.path.join(PKG, "bin", "knng_main")
 KNNG_SYMBOLS = [
     "knng_default_params", "knng_create", "knng_destroy", "knng_last_error",
     "knng_ingest_chunk", "knng_ingest_device", "knng_attach_fp32_device",
+    "knng_host_alloc", "knng_host_free",
     "knng_build", "knng_build_forest", "knng_build_init", "knng_join_iteration",
     "knng_export", "knng_export_device", "knng_get_stats", "knng_synchronize",
     "knng_copy_quantized", "knng_copy_forest", "knng_copy_graph", "knng_load_graph",
@@ -98,6 +99,8 @@ def lib() -> C.CDLL:
             "knng_last_error": ([V], C.c_char_p),
             "knng_ingest_chunk": ([V, V, i64, i64], i32),
             "knng_ingest_device": ([V, V, i64, i64], i32),
+            "knng_host_alloc": ([V, P(V), i64], i32),
+            "knng_host_free": ([V, V], i32),
             "knng_attach_fp32_device": ([V, V], i32),
             "knng_build": ([V], i32),
             "knng_build_forest": ([V], i32),

@@ -120,6 +120,24 @@ def test_forest_and_init_bitexact(torch_cuda, n, d, T, leaf):
     assert np.array_equal(sc, o_sc)
 
 
+def test_duplicate_rows_build_bitexact(torch_cuda):
+    """Many identical rows: the depth cap cuts them into leaves; whole build
+    still bit-exact."""
+    say("duplicates")
+    x = synth.gaussian_mixture(31, 3000, 256, 8)
+    x[100:2100] = x[100]
+    q, sk = oracle.quantize(x)
+    b = _builder(x, num_trees=4, leaf_size=16, k_build=20, k_eval=15)
+    b.build_forest()
+    perm, leaves = b.forest()
+    o_perm, o_leaves = oracle.forest_device(q, 4, 16, 1337)
+    assert np.array_equal(perm, o_perm) and np.array_equal(leaves, o_leaves)
+    b.build_init()
+    o_ids, o_sc = oracle.init_device(q, o_perm, o_leaves, 20)
+    ids, sc = b.graph()
+    assert np.array_equal(ids, o_ids) and np.array_equal(sc, o_sc)
+
+
 # ------------------------------------------------------------- K5-K7
 
 def test_join_iterations_bitexact(torch_cuda):
@@ -251,6 +269,7 @@ def test_large_build_properties(torch_cuda):
     n = 200000
     say("large", n)
     xt = synth.gaussian_mixture_torch(2, n, 1024, 1024)
+    torch.cuda.synchronize()
     b = knng.Builder(n, _params(keep_fp32=0))
     b.ingest_device(xt.data_ptr(), 0, n)
     b.attach_fp32(xt.data_ptr())

@@ -143,8 +143,9 @@ def _forest_bruteforce(q, T, leaf, seed):
         perm = list(range(n))
 
         def rec(start, size, depth):
-            if size <= leaf:
-                leaves.append((t, start, size))
+            if size <= leaf or depth >= 64:   # depth cap: cut into leaves of <= leaf
+                for s0 in range(0, size, leaf):
+                    leaves.append((t, start + s0, min(leaf, size - s0)))
                 return
             a, b = _pivots(seed, t, depth, start, size)
             seg = perm[start:start + size]
@@ -177,6 +178,19 @@ def test_forest_device_bruteforce():
         lt = leaves[leaves[:, 0] == t]
         assert lt[0, 1] == 0 and np.all(lt[1:, 1] == lt[:-1, 1] + lt[:-1, 2])
         assert lt[:, 2].sum() == 300 and lt[:, 2].max() <= 8 and lt[:, 2].min() >= 1
+
+
+def test_forest_device_duplicate_rows_terminate():
+    """Identical rows always tie (ties go right), so a node of duplicates only
+    sheds its pivot each level; past depth 64 the node is cut into leaves."""
+    x, q, sk = _small(n=300)
+    q[50:260] = q[50]
+    sys_limit = __import__("sys").getrecursionlimit()
+    __import__("sys").setrecursionlimit(max(sys_limit, 5000))
+    perm, leaves = oracle.forest_device(q, 2, 8, seed=5)
+    bp, bl = _forest_bruteforce(q, 2, 8, 5)
+    assert np.array_equal(perm, bp) and np.array_equal(leaves, bl)
+    assert leaves[:, 2].max() <= 8 and leaves[:, 2].sum() == 2 * 300
 
 
 def _topk_unique(cands, K):
