@@ -45,7 +45,13 @@ typedef struct knng_params {
   uint64_t seed;           /* pivot-hash seed (1337, the reference's tree seed base) */
   int32_t keep_fp32;       /* 1: keep the float32 rows in HBM for the FP32 rerank */
   int32_t rerank_width;    /* list entries re-scored in FP32 (k_eval..k_build); 0 = k_build */
+  int32_t stop_rule;       /* KNNG_STOP_*: which rate the stop rule of src/main.cpp:432-439 watches */
+  int32_t pad0;
 } knng_params;
+
+#define KNNG_STOP_REFERENCE 0  /* slots whose id changed, shifts included (src/main.cpp:421-427) */
+#define KNNG_STOP_INSERTED 1   /* ids that newly entered a list */
+#define KNNG_STOP_NONE 2       /* always run max_iters */
 
 /* Per-build statistics.  Times are HIP-event milliseconds on the context's
  * stream; bytes are algorithmic (counted from the work done, no cache credit). */
@@ -68,6 +74,8 @@ typedef struct knng_stats {
   double join_bytes[32];       /* algorithmic HBM bytes per join iteration */
   int64_t join_candidates[32]; /* unique 2-hop candidates scored by sketch */
   int64_t join_rescored[32];   /* candidates re-scored with int8 dots */
+  int64_t join_inserted[32];   /* ids that newly entered a list */
+  float insert_rates[32];      /* join_inserted / (n * k_build) */
 } knng_stats;
 
 typedef struct knng_ctx knng_ctx;
@@ -136,6 +144,63 @@ int knng_quantize_device(const float* dev_x, int64_t n, int32_t d, int8_t* dev_q
 int knng_score_pairs_device(const int8_t* dev_q, const uint64_t* dev_sketch, int32_t d,
                             const int32_t* dev_a, const int32_t* dev_b, int64_t npairs,
                             int32_t* dev_dot, int32_t* dev_sim);
+
+/* ------------------------------------------------------------------------
+ * Serving: kNN queries and inserts against a built, device-resident graph.
+ * The reference has no query path (it writes querytime = 0,
+ * src/main.cpp:479); these entry points are new.  Search = descent of
+ * `entry_trees` RP trees to their leaves for entry points, then beam search
+ * over the K-list graph with int8 scores, then (rerank=1) FP32 re-scoring of
+ * the final beam.  Result ids are 0-based, best first.
+ * ------------------------------------------------------------------------ */
+typedef struct knng_server knng_server;
+
+typedef struct knng_server_params {
+  int32_t beam;            /* beam width (<= 64) */
+  int32_t k;               /* neighbours returned per query (<= beam) */
+  int32_t entry_trees;     /* RP trees descended for entry points (<= num_trees) */
+  int32_t rerank;          /* 1: FP32 re-score of the final beam */
+  int32_t max_steps;       /* beam-search steps (4 expansions each) */
+  int32_t max_batch;       /* queries per launch (per slot) */
+  int32_t max_slots;       /* concurrent launches, one HIP stream each */
+  int32_t pad0;
+  int64_t insert_capacity; /* rows that later inserts may add */
+  int64_t reserve_bytes;   /* HBM left free when admitting a new slot */
+} knng_server_params;
+
+int knng_server_default_params(knng_server_params* sp);
+
+/* Copy the built graph, int8 rows and float32 rows of `ctx` into a
+ * device-resident index with room for insert_capacity more rows, and start
+ * the scheduler thread.  ctx must stay alive (its forest is used for entry
+ * points). */
+int knng_server_create(knng_server** out, knng_ctx* ctx, const knng_server_params* sp);
+int knng_server_destroy(knng_server* srv);
+const char* knng_server_last_error(const knng_server* srv);
+
+/* Queue a request (rows are copied): kind 0 = query batch of nrows vectors,
+ * kind 1 = insert of nrows new vectors (ids n, n+1, ... in submission order).
+ * Returns at once with a ticket.  Queries are split into max_batch pieces and
+ * admitted as slots and HBM free up; pieces of different requests run
+ * concurrently on different streams.  An insert waits until the queries
+ * admitted before it have finished, then updates the graph; queries admitted
+ * after it see the new rows. */
+int knng_server_submit(knng_server* srv, int32_t kind, const float* rows, int32_t nrows,
+                       int64_t* ticket);
+/* state: 0 queued, 1 running, 2 done. */
+int knng_server_poll(knng_server* srv, int64_t ticket, int32_t* state);
+/* Block until done; copy results (queries: nrows*k ids and scores; inserts:
+ * the new ids in ids[0..nrows)) and the submit-to-done latency; release the
+ * ticket.  ids/scores may be NULL. */
+int knng_server_wait(knng_server* srv, int64_t ticket, int32_t* ids, float* scores,
+                     double* latency_ms);
+/* Rows currently in the index. */
+int64_t knng_server_size(const knng_server* srv);
+
+/* One synchronous batch search on the server's index (no scheduler): host
+ * queries in, host results out.  Same kernel as the server's slots. */
+int knng_server_search(knng_server* srv, const float* queries, int32_t nq, int32_t* ids,
+                       float* scores);
 
 #ifdef __cplusplus
 }

@@ -350,8 +350,22 @@ int orc_export(const int32_t* ids, const int32_t* scores, int64_t n, int32_t K,
 // Forest with hash-chosen pivots, all trees.  perm is T x n; leaves are
 // written as (tree, start, size) triples in (tree, start) order.
 // Returns the number of leaves, or -1.
-int64_t orc_forest_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t leaf_size,
-                          uint64_t seed, int32_t* perm, int64_t* leaf_tri, int64_t leaf_cap) {
+}  // extern "C"
+
+namespace {
+
+struct Split {             // an internal node: pivot ids and left-child size
+  int32_t p1id, p2id;
+  int64_t nleft;
+};
+
+inline uint64_t node_key(int32_t tree, int32_t depth, int64_t start) {
+  return (uint64_t(uint32_t(tree)) << 48) ^ (uint64_t(uint32_t(depth)) << 40) ^ uint64_t(start);
+}
+
+int64_t forest_impl(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t leaf_size,
+                    uint64_t seed, int32_t* perm, int64_t* leaf_tri, int64_t leaf_cap,
+                    std::unordered_map<uint64_t, Split>* rec) {
   if (n < 2 || leaf_size < 2) return -1;
   int64_t nleaf = 0;
   std::vector<int32_t> lo, hi;
@@ -393,11 +407,110 @@ int64_t orc_forest_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int3
       std::copy(lo.begin(), lo.end(), P + nd.start);
       std::copy(hi.begin(), hi.end(), P + nd.start + int64_t(lo.size()));
       const int64_t nl = int64_t(lo.size());
+      if (rec) (*rec)[node_key(t, nd.depth, nd.start)] = Split{pa, pb, nl};
       stack.push_back(Node{nd.start + nl, nd.size - nl, nd.depth + 1});
       stack.push_back(Node{nd.start, nl, nd.depth + 1});
     }
   }
   return nleaf;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t orc_forest_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t leaf_size,
+                          uint64_t seed, int32_t* perm, int64_t* leaf_tri, int64_t leaf_cap) {
+  return forest_impl(q, n, d, T, leaf_size, seed, perm, leaf_tri, leaf_cap, nullptr);
+}
+
+// Beam search over a K-list graph (device semantics of the serving kernel):
+// entry points = members of the leaf reached by descending each of the first
+// min(entry_trees, T, 4) trees (left iff <q,p1> - <q,p2> > 0; a depth-capped
+// node enters its first <= leaf_size rows); then, per step, merge the scored
+// candidates into the beam (best `beam` by (score desc, id asc), expanded
+// flags kept), expand the first 4 unexpanded beam entries in beam order (at
+// most max_steps times), taking their neighbours that are not in the visited
+// set; the visited set is reset to the beam's ids when its insert count plus
+// 4K would exceed 3072.  Output: best k of the beam (ids, int8 scores),
+// padded with -1 / INT32_MIN.
+int orc_search_device(const int8_t* q, int64_t n, int32_t d, const int32_t* gid, int32_t K,
+                      int32_t T, int32_t leaf_size, uint64_t seed, int32_t entry_trees,
+                      int32_t beam, int32_t k, int32_t max_steps, const int8_t* qq, int64_t nq,
+                      int32_t* out_ids, int32_t* out_sc) {
+  std::vector<int32_t> perm(size_t(T) * n);
+  std::unordered_map<uint64_t, Split> rec;
+  if (forest_impl(q, n, d, T, leaf_size, seed, perm.data(), nullptr, 0, &rec) < 0) return -1;
+  const int et = std::min(std::min(entry_trees, T), 4);
+  constexpr int kExpand = 4;
+  struct B { int32_t id, sc, ex; };
+  for (int64_t qi = 0; qi < nq; ++qi) {
+    const int8_t* y = qq + qi * d;
+    std::unordered_set<int32_t> vis;
+    int64_t used = 0;
+    std::vector<int32_t> cands;
+    for (int t = 0; t < et; ++t) {
+      int64_t start = 0, size = n;
+      int32_t depth = 0;
+      while (size > leaf_size && depth < kMaxDepth) {
+        const Split& sp = rec.at(node_key(t, depth, start));
+        const int32_t diff = dot_i8(y, q + int64_t(sp.p1id) * d, d) - dot_i8(y, q + int64_t(sp.p2id) * d, d);
+        if (diff > 0) {
+          size = sp.nleft;
+        } else {
+          start += sp.nleft;
+          size -= sp.nleft;
+        }
+        ++depth;
+      }
+      const int64_t m = std::min<int64_t>(size, leaf_size);
+      for (int64_t j = 0; j < m; ++j) {
+        const int32_t id = perm[size_t(t) * n + start + j];
+        if (vis.insert(id).second) {
+          ++used;
+          cands.push_back(id);
+        }
+      }
+    }
+    std::vector<B> bm, all;
+    for (int step = 0;; ++step) {
+      all = bm;
+      for (int32_t c : cands) all.push_back(B{c, dot_i8(y, q + int64_t(c) * d, d), 0});
+      std::sort(all.begin(), all.end(), [](const B& a, const B& b) {
+        return a.sc != b.sc ? a.sc > b.sc : a.id < b.id;
+      });
+      if (int(all.size()) > beam) all.resize(beam);
+      bm = all;
+      std::vector<int32_t> sel;
+      if (step < max_steps)
+        for (auto& e : bm)
+          if (!e.ex && int(sel.size()) < kExpand) {
+            e.ex = 1;
+            sel.push_back(e.id);
+          }
+      if (sel.empty()) break;
+      cands.clear();
+      if (used + int64_t(kExpand) * K > 3072) {
+        vis.clear();
+        for (auto& e : bm) vis.insert(e.id);
+        used = int64_t(bm.size());
+      }
+      for (int32_t sid : sel)
+        for (int e = 0; e < K; ++e) {
+          const int32_t id = gid[int64_t(sid) * K + e];
+          if (vis.insert(id).second) {
+            ++used;
+            cands.push_back(id);
+          }
+        }
+    }
+    for (int j = 0; j < k; ++j) {
+      const bool ok = j < int(bm.size());
+      out_ids[qi * k + j] = ok ? bm[j].id : -1;
+      out_sc[qi * k + j] = ok ? bm[j].sc : kScoreMin;
+    }
+  }
+  return 0;
 }
 
 static inline bool edge_before(const Edge& x, const Edge& y) {

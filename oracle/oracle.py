@@ -55,6 +55,8 @@ def lib() -> C.CDLL:
             "orc_sketch_threshold": ([V, i64, i32, i32], i32),
             "orc_join_device": ([V, V, i64, i32, i32, i32, V, V, V, V], i64),
             "orc_rerank_device": ([V, i64, i32, i32, i32, i32, V, V, V], i32),
+            "orc_search_device": ([V, i64, i32, V, i32, i32, i32, u64, i32, i32, i32, i32, V, i64,
+                                   V, V], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -203,3 +205,18 @@ def run_reference_binary(h5_path: str, workdir: str, threads: int = 1, native: b
                        timeout=timeout)
     return {"returncode": r.returncode, "stdout": r.stdout,
             "out": os.path.join(workdir, "results", task, "apex_simd.h5")}
+
+
+def search_device(q, gid, queries_q, T=16, leaf=64, seed=1337, entry_trees=4, beam=64, k=15,
+                  max_steps=128):
+    """Beam search over graph `gid` (device semantics of the serving kernel)."""
+    n, d = q.shape
+    K = gid.shape[1]
+    nq = queries_q.shape[0]
+    ids = np.empty((nq, k), np.int32)
+    sc = np.empty((nq, k), np.int32)
+    rc = lib().orc_search_device(_p(np.ascontiguousarray(q)), n, d, _p(np.ascontiguousarray(gid)), K,
+                                 T, leaf, seed, entry_trees, beam, k, max_steps,
+                                 _p(np.ascontiguousarray(queries_q)), nq, _p(ids), _p(sc))
+    assert rc == 0
+    return ids, sc

@@ -544,7 +544,7 @@ struct JoinScratch {
   int32_t new_sc[kMaxK];
   uint64_t own_sk[kMaxDim / 64];
   int32_t hist[kMaxDim + 1];
-  int32_t cnt[4];   // 0 unique candidates, 1 selected, 2 survivors, 3 threshold
+  int32_t cnt[5];   // 0 unique candidates, 1 selected, 2 survivors, 3 threshold, 4 new ids
   int lds[8];
 };
 
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(256) void k_join(
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int words = d >> 6, nch = d >> 4;
-  unsigned long long my_upd = 0, my_cand = 0, my_sel = 0;
+  unsigned long long my_upd = 0, my_cand = 0, my_sel = 0, my_ins = 0;
 
   // Schedule: blocks are dealt to the 8 XCDs round-robin, so XCD x = block % 8
   // takes the contiguous slice [x*n/8, (x+1)*n/8) of the processing order and
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(256) void k_join(
     // ---- 0/1: reset, own list
     for (int j = tid; j < HS; j += 256) hs[j] = -1;
     for (int j = tid; j <= d; j += 256) S.hist[j] = 0;
-    if (tid < 4) S.cnt[tid] = 0;
+    if (tid < 5) S.cnt[tid] = 0;
     for (int c = tid; c < nch; c += 256)
       *reinterpret_cast<v4i*>(own_row + 16 * c) = *reinterpret_cast<const v4i*>(q + i * d + 16 * c);
     if (tid < words) S.own_sk[tid] = sk[i * words + tid];
@@ -766,6 +766,7 @@ __global__ __launch_bounds__(256) void k_join(
       if (r < K) {
         S.new_id[r] = id;
         S.new_sc[r] = sc;
+        if (e >= K) atomicAdd(&S.cnt[4], 1);   // a new id entered the list
       }
     }
     __syncthreads();
@@ -782,6 +783,7 @@ __global__ __launch_bounds__(256) void k_join(
         my_upd += __popcll(m);
         my_cand += U;
         my_sel += NS;
+        my_ins += S.cnt[4];
       }
     }
     __syncthreads();
@@ -791,23 +793,19 @@ __global__ __launch_bounds__(256) void k_join(
     atomicAdd(c + 0, my_upd);
     atomicAdd(c + 1, my_cand);
     atomicAdd(c + 2, my_sel);
+    atomicAdd(c + 3, my_ins);
   }
 }
 
 __global__ void k_sum_counters(const unsigned long long* __restrict__ c,
                                unsigned long long* __restrict__ out) {
-  __shared__ unsigned long long part[3][256];
-  unsigned long long a = 0, b = 0, e = 0;
-  for (int j = threadIdx.x; j < kNumCounters; j += 256) {
-    a += c[4 * j];
-    b += c[4 * j + 1];
-    e += c[4 * j + 2];
-  }
-  part[0][threadIdx.x] = a;
-  part[1][threadIdx.x] = b;
-  part[2][threadIdx.x] = e;
+  __shared__ unsigned long long part[4][256];
+  unsigned long long a[4] = {0, 0, 0, 0};
+  for (int j = threadIdx.x; j < kNumCounters; j += 256)
+    for (int f = 0; f < 4; ++f) a[f] += c[4 * j + f];
+  for (int f = 0; f < 4; ++f) part[f][threadIdx.x] = a[f];
   __syncthreads();
-  if (threadIdx.x < 3) {
+  if (threadIdx.x < 4) {
     unsigned long long s = 0;
     for (int j = 0; j < 256; ++j) s += part[threadIdx.x][j];
     out[threadIdx.x] = s;
@@ -959,6 +957,7 @@ int check_params(const knng_params* p, int64_t n, std::string* why) {
   if (p->max_iters < 0 || p->max_iters > 32) return *why = "max_iters must be in [0, 32]", KNNG_ERANGE;
   if (p->rerank_width < 0 || p->rerank_width > p->k_build) return *why = "rerank_width must be in [0, k_build]", KNNG_EINVAL;
   if (p->rerank_width && p->rerank_width < p->k_eval) return *why = "rerank_width must be >= k_eval", KNNG_EINVAL;
+  if (p->stop_rule < 0 || p->stop_rule > 2) return *why = "stop_rule must be 0, 1 or 2", KNNG_EINVAL;
   if (int64_t(p->num_trees) * n >= (int64_t(1) << 31) - 1) return *why = "num_trees * n must fit in int32", KNNG_ERANGE;
   if (n >= (int64_t(1) << 31) - 1) return *why = "n must fit in int32", KNNG_ERANGE;
   return 0;
@@ -982,6 +981,8 @@ int knng_default_params(knng_params* p) {
   p->seed = 1337;
   p->keep_fp32 = 1;
   p->rerank_width = 0;
+  p->stop_rule = KNNG_STOP_REFERENCE;
+  p->pad0 = 0;
   return 0;
 }
 
@@ -1324,7 +1325,7 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(256), 0, c->stream, c->counters, c->csum);
-  unsigned long long sums[3];
+  unsigned long long sums[4];
   HIPCHK(c, hipMemcpyAsync(sums, c->csum, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0;
@@ -1337,6 +1338,8 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
     c->st.ms_join_iter[it] = ms;
     c->st.join_candidates[it] = int64_t(sums[1]);
     c->st.join_rescored[it] = int64_t(sums[2]);
+    c->st.join_inserted[it] = int64_t(sums[3]);
+    c->st.insert_rates[it] = float(sums[3]) / float(size_t(n) * size_t(K));
     // algorithmic bytes: own list + own row/sketch, K neighbour id rows,
     // one sketch per unique candidate, one row per re-scored candidate, writes
     c->st.join_bytes[it] = double(n) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) +
@@ -1360,8 +1363,13 @@ int knng_build(knng_ctx* c) {
   for (int it = 0; it < c->p.max_iters; ++it) {
     int64_t upd = 0;
     if ((rc = knng_join_iteration(c, &upd))) return rc;
-    // stop rule: src/main.cpp:432-439
-    const float rate = float(upd) / float(size_t(c->n) * size_t(c->p.k_build));
+    // stop rule of src/main.cpp:432-439 on the chosen rate: the reference's
+    // (slots whose id changed, shifts included) or the new-entry rate
+    const int j = c->st.iterations - 1 < 31 ? c->st.iterations - 1 : 31;
+    const float rate = c->p.stop_rule == KNNG_STOP_INSERTED
+                           ? c->st.insert_rates[j]
+                           : float(upd) / float(size_t(c->n) * size_t(c->p.k_build));
+    if (c->p.stop_rule == KNNG_STOP_NONE) continue;
     if (rate < c->p.early_stop_rate || (it > 0 && (prev - rate) < c->p.rate_diff_tol)) break;
     prev = rate;
   }

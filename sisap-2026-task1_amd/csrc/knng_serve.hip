@@ -1,0 +1,842 @@
+// knng_serve.hip -- device-resident kNN serving on a built graph (the
+// knng_server_* entry points of include/knng.h).  The reference engine has
+// no query path (src/main.cpp:479 writes querytime = 0); this is new.
+//
+// Index in HBM: int8 rows + sketches + float32 rows + the K-list graph, with
+// room for insert_capacity more rows.  Search (k_search, one 256-thread block
+// per query):
+//   1. quantise the query with K1 (same int8 rules as the build);
+//   2. descend `entry_trees` RP trees of the build's forest (same two-pivot
+//      rule) and take the leaves' members as entry candidates;
+//   3. beam search: the best unexpanded beam entries (4 per step, one per
+//      wave) are expanded, their K neighbours filtered through an LDS visited
+//      set, scored with int8 dots (16 lanes per 1 KB row gather) and merged
+//      into the beam by rank in (score desc, id asc) order;
+//   4. optional FP32 re-score of the final beam, best k out.
+// Results depend only on the index contents, never on what else is running.
+//
+// Scheduler (one host thread): requests are split into max_batch pieces;
+// each piece runs on a "slot" (HIP stream + buffers).  Slots are created on
+// demand while fewer than max_slots exist and hipMemGetInfo leaves
+// reserve_bytes free; pieces of different requests overlap on the device.
+// Inserts drain the running queries, then append rows, search their
+// neighbours, and merge reverse edges into the neighbours' lists.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "knng.h"
+#include "knng_device.h"
+#include "knng_internal.h"
+
+using namespace knng_dev;
+
+namespace {
+
+constexpr int kBeamMax = 64;
+constexpr int kExpand = 4;        // beam entries expanded per step (one per wave)
+constexpr int kCandCap = 256;     // >= kExpand * kMaxK and >= 4 entry leaves of 64
+constexpr int kVisitBits = 12;    // visited set: 4096 slots, reset when 3/4 full
+
+__device__ __forceinline__ bool ahead(int32_t sa, int32_t ia, int32_t sb, int32_t ib) {
+  return sa != sb ? sa > sb : ia < ib;   // (score desc, id asc)
+}
+
+__global__ __launch_bounds__(256) void k_search(
+    const int8_t* __restrict__ q, const float* __restrict__ x, int d,
+    const int32_t* __restrict__ gid, int K, const Node* __restrict__ hier, int64_t hier_count,
+    const int4* __restrict__ leaves, const int32_t* __restrict__ perm, int64_t n_forest,
+    int entry_trees, const int8_t* __restrict__ qq, const float* __restrict__ qx, int nq, int beam,
+    int k, int rerank, int max_steps, int32_t* __restrict__ out_ids, float* __restrict__ out_sc) {
+  constexpr int HS = 1 << kVisitBits;
+  __shared__ int32_t hs[HS];
+  __shared__ __attribute__((aligned(16))) int8_t qrow[kMaxDim];
+  __shared__ int32_t cid[kCandCap], csc[kCandCap], surv[kCandCap];
+  __shared__ int32_t bid[kBeamMax], bsc[kBeamMax], bexp[kBeamMax];
+  __shared__ int32_t nid[kBeamMax], nsc[kBeamMax], nexp[kBeamMax];
+  __shared__ float fsc[kBeamMax];
+  __shared__ int32_t sel[kExpand];
+  __shared__ int32_t cnt[8];   // 0 candidates, 1 beam size, 2 visited used, 3 selected, 4 survivors
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int g = lane >> 4, s16 = lane & 15;
+  const int nch = d >> 4;
+
+  for (int qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+    for (int j = tid; j < HS; j += 256) hs[j] = -1;
+    if (tid < 8) cnt[tid] = 0;
+    for (int c = tid; c < nch; c += 256)
+      *reinterpret_cast<v4i*>(qrow + 16 * c) = *reinterpret_cast<const v4i*>(qq + int64_t(qi) * d + 16 * c);
+    __syncthreads();
+    v4i own[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int chn = s16 + 16 * c;
+      own[c] = chn < nch ? *reinterpret_cast<const v4i*>(qrow + 16 * chn) : v4i{0, 0, 0, 0};
+    }
+
+    // ---- entry points: descend trees w, w+4, ... to a leaf
+    for (int t = w; t < entry_trees; t += 4) {
+      int leaf_idx = -1;
+      if (hier_count == 0) {
+        leaf_idx = t;   // every tree is a single leaf
+      } else {
+        int node = t;
+        for (int depth = 0; depth < 4096 && leaf_idx < 0; ++depth) {
+          const Node nd = hier[node];
+          const int8_t* ra = q + int64_t(nd.p1id) * d;
+          const int8_t* rb = q + int64_t(nd.p2id) * d;
+          int s = 0;
+          for (int c = lane; c < nch; c += 64) {
+            const v4i v = *reinterpret_cast<const v4i*>(qrow + 16 * c);
+            s = dot16_i8(v, *reinterpret_cast<const v4i*>(ra + 16 * c), s);
+            s -= dot16_i8(v, *reinterpret_cast<const v4i*>(rb + 16 * c), 0);
+          }
+          s = wave_sum_i(s);
+          const int child = s > 0 ? nd.left : nd.right;
+          if (child < 0)
+            leaf_idx = -child - 1;
+          else
+            node = child;
+        }
+      }
+      if (leaf_idx >= 0) {
+        const int4 lf = leaves[leaf_idx];
+        for (int j = lane; j < lf.z; j += 64) {
+          const int32_t id = perm[int64_t(lf.x) * n_forest + lf.y + j];
+          if (hs_insert(hs, kVisitBits, id)) {
+            atomicAdd(&cnt[2], 1);
+            const int slot = atomicAdd(&cnt[0], 1);
+            if (slot < kCandCap) cid[slot] = id;
+          }
+        }
+      }
+    }
+    __syncthreads();
+
+    for (int step = 0;; ++step) {
+      // ---- score the candidates: 16 lanes per 1 KB row, 2 rows per lane group
+      const int nc = cnt[0] < kCandCap ? cnt[0] : kCandCap;
+      for (int s0 = 0; s0 < nc; s0 += 32) {
+        const int sa = s0 + w * 4 + g, sb = sa + 16;
+        const int8_t* ra = sa < nc ? q + int64_t(cid[sa]) * d : nullptr;
+        const int8_t* rb = sb < nc ? q + int64_t(cid[sb]) * d : nullptr;
+        int da = 0, db = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int chn = s16 + 16 * c;
+          if (chn < nch) {
+            if (ra) da = dot16_i8(*reinterpret_cast<const v4i*>(ra + 16 * chn), own[c], da);
+            if (rb) db = dot16_i8(*reinterpret_cast<const v4i*>(rb + 16 * chn), own[c], db);
+          }
+        }
+#pragma unroll
+        for (int m = 8; m >= 1; m >>= 1) {
+          da += __shfl_xor(da, m);
+          db += __shfl_xor(db, m);
+        }
+        if (s16 == 0) {
+          if (sa < nc) csc[sa] = da;
+          if (sb < nc) csc[sb] = db;
+        }
+      }
+      if (tid == 0) cnt[4] = 0;
+      __syncthreads();
+      // ---- merge candidates into the beam by rank
+      const int nb = cnt[1];
+      for (int u = tid; u < nc; u += 256) {
+        int r = 0;
+        for (int j = 0; j < nb; ++j) r += ahead(bsc[j], bid[j], csc[u], cid[u]) ? 1 : 0;
+        if (r < beam) surv[atomicAdd(&cnt[4], 1)] = u;
+      }
+      __syncthreads();
+      const int P = cnt[4];
+      for (int e = tid; e < nb + P; e += 256) {
+        int32_t sc, id, ex;
+        int r;
+        if (e < nb) {
+          sc = bsc[e];
+          id = bid[e];
+          ex = bexp[e];
+          r = e;
+        } else {
+          const int u = surv[e - nb];
+          sc = csc[u];
+          id = cid[u];
+          ex = 0;
+          r = 0;
+          for (int j = 0; j < nb; ++j) r += ahead(bsc[j], bid[j], sc, id) ? 1 : 0;
+        }
+        for (int p = 0; p < P && r < beam; ++p) {
+          const int u = surv[p];
+          r += ahead(csc[u], cid[u], sc, id) ? 1 : 0;
+        }
+        if (r < beam) {
+          nid[r] = id;
+          nsc[r] = sc;
+          nexp[r] = ex;
+        }
+      }
+      __syncthreads();
+      const int nnew = (nb + P) < beam ? (nb + P) : beam;
+      if (tid < nnew) {
+        bid[tid] = nid[tid];
+        bsc[tid] = nsc[tid];
+        bexp[tid] = nexp[tid];
+      }
+      __syncthreads();
+      // ---- pick the best unexpanded entries (wave 0)
+      if (w == 0) {
+        unsigned long long m = __ballot(lane < nnew && !bexp[lane]);
+        int ns = 0;
+        for (int e = 0; e < kExpand && m; ++e) {
+          const int j = __ffsll(m) - 1;
+          m &= m - 1;
+          if (lane == 0) {
+            sel[e] = bid[j];
+            bexp[j] = 1;
+          }
+          ++ns;
+        }
+        if (lane == 0) {
+          cnt[1] = nnew;
+          cnt[3] = (step < max_steps) ? ns : 0;
+          cnt[0] = 0;
+        }
+      }
+      __syncthreads();
+      const int ns = cnt[3];
+      if (ns == 0) break;
+      // ---- keep the visited set below 3/4 full: reset it to the beam
+      if (cnt[2] + kExpand * K > (HS * 3) / 4) {
+        for (int j = tid; j < HS; j += 256) hs[j] = -1;
+        __syncthreads();
+        if (tid < nnew) hs_insert(hs, kVisitBits, bid[tid]);
+        if (tid == 0) cnt[2] = nnew;
+        __syncthreads();
+      }
+      // ---- expand: wave w takes selected entry w
+      if (w < ns && lane < K) {
+        const int32_t id = gid[int64_t(sel[w]) * K + lane];
+        if (hs_insert(hs, kVisitBits, id)) {
+          atomicAdd(&cnt[2], 1);
+          cid[atomicAdd(&cnt[0], 1)] = id;
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- output
+    const int nb = cnt[1];
+    if (rerank) {
+      float4 ox[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int chn = lane + 64 * c;
+        ox[c] = chn < (d >> 2) ? reinterpret_cast<const float4*>(qx + int64_t(qi) * d)[chn]
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      for (int j = w; j < nb; j += 4) {
+        const float4* row = reinterpret_cast<const float4*>(x + int64_t(bid[j]) * d);
+        float s = 0.f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int chn = lane + 64 * c;
+          if (chn < (d >> 2)) {
+            const float4 v = row[chn];
+            s = fmaf(ox[c].x, v.x, s);
+            s = fmaf(ox[c].y, v.y, s);
+            s = fmaf(ox[c].z, v.z, s);
+            s = fmaf(ox[c].w, v.w, s);
+          }
+        }
+        s = wave_sum_f(s);
+        if (lane == 0) fsc[j] = s;
+      }
+      __syncthreads();
+      if (tid < nb) {
+        const float s = fsc[tid];
+        const int32_t id = bid[tid];
+        int r = 0;
+        for (int j = 0; j < nb; ++j) r += (fsc[j] > s || (fsc[j] == s && bid[j] < id)) ? 1 : 0;
+        if (r < k) {
+          out_ids[int64_t(qi) * k + r] = id;
+          out_sc[int64_t(qi) * k + r] = s;
+        }
+      }
+    } else if (tid < nb && tid < k) {
+      out_ids[int64_t(qi) * k + tid] = bid[tid];
+      out_sc[int64_t(qi) * k + tid] = float(bsc[tid]);
+    }
+    if (tid >= nb && tid < k) {
+      out_ids[int64_t(qi) * k + tid] = -1;
+      out_sc[int64_t(qi) * k + tid] = -INFINITY;
+    }
+    __syncthreads();
+  }
+}
+
+// Merge reverse-edge proposals (new row p with score s) into target rows'
+// lists; one block per target.  The K kept entries are the forced proposals
+// first (a new row always enters the lists of its nearest neighbours, so a
+// search that reaches them can reach it), then the best of the rest; the
+// kept entries are stored in (score desc, id asc) order.
+__global__ __launch_bounds__(64) void k_merge_reverse(int32_t* __restrict__ gid,
+                                                      int32_t* __restrict__ gsc, int K,
+                                                      const int32_t* __restrict__ targets,
+                                                      const int32_t* __restrict__ offs,
+                                                      const int32_t* __restrict__ pid,
+                                                      const int32_t* __restrict__ psc,
+                                                      const int32_t* __restrict__ pforced,
+                                                      int ntargets) {
+  __shared__ int32_t li[2 * kMaxK], ls[2 * kMaxK], lf[2 * kMaxK], keep[2 * kMaxK];
+  __shared__ int32_t oi[kMaxK], os[kMaxK];
+  const int lane = threadIdx.x;
+  for (int t = blockIdx.x; t < ntargets; t += gridDim.x) {
+    const int64_t row = targets[t];
+    const int p0 = offs[t], np = offs[t + 1] - offs[t];
+    const int m = np < kMaxK ? np : kMaxK;
+    if (lane < K) {
+      li[lane] = gid[row * K + lane];
+      ls[lane] = gsc[row * K + lane];
+      lf[lane] = 0;
+    }
+    if (lane < m) {
+      li[K + lane] = pid[p0 + lane];
+      ls[K + lane] = psc[p0 + lane];
+      lf[K + lane] = pforced[p0 + lane];
+    }
+    __syncthreads();
+    auto first = [&](int a, int b, bool use_forced) {   // a ahead of b
+      if (use_forced && lf[a] != lf[b]) return lf[a] > lf[b];
+      if (ls[a] != ls[b]) return ls[a] > ls[b];
+      if (li[a] != li[b]) return li[a] < li[b];
+      return a < b;
+    };
+    for (int e = lane; e < K + m; e += 64) {   // which K survive
+      int r = 0;
+      for (int j = 0; j < K + m; ++j) r += first(j, e, true) ? 1 : 0;
+      keep[e] = r < K;
+    }
+    __syncthreads();
+    for (int e = lane; e < K + m; e += 64) {   // their order
+      if (!keep[e]) continue;
+      int r = 0;
+      for (int j = 0; j < K + m; ++j) r += (keep[j] && first(j, e, false)) ? 1 : 0;
+      oi[r] = li[e];
+      os[r] = ls[e];
+    }
+    __syncthreads();
+    if (lane < K) {
+      gid[row * K + lane] = oi[lane];
+      gsc[row * K + lane] = os[lane];
+    }
+    __syncthreads();
+  }
+}
+
+using clk = std::chrono::steady_clock;
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  float* dx = nullptr;
+  int8_t* dq = nullptr;
+  uint64_t* dsk = nullptr;
+  int32_t* did = nullptr;
+  float* dsc = nullptr;
+  float* hx = nullptr;        // pinned staging
+  int32_t* hid = nullptr;
+  float* hsc = nullptr;
+  bool busy = false;
+  int64_t ticket = -1;
+  int32_t row0 = 0, nrows = 0;
+};
+
+struct Request {
+  int64_t ticket = 0;
+  int kind = 0;
+  int32_t nrows = 0;
+  std::vector<float> rows;
+  std::vector<int32_t> ids;
+  std::vector<float> scores;
+  int32_t next_row = 0;
+  int32_t running = 0;
+  int state = 0;   // 0 queued, 1 running, 2 done
+  int error = 0;
+  clk::time_point t_submit, t_done;
+};
+
+}  // namespace
+
+struct knng_server {
+  knng_ctx* ctx = nullptr;
+  knng_server_params sp{};
+  int device = 0, num_cu = 256, d = 0, words = 0, K = 0;
+  std::atomic<int64_t> n{0};   // rows in the index (grows with inserts)
+  int64_t cap = 0, n_forest = 0;
+  int8_t* q = nullptr;
+  uint64_t* sk = nullptr;
+  float* x = nullptr;
+  int32_t* gid = nullptr;
+  int32_t* gsc = nullptr;
+  hipStream_t ins_stream = nullptr;
+  std::vector<Slot*> slots;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Request*> queue;
+  std::map<int64_t, Request*> reqs;
+  int64_t next_ticket = 1;
+  bool stop = false;
+  std::thread th;
+  std::string err;
+};
+
+namespace {
+
+int sfail(knng_server* s, int code, const std::string& m) {
+  if (s) s->err = m;
+  return code;
+}
+
+#define SCHK(srv, expr)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return sfail(srv, KNNG_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+size_t slot_device_bytes(const knng_server* s) {
+  const size_t b = size_t(s->sp.max_batch);
+  return b * (size_t(s->d) * 5 + size_t(s->words) * 8 + size_t(s->sp.k) * 8) + 4096;
+}
+
+void free_slot(Slot* sl) {
+  if (!sl) return;
+  if (sl->stream) (void)hipStreamSynchronize(sl->stream);
+  for (void* p : {static_cast<void*>(sl->dx), static_cast<void*>(sl->dq),
+                  static_cast<void*>(sl->dsk), static_cast<void*>(sl->did),
+                  static_cast<void*>(sl->dsc)})
+    if (p) (void)hipFree(p);
+  for (void* p : {static_cast<void*>(sl->hx), static_cast<void*>(sl->hid), static_cast<void*>(sl->hsc)})
+    if (p) (void)hipHostFree(p);
+  if (sl->done) (void)hipEventDestroy(sl->done);
+  if (sl->stream) (void)hipStreamDestroy(sl->stream);
+  delete sl;
+}
+
+Slot* new_slot(knng_server* s) {
+  auto* sl = new Slot();
+  const size_t B = size_t(s->sp.max_batch);
+  bool ok = hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&sl->done, hipEventDisableTiming) == hipSuccess &&
+            hipMalloc(&sl->dx, B * s->d * sizeof(float)) == hipSuccess &&
+            hipMalloc(&sl->dq, B * s->d) == hipSuccess &&
+            hipMalloc(&sl->dsk, B * s->words * sizeof(uint64_t)) == hipSuccess &&
+            hipMalloc(&sl->did, B * s->sp.k * sizeof(int32_t)) == hipSuccess &&
+            hipMalloc(&sl->dsc, B * s->sp.k * sizeof(float)) == hipSuccess &&
+            hipHostMalloc(&sl->hx, B * s->d * sizeof(float), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc(&sl->hid, B * s->sp.k * sizeof(int32_t), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc(&sl->hsc, B * s->sp.k * sizeof(float), hipHostMallocDefault) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    free_slot(sl);
+    return nullptr;
+  }
+  return sl;
+}
+
+// queue one search launch on `stream`: device queries dx (float32) -> ids/scores
+int launch_search(knng_server* s, hipStream_t stream, const float* dx, int8_t* dq, uint64_t* dsk,
+                  int32_t nq, int beam, int k, int rerank, int32_t* did, float* dsc) {
+  knng_ctx* c = s->ctx;
+  if (knng_host::launch_quantize(stream, dx, nq, s->d, dq, dsk, s->num_cu))
+    return sfail(s, KNNG_EHIP, "quantise launch failed");
+  const int grid = std::max(1, std::min<int>(nq, s->num_cu * 8));
+  // at most 4 trees: their leaves (<= 64 rows each) fill the candidate buffer
+  const int et = std::min(std::min<int>(s->sp.entry_trees, c->p.num_trees), kCandCap / 64);
+  hipLaunchKernelGGL(k_search, dim3(grid), dim3(256), 0, stream, s->q, s->x, s->d, s->gid, s->K,
+                     c->hier, c->hier_count, c->leaves, c->perm, s->n_forest, et, dq, dx, nq, beam, k,
+                     rerank, s->sp.max_steps, did, dsc);
+  SCHK(s, hipGetLastError());
+  return 0;
+}
+
+// insert: runs on the scheduler thread with every slot idle
+int do_insert(knng_server* s, Request* r) {
+  const int32_t m = r->nrows;
+  const int K = s->K;
+  const int64_t base = s->n.load();
+  if (base + m > s->cap) return sfail(s, KNNG_ERANGE, "insert exceeds insert_capacity");
+  hipStream_t st = s->ins_stream;
+  SCHK(s, hipMemcpyAsync(s->x + base * s->d, r->rows.data(), size_t(m) * s->d * sizeof(float),
+                         hipMemcpyHostToDevice, st));
+  int32_t* did = nullptr;
+  float* dsc = nullptr;
+  SCHK(s, hipMalloc(&did, size_t(m) * K * sizeof(int32_t)));
+  SCHK(s, hipMalloc(&dsc, size_t(m) * K * sizeof(float)));
+  // neighbours of the new rows among the current ones (int8 scores, no rerank)
+  const int beam = std::max(s->sp.beam, K);
+  int rc = launch_search(s, st, s->x + base * s->d, s->q + base * s->d, s->sk + base * s->words, m,
+                         std::min(beam, kBeamMax), K, 0, did, dsc);
+  std::vector<int32_t> ids(size_t(m) * K);
+  std::vector<float> sc(size_t(m) * K);
+  if (!rc) {
+    SCHK(s, hipMemcpyAsync(ids.data(), did, ids.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SCHK(s, hipMemcpyAsync(sc.data(), dsc, sc.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    SCHK(s, hipStreamSynchronize(st));
+  }
+  (void)hipFree(did);
+  (void)hipFree(dsc);
+  if (rc) return rc;
+  // lists of the new rows; reverse proposals (target, new id, score); a new
+  // row is forced into the lists of its kForced nearest neighbours
+  constexpr int kForced = 16;
+  struct Prop {
+    int32_t target, score, self, forced;
+  };
+  std::vector<int32_t> gi(size_t(m) * K), gs(size_t(m) * K);
+  std::vector<Prop> prop;
+  prop.reserve(size_t(m) * K);
+  for (int32_t j = 0; j < m; ++j) {
+    const int32_t self = int32_t(base + j);
+    for (int e = 0; e < K; ++e) {
+      int32_t id = ids[size_t(j) * K + e];
+      const int32_t scv = id >= 0 ? int32_t(sc[size_t(j) * K + e]) : INT32_MIN;
+      if (id < 0) id = ids[size_t(j) * K] >= 0 ? ids[size_t(j) * K] : 0;
+      gi[size_t(j) * K + e] = id;
+      gs[size_t(j) * K + e] = scv;
+      if (scv != INT32_MIN) prop.push_back(Prop{id, scv, self, e < kForced ? 1 : 0});
+    }
+  }
+  SCHK(s, hipMemcpyAsync(s->gid + base * K, gi.data(), gi.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice, st));
+  SCHK(s, hipMemcpyAsync(s->gsc + base * K, gs.data(), gs.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice, st));
+  std::sort(prop.begin(), prop.end(), [](const Prop& a, const Prop& b) {
+    if (a.target != b.target) return a.target < b.target;
+    if (a.forced != b.forced) return a.forced > b.forced;
+    if (a.score != b.score) return a.score > b.score;
+    return a.self < b.self;
+  });
+  std::vector<int32_t> targets, offs, pid, psc, pfo;
+  for (size_t a = 0; a < prop.size();) {
+    size_t b = a;
+    while (b < prop.size() && prop[b].target == prop[a].target) ++b;
+    targets.push_back(prop[a].target);
+    offs.push_back(int32_t(pid.size()));
+    for (size_t e = a; e < b && e < a + size_t(kMaxK); ++e) {   // first kMaxK per target
+      pid.push_back(prop[e].self);
+      psc.push_back(prop[e].score);
+      pfo.push_back(prop[e].forced);
+    }
+    a = b;
+  }
+  offs.push_back(int32_t(pid.size()));
+  if (!targets.empty()) {
+    int32_t *dt = nullptr, *doff = nullptr, *dpi = nullptr, *dps = nullptr, *dpf = nullptr;
+    SCHK(s, hipMalloc(&dt, targets.size() * 4));
+    SCHK(s, hipMalloc(&doff, offs.size() * 4));
+    SCHK(s, hipMalloc(&dpi, pid.size() * 4));
+    SCHK(s, hipMalloc(&dps, psc.size() * 4));
+    SCHK(s, hipMalloc(&dpf, pfo.size() * 4));
+    SCHK(s, hipMemcpyAsync(dt, targets.data(), targets.size() * 4, hipMemcpyHostToDevice, st));
+    SCHK(s, hipMemcpyAsync(doff, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, st));
+    SCHK(s, hipMemcpyAsync(dpi, pid.data(), pid.size() * 4, hipMemcpyHostToDevice, st));
+    SCHK(s, hipMemcpyAsync(dps, psc.data(), psc.size() * 4, hipMemcpyHostToDevice, st));
+    SCHK(s, hipMemcpyAsync(dpf, pfo.data(), pfo.size() * 4, hipMemcpyHostToDevice, st));
+    const int grid = std::max(1, std::min<int>(int(targets.size()), s->num_cu * 16));
+    hipLaunchKernelGGL(k_merge_reverse, dim3(grid), dim3(64), 0, st, s->gid, s->gsc, K, dt, doff, dpi,
+                       dps, dpf, int(targets.size()));
+    SCHK(s, hipGetLastError());
+    SCHK(s, hipStreamSynchronize(st));
+    for (void* ptr : {static_cast<void*>(dt), static_cast<void*>(doff), static_cast<void*>(dpi),
+                      static_cast<void*>(dps), static_cast<void*>(dpf)})
+      (void)hipFree(ptr);
+  }
+  SCHK(s, hipStreamSynchronize(st));
+  s->n.store(base + m);
+  r->ids.resize(m);
+  for (int32_t j = 0; j < m; ++j) r->ids[j] = int32_t(base + j);
+  return 0;
+}
+
+// A free slot, or a new one while fewer than max_slots exist and HBM beyond
+// reserve_bytes is free.  The first slot ignores the reserve (some progress is
+// always possible); *fatal is set when even that cannot be allocated.
+Slot* acquire_slot(knng_server* s, bool* fatal) {
+  *fatal = false;
+  for (Slot* sl : s->slots)
+    if (!sl->busy) return sl;
+  if (int(s->slots.size()) >= s->sp.max_slots) return nullptr;
+  if (!s->slots.empty()) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return nullptr;
+    if (free_b < slot_device_bytes(s) + size_t(s->sp.reserve_bytes)) return nullptr;   // admission
+  }
+  Slot* sl = new_slot(s);
+  if (sl)
+    s->slots.push_back(sl);
+  else if (s->slots.empty())
+    *fatal = true;
+  return sl;
+}
+
+void dispatch(knng_server* s, Request* r, Slot* sl) {
+  const int32_t m = std::min(s->sp.max_batch, r->nrows - r->next_row);
+  memcpy(sl->hx, r->rows.data() + size_t(r->next_row) * s->d, size_t(m) * s->d * sizeof(float));
+  bool ok = hipMemcpyAsync(sl->dx, sl->hx, size_t(m) * s->d * sizeof(float), hipMemcpyHostToDevice,
+                           sl->stream) == hipSuccess;
+  ok = ok && !launch_search(s, sl->stream, sl->dx, sl->dq, sl->dsk, m, s->sp.beam, s->sp.k,
+                            s->sp.rerank, sl->did, sl->dsc);
+  ok = ok && hipMemcpyAsync(sl->hid, sl->did, size_t(m) * s->sp.k * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
+  ok = ok && hipMemcpyAsync(sl->hsc, sl->dsc, size_t(m) * s->sp.k * sizeof(float),
+                            hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
+  ok = ok && hipEventRecord(sl->done, sl->stream) == hipSuccess;
+  if (!ok) r->error = KNNG_EHIP;
+  sl->busy = true;
+  sl->ticket = r->ticket;
+  sl->row0 = r->next_row;
+  sl->nrows = m;
+  r->next_row += m;
+  r->running += 1;
+  r->state = 1;
+}
+
+void scheduler(knng_server* s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (true) {
+    bool progress = false, busy = false;
+    for (Slot* sl : s->slots) {
+      if (!sl->busy) continue;
+      const hipError_t e = hipEventQuery(sl->done);
+      if (e == hipErrorNotReady) {
+        busy = true;
+        continue;
+      }
+      Request* r = s->reqs[sl->ticket];
+      if (e != hipSuccess) r->error = KNNG_EHIP;
+      const size_t kk = size_t(s->sp.k);
+      memcpy(r->ids.data() + size_t(sl->row0) * kk, sl->hid, size_t(sl->nrows) * kk * 4);
+      memcpy(r->scores.data() + size_t(sl->row0) * kk, sl->hsc, size_t(sl->nrows) * kk * 4);
+      sl->busy = false;
+      r->running -= 1;
+      if (r->running == 0 && r->next_row == r->nrows) {
+        r->state = 2;
+        r->t_done = clk::now();
+        s->cv.notify_all();
+      }
+      progress = true;
+    }
+    while (!s->queue.empty()) {
+      Request* r = s->queue.front();
+      if (r->kind == 1) {
+        bool any = false;
+        for (Slot* sl : s->slots) any = any || sl->busy;
+        if (any) break;   // drain the queries admitted before the insert
+        r->state = 1;
+        lk.unlock();
+        const int rc = do_insert(s, r);
+        lk.lock();
+        r->error = rc;
+        r->state = 2;
+        r->t_done = clk::now();
+        s->queue.pop_front();
+        s->cv.notify_all();
+        progress = true;
+        continue;
+      }
+      bool fatal = false;
+      while (r->next_row < r->nrows) {
+        Slot* sl = acquire_slot(s, &fatal);
+        if (!sl) break;
+        dispatch(s, r, sl);
+        progress = true;
+        busy = true;
+      }
+      if (fatal) {   // not even one slot fits in HBM: fail the request
+        s->err = "cannot allocate a search slot";
+        r->error = KNNG_ENOMEM;
+        r->next_row = r->nrows;
+        if (r->running == 0) {
+          r->state = 2;
+          r->t_done = clk::now();
+          s->cv.notify_all();
+        }
+        s->queue.pop_front();
+        progress = true;
+        continue;
+      }
+      if (r->next_row < r->nrows) break;
+      s->queue.pop_front();
+    }
+    if (s->stop && s->queue.empty() && !busy) break;
+    if (!progress) s->cv.wait_for(lk, std::chrono::microseconds(busy ? 20 : 500));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int knng_server_default_params(knng_server_params* sp) {
+  if (!sp) return KNNG_EINVAL;
+  sp->beam = 64;
+  sp->k = 15;
+  sp->entry_trees = 4;
+  sp->rerank = 1;
+  sp->max_steps = 128;
+  sp->max_batch = 1024;
+  sp->max_slots = 8;
+  sp->pad0 = 0;
+  sp->insert_capacity = 1 << 20;
+  sp->reserve_bytes = int64_t(2) << 30;
+  return 0;
+}
+
+const char* knng_server_last_error(const knng_server* s) { return s ? s->err.c_str() : "null server"; }
+
+int knng_server_create(knng_server** out, knng_ctx* c, const knng_server_params* sp) {
+  if (!out || !c || !sp) return KNNG_EINVAL;
+  *out = nullptr;
+  if (!c->have_graph || !c->have_forest) return KNNG_ESTATE;
+  if (sp->beam < 1 || sp->beam > kBeamMax || sp->k < 1 || sp->k > sp->beam || sp->entry_trees < 1 ||
+      sp->max_batch < 1 || sp->max_slots < 1 || sp->max_steps < 0 || sp->insert_capacity < 0)
+    return KNNG_EINVAL;
+  const float* xsrc = c->x_ext ? c->x_ext : c->x;
+  if (!xsrc) return KNNG_ESTATE;   // the index keeps float32 rows (rerank, inserts)
+  auto* s = new knng_server();
+  s->ctx = c;
+  s->sp = *sp;
+  s->device = c->device;
+  s->num_cu = c->num_cu;
+  s->d = c->d;
+  s->words = c->words;
+  s->K = c->p.k_build;
+  s->n.store(c->n);
+  s->n_forest = c->n;
+  s->cap = c->n + sp->insert_capacity;
+  auto bail = [&](int rc) {
+    knng_server_destroy(s);
+    return rc;
+  };
+  if (hipSetDevice(s->device) != hipSuccess) return bail(KNNG_EHIP);
+  const size_t cap = size_t(s->cap);
+  if (hipMalloc(&s->q, cap * s->d) != hipSuccess || hipMalloc(&s->sk, cap * s->words * 8) != hipSuccess ||
+      hipMalloc(&s->x, cap * s->d * sizeof(float)) != hipSuccess ||
+      hipMalloc(&s->gid, cap * s->K * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&s->gsc, cap * s->K * sizeof(int32_t)) != hipSuccess)
+    return bail(KNNG_ENOMEM);
+  if (hipStreamCreateWithFlags(&s->ins_stream, hipStreamNonBlocking) != hipSuccess) return bail(KNNG_EHIP);
+  const size_t n = size_t(c->n);
+  if (hipStreamSynchronize(c->stream) != hipSuccess ||
+      hipMemcpy(s->q, c->q, n * s->d, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(s->sk, c->sk, n * s->words * 8, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(s->x, xsrc, n * s->d * sizeof(float), hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(s->gid, c->gid[c->cur], n * s->K * 4, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(s->gsc, c->gsc[c->cur], n * s->K * 4, hipMemcpyDeviceToDevice) != hipSuccess)
+    return bail(KNNG_EHIP);
+  s->th = std::thread(scheduler, s);
+  *out = s;
+  return 0;
+}
+
+int knng_server_destroy(knng_server* s) {
+  if (!s) return 0;
+  if (s->th.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+    }
+    s->cv.notify_all();
+    s->th.join();
+  }
+  (void)hipSetDevice(s->device);
+  for (Slot* sl : s->slots) free_slot(sl);
+  for (auto& kv : s->reqs) delete kv.second;
+  for (void* p : {static_cast<void*>(s->q), static_cast<void*>(s->sk), static_cast<void*>(s->x),
+                  static_cast<void*>(s->gid), static_cast<void*>(s->gsc)})
+    if (p) (void)hipFree(p);
+  if (s->ins_stream) (void)hipStreamDestroy(s->ins_stream);
+  delete s;
+  return 0;
+}
+
+int knng_server_submit(knng_server* s, int32_t kind, const float* rows, int32_t nrows,
+                       int64_t* ticket) {
+  if (!s || !rows || nrows < 1 || (kind != 0 && kind != 1) || !ticket) return KNNG_EINVAL;
+  auto* r = new Request();
+  r->kind = kind;
+  r->nrows = nrows;
+  r->rows.assign(rows, rows + size_t(nrows) * s->d);
+  if (kind == 0) {
+    r->ids.assign(size_t(nrows) * s->sp.k, -1);
+    r->scores.assign(size_t(nrows) * s->sp.k, 0.f);
+  }
+  r->t_submit = clk::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->ticket = s->next_ticket++;
+    s->reqs[r->ticket] = r;
+    s->queue.push_back(r);
+  }
+  s->cv.notify_all();
+  *ticket = r->ticket;
+  return 0;
+}
+
+int knng_server_poll(knng_server* s, int64_t ticket, int32_t* state) {
+  if (!s || !state) return KNNG_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return KNNG_EINVAL;
+  *state = it->second->state;
+  return 0;
+}
+
+int knng_server_wait(knng_server* s, int64_t ticket, int32_t* ids, float* scores, double* latency_ms) {
+  if (!s) return KNNG_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return KNNG_EINVAL;
+  Request* r = it->second;
+  s->cv.wait(lk, [&] { return r->state == 2; });
+  s->reqs.erase(it);
+  lk.unlock();
+  const int rc = r->error;
+  if (!rc) {
+    if (ids) memcpy(ids, r->ids.data(), r->ids.size() * sizeof(int32_t));
+    if (scores && r->kind == 0) memcpy(scores, r->scores.data(), r->scores.size() * sizeof(float));
+  } else if (s->err.empty()) {
+    s->err = "request failed";
+  }
+  if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  delete r;
+  return rc;
+}
+
+int64_t knng_server_size(const knng_server* s) {
+  if (!s) return -1;
+  return s->n.load();
+}
+
+int knng_server_search(knng_server* s, const float* queries, int32_t nq, int32_t* ids, float* scores) {
+  int64_t t = 0;
+  int rc = knng_server_submit(s, 0, queries, nq, &t);
+  if (rc) return rc;
+  return knng_server_wait(s, t, ids, scores, nullptr);
+}
+
+}  // extern "C"

@@ -25,6 +25,9 @@ KNNG_SYMBOLS = [
     "knng_export", "knng_export_device", "knng_get_stats", "knng_synchronize",
     "knng_copy_quantized", "knng_copy_forest", "knng_copy_graph", "knng_load_graph",
     "knng_quantize_device", "knng_score_pairs_device",
+    "knng_server_default_params", "knng_server_create", "knng_server_destroy",
+    "knng_server_last_error", "knng_server_submit", "knng_server_poll", "knng_server_wait",
+    "knng_server_size", "knng_server_search",
 ]
 H5_SYMBOLS = [
     "knng_h5_open", "knng_h5_shape_of", "knng_h5_read_rows", "knng_h5_close",
@@ -32,6 +35,9 @@ H5_SYMBOLS = [
     "knng_h5_write_i32", "knng_h5_create_f32", "knng_h5_write_rows", "knng_h5_finish",
     "knng_h5_write_result", "knng_h5_read_attr_str", "knng_h5_read_attr_f32",
 ]
+
+
+STOP_REFERENCE, STOP_INSERTED, STOP_NONE = 0, 1, 2
 
 
 class KnngError(RuntimeError):
@@ -45,6 +51,7 @@ class Params(C.Structure):
         ("num_trees", C.c_int32), ("rerank_mult", C.c_int32), ("leaf_size", C.c_int32),
         ("max_iters", C.c_int32), ("early_stop_rate", C.c_float), ("rate_diff_tol", C.c_float),
         ("seed", C.c_uint64), ("keep_fp32", C.c_int32), ("rerank_width", C.c_int32),
+        ("stop_rule", C.c_int32), ("pad0", C.c_int32),
     ]
 
 
@@ -57,6 +64,7 @@ class Stats(C.Structure):
         ("ms_join", C.c_float), ("ms_join_iter", C.c_float * 32), ("ms_export", C.c_float),
         ("ms_build", C.c_float), ("join_bytes", C.c_double * 32),
         ("join_candidates", C.c_int64 * 32), ("join_rescored", C.c_int64 * 32),
+        ("join_inserted", C.c_int64 * 32), ("insert_rates", C.c_float * 32),
     ]
 
     def as_dict(self) -> dict:
@@ -71,6 +79,8 @@ class Stats(C.Structure):
             "join_bytes": list(self.join_bytes[:it]),
             "join_candidates": list(self.join_candidates[:it]),
             "join_rescored": list(self.join_rescored[:it]),
+            "join_inserted": list(self.join_inserted[:it]),
+            "insert_rates": [float(r) for r in self.insert_rates[:it]],
         }
 
 
@@ -337,3 +347,111 @@ def h5_attr_f32(path: str, name: str) -> float:
     if rc:
         raise KnngError(f"h5 attribute {name} of {path}: {rc}")
     return float(v.value)
+
+
+# -- serving ------------------------------------------------------------------
+
+class ServerParams(C.Structure):
+    _fields_ = [
+        ("beam", C.c_int32), ("k", C.c_int32), ("entry_trees", C.c_int32), ("rerank", C.c_int32),
+        ("max_steps", C.c_int32), ("max_batch", C.c_int32), ("max_slots", C.c_int32),
+        ("pad0", C.c_int32), ("insert_capacity", C.c_int64), ("reserve_bytes", C.c_int64),
+    ]
+
+
+def _server_lib() -> C.CDLL:
+    L = lib()
+    if not getattr(L, "_server_sigs", False):
+        V, i32, i64, P = C.c_void_p, C.c_int32, C.c_int64, C.POINTER
+        sig = {
+            "knng_server_default_params": ([P(ServerParams)], i32),
+            "knng_server_create": ([P(V), V, P(ServerParams)], i32),
+            "knng_server_destroy": ([V], i32),
+            "knng_server_last_error": ([V], C.c_char_p),
+            "knng_server_submit": ([V, i32, V, i32, P(i64)], i32),
+            "knng_server_poll": ([V, i64, P(i32)], i32),
+            "knng_server_wait": ([V, i64, V, V, P(C.c_double)], i32),
+            "knng_server_size": ([V], i64),
+            "knng_server_search": ([V, V, i32, V, V], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes, f.restype = args, res
+        L._server_sigs = True
+    return L
+
+
+def server_params(**over) -> ServerParams:
+    p = ServerParams()
+    _server_lib().knng_server_default_params(C.byref(p))
+    for k, v in over.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class Server:
+    """Device-resident kNN serving over a built Builder (knng_server)."""
+
+    def __init__(self, builder: Builder, params: ServerParams | None = None, **over):
+        self.L = _server_lib()
+        self.builder = builder          # keeps the forest alive
+        self.p = params if params is not None else server_params(**over)
+        self.d = builder.p.dim
+        self.srv = C.c_void_p()
+        rc = self.L.knng_server_create(C.byref(self.srv), builder.ctx, C.byref(self.p))
+        if rc:
+            raise KnngError(f"knng_server_create failed ({rc})")
+
+    def _ck(self, rc, what):
+        if rc:
+            msg = self.L.knng_server_last_error(self.srv).decode(errors="replace")
+            raise KnngError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if self.srv:
+            self.L.knng_server_destroy(self.srv)
+            self.srv = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, rows: np.ndarray, insert: bool = False) -> int:
+        rows = np.ascontiguousarray(rows, np.float32)
+        assert rows.ndim == 2 and rows.shape[1] == self.d
+        t = C.c_int64()
+        self._ck(self.L.knng_server_submit(self.srv, int(insert), _ptr(rows), rows.shape[0],
+                                           C.byref(t)), "submit")
+        return t.value
+
+    def poll(self, ticket: int) -> int:
+        st = C.c_int32()
+        self._ck(self.L.knng_server_poll(self.srv, ticket, C.byref(st)), "poll")
+        return st.value
+
+    def wait(self, ticket: int, nrows: int, insert: bool = False):
+        """-> (ids, scores, latency_ms); inserts return (new ids, None, latency)."""
+        lat = C.c_double()
+        if insert:
+            ids = np.empty(nrows, np.int32)
+            self._ck(self.L.knng_server_wait(self.srv, ticket, _ptr(ids), None, C.byref(lat)), "wait")
+            return ids, None, lat.value
+        ids = np.empty((nrows, self.p.k), np.int32)
+        sc = np.empty((nrows, self.p.k), np.float32)
+        self._ck(self.L.knng_server_wait(self.srv, ticket, _ptr(ids), _ptr(sc), C.byref(lat)), "wait")
+        return ids, sc, lat.value
+
+    def search(self, queries: np.ndarray):
+        queries = np.ascontiguousarray(queries, np.float32)
+        nq = queries.shape[0]
+        ids = np.empty((nq, self.p.k), np.int32)
+        sc = np.empty((nq, self.p.k), np.float32)
+        self._ck(self.L.knng_server_search(self.srv, _ptr(queries), nq, _ptr(ids), _ptr(sc)), "search")
+        return ids, sc
+
+    def size(self) -> int:
+        return int(self.L.knng_server_size(self.srv))

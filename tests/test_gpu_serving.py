@@ -1,0 +1,113 @@
+"""GPU tests of the serving path (knng_server_*): the int8 beam search is
+bit-exact against the oracle's restatement, FP32-reranked answers reach high
+recall against brute force, every result is the same alone and in a
+concurrent mix, and inserts make new rows findable."""
+import sys
+
+import numpy as np
+import pytest
+
+import knng
+import oracle
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _built(x, **kw):
+    n, d = x.shape
+    b = knng.Builder(n, knng.default_params(dim=d, **kw))
+    b.ingest(x)
+    b.build()
+    return b
+
+
+def test_search_int8_bitexact_vs_oracle(torch_cuda):
+    say("search bit-exact")
+    x = synth.gaussian_mixture(41, 4000, 256, 16)
+    b = _built(x, num_trees=4, leaf_size=32, k_build=20, k_eval=15)
+    ids, _ = b.graph()
+    q, _ = oracle.quantize(x)
+    queries = synth.gaussian_mixture(42, 300, 256, 16)
+    qq, _ = oracle.quantize(queries)
+    with knng.Server(b, beam=32, k=10, rerank=0, entry_trees=4, max_batch=64, max_slots=3) as s:
+        g_ids, g_sc = s.search(queries)
+    o_ids, o_sc = oracle.search_device(q, ids, qq, T=4, leaf=32, entry_trees=4, beam=32, k=10,
+                                       max_steps=128)
+    assert np.array_equal(g_ids, o_ids)
+    assert np.array_equal(g_sc, o_sc.astype(np.float32))
+
+
+def test_search_recall_rerank(torch_cuda):
+    say("search recall")
+    n = 20000
+    x = synth.gaussian_mixture(43, n + 500, 1024, 64)
+    base, queries = x[:n], x[n:]
+    b = _built(base)
+    with knng.Server(b, beam=64, k=15, rerank=1) as s:
+        ids, sc = s.search(queries)
+    full = np.vstack([base, queries])
+    gt = synth.exact_knn(full, np.arange(n, n + 500), 15)
+    rec = np.mean([len(np.intersect1d(ids[j], gt[j])) / 15 for j in range(500)])
+    say(f"query recall@15 {rec:.4f}")
+    assert rec > 0.9
+    # FP32 scores are the exact dot products, best first
+    exact = np.einsum("qd,qkd->qk", queries.astype(np.float64), base[ids].astype(np.float64))
+    assert np.abs(sc - exact).max() < 1e-5
+    assert np.all(np.diff(sc, axis=1) <= 0)
+
+
+def test_alone_equals_in_the_mix(torch_cuda):
+    """Continuous batching: results of requests that ran concurrently with
+    others (split into pieces on several streams) equal their results alone."""
+    say("alone vs mix")
+    x = synth.gaussian_mixture(44, 12000, 512, 32)
+    b = _built(x)
+    rng = np.random.default_rng(0)
+    reqs = [synth.gaussian_mixture(100 + j, int(rng.integers(1, 300)), 512, 32) for j in range(40)]
+    with knng.Server(b, beam=48, k=15, rerank=1, max_batch=64, max_slots=4) as s:
+        alone = [s.search(r) for r in reqs]
+        tickets = [s.submit(r) for r in reqs]         # all in flight together
+        states = [s.poll(t) for t in tickets]
+        mixed = [s.wait(t, len(r)) for t, r in zip(tickets, reqs)]
+    assert all(st in (0, 1, 2) for st in states)
+    for (ai, asc), (mi, msc, lat) in zip(alone, mixed):
+        assert np.array_equal(ai, mi)
+        assert np.array_equal(asc, msc)
+        assert lat >= 0
+
+
+def test_insert_then_find(torch_cuda):
+    say("insert")
+    x = synth.gaussian_mixture(45, 10000, 1024, 32)
+    new = synth.gaussian_mixture(46, 300, 1024, 32)
+    b = _built(x)
+    with knng.Server(b, beam=64, k=15, rerank=1, insert_capacity=400) as s:
+        t = s.submit(new, insert=True)
+        ids, _, _ = s.wait(t, 300, insert=True)
+        assert ids.tolist() == list(range(10000, 10300))
+        assert s.size() == 10300
+        got, sc = s.search(new)
+        assert (got[:, 0] == np.arange(10000, 10300)).mean() > 0.98   # each finds itself first
+        with pytest.raises(knng.KnngError):
+            s.wait(s.submit(new, insert=True), 300, insert=True)   # beyond insert_capacity
+        assert s.size() == 10300
+
+
+def test_admission_with_tight_memory_reserve(torch_cuda):
+    """A reserve larger than free HBM still admits the first slot, so
+    requests complete (one at a time)."""
+    say("admission")
+    x = synth.gaussian_mixture(47, 5000, 256, 8)
+    b = _built(x)
+    queries = synth.gaussian_mixture(48, 700, 256, 8)
+    with knng.Server(b, beam=32, k=10, max_batch=100, max_slots=8, reserve_bytes=1 << 50) as s:
+        ts = [s.submit(queries[j:j + 100]) for j in range(0, 700, 100)]
+        outs = [s.wait(t, 100)[0] for t in ts]
+        ref, _ = s.search(queries)
+    assert np.array_equal(np.vstack(outs), ref)
