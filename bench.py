@@ -79,7 +79,9 @@ def cpu_baseline(args, nproc):
     import synth
 
     ns = args.cpu_sample
-    x = synth.gaussian_mixture(args.seed, ns, args.dim, args.clusters)
+    # same points per cluster as the workload, so the sample's kNN structure matches
+    cl = max(1, round(args.clusters * ns / args.n))
+    x = synth.gaussian_mixture(args.seed, ns, args.dim, cl)
     threads = min(8, nproc)
     if os.path.exists(oracle.REF_BIN) and args.dim == 1024:
         with tempfile.TemporaryDirectory() as td:
@@ -91,12 +93,17 @@ def cpu_baseline(args, nproc):
                 wall = time.time() - t0
                 if r["returncode"] == 0:
                     bt = knng.h5_attr_f32(r["out"], "buildtime")
+                    kn = knng.h5_read(r["out"], "knns", np.int32)
+                    rows = np.sort(np.random.default_rng(7).choice(ns, min(2000, ns), replace=False))
+                    rec = synth.recall_at_k(kn, synth.exact_knn(x, rows, 15), rows, 15)
+                    iters = r["stdout"].count("-> Iteration")
                     return {"value": round(bt * args.n / ns, 2), "unit": "s (extrapolated)",
                             "cores": threads, "kind": "reference",
-                            "sample": f"{ns} rows of the same generator (seed {args.seed}, "
-                                      f"{args.clusters} clusters); reference buildtime {bt:.2f} s "
-                                      f"(wall {wall:.2f} s incl. HDF5 read) x {args.n}/{ns}",
-                            "measured_s": round(bt, 3)}
+                            "sample": f"{ns} rows of the same generator (seed {args.seed}, {cl} "
+                                      f"clusters: the workload's rows per cluster); reference "
+                                      f"buildtime {bt:.2f} s (wall {wall:.2f} s incl. HDF5 read, "
+                                      f"{iters} iterations) x {args.n}/{ns}",
+                            "measured_s": round(bt, 3), "sample_recall_at_15": round(rec, 4)}
                 log("reference binary failed:", r["stdout"][-400:])
             except subprocess.TimeoutExpired:
                 log("reference binary timed out")
@@ -233,6 +240,8 @@ def main():
                       "join_iter": [round(v, 1) for v in st["ms_join_iter"]],
                       "export_rerank": round(st["ms_export"], 1)},
         "iterations": it, "update_rates": [round(r, 5) for r in st["rates"]],
+        "insert_rates": [round(r, 5) for r in st["insert_rates"]],
+        "stop_rule": "new-entry rate < 0.01 (reference rule's rate reported as update_rates)",
         "roofline": roofline, "cpu_baseline": cpu,
     }
     b.close()

@@ -46,11 +46,11 @@ typedef struct knng_params {
   int32_t keep_fp32;       /* 1: keep the float32 rows in HBM for the FP32 rerank */
   int32_t rerank_width;    /* list entries re-scored in FP32 (k_eval..k_build); 0 = k_build */
   int32_t stop_rule;       /* KNNG_STOP_*: which rate the stop rule of src/main.cpp:432-439 watches */
-  int32_t pad0;
+  float insert_stop_rate;  /* KNNG_STOP_INSERTED: stop once fewer than this share of slots got a new id */
 } knng_params;
 
 #define KNNG_STOP_REFERENCE 0  /* slots whose id changed, shifts included (src/main.cpp:421-427) */
-#define KNNG_STOP_INSERTED 1   /* ids that newly entered a list */
+#define KNNG_STOP_INSERTED 1   /* ids that newly entered a list (default) */
 #define KNNG_STOP_NONE 2       /* always run max_iters */
 
 /* Per-build statistics.  Times are HIP-event milliseconds on the context's

@@ -593,7 +593,8 @@ int32_t orc_sketch_threshold(const int32_t* h, int64_t cnt, int32_t M, int32_t b
 // slots whose id changed (src/main.cpp:421-427 counting rule), or -1.
 int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t d, int32_t K,
                         int32_t M, const int32_t* ids_in, const int32_t* sc_in,
-                        int32_t* ids_out, int32_t* sc_out) {
+                        int32_t* ids_out, int32_t* sc_out, int64_t* inserted) {
+  int64_t ins = 0;
   const int words = d / 64;
   int64_t updates = 0;
   std::unordered_set<int32_t> seen;
@@ -628,10 +629,12 @@ int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t 
     });
     for (int k = 0; k < K; ++k) {
       if (all[k].e.id != own[k]) ++updates;
+      if (all[k].slot == K) ++ins;   // a candidate (new id) entered the list
       ids_out[i * K + k] = all[k].e.id;
       sc_out[i * K + k] = all[k].e.score;
     }
   }
+  if (inserted) *inserted = ins;
   return updates;
 }
 

@@ -981,8 +981,8 @@ int knng_default_params(knng_params* p) {
   p->seed = 1337;
   p->keep_fp32 = 1;
   p->rerank_width = 0;
-  p->stop_rule = KNNG_STOP_REFERENCE;
-  p->pad0 = 0;
+  p->stop_rule = KNNG_STOP_INSERTED;
+  p->insert_stop_rate = 0.01f;
   return 0;
 }
 
@@ -1363,13 +1363,18 @@ int knng_build(knng_ctx* c) {
   for (int it = 0; it < c->p.max_iters; ++it) {
     int64_t upd = 0;
     if ((rc = knng_join_iteration(c, &upd))) return rc;
-    // stop rule of src/main.cpp:432-439 on the chosen rate: the reference's
-    // (slots whose id changed, shifts included) or the new-entry rate
+    // KNNG_STOP_REFERENCE: src/main.cpp:432-439 on the rate of slots whose id
+    // changed (a shift counts).  That rate stays near 1 while lists improve
+    // anywhere near their top, so from a weak init it stops after two
+    // iterations; KNNG_STOP_INSERTED instead stops once fewer than
+    // insert_stop_rate of all slots received a new id.
     const int j = c->st.iterations - 1 < 31 ? c->st.iterations - 1 : 31;
-    const float rate = c->p.stop_rule == KNNG_STOP_INSERTED
-                           ? c->st.insert_rates[j]
-                           : float(upd) / float(size_t(c->n) * size_t(c->p.k_build));
     if (c->p.stop_rule == KNNG_STOP_NONE) continue;
+    if (c->p.stop_rule == KNNG_STOP_INSERTED) {
+      if (c->st.insert_rates[j] < c->p.insert_stop_rate) break;
+      continue;
+    }
+    const float rate = float(upd) / float(size_t(c->n) * size_t(c->p.k_build));
     if (rate < c->p.early_stop_rate || (it > 0 && (prev - rate) < c->p.rate_diff_tol)) break;
     prev = rate;
   }

@@ -51,7 +51,7 @@ class Params(C.Structure):
         ("num_trees", C.c_int32), ("rerank_mult", C.c_int32), ("leaf_size", C.c_int32),
         ("max_iters", C.c_int32), ("early_stop_rate", C.c_float), ("rate_diff_tol", C.c_float),
         ("seed", C.c_uint64), ("keep_fp32", C.c_int32), ("rerank_width", C.c_int32),
-        ("stop_rule", C.c_int32), ("pad0", C.c_int32),
+        ("stop_rule", C.c_int32), ("insert_stop_rate", C.c_float),
     ]
 
 

@@ -175,13 +175,14 @@ def test_join_other_shapes_bitexact(torch_cuda, K, mult):
 
 # ---------------------------------------------------- whole build + export
 
-@pytest.mark.parametrize("n", [3000, 40, 2])
-def test_full_build_bitexact(torch_cuda, n):
-    say("full build", n)
+@pytest.mark.parametrize("n,rule", [(3000, knng.STOP_INSERTED), (3000, knng.STOP_REFERENCE),
+                                    (40, knng.STOP_INSERTED), (2, knng.STOP_INSERTED)])
+def test_full_build_bitexact(torch_cuda, n, rule):
+    say("full build", n, rule)
     x = synth.gaussian_mixture(9 + n, n, 1024, 8) if n > 100 else make_golden.case_input(15, n, 2, False)
     q, sk = oracle.quantize(x)
-    o_ids, o_sc, o_rates = oracle.build_device_semantics(q, sk)
-    b = _builder(x)
+    o_ids, o_sc, o_rates = oracle.build_device_semantics(q, sk, stop_rule=rule)
+    b = _builder(x, stop_rule=rule)
     b.build()
     ids, sc = b.graph()
     st = b.stats()
