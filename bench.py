@@ -133,6 +133,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=50000)
     ap.add_argument("--cpu-timeout", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["build", "serve", "batch"], default="build",
+                    help="build: the graph build (default); serve: the request loop on a built "
+                         "graph; batch: many uneven independent builds")
+    ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--rate", type=float, default=2000.0, help="Poisson request rate (req/s)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -140,6 +147,12 @@ def main():
         print("bench.py: this kNN-graph build runs on one GPU only; --gpus must be 1 "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serve":
+        serve_main(args)
+        return
+    if args.workload == "batch":
+        batch_main(args)
+        return
 
     import numpy as np
     import torch
@@ -243,6 +256,191 @@ def main():
         "insert_rates": [round(r, 5) for r in st["insert_rates"]],
         "stop_rule": "new-entry rate < 0.01 (reference rule's rate reported as update_rates)",
         "roofline": roofline, "cpu_baseline": cpu,
+    }
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
+def batch_main(args):
+    """Batched uneven builds: `jobs` independent graphs, n log-uniform in
+    [2000, 200000], d from {384, 768, 1024}, 16-512 clusters, k from
+    {10, 15, 32}; rows generated in HBM before the timed region."""
+    import numpy as np
+    import torch
+
+    import knng
+    import synth
+
+    torch.backends.cuda.matmul.allow_tf32 = False
+    rng = np.random.default_rng(4)
+    jobs, data = [], []
+    for j in range(args.jobs):
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(200000))))
+        d = int(rng.choice([384, 768, 1024]))
+        cl = int(np.exp(rng.uniform(np.log(16), np.log(512))))
+        k = int(rng.choice([10, 15, 32]))
+        x = synth.gaussian_mixture_torch(4 * (1 << 20) + j, n, d, cl, device="cuda")
+        data.append(x)
+        p = knng.default_params(dim=d, k_eval=k, k_build=max(50, k), num_trees=8, keep_fp32=0)
+        jobs.append({"rows": x.data_ptr(), "n": n, "params": p, "rerank": 1, "k": k, "d": d})
+    torch.cuda.synchronize()
+    total_rows = sum(j["n"] for j in jobs)
+    log(f"batch: {len(jobs)} jobs, {total_rows} rows")
+    t0 = time.time()
+    rc = knng.build_batch(jobs, 0, args.workers)
+    wall = time.time() - t0
+    assert rc == 0, [j["status"] for j in jobs if j["status"]][:5]
+    recs = []
+    for j, job in zip(range(len(jobs)), jobs):
+        x = data[j]
+        rows = torch.from_numpy(np.sort(np.random.default_rng(j).choice(job["n"], min(200, job["n"]),
+                                                                       replace=False))).cuda()
+        gt = exact_topk(torch, x, rows, job["k"]).cpu().numpy()
+        recs.append(recall(job["knns"][rows.cpu().numpy()], gt, job["k"]))
+    recs = np.array(recs)
+    out = {
+        "metric": "batched uneven builds per second", "value": round(len(jobs) / wall, 3),
+        "unit": "builds/s", "higher_is_better": True, "n_gpus": 1, "wall_s": round(wall, 3),
+        "config": {"workload": f"{len(jobs)} independent builds, n log-uniform [2000,200000], d in "
+                               f"{{384,768,1024}}, 16-512 clusters, k in {{10,15,32}}, 8 trees, FP32 "
+                               f"rerank export, {args.workers} workers", "total_rows": total_rows},
+        "recall_at_k": {"mean": round(float(recs.mean()), 4), "min": round(float(recs.min()), 4),
+                        "p10": round(float(np.percentile(recs, 10)), 4)},
+        "iterations_mean": round(float(np.mean([j["iterations"] for j in jobs])), 2),
+        "rows_per_s": round(total_rows / wall, 1), "data": "synthetic",
+    }
+    print(json.dumps(out), flush=True)
+
+
+def exact_topk_prefix(torch, x, qv, limits, k, chunk=262144):
+    """Exact FP32 top-k of query vectors qv over rows x[:limits[j]] (per query)."""
+    R = qv.shape[0]
+    best_s = torch.full((R, k), float("-inf"), device=x.device)
+    best_i = torch.zeros((R, k), dtype=torch.int64, device=x.device)
+    lim = limits.view(-1, 1)
+    top = int(limits.max().item())
+    for a in range(0, top, chunk):
+        b = min(top, a + chunk)
+        s = qv @ x[a:b].T
+        cols = torch.arange(a, b, device=x.device).view(1, -1)
+        s.masked_fill_(cols >= lim, float("-inf"))
+        ts, ti = s.topk(min(k, b - a), dim=1)
+        cs = torch.cat([best_s, ts], 1)
+        ci = torch.cat([best_i, ti + a], 1)
+        best_s, o = cs.topk(k, dim=1)
+        best_i = ci.gather(1, o)
+    return best_i
+
+
+def serve_main(args):
+    """Serving loop over a device-resident graph: a Poisson request stream
+    (query batches log-uniform in [1, 4096], 10% inserts of 1-1024 rows) is
+    replayed against knng_server; latencies are measured by the server from
+    submission to completion."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    import knng
+    import synth
+
+    torch.backends.cuda.matmul.allow_tf32 = False
+    n, d = args.n, args.dim
+    rng = np.random.default_rng(7)
+    reqs, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / args.rate)
+        if t > args.serve_seconds:
+            break
+        if rng.random() < 0.1:
+            reqs.append([t, 1, int(rng.integers(1, 1025))])
+        else:
+            reqs.append([t, 0, int(np.exp(rng.uniform(0.0, np.log(4096.0))))])
+    n_ins = sum(r[2] for r in reqs if r[1] == 1)
+    n_q = sum(r[2] for r in reqs if r[1] == 0)
+    log(f"serve: {len(reqs)} requests over {args.serve_seconds}s, {n_q} queries, {n_ins} inserted rows")
+    # rows: [base n][inserted rows, submission order][query rows]; one mixture
+    x = synth.gaussian_mixture_torch(5, n + n_ins + n_q, d, args.clusters, device="cuda")
+    torch.cuda.synchronize()
+    b = knng.Builder(n, knng.default_params(dim=d, num_trees=args.trees, k_build=args.k_build,
+                                            keep_fp32=0))
+    b.attach_fp32(x.data_ptr())
+    b.ingest_device(x.data_ptr(), 0, n)
+    t0 = time.time()
+    b.build()
+    b.synchronize()
+    log(f"graph built in {time.time() - t0:.2f}s")
+    xh = x[n:].cpu().numpy()
+    ins_rows, q_rows = xh[:n_ins], xh[n_ins:]
+    srv = knng.Server(b, beam=64, k=15, rerank=1, entry_trees=4, max_batch=1024, max_slots=8,
+                      insert_capacity=n_ins + 16)
+    srv.search(q_rows[:2048])   # warm-up (code objects, slots)
+    qoff = ioff = 0
+    plan = []
+    for tt, kind, sz in reqs:
+        if kind == 1:
+            plan.append((tt, 1, ioff, sz))
+            ioff += sz
+        else:
+            plan.append((tt, 0, qoff, sz))
+            qoff += sz
+    tickets = [None] * len(plan)
+    start = time.time()
+
+    def submitter():
+        for j, (tt, kind, off, sz) in enumerate(plan):
+            delay = start + tt - time.time()
+            if delay > 0:
+                time.sleep(delay)
+            rows = ins_rows[off:off + sz] if kind else q_rows[off:off + sz]
+            tickets[j] = srv.submit(rows, insert=bool(kind))
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    results = []
+    for j, (tt, kind, off, sz) in enumerate(plan):
+        while tickets[j] is None:
+            time.sleep(0.0002)
+        ids, sc, lat = srv.wait(tickets[j], sz, insert=bool(kind))
+        results.append((kind, off, sz, ids, lat))
+    th.join()
+    wall = time.time() - start
+    srv.close()
+    qlat = np.array([r[4] for r in results if r[0] == 0])
+    small = np.array([r[4] for r in results if r[0] == 0 and r[2] <= 64])
+    ilat = np.array([r[4] for r in results if r[0] == 1])
+    # recall: sampled query rows against brute force over base + the inserts
+    # submitted before the query (the server applies inserts in order)
+    samp, seen_ins = [], 0
+    for kind, off, sz, ids, lat in results:
+        if kind == 1:
+            seen_ins += sz
+            continue
+        for r in range(0, sz, max(1, sz // 4)):
+            samp.append((off + r, n + seen_ins, ids[r]))
+    pick = np.random.default_rng(1).choice(len(samp), min(2000, len(samp)), replace=False)
+    qv = torch.from_numpy(np.stack([q_rows[samp[p][0]] for p in pick])).cuda()
+    lim = torch.tensor([samp[p][1] for p in pick], device="cuda")
+    gt = exact_topk_prefix(torch, x[: n + n_ins], qv, lim, 15).cpu().numpy()
+    rec = np.mean([len(np.intersect1d(samp[p][2], gt[j])) / 15 for j, p in enumerate(pick)])
+    out = {
+        "metric": "query p50/p99 ms (serving a 6.35M x 1024 graph)", "value": round(float(np.percentile(small, 99)), 3) if len(small) else None,
+        "unit": "ms (p99, batches <= 64)", "higher_is_better": False, "n_gpus": 1,
+        "config": {"workload": f"Poisson {args.rate:.0f} req/s for {args.serve_seconds}s over a {n}x{d} "
+                               f"graph; query batches log-uniform [1,4096], 10% inserts of 1-1024 rows; "
+                               f"beam 64, k 15, FP32 rerank", "n": n, "dim": d,
+                   "clusters": args.clusters},
+        "requests": len(plan), "queries": n_q, "inserted_rows": n_ins, "wall_s": round(wall, 3),
+        "queries_per_s": round(n_q / wall, 1),
+        "latency_ms": {"p50": round(float(np.percentile(qlat, 50)), 3),
+                       "p99": round(float(np.percentile(qlat, 99)), 3),
+                       "p50_batch_le_64": round(float(np.percentile(small, 50)), 3) if len(small) else None,
+                       "p99_batch_le_64": round(float(np.percentile(small, 99)), 3) if len(small) else None,
+                       "insert_p50": round(float(np.percentile(ilat, 50)), 3) if len(ilat) else None,
+                       "insert_p99": round(float(np.percentile(ilat, 99)), 3) if len(ilat) else None},
+        "query_recall_at_15": round(float(rec), 4), "recall_rows": int(len(pick)),
+        "data": "synthetic",
     }
     b.close()
     print(json.dumps(out), flush=True)

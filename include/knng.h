@@ -47,6 +47,9 @@ typedef struct knng_params {
   int32_t rerank_width;    /* list entries re-scored in FP32 (k_eval..k_build); 0 = k_build */
   int32_t stop_rule;       /* KNNG_STOP_*: which rate the stop rule of src/main.cpp:432-439 watches */
   float insert_stop_rate;  /* KNNG_STOP_INSERTED: stop once fewer than this share of slots got a new id */
+  int32_t incremental;     /* 1: a 2-hop pair is a candidate only if one of its edges is new
+                              since the previous iteration (NN-Descent's new/old rule) */
+  int32_t pad0;
 } knng_params;
 
 #define KNNG_STOP_REFERENCE 0  /* slots whose id changed, shifts included (src/main.cpp:421-427) */
@@ -144,6 +147,28 @@ int knng_quantize_device(const float* dev_x, int64_t n, int32_t d, int8_t* dev_q
 int knng_score_pairs_device(const int8_t* dev_q, const uint64_t* dev_sketch, int32_t d,
                             const int32_t* dev_a, const int32_t* dev_b, int64_t npairs,
                             int32_t* dev_dot, int32_t* dev_sim);
+
+/* ------------------------------------------------------------------------
+ * Batched builds of many independent graphs of uneven size (no reference
+ * counterpart: the reference builds one graph of DIM = 1024).  Each job is a
+ * whole build (ingest, forest, init, join, export); `workers` host threads
+ * each drive their own context and HIP stream and take jobs largest-first
+ * from a shared counter, so small builds overlap on the device.
+ * ------------------------------------------------------------------------ */
+typedef struct knng_job {
+  const float* rows;       /* n x params.dim float32 rows */
+  int32_t rows_on_device;  /* 1: rows is a device pointer on `device` */
+  int32_t rerank;          /* export with the FP32 rerank */
+  int64_t n;
+  knng_params params;      /* per-job dim, k_build, k_eval, trees, ... */
+  int32_t* knns;           /* host out, n x k_eval (1-based, ascending, best last) */
+  float* dists;            /* host out, n x k_eval */
+  int32_t status;          /* out: 0 ok, else a KNNG_E* code */
+  int32_t iterations;      /* out: join iterations run */
+  double build_ms;         /* out: wall time of this job's build + export */
+} knng_job;
+
+int knng_build_batch(knng_job* jobs, int32_t njobs, int32_t device, int32_t workers);
 
 /* ------------------------------------------------------------------------
  * Serving: kNN queries and inserts against a built, device-resident graph.

@@ -591,9 +591,15 @@ int32_t orc_sketch_threshold(const int32_t* h, int64_t cnt, int32_t M, int32_t b
 
 // One Jacobi refinement iteration (device semantics).  Returns the number of
 // slots whose id changed (src/main.cpp:421-427 counting rule), or -1.
+//
+// fresh_in (optional, one u64 per row, bit k = entry k entered the list in
+// the previous iteration): incremental join -- the pair (i -> n1 -> c) is a
+// candidate only if n1 is fresh in i's list or c is fresh in n1's list.
+// fresh_out (optional) receives the same masks for the new lists.
 int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t d, int32_t K,
                         int32_t M, const int32_t* ids_in, const int32_t* sc_in,
-                        int32_t* ids_out, int32_t* sc_out, int64_t* inserted) {
+                        int32_t* ids_out, int32_t* sc_out, int64_t* inserted,
+                        const uint64_t* fresh_in, uint64_t* fresh_out) {
   int64_t ins = 0;
   const int words = d / 64;
   int64_t updates = 0;
@@ -610,8 +616,11 @@ int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t 
     for (int k = 0; k < K; ++k) seen.insert(own[k]);
     for (int k = 0; k < K; ++k) {
       const int32_t* nb = ids_in + int64_t(own[k]) * K;
-      for (int k2 = 0; k2 < K; ++k2)
+      const bool own_fresh = !fresh_in || ((fresh_in[i] >> k) & 1ull);
+      for (int k2 = 0; k2 < K; ++k2) {
+        if (!own_fresh && !((fresh_in[own[k]] >> k2) & 1ull)) continue;
         if (seen.insert(nb[k2]).second) cand.push_back(nb[k2]);
+      }
     }
     const uint64_t* si = sk + i * words;
     for (int32_t c : cand) hval.push_back(sketch_sim(si, sk + int64_t(c) * words, words));
@@ -627,12 +636,17 @@ int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t 
       if (x.e.id != y.e.id) return x.e.id < y.e.id;
       return x.slot < y.slot;
     });
+    uint64_t fm = 0;
     for (int k = 0; k < K; ++k) {
       if (all[k].e.id != own[k]) ++updates;
-      if (all[k].slot == K) ++ins;   // a candidate (new id) entered the list
+      if (all[k].slot == K) {   // a candidate (new id) entered the list
+        ++ins;
+        fm |= 1ull << k;
+      }
       ids_out[i * K + k] = all[k].e.id;
       sc_out[i * K + k] = all[k].e.score;
     }
+    if (fresh_out) fresh_out[i] = fm;
   }
   if (inserted) *inserted = ins;
   return updates;

@@ -53,7 +53,7 @@ def lib() -> C.CDLL:
             "orc_forest_device": ([V, i64, i32, i32, i32, u64, V, V, i64], i64),
             "orc_init_device": ([V, i64, i32, i32, i32, V, V, i64, V, V], i32),
             "orc_sketch_threshold": ([V, i64, i32, i32], i32),
-            "orc_join_device": ([V, V, i64, i32, i32, i32, V, V, V, V, C.POINTER(i64)], i64),
+            "orc_join_device": ([V, V, i64, i32, i32, i32, V, V, V, V, C.POINTER(i64), V, V], i64),
             "orc_rerank_device": ([V, i64, i32, i32, i32, i32, V, V, V], i32),
             "orc_search_device": ([V, i64, i32, V, i32, i32, i32, u64, i32, i32, i32, i32, V, i64,
                                    V, V], i32),
@@ -152,19 +152,26 @@ def sketch_threshold(h: np.ndarray, M: int, bits: int) -> int:
     return int(lib().orc_sketch_threshold(_p(h), h.shape[0], M, bits))
 
 
-def join_device(q, sk, ids, sc, M: int, with_inserted: bool = False):
-    """One Jacobi join iteration -> (ids, scores, updates[, inserted])."""
+def join_device(q, sk, ids, sc, M: int, with_inserted: bool = False, fresh=None):
+    """One Jacobi join iteration -> (ids, scores, updates[, inserted][, fresh_out]).
+    fresh (u64 per row) selects the incremental join and adds fresh_out."""
     n, d = q.shape
     K = ids.shape[1]
     oi = np.empty_like(ids)
     os_ = np.empty_like(sc)
     ins = C.c_int64()
+    fo = np.empty(n, np.uint64)
+    fi = None if fresh is None else _p(np.ascontiguousarray(fresh, np.uint64))
     upd = lib().orc_join_device(_p(q), _p(sk), n, d, K, M, _p(np.ascontiguousarray(ids)),
-                                _p(np.ascontiguousarray(sc)), _p(oi), _p(os_), C.byref(ins))
+                                _p(np.ascontiguousarray(sc)), _p(oi), _p(os_), C.byref(ins),
+                                fi, _p(fo))
     assert upd >= 0
+    out = [oi, os_, int(upd)]
     if with_inserted:
-        return oi, os_, int(upd), int(ins.value)
-    return oi, os_, int(upd)
+        out.append(int(ins.value))
+    if fresh is not None:
+        out.append(fo)
+    return tuple(out)
 
 
 def rerank_device(x, ids, R: int, k_eval: int = 15):
@@ -183,7 +190,7 @@ STOP_REFERENCE, STOP_INSERTED, STOP_NONE = 0, 1, 2
 
 def build_device_semantics(q, sk, trees=16, leaf=64, K=50, mult=8, max_iters=30,
                            stop=0.04, tol=0.01, seed=1337, stop_rule=STOP_INSERTED,
-                           insert_stop=0.01):
+                           insert_stop=0.01, incremental=False):
     """Full device-mode build.  Stop rules: STOP_REFERENCE is src/main.cpp:432-439
     on the rate of slots whose id changed; STOP_INSERTED stops once fewer than
     insert_stop of all slots received a new id; STOP_NONE runs max_iters."""
@@ -192,8 +199,13 @@ def build_device_semantics(q, sk, trees=16, leaf=64, K=50, mult=8, max_iters=30,
     n = q.shape[0]
     prev = np.float32(1.0)
     rates = []
+    fresh = np.full(n, np.uint64(0xFFFFFFFFFFFFFFFF)) if incremental else None
     for it in range(max_iters):
-        ids, sc, upd, ins = join_device(q, sk, ids, sc, K * mult, with_inserted=True)
+        if incremental:
+            ids, sc, upd, ins, fresh = join_device(q, sk, ids, sc, K * mult, with_inserted=True,
+                                                   fresh=fresh)
+        else:
+            ids, sc, upd, ins = join_device(q, sk, ids, sc, K * mult, with_inserted=True)
         rate = np.float32(upd) / np.float32(n * K)
         rates.append(float(rate))
         if stop_rule == STOP_INSERTED:

@@ -53,7 +53,8 @@ def build_knng() -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libknng.so")
     _run([hipcc(), *HIP_FLAGS, "-shared", f"-I{INC}", os.path.join(CSRC, "knng.hip"),
-          os.path.join(CSRC, "knng_serve.hip"), "-o", out + ".tmp"])
+          os.path.join(CSRC, "knng_serve.hip"), os.path.join(CSRC, "knng_batch.cpp"),
+          "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -542,6 +542,8 @@ struct JoinScratch {
   int32_t own_sc[kMaxK];
   int32_t new_id[kMaxK];
   int32_t new_sc[kMaxK];
+  int32_t new_fresh[kMaxK];   // 1: the entry came from this iteration's candidates
+  uint64_t own_new;           // bit k: own entry k entered the list last iteration
   uint64_t own_sk[kMaxDim / 64];
   int32_t hist[kMaxDim + 1];
   int32_t cnt[5];   // 0 unique candidates, 1 selected, 2 survivors, 3 threshold, 4 new ids
@@ -555,7 +557,11 @@ __global__ __launch_bounds__(256) void k_join(
     const int8_t* __restrict__ q, const uint64_t* __restrict__ sk, int d, int64_t n, int K, int M,
     const int32_t* __restrict__ ids_in, const int32_t* __restrict__ sc_in,
     int32_t* __restrict__ ids_out, int32_t* __restrict__ sc_out,
-    unsigned long long* __restrict__ counters, const int32_t* __restrict__ order) {
+    unsigned long long* __restrict__ counters, const int32_t* __restrict__ order,
+    const uint64_t* __restrict__ fresh_in, uint64_t* __restrict__ fresh_out) {
+  // fresh_in != nullptr: incremental join -- the pair (i -> n1 -> c) is a
+  // candidate only if n1 is fresh in i's list or c is fresh in n1's list
+  // (otherwise the previous iteration already considered it)
   constexpr int HS = 1 << HSBITS;
   static_assert(HS * 4 >= CAP * 2 + CAP * 4, "selection arrays must fit in the hash area");
   __shared__ __attribute__((aligned(16))) int32_t hs[HS];
@@ -591,6 +597,7 @@ __global__ __launch_bounds__(256) void k_join(
       S.own_id[tid] = ids_in[i * K + tid];
       S.own_sc[tid] = sc_in[i * K + tid];
     }
+    if (tid == 0) S.own_new = fresh_in ? fresh_in[i] : ~0ull;
     __syncthreads();
     if (tid < K) hs_insert(hs, HSBITS, S.own_id[tid]);
     if (tid == K) hs_insert(hs, HSBITS, int32_t(i));
@@ -599,10 +606,16 @@ __global__ __launch_bounds__(256) void k_join(
     {
       constexpr int kRowsPerWave = kMaxK / 4;
       int32_t c[kRowsPerWave];
+      const uint64_t own_new = S.own_new;
 #pragma unroll
       for (int r = 0; r < kRowsPerWave; ++r) {
         const int k = w + 4 * r;
-        c[r] = (k < K && lane < K) ? ids_in[int64_t(S.own_id[k]) * K + lane] : -1;
+        c[r] = -1;
+        if (k < K && lane < K) {
+          const int32_t n1 = S.own_id[k];
+          const bool take = ((own_new >> k) & 1ull) || !fresh_in || ((fresh_in[n1] >> lane) & 1ull);
+          if (take) c[r] = ids_in[int64_t(n1) * K + lane];
+        }
       }
 #pragma unroll
       for (int r = 0; r < kRowsPerWave; ++r)
@@ -766,19 +779,23 @@ __global__ __launch_bounds__(256) void k_join(
       if (r < K) {
         S.new_id[r] = id;
         S.new_sc[r] = sc;
+        S.new_fresh[r] = e >= K ? 1 : 0;
         if (e >= K) atomicAdd(&S.cnt[4], 1);   // a new id entered the list
       }
     }
     __syncthreads();
     // ---- 7: write back, count changes
     if (w == 0) {
-      bool changed = false;
+      bool changed = false, fresh = false;
       if (lane < K) {
         ids_out[i * K + lane] = S.new_id[lane];
         sc_out[i * K + lane] = S.new_sc[lane];
         changed = S.new_id[lane] != S.own_id[lane];
+        fresh = S.new_fresh[lane] != 0;
       }
       const unsigned long long m = __ballot(changed);
+      const unsigned long long fm = __ballot(fresh);
+      if (fresh_out && lane == 0) fresh_out[i] = fm;
       if (lane == 0) {
         my_upd += __popcll(m);
         my_cand += U;
@@ -983,6 +1000,7 @@ int knng_default_params(knng_params* p) {
   p->rerank_width = 0;
   p->stop_rule = KNNG_STOP_INSERTED;
   p->insert_stop_rate = 0.01f;
+  p->incremental = 0;
   return 0;
 }
 
@@ -1051,6 +1069,7 @@ int knng_destroy(knng_ctx* c) {
   for (int b = 0; b < 2; ++b) {
     dfree(&c->gid[b]);
     dfree(&c->gsc[b]);
+    dfree(&c->fresh[b]);
     dfree(&c->dstage[b]);
     if (c->pin[b]) hipHostFree(c->pin[b]);
     if (c->pin_ev[b]) hipEventDestroy(c->pin_ev[b]);
@@ -1268,8 +1287,10 @@ int knng_build_init(knng_ctx* c) {
   int rc;
   for (int b = 0; b < 2; ++b)
     if ((rc = dalloc(c, &c->gid[b], size_t(n) * K, "graph ids")) ||
-        (rc = dalloc(c, &c->gsc[b], size_t(n) * K, "graph scores")))
+        (rc = dalloc(c, &c->gsc[b], size_t(n) * K, "graph scores")) ||
+        (rc = dalloc(c, &c->fresh[b], size_t(n), "fresh masks")))
       return rc;
+  HIPCHK(c, hipMemsetAsync(c->fresh[0], 0xFF, size_t(n) * sizeof(uint64_t), c->stream));   // all fresh
   uint64_t* cand = nullptr;
   if ((rc = dalloc(c, &cand, size_t(n) * T * kLeafSlots, "init candidates"))) return rc;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -1318,10 +1339,12 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   const int32_t* order = c->have_forest ? c->perm : nullptr;   // tree 0's leaf order
   if (K <= 50)
     hipLaunchKernelGGL((k_join<12, 2560>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n, K,
-                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, order);
+                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, order,
+                       c->p.incremental ? c->fresh[a] : nullptr, c->fresh[b]);
   else
     hipLaunchKernelGGL((k_join<13, 4096>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n, K,
-                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, order);
+                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, order,
+                       c->p.incremental ? c->fresh[a] : nullptr, c->fresh[b]);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(256), 0, c->stream, c->counters, c->csum);
@@ -1491,12 +1514,14 @@ int knng_load_graph(knng_ctx* c, const int32_t* ids, const int32_t* scores) {
   int rc;
   for (int b = 0; b < 2; ++b)
     if ((rc = dalloc(c, &c->gid[b], size_t(n) * K, "graph ids")) ||
-        (rc = dalloc(c, &c->gsc[b], size_t(n) * K, "graph scores")))
+        (rc = dalloc(c, &c->gsc[b], size_t(n) * K, "graph scores")) ||
+        (rc = dalloc(c, &c->fresh[b], size_t(n), "fresh masks")))
       return rc;
   c->cur = 0;
   const size_t bytes = size_t(n) * K * sizeof(int32_t);
   HIPCHK(c, hipMemcpyAsync(c->gid[0], ids, bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->gsc[0], scores, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->fresh[0], 0xFF, size_t(n) * sizeof(uint64_t), c->stream));   // all fresh
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_graph = true;
   return 0;

@@ -42,6 +42,7 @@ struct knng_ctx {
 
   // graph
   int32_t *gid[2] = {nullptr, nullptr}, *gsc[2] = {nullptr, nullptr};
+  uint64_t* fresh[2] = {nullptr, nullptr};   // per row: bit k = entry k entered last iteration
   int cur = 0;
   bool have_graph = false;
   unsigned long long *counters = nullptr, *csum = nullptr;

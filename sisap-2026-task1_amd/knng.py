@@ -27,7 +27,7 @@ KNNG_SYMBOLS = [
     "knng_quantize_device", "knng_score_pairs_device",
     "knng_server_default_params", "knng_server_create", "knng_server_destroy",
     "knng_server_last_error", "knng_server_submit", "knng_server_poll", "knng_server_wait",
-    "knng_server_size", "knng_server_search",
+    "knng_server_size", "knng_server_search", "knng_build_batch",
 ]
 H5_SYMBOLS = [
     "knng_h5_open", "knng_h5_shape_of", "knng_h5_read_rows", "knng_h5_close",
@@ -52,6 +52,7 @@ class Params(C.Structure):
         ("max_iters", C.c_int32), ("early_stop_rate", C.c_float), ("rate_diff_tol", C.c_float),
         ("seed", C.c_uint64), ("keep_fp32", C.c_int32), ("rerank_width", C.c_int32),
         ("stop_rule", C.c_int32), ("insert_stop_rate", C.c_float),
+        ("incremental", C.c_int32), ("pad0", C.c_int32),
     ]
 
 
@@ -455,3 +456,51 @@ class Server:
 
     def size(self) -> int:
         return int(self.L.knng_server_size(self.srv))
+
+
+# -- batched builds ------------------------------------------------------------
+
+class Job(C.Structure):
+    _fields_ = [
+        ("rows", C.c_void_p), ("rows_on_device", C.c_int32), ("rerank", C.c_int32),
+        ("n", C.c_int64), ("params", Params), ("knns", C.c_void_p), ("dists", C.c_void_p),
+        ("status", C.c_int32), ("iterations", C.c_int32), ("build_ms", C.c_double),
+    ]
+
+
+def build_batch(jobs, device: int = 0, workers: int = 8) -> int:
+    """jobs: list of dicts {rows: device ptr or numpy array, n, params, rerank};
+    returns the batch status; fills job['knns'], ['dists'], ['status'],
+    ['iterations'], ['build_ms']."""
+    L = lib()
+    if not getattr(L, "_batch_sig", False):
+        L.knng_build_batch.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
+        L.knng_build_batch.restype = C.c_int32
+        L._batch_sig = True
+    arr = (Job * len(jobs))()
+    keep = []
+    for j, job in enumerate(jobs):
+        p = job["params"]
+        k = p.k_eval
+        job["knns"] = np.empty((job["n"], k), np.int32)
+        job["dists"] = np.empty((job["n"], k), np.float32)
+        rows = job["rows"]
+        if isinstance(rows, np.ndarray):
+            rows = np.ascontiguousarray(rows, np.float32)
+            keep.append(rows)
+            arr[j].rows = rows.ctypes.data
+            arr[j].rows_on_device = 0
+        else:
+            arr[j].rows = int(rows)
+            arr[j].rows_on_device = 1
+        arr[j].rerank = int(job.get("rerank", 0))
+        arr[j].n = job["n"]
+        arr[j].params = p
+        arr[j].knns = job["knns"].ctypes.data
+        arr[j].dists = job["dists"].ctypes.data
+    rc = L.knng_build_batch(C.cast(arr, C.c_void_p), len(jobs), device, workers)
+    for j, job in enumerate(jobs):
+        job["status"] = arr[j].status
+        job["iterations"] = arr[j].iterations
+        job["build_ms"] = arr[j].build_ms
+    return rc

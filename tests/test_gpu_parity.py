@@ -157,6 +157,36 @@ def test_join_iterations_bitexact(torch_cuda):
         assert np.array_equal(g_sc, sc), it
 
 
+def test_incremental_join_bitexact(torch_cuda):
+    """New/old rule: only pairs with a fresh edge are candidates."""
+    say("incremental join")
+    x = synth.gaussian_mixture(17, 5000, 1024, 32)
+    q, sk = oracle.quantize(x)
+    perm, leaves = oracle.forest_device(q, 4, 64, 1337)
+    ids, sc = oracle.init_device(q, perm, leaves, 50)
+    b = _builder(x, incremental=1)
+    b.load_graph(ids, sc)
+    fresh = np.full(5000, np.uint64(0xFFFFFFFFFFFFFFFF))
+    for it in range(4):
+        upd = b.join_iteration()
+        ids, sc, o_upd, fresh = oracle.join_device(q, sk, ids, sc, 400, fresh=fresh)
+        g_ids, g_sc = b.graph()
+        assert upd == o_upd, it
+        assert np.array_equal(g_ids, ids) and np.array_equal(g_sc, sc), it
+
+
+def test_incremental_full_build_bitexact(torch_cuda):
+    say("incremental build")
+    x = synth.gaussian_mixture(18, 4000, 1024, 16)
+    q, sk = oracle.quantize(x)
+    o_ids, o_sc, o_rates = oracle.build_device_semantics(q, sk, incremental=True)
+    b = _builder(x, incremental=1)
+    b.build()
+    ids, sc = b.graph()
+    assert np.array_equal(ids, o_ids) and np.array_equal(sc, o_sc)
+    assert b.stats()["iterations"] == len(o_rates)
+
+
 @pytest.mark.parametrize("K,mult", [(64, 8), (10, 3)])
 def test_join_other_shapes_bitexact(torch_cuda, K, mult):
     say("join shapes", K, mult)

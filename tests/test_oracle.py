@@ -300,6 +300,41 @@ def test_join_device_bruteforce():
     assert upd == changes
 
 
+def test_incremental_join_bruteforce():
+    """Incremental rule: (i -> n1 -> c) is a candidate iff n1 is fresh in i's
+    list or c is fresh in n1's list; fresh_out marks entries that came from
+    candidates."""
+    x, q, sk = _small(n=250, d=128)
+    K, M = 10, 30
+    perm, leaves = oracle.forest_device(q, 2, 16)
+    ids, sc = oracle.init_device(q, perm, leaves, K)
+    rng = np.random.default_rng(3)
+    fresh = rng.integers(0, 2 ** 63, 250, dtype=np.uint64) & np.uint64((1 << K) - 1)
+    oi, osc, upd, fo = oracle.join_device(q, sk, ids, sc, M, fresh=fresh)
+    Q = q.astype(np.int64)
+    bits = np.unpackbits(sk.view(np.uint8), axis=1, bitorder="little")
+    for i in range(250):
+        excl = set(ids[i].tolist()) | {i}
+        cands = []
+        for k, n1 in enumerate(ids[i]):
+            for k2, c in enumerate(ids[n1]):
+                c = int(c)
+                if not ((int(fresh[i]) >> k) & 1 or (int(fresh[n1]) >> k2) & 1):
+                    continue
+                if c not in excl and c not in cands:
+                    cands.append(c)
+        h = np.array([128 - int((bits[i] ^ bits[c]).sum()) for c in cands], np.int32)
+        sel = cands
+        if len(cands) > M:
+            thr = max(t for t in range(129) if (h >= t).sum() >= M)
+            sel = [c for c, hv in zip(cands, h) if hv >= thr]
+        pool = [(int(ids[i, k]), int(sc[i, k]), k) for k in range(K)]
+        pool += [(c, int(Q[i] @ Q[c]), K) for c in sel]
+        pool.sort(key=lambda e: (-e[1], e[0], e[2]))
+        assert oi[i].tolist() == [e[0] for e in pool[:K]]
+        assert int(fo[i]) == sum(1 << r for r, e in enumerate(pool[:K]) if e[2] == K)
+
+
 def test_rerank_device_fp64():
     x, q, sk = _small(n=200, d=128)
     perm, leaves = oracle.forest_device(q, 4, 16)
