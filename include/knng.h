@@ -49,6 +49,10 @@ typedef struct knng_params {
   float insert_stop_rate;  /* KNNG_STOP_INSERTED: stop once fewer than this share of slots got a new id */
   int32_t incremental;     /* 1: a 2-hop pair is a candidate only if one of its edges is new
                               since the previous iteration (NN-Descent's new/old rule) */
+  int32_t cells;           /* k-means cells that seed every tree (0: plain RP trees over all
+                              rows, as the reference; -1: n / 1536) */
+  int32_t kmeans_iters;    /* Lloyd iterations per cell partition */
+  int32_t cell_partitions; /* independently seeded partitions; tree t uses t % partitions */
   int32_t pad0;
 } knng_params;
 
@@ -74,6 +78,8 @@ typedef struct knng_stats {
   float ms_join_iter[32];
   float ms_export;
   float ms_build;              /* forest + init + join, the reference's buildtime span */
+  float ms_kmeans;             /* cell k-means (part of ms_forest) */
+  int32_t cells;               /* cells per partition (0: none) */
   double join_bytes[32];       /* algorithmic HBM bytes per join iteration */
   int64_t join_candidates[32]; /* unique 2-hop candidates scored by sketch */
   int64_t join_rescored[32];   /* candidates re-scored with int8 dots */

@@ -30,6 +30,11 @@
 //   orc_build_reference  golden knns/dists from the compiled reference
 //   orc_forest_device    invariants (partition, stable order, leaf bound) +
 //                        brute-force recursion in tests/test_oracle.py
+//   orc_forest_cells_device  same invariants + cells as contiguous runs
+//   orc_assign_device    float brute-force argmax in numpy (tests)
+//   orc_kmeans_device    fixed points: iters=0 gives the quantised seed rows;
+//                        separated clusters are recovered exactly; each
+//                        centroid equals the requantised sum of its cell
 //   orc_init_device      brute-force restatement in numpy (tests)
 //   orc_join_device      brute-force restatement in numpy (tests)
 //   orc_rerank_device    FP64 numpy dot products
@@ -41,6 +46,7 @@
 #include <cstring>
 #include <limits>
 #include <random>
+#include <unordered_map>
 #include <unordered_set>
 #include <utility>
 #include <vector>
@@ -363,17 +369,43 @@ inline uint64_t node_key(int32_t tree, int32_t depth, int64_t start) {
   return (uint64_t(uint32_t(tree)) << 48) ^ (uint64_t(uint32_t(depth)) << 40) ^ uint64_t(start);
 }
 
+// Cell seeding of the forest: `assign` holds G partitions' cell of every row
+// ([G][n], cells 0..C-1); tree t starts from partition t % G.
+struct CellSeed {
+  const int32_t* assign;
+  int32_t C, G;
+};
+
+// stable counting sort of rows by cell; offs has C + 1 entries
+void cell_order(const int32_t* a, int64_t n, int32_t C, int32_t* rows, std::vector<int64_t>* offs) {
+  offs->assign(size_t(C) + 1, 0);
+  for (int64_t i = 0; i < n; ++i) (*offs)[size_t(a[i]) + 1]++;
+  for (int32_t c = 0; c < C; ++c) (*offs)[size_t(c) + 1] += (*offs)[size_t(c)];
+  std::vector<int64_t> cur(offs->begin(), offs->end() - 1);
+  for (int64_t i = 0; i < n; ++i) rows[cur[size_t(a[i])]++] = int32_t(i);
+}
+
 int64_t forest_impl(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t leaf_size,
                     uint64_t seed, int32_t* perm, int64_t* leaf_tri, int64_t leaf_cap,
-                    std::unordered_map<uint64_t, Split>* rec) {
+                    std::unordered_map<uint64_t, Split>* rec, const CellSeed* cs = nullptr) {
   if (n < 2 || leaf_size < 2) return -1;
   int64_t nleaf = 0;
   std::vector<int32_t> lo, hi;
   struct Node { int64_t start, size; int32_t depth; };
+  // cells replace the root split (as the library does) unless the whole set is one leaf
+  const bool cells = cs && cs->C >= 2 && n > leaf_size;
   for (int t = 0; t < T; ++t) {
     int32_t* P = perm + int64_t(t) * n;
-    for (int64_t i = 0; i < n; ++i) P[i] = int32_t(i);
-    std::vector<Node> stack{{0, n, 0}};
+    std::vector<Node> stack;
+    if (cells) {
+      std::vector<int64_t> offs;
+      cell_order(cs->assign + int64_t(t % cs->G) * n, n, cs->C, P, &offs);
+      for (int32_t c = cs->C - 1; c >= 0; --c)
+        stack.push_back(Node{offs[size_t(c)], offs[size_t(c) + 1] - offs[size_t(c)], 1});
+    } else {
+      for (int64_t i = 0; i < n; ++i) P[i] = int32_t(i);
+      stack.push_back(Node{0, n, 0});
+    }
     while (!stack.empty()) {
       Node nd = stack.back();
       stack.pop_back();
@@ -415,6 +447,43 @@ int64_t forest_impl(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t le
   return nleaf;
 }
 
+// Centroid of a cell from its exact int64 member sum S: the int8 direction
+// z_k = trunc(S_k * 127 / max_k |S_k|) and w = float(1 / sqrt(sum z_k^2));
+// an all-zero sum gives z = 0, w = 0.
+void quantize_centroid(const int64_t* S, int32_t d, int8_t* z, float* w) {
+  int64_t M = 0;
+  for (int32_t k = 0; k < d; ++k) M = std::max<int64_t>(M, S[k] < 0 ? -S[k] : S[k]);
+  if (M == 0) {
+    std::fill(z, z + d, int8_t(0));
+    *w = 0.0f;
+    return;
+  }
+  double ss = 0.0;
+  for (int32_t k = 0; k < d; ++k) {
+    z[k] = int8_t((S[k] * 127) / M);
+    ss += double(z[k]) * double(z[k]);
+  }
+  *w = float(1.0 / std::sqrt(ss));
+}
+
+// cell of every row: argmax_c float(<q_i, z_c>) * w_c, the first (smallest c) maximum
+void assign_rows(const int8_t* q, int64_t n, int32_t d, const int8_t* z, const float* w, int32_t C,
+                 int32_t* out) {
+  for (int64_t i = 0; i < n; ++i) {
+    const int8_t* v = q + i * d;
+    float best = -std::numeric_limits<float>::infinity();
+    int32_t bi = std::numeric_limits<int32_t>::max();
+    for (int32_t c = 0; c < C; ++c) {
+      const float s = float(dot_i8(v, z + int64_t(c) * d, d)) * w[c];
+      if (s > best) {
+        best = s;
+        bi = c;
+      }
+    }
+    out[i] = bi;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -422,6 +491,52 @@ extern "C" {
 int64_t orc_forest_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t leaf_size,
                           uint64_t seed, int32_t* perm, int64_t* leaf_tri, int64_t leaf_cap) {
   return forest_impl(q, n, d, T, leaf_size, seed, perm, leaf_tri, leaf_cap, nullptr);
+}
+
+int32_t orc_assign_device(const int8_t* q, int64_t n, int32_t d, const int8_t* z, const float* w,
+                          int32_t C, int32_t* out) {
+  if (C < 1) return -1;
+  assign_rows(q, n, d, z, w, C, out);
+  return 0;
+}
+
+// Spherical k-means on the int8 rows (partition g): seeds are rows
+// ((2c + 1) n / (2C) + 7919 g) mod n; then `iters` rounds of (assign, exact
+// integer member sums, requantised centroid; an empty cell keeps its
+// centroid) and a final assignment with the last centroids.
+int32_t orc_kmeans_device(const int8_t* q, int64_t n, int32_t d, int32_t C, int32_t g,
+                          int32_t iters, int8_t* z, float* w, int32_t* assign) {
+  if (C < 1 || n < 1) return -1;
+  std::vector<int64_t> S(size_t(C) * d);
+  for (int32_t c = 0; c < C; ++c) {
+    const int64_t id = ((2 * int64_t(c) + 1) * n / (2 * int64_t(C)) + int64_t(g) * 7919) % n;
+    for (int32_t k = 0; k < d; ++k) S[size_t(c) * d + k] = q[id * d + k];
+    quantize_centroid(&S[size_t(c) * d], d, z + int64_t(c) * d, w + c);
+  }
+  std::vector<int64_t> cnt(C);
+  for (int32_t it = 0;; ++it) {
+    assign_rows(q, n, d, z, w, C, assign);
+    if (it == iters) break;
+    std::fill(S.begin(), S.end(), 0);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t* s = &S[size_t(assign[i]) * d];
+      for (int32_t k = 0; k < d; ++k) s[k] += q[i * d + k];
+      ++cnt[size_t(assign[i])];
+    }
+    for (int32_t c = 0; c < C; ++c)
+      if (cnt[size_t(c)]) quantize_centroid(&S[size_t(c) * d], d, z + int64_t(c) * d, w + c);
+  }
+  return 0;
+}
+
+// Forest whose trees start from cells: assign = [G][n] cells of the rows.
+int64_t orc_forest_cells_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t leaf_size,
+                                uint64_t seed, const int32_t* assign, int32_t C, int32_t G,
+                                int32_t* perm, int64_t* leaf_tri, int64_t leaf_cap) {
+  if (G < 1) return -1;
+  const CellSeed cs{assign, C, G};
+  return forest_impl(q, n, d, T, leaf_size, seed, perm, leaf_tri, leaf_cap, nullptr, &cs);
 }
 
 // Beam search over a K-list graph (device semantics of the serving kernel):
@@ -437,10 +552,19 @@ int64_t orc_forest_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int3
 int orc_search_device(const int8_t* q, int64_t n, int32_t d, const int32_t* gid, int32_t K,
                       int32_t T, int32_t leaf_size, uint64_t seed, int32_t entry_trees,
                       int32_t beam, int32_t k, int32_t max_steps, const int8_t* qq, int64_t nq,
+                      const int32_t* assign, int32_t C, int32_t G, const int32_t* qcell,
                       int32_t* out_ids, int32_t* out_sc) {
   std::vector<int32_t> perm(size_t(T) * n);
   std::unordered_map<uint64_t, Split> rec;
-  if (forest_impl(q, n, d, T, leaf_size, seed, perm.data(), nullptr, 0, &rec) < 0) return -1;
+  const CellSeed cs{assign, C, G};
+  const bool cells = assign && C >= 2 && G >= 1 && n > leaf_size;
+  if (forest_impl(q, n, d, T, leaf_size, seed, perm.data(), nullptr, 0, &rec, cells ? &cs : nullptr) < 0)
+    return -1;
+  std::vector<std::vector<int64_t>> coffs(cells ? G : 0);
+  for (int32_t g = 0; g < int32_t(coffs.size()); ++g) {
+    std::vector<int32_t> rows(n);
+    cell_order(assign + int64_t(g) * n, n, C, rows.data(), &coffs[g]);
+  }
   const int et = std::min(std::min(entry_trees, T), 4);
   constexpr int kExpand = 4;
   struct B { int32_t id, sc, ex; };
@@ -452,6 +576,13 @@ int orc_search_device(const int8_t* q, int64_t n, int32_t d, const int32_t* gid,
     for (int t = 0; t < et; ++t) {
       int64_t start = 0, size = n;
       int32_t depth = 0;
+      if (cells) {   // enter at the query's cell of this tree's partition
+        const int32_t g = t % G, c = qcell[int64_t(g) * nq + qi];
+        start = coffs[g][size_t(c)];
+        size = coffs[g][size_t(c) + 1] - start;
+        depth = 1;
+        if (size == 0) continue;
+      }
       while (size > leaf_size && depth < kMaxDepth) {
         const Split& sp = rec.at(node_key(t, depth, start));
         const int32_t diff = dot_i8(y, q + int64_t(sp.p1id) * d, d) - dot_i8(y, q + int64_t(sp.p2id) * d, d);

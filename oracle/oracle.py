@@ -56,7 +56,10 @@ def lib() -> C.CDLL:
             "orc_join_device": ([V, V, i64, i32, i32, i32, V, V, V, V, C.POINTER(i64), V, V], i64),
             "orc_rerank_device": ([V, i64, i32, i32, i32, i32, V, V, V], i32),
             "orc_search_device": ([V, i64, i32, V, i32, i32, i32, u64, i32, i32, i32, i32, V, i64,
-                                   V, V], i32),
+                                   V, i32, i32, V, V, V], i32),
+            "orc_assign_device": ([V, i64, i32, V, V, i32, V], i32),
+            "orc_kmeans_device": ([V, i64, i32, i32, i32, i32, V, V, V], i32),
+            "orc_forest_cells_device": ([V, i64, i32, i32, i32, u64, V, i32, i32, V, V, i64], i64),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -136,6 +139,63 @@ def forest_device(q: np.ndarray, trees: int, leaf: int, seed: int = 1337):
     return perm, leaves[:nl].copy()
 
 
+def cells_for(n: int, cells: int = -1) -> int:
+    """Cell count the library picks: `cells` if >= 0, else n // 1536 when that is >= 2."""
+    if cells >= 0:
+        return cells
+    c = n // 1536
+    return c if c >= 2 else 0
+
+
+def assign_device(q: np.ndarray, z: np.ndarray, w: np.ndarray) -> np.ndarray:
+    n, d = q.shape
+    C = z.shape[0]
+    out = np.empty(n, np.int32)
+    assert lib().orc_assign_device(_p(np.ascontiguousarray(q)), n, d, _p(np.ascontiguousarray(z)),
+                                   _p(np.ascontiguousarray(w, np.float32)), C, _p(out)) == 0
+    return out
+
+
+def kmeans_device(q: np.ndarray, C: int, g: int = 0, iters: int = 6):
+    """Spherical int8 k-means of partition g -> (z [C, d] int8, w [C] f32, assign [n] int32)."""
+    n, d = q.shape
+    z = np.empty((C, d), np.int8)
+    w = np.empty(C, np.float32)
+    a = np.empty(n, np.int32)
+    assert lib().orc_kmeans_device(_p(np.ascontiguousarray(q)), n, d, C, g, iters, _p(z), _p(w),
+                                   _p(a)) == 0
+    return z, w, a
+
+
+def cells_device(q: np.ndarray, trees: int, leaf: int, cells: int = -1, iters: int = 6,
+                 partitions: int = 2):
+    """The library's cell plan for a build: None when no cells are used, else
+    (C, G, z [G, C, d], w [G, C], assign [G, n])."""
+    n = q.shape[0]
+    C = min(cells_for(n, cells), n)
+    if C < 2 or n <= leaf:
+        return None
+    G = max(1, min(partitions, trees))
+    zs, ws, As = zip(*(kmeans_device(q, C, g, iters) for g in range(G)))
+    return C, G, np.stack(zs), np.stack(ws), np.stack(As)
+
+
+def forest_cells_device(q: np.ndarray, trees: int, leaf: int, plan, seed: int = 1337):
+    """Forest seeded by a cell plan (from cells_device); plan None = plain forest."""
+    if plan is None:
+        return forest_device(q, trees, leaf, seed)
+    C, G, _, _, A = plan
+    n, d = q.shape
+    perm = np.empty((trees, n), np.int32)
+    cap = trees * n + 16
+    leaves = np.empty((cap, 3), np.int64)
+    nl = lib().orc_forest_cells_device(_p(q), n, d, trees, leaf, seed,
+                                       _p(np.ascontiguousarray(A, np.int32)), C, G, _p(perm),
+                                       _p(leaves), cap)
+    assert nl >= 0
+    return perm, leaves[:nl].copy()
+
+
 def init_device(q: np.ndarray, perm: np.ndarray, leaves: np.ndarray, K: int):
     n, d = q.shape
     T = perm.shape[0]
@@ -190,11 +250,15 @@ STOP_REFERENCE, STOP_INSERTED, STOP_NONE = 0, 1, 2
 
 def build_device_semantics(q, sk, trees=16, leaf=64, K=50, mult=8, max_iters=30,
                            stop=0.04, tol=0.01, seed=1337, stop_rule=STOP_INSERTED,
-                           insert_stop=0.01, incremental=False):
+                           insert_stop=0.01, incremental=False, cells=-1, kmeans_iters=6,
+                           partitions=2):
     """Full device-mode build.  Stop rules: STOP_REFERENCE is src/main.cpp:432-439
     on the rate of slots whose id changed; STOP_INSERTED stops once fewer than
-    insert_stop of all slots received a new id; STOP_NONE runs max_iters."""
-    perm, leaves = forest_device(q, trees, leaf, seed)
+    insert_stop of all slots received a new id; STOP_NONE runs max_iters.
+    cells / kmeans_iters / partitions select the cell seeding of the forest
+    with the library's rule (cells=-1: n // 1536 when >= 2, 0: none)."""
+    plan = cells_device(q, trees, leaf, cells, kmeans_iters, partitions)
+    perm, leaves = forest_cells_device(q, trees, leaf, plan, seed)
     ids, sc = init_device(q, perm, leaves, K)
     n = q.shape[0]
     prev = np.float32(1.0)
@@ -234,15 +298,24 @@ def run_reference_binary(h5_path: str, workdir: str, threads: int = 1, native: b
 
 
 def search_device(q, gid, queries_q, T=16, leaf=64, seed=1337, entry_trees=4, beam=64, k=15,
-                  max_steps=128):
-    """Beam search over graph `gid` (device semantics of the serving kernel)."""
+                  max_steps=128, plan=None):
+    """Beam search over graph `gid` (device semantics of the serving kernel);
+    `plan` (from cells_device) routes each tree's entry through the query's cell."""
     n, d = q.shape
     K = gid.shape[1]
+    queries_q = np.ascontiguousarray(queries_q)
     nq = queries_q.shape[0]
     ids = np.empty((nq, k), np.int32)
     sc = np.empty((nq, k), np.int32)
+    A = qc = None
+    C = G = 0
+    if plan is not None:
+        C, G, Z, W, A = plan
+        qc = np.ascontiguousarray(np.stack([assign_device(queries_q, Z[g], W[g]) for g in range(G)]))
+        A = np.ascontiguousarray(A, np.int32)
     rc = lib().orc_search_device(_p(np.ascontiguousarray(q)), n, d, _p(np.ascontiguousarray(gid)), K,
                                  T, leaf, seed, entry_trees, beam, k, max_steps,
-                                 _p(np.ascontiguousarray(queries_q)), nq, _p(ids), _p(sc))
+                                 _p(queries_q), nq, None if A is None else _p(A), C, G,
+                                 None if qc is None else _p(qc), _p(ids), _p(sc))
     assert rc == 0
     return ids, sc

@@ -323,6 +323,138 @@ __global__ void k_hier_append(const Node* __restrict__ nodes, int nnodes, Node* 
 }
 
 // ---------------------------------------------------------------------------
+// Cells (spherical k-means on the int8 rows), used to seed the forest.
+// k_assign: a[i] = argmax_c float(<q_i, Z_c>) * w[c], ties to the smaller c.
+// One block = 128 rows, wave v owns rows 32v..32v+31 and keeps their A
+// fragments for the whole pass in registers; 64-cell centroid tiles are
+// staged in LDS and shared by the four waves; each wave runs
+// v_mfma_i32_32x32x32_i8 over the two 32-cell halves of a tile and keeps a
+// running best per accumulator slot, reduced across lanes once at the end.
+// ---------------------------------------------------------------------------
+
+constexpr int kAssignRows = 128;
+constexpr int kAssignTile = 32;
+
+__global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, int64_t nrows, int d,
+                                                const int8_t* __restrict__ Z,
+                                                const float* __restrict__ w, int C,
+                                                int32_t* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) int8_t ztile[kAssignTile * (kMaxDim + 16)];
+  const int stride = d + 16;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int ksteps = d >> 5;
+  for (int64_t r0 = int64_t(blockIdx.x) * kAssignRows; r0 < nrows;
+       r0 += int64_t(gridDim.x) * kAssignRows) {
+    // A fragments: row r0 + 32*wv + (lane&31), bytes 32*ks + 16*(lane>>5)
+    const int64_t arow = r0 + 32 * wv + (lane & 31);
+    const bool arow_ok = arow < nrows;
+    v4i a[32];
+#pragma unroll
+    for (int ks = 0; ks < 32; ++ks)
+      a[ks] = (arow_ok && ks < ksteps)
+                  ? *reinterpret_cast<const v4i*>(q + arow * d + 32 * ks + 16 * (lane >> 5))
+                  : v4i{0, 0, 0, 0};
+    float best[16];
+    int32_t bidx[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      best[j] = -INFINITY;
+      bidx[j] = 0x7FFFFFFF;
+    }
+    for (int c0 = 0; c0 < C; c0 += kAssignTile) {
+      const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
+      __syncthreads();   // previous tile fully consumed
+      for (int u = tid; u < nc * (d >> 4); u += 256) {
+        const int r = u / (d >> 4), c = u - r * (d >> 4);
+        *reinterpret_cast<v4i*>(ztile + r * stride + 16 * c) =
+            *reinterpret_cast<const v4i*>(Z + int64_t(c0 + r) * d + 16 * c);
+      }
+      __syncthreads();
+      {
+        const int cl = lane & 31;   // B column (cell) of this lane
+        const int8_t* rb = ztile + (cl < nc ? cl : 0) * stride + 16 * (lane >> 5);
+        v16i acc;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[j] = 0;
+#pragma unroll
+        for (int ks = 0; ks < 32; ++ks)
+          if (ks < ksteps)
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], *reinterpret_cast<const v4i*>(rb + 32 * ks),
+                                                        acc, 0, 0, 0);
+        const int cell = c0 + cl;
+        const bool ok = cl < nc;
+        const float wc = ok ? w[cell] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const float s = float(acc[j]) * wc;
+          if (ok && s > best[j]) {   // cells arrive in increasing order: ties keep the smaller
+            best[j] = s;
+            bidx[j] = cell;
+          }
+        }
+      }
+    }
+    // reduce over the 32 lanes that share the accumulator rows
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+#pragma unroll
+      for (int m = 16; m >= 1; m >>= 1) {
+        const float ov = __shfl_xor(best[j], m);
+        const int32_t oi = __shfl_xor(bidx[j], m);
+        if (ov > best[j] || (ov == best[j] && oi < bidx[j])) {
+          best[j] = ov;
+          bidx[j] = oi;
+        }
+      }
+      // acc element j of lane l holds row 8*(j/4) + 4*(l/32) + (j%4) of the wave's 32
+      const int64_t row = r0 + 32 * wv + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+      if ((lane & 31) == 0 && row < nrows) out[row] = bidx[j];
+    }
+  }
+}
+
+// Per-cell exact sums of member rows: block per cell, members listed in
+// `members` (cells contiguous, offsets in `offs`).  int64 accumulators: a
+// cell may hold more than 2^24 rows of +-128.
+__global__ __launch_bounds__(256) void k_cell_sum(const int8_t* __restrict__ q, int d,
+                                                  const int32_t* __restrict__ members,
+                                                  const int64_t* __restrict__ offs, int C,
+                                                  int64_t* __restrict__ sums) {
+  for (int c = blockIdx.x; c < C; c += gridDim.x) {
+    const int64_t b = offs[c], e = offs[c + 1];
+    for (int k0 = threadIdx.x * 4; k0 < d; k0 += 1024) {
+      int64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+      int64_t m = b;
+      for (; m + 4 <= e; m += 4) {
+        uint32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          v[u] = *reinterpret_cast<const uint32_t*>(q + int64_t(members[m + u]) * d + k0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          s0 += int8_t(v[u] & 0xFF);
+          s1 += int8_t((v[u] >> 8) & 0xFF);
+          s2 += int8_t((v[u] >> 16) & 0xFF);
+          s3 += int8_t(v[u] >> 24);
+        }
+      }
+      for (; m < e; ++m) {
+        const uint32_t v = *reinterpret_cast<const uint32_t*>(q + int64_t(members[m]) * d + k0);
+        s0 += int8_t(v & 0xFF);
+        s1 += int8_t((v >> 8) & 0xFF);
+        s2 += int8_t((v >> 16) & 0xFF);
+        s3 += int8_t(v >> 24);
+      }
+      int64_t* o = sums + int64_t(c) * d + k0;
+      o[0] = s0;
+      o[1] = s1;
+      o[2] = s2;
+      o[3] = s3;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // K3: leaf Gram tiles on int8 MFMA.  One block per leaf (<= 64 members), four
 // waves, wave w computes the 32x32 quadrant (w>>1, w&1) with
 // v_mfma_i32_32x32x32_i8 over K-chunks staged in LDS.  A and B fragments are
@@ -949,6 +1081,16 @@ int launch_quantize(hipStream_t stream, const float* dx, int64_t nrows, int d, i
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+int launch_assign(hipStream_t stream, const int8_t* q, int64_t nrows, int d, const int8_t* Z,
+                  const float* w, int C, int32_t* out, int num_cu) {
+  if (nrows <= 0) return 0;
+  int64_t g = (nrows + kAssignRows - 1) / kAssignRows;
+  const int64_t cap = int64_t(num_cu) * 8;
+  if (g > cap) g = cap;
+  hipLaunchKernelGGL(k_assign, dim3(g), dim3(256), 0, stream, q, nrows, d, Z, w, C, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace knng_host
 
 namespace {
@@ -960,6 +1102,175 @@ int quantize_rows(knng_ctx* c, const float* dx, int64_t row0, int64_t nrows) {
   if (launch_quantize(c->stream, dx, nrows, c->d, c->q + row0 * c->d, c->sk + row0 * c->words,
                       c->num_cu))
     return fail(c, KNNG_EHIP, std::string("k_quantize launch: ") + hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Cells.  A centroid is stored as an int8 direction z (largest component
+// +-127) with w = 1/|z|; a row goes to argmax_c float(<q, z_c>) * w_c.
+// The update is exact integer arithmetic on the host: S_c = sum of the int8
+// member rows, z_k = trunc(S_k * 127 / max_k |S_k|), w = float(1/sqrt(sum z^2)).
+// ---------------------------------------------------------------------------
+
+int32_t cells_for(const knng_params& p, int64_t n) {
+  if (p.cells >= 0) return p.cells;
+  const int64_t c = n / 1536;
+  return c >= 2 ? int32_t(c) : 0;
+}
+
+void quantize_centroid(const int64_t* S, int d, int8_t* z, float* w) {
+  int64_t M = 0;
+  for (int k = 0; k < d; ++k) M = std::max<int64_t>(M, S[k] < 0 ? -S[k] : S[k]);
+  if (M == 0) {
+    std::fill(z, z + d, int8_t(0));
+    *w = 0.0f;
+    return;
+  }
+  double ss = 0.0;
+  for (int k = 0; k < d; ++k) {
+    z[k] = int8_t((S[k] * 127) / M);
+    ss += double(z[k]) * double(z[k]);
+  }
+  *w = ss > 0.0 ? float(1.0 / std::sqrt(ss)) : 0.0f;
+}
+
+// stable counting sort of rows by cell: members[offs[c]..offs[c+1]) ascending ids
+void cell_lists(const std::vector<int32_t>& a, int C, std::vector<int32_t>* members,
+                std::vector<int64_t>* offs) {
+  offs->assign(size_t(C) + 1, 0);
+  for (int32_t c : a) (*offs)[size_t(c) + 1]++;
+  for (int c = 0; c < C; ++c) (*offs)[size_t(c) + 1] += (*offs)[size_t(c)];
+  std::vector<int64_t> cur(offs->begin(), offs->end() - 1);
+  members->resize(a.size());
+  for (size_t i = 0; i < a.size(); ++i) (*members)[size_t(cur[size_t(a[i])]++)] = int32_t(i);
+}
+
+// spherical k-means on the int8 rows for partition g; leaves the final
+// assignment in c->cassign + g*n and the centroids in c->cz / c->cw
+int run_kmeans(knng_ctx* c, int g, int32_t* d_members, int64_t* d_offs, int64_t* d_sums) {
+  const int64_t n = c->n;
+  const int d = c->d, C = c->ncells;
+  std::vector<int8_t> z(size_t(C) * d);
+  std::vector<float> w(C);
+  // seed rows: evenly spaced ids, shifted per partition
+  {
+    std::vector<int8_t> row(d);
+    std::vector<int64_t> S(d);
+    for (int cc = 0; cc < C; ++cc) {
+      const int64_t id = ((2 * int64_t(cc) + 1) * n / (2 * int64_t(C)) + int64_t(g) * 7919) % n;
+      HIPCHK(c, hipMemcpy(row.data(), c->q + id * d, d, hipMemcpyDeviceToHost));
+      for (int k = 0; k < d; ++k) S[k] = row[k];
+      quantize_centroid(S.data(), d, &z[size_t(cc) * d], &w[cc]);
+    }
+  }
+  int8_t* dz = c->cz + size_t(g) * C * d;
+  float* dw = c->cw + size_t(g) * C;
+  int32_t* da = c->cassign + size_t(g) * n;
+  std::vector<int32_t> a(n), members;
+  std::vector<int64_t> offs;
+  std::vector<int64_t> sums(size_t(C) * d);
+  const int agrid = grid_for(c, (n + kAssignRows - 1) / kAssignRows, 1, 8);
+  for (int it = 0;; ++it) {
+    HIPCHK(c, hipMemcpyAsync(dz, z.data(), z.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dw, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_assign, dim3(agrid), dim3(256), 0, c->stream, c->q, n, d, dz, dw, C, da);
+    HIPCHK(c, hipGetLastError());
+    if (it == c->p.kmeans_iters) break;
+    HIPCHK(c, hipMemcpyAsync(a.data(), da, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    cell_lists(a, C, &members, &offs);
+    HIPCHK(c, hipMemcpyAsync(d_members, members.data(), size_t(n) * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_offs, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_cell_sum, dim3(std::min(C, c->num_cu * 8)), dim3(256), 0, c->stream, c->q, d,
+                       d_members, d_offs, C, d_sums);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(sums.data(), d_sums, sums.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int cc = 0; cc < C; ++cc) {
+      if (offs[size_t(cc) + 1] == offs[size_t(cc)]) continue;   // empty: keep the centroid
+      quantize_centroid(&sums[size_t(cc) * d], d, &z[size_t(cc) * d], &w[cc]);
+    }
+  }
+  return 0;
+}
+
+// Seed every tree with its partition's cells: perm = rows by (cell, id);
+// cells larger than a leaf become the first level's nodes (depth 1), the
+// others leaves.  Returns the node count in *nnodes.
+int seed_from_cells(knng_ctx* c, int* nnodes) {
+  const int64_t n = c->n, T = c->p.num_trees;
+  const int d = c->d, leaf = c->p.leaf_size;
+  const int C = c->ncells, G = c->nparts;
+  int rc;
+  if ((rc = dalloc(c, &c->cz, size_t(G) * C * d, "centroids")) ||
+      (rc = dalloc(c, &c->cw, size_t(G) * C, "centroid norms")) ||
+      (rc = dalloc(c, &c->cassign, size_t(G) * n, "cell assignment")) ||
+      (rc = dalloc(c, &c->cell_root, size_t(T) * C, "cell roots")))
+    return rc;
+  int32_t* d_members = nullptr;
+  int64_t* d_offs = nullptr;
+  int64_t* d_sums = nullptr;
+  if ((rc = dalloc(c, &d_members, size_t(n), "cell members")) ||
+      (rc = dalloc(c, &d_offs, size_t(C) + 1, "cell offsets")) ||
+      (rc = dalloc(c, &d_sums, size_t(C) * d, "cell sums")))
+    return rc;
+  for (int g = 0; g < G && !rc; ++g) rc = run_kmeans(c, g, d_members, d_offs, d_sums);
+  dfree(&d_members);
+  dfree(&d_offs);
+  dfree(&d_sums);
+  if (rc) return rc;
+  // host build of perm, nodepos, first-level nodes, small-cell leaves, cell roots
+  std::vector<std::vector<int32_t>> mem(G);
+  std::vector<std::vector<int64_t>> off(G);
+  std::vector<int32_t> a(n);
+  for (int g = 0; g < G; ++g) {
+    HIPCHK(c, hipMemcpy(a.data(), c->cassign + size_t(g) * n, size_t(n) * 4, hipMemcpyDeviceToHost));
+    cell_lists(a, C, &mem[g], &off[g]);
+  }
+  std::vector<int32_t> perm(size_t(T) * n), npos(size_t(T) * n, -1), roots(size_t(T) * C);
+  std::vector<Node> nodes;
+  std::vector<int4> leaves;
+  for (int t = 0; t < T; ++t) {
+    const int g = t % G;
+    std::copy(mem[g].begin(), mem[g].end(), perm.begin() + size_t(t) * n);
+    for (int cc = 0; cc < C; ++cc) {
+      const int64_t s = off[g][size_t(cc)], z = off[g][size_t(cc) + 1] - s;
+      if (z == 0) {
+        roots[size_t(t) * C + cc] = INT32_MIN;
+      } else if (z <= leaf) {
+        roots[size_t(t) * C + cc] = -int32_t(leaves.size() + 1);
+        leaves.push_back(make_int4(t, int(s), int(z), 0));
+      } else {
+        Node nd;
+        nd.tree = t;
+        nd.start = int32_t(s);
+        nd.size = int32_t(z);
+        node_pivots(c->p.seed, t, 1, s, z, &nd.p1, &nd.p2);
+        nd.left = nd.right = -1;
+        nd.nleft = 0;
+        nd.p1id = nd.p2id = -1;
+        roots[size_t(t) * C + cc] = int32_t(nodes.size());
+        for (int64_t j = s; j < s + z; ++j) npos[size_t(t) * n + j] = int32_t(nodes.size());
+        nodes.push_back(nd);
+      }
+    }
+  }
+  if (int64_t(nodes.size()) > c->node_cap || int64_t(leaves.size()) > c->leaf_cap)
+    return fail(c, KNNG_ERANGE, "cell seeding exceeds the node/leaf tables");
+  HIPCHK(c, hipMemcpyAsync(c->perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->nodepos, npos.data(), npos.size() * 4, hipMemcpyHostToDevice, c->stream));
+  if (!nodes.empty())
+    HIPCHK(c, hipMemcpyAsync(c->nodes, nodes.data(), nodes.size() * sizeof(Node), hipMemcpyHostToDevice,
+                             c->stream));
+  if (!leaves.empty())
+    HIPCHK(c, hipMemcpyAsync(c->leaves, leaves.data(), leaves.size() * sizeof(int4),
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->cell_root, roots.data(), roots.size() * 4, hipMemcpyHostToDevice,
+                           c->stream));
+  int32_t cnt[2] = {0, int32_t(leaves.size())};
+  HIPCHK(c, hipMemcpyAsync(c->dcount, cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *nnodes = int(nodes.size());
   return 0;
 }
 
@@ -975,6 +1286,9 @@ int check_params(const knng_params* p, int64_t n, std::string* why) {
   if (p->rerank_width < 0 || p->rerank_width > p->k_build) return *why = "rerank_width must be in [0, k_build]", KNNG_EINVAL;
   if (p->rerank_width && p->rerank_width < p->k_eval) return *why = "rerank_width must be >= k_eval", KNNG_EINVAL;
   if (p->stop_rule < 0 || p->stop_rule > 2) return *why = "stop_rule must be 0, 1 or 2", KNNG_EINVAL;
+  if (p->cells < -1) return *why = "cells must be -1 (auto), 0 (none) or a count", KNNG_EINVAL;
+  if (p->kmeans_iters < 0 || p->kmeans_iters > 100) return *why = "kmeans_iters must be in [0, 100]", KNNG_EINVAL;
+  if (p->cell_partitions < 1) return *why = "cell_partitions must be >= 1", KNNG_EINVAL;
   if (int64_t(p->num_trees) * n >= (int64_t(1) << 31) - 1) return *why = "num_trees * n must fit in int32", KNNG_ERANGE;
   if (n >= (int64_t(1) << 31) - 1) return *why = "n must fit in int32", KNNG_ERANGE;
   return 0;
@@ -1001,6 +1315,10 @@ int knng_default_params(knng_params* p) {
   p->stop_rule = KNNG_STOP_INSERTED;
   p->insert_stop_rate = 0.01f;
   p->incremental = 0;
+  p->cells = -1;
+  p->kmeans_iters = 6;
+  p->cell_partitions = 2;
+  p->pad0 = 0;
   return 0;
 }
 
@@ -1065,6 +1383,10 @@ int knng_destroy(knng_ctx* c) {
   dfree(&c->nodes2);
   dfree(&c->leaves);
   dfree(&c->hier);
+  dfree(&c->cz);
+  dfree(&c->cw);
+  dfree(&c->cassign);
+  dfree(&c->cell_root);
   dfree(&c->dcount);
   for (int b = 0; b < 2; ++b) {
     dfree(&c->gid[b]);
@@ -1200,7 +1522,8 @@ int knng_build_forest(knng_ctx* c) {
   const int64_t n = c->n, T = c->p.num_trees, total = T * n;
   const int leaf = c->p.leaf_size;
   c->node_cap = T * (n / (leaf + 1) + 1) + 16;
-  c->leaf_cap = total / 4 + T + 1024;
+  // every non-empty cell below the leaf size is a leaf of its own
+  c->leaf_cap = total / 4 + T + 1024 + T * int64_t(std::min<int64_t>(cells_for(c->p, n), n));
   int rc;
   if ((rc = dalloc(c, &c->perm, total, "perm")) || (rc = dalloc(c, &c->perm2, total, "perm2")) ||
       (rc = dalloc(c, &c->nodepos, total, "nodepos")) ||
@@ -1230,6 +1553,28 @@ int knng_build_forest(knng_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     nnodes = 0;
   }
+  c->ncells = 0;
+  c->nparts = 0;
+  c->st.ms_kmeans = 0;
+  const int32_t ncells = cells_for(c->p, n);
+  if (nnodes > 0 && ncells >= 2) {
+    // k-means cells replace the root split of every tree
+    c->ncells = std::min<int32_t>(ncells, int32_t(n));
+    c->nparts = std::max(1, std::min<int>(c->p.cell_partitions, int(T)));
+    hipEvent_t k0, k1;
+    HIPCHK(c, hipEventCreate(&k0));
+    HIPCHK(c, hipEventCreate(&k1));
+    HIPCHK(c, hipEventRecord(k0, c->stream));
+    rc = seed_from_cells(c, &nnodes);
+    (void)hipEventRecord(k1, c->stream);
+    (void)hipEventSynchronize(k1);
+    (void)hipEventElapsedTime(&c->st.ms_kmeans, k0, k1);
+    (void)hipEventDestroy(k0);
+    (void)hipEventDestroy(k1);
+    if (rc) return rc;
+    levels = 1;
+  }
+  c->st.cells = c->ncells;
   const int nch = c->d / 16;
   while (nnodes > 0) {
     const int run = 16;

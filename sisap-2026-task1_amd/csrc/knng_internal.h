@@ -34,8 +34,14 @@ struct knng_ctx {
   int32_t *flags = nullptr, *scan = nullptr, *blocksums = nullptr;
   Node *nodes = nullptr, *nodes2 = nullptr;
   int4* leaves = nullptr;
-  Node* hier = nullptr;        // every level's nodes, global child links (roots 0..T-1)
+  Node* hier = nullptr;        // every level's nodes, global child links
   int64_t hier_count = 0;
+  // cells: spherical k-means partitions of the int8 rows that seed the trees
+  int32_t ncells = 0, nparts = 0;
+  int8_t* cz = nullptr;        // [nparts][ncells][d] int8 centroid directions
+  float* cw = nullptr;         // [nparts][ncells] 1 / |centroid|
+  int32_t* cassign = nullptr;  // [nparts][n] cell of every row
+  int32_t* cell_root = nullptr;   // [T][ncells]: subtree root (node >= 0, leaf -(l+1), empty INT32_MIN)
   int32_t* dcount = nullptr;   // [0] next nodes, [1] leaves, [2] overflow, [3] scan total
   int64_t node_cap = 0, leaf_cap = 0, nleaves = 0;
   bool have_forest = false;
@@ -106,5 +112,8 @@ inline int grid_for(const knng_ctx* c, int64_t work, int per_block, int blocks_p
 // K1 on an arbitrary stream (defined in knng.hip)
 int launch_quantize(hipStream_t stream, const float* dx, int64_t nrows, int d, int8_t* q,
                     uint64_t* sk, int num_cu);
+// route int8 rows to cells: out[i] = argmax_c float(<q_i, Z_c>) * w[c] (knng.hip)
+int launch_assign(hipStream_t stream, const int8_t* q, int64_t nrows, int d, const int8_t* Z,
+                  const float* w, int C, int32_t* out, int num_cu);
 
 }  // namespace knng_host

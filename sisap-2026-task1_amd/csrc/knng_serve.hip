@@ -57,8 +57,10 @@ __global__ __launch_bounds__(256) void k_search(
     const int8_t* __restrict__ q, const float* __restrict__ x, int d,
     const int32_t* __restrict__ gid, int K, const Node* __restrict__ hier, int64_t hier_count,
     const int4* __restrict__ leaves, const int32_t* __restrict__ perm, int64_t n_forest,
-    int entry_trees, const int8_t* __restrict__ qq, const float* __restrict__ qx, int nq, int beam,
-    int k, int rerank, int max_steps, int32_t* __restrict__ out_ids, float* __restrict__ out_sc) {
+    const int32_t* __restrict__ cell_root, const int32_t* __restrict__ qcell, int ncells,
+    int nparts, int entry_trees, const int8_t* __restrict__ qq, const float* __restrict__ qx,
+    int nq, int beam, int k, int rerank, int max_steps, int32_t* __restrict__ out_ids,
+    float* __restrict__ out_sc) {
   constexpr int HS = 1 << kVisitBits;
   __shared__ int32_t hs[HS];
   __shared__ __attribute__((aligned(16))) int8_t qrow[kMaxDim];
@@ -85,13 +87,24 @@ __global__ __launch_bounds__(256) void k_search(
       own[c] = chn < nch ? *reinterpret_cast<const v4i*>(qrow + 16 * chn) : v4i{0, 0, 0, 0};
     }
 
-    // ---- entry points: descend trees w, w+4, ... to a leaf
+    // ---- entry points: descend trees w, w+4, ... to a leaf; with cells, each
+    // tree starts at the subtree of the query's cell in that tree's partition
     for (int t = w; t < entry_trees; t += 4) {
       int leaf_idx = -1;
-      if (hier_count == 0) {
+      int node = t;
+      bool skip = false;
+      if (ncells > 0) {
+        const int32_t r = cell_root[int64_t(t) * ncells + qcell[int64_t(t % nparts) * nq + qi]];
+        if (r == INT32_MIN)
+          skip = true;   // empty cell
+        else if (r < 0)
+          leaf_idx = -r - 1;
+        else
+          node = r;
+      } else if (hier_count == 0) {
         leaf_idx = t;   // every tree is a single leaf
-      } else {
-        int node = t;
+      }
+      if (!skip && leaf_idx < 0) {
         for (int depth = 0; depth < 4096 && leaf_idx < 0; ++depth) {
           const Node nd = hier[node];
           const int8_t* ra = q + int64_t(nd.p1id) * d;
@@ -355,6 +368,7 @@ struct Slot {
   uint64_t* dsk = nullptr;
   int32_t* did = nullptr;
   float* dsc = nullptr;
+  int32_t* dcell = nullptr;   // query cells, one row of max_batch per partition
   float* hx = nullptr;        // pinned staging
   int32_t* hid = nullptr;
   float* hsc = nullptr;
@@ -426,7 +440,7 @@ void free_slot(Slot* sl) {
   if (sl->stream) (void)hipStreamSynchronize(sl->stream);
   for (void* p : {static_cast<void*>(sl->dx), static_cast<void*>(sl->dq),
                   static_cast<void*>(sl->dsk), static_cast<void*>(sl->did),
-                  static_cast<void*>(sl->dsc)})
+                  static_cast<void*>(sl->dsc), static_cast<void*>(sl->dcell)})
     if (p) (void)hipFree(p);
   for (void* p : {static_cast<void*>(sl->hx), static_cast<void*>(sl->hid), static_cast<void*>(sl->hsc)})
     if (p) (void)hipHostFree(p);
@@ -445,6 +459,7 @@ Slot* new_slot(knng_server* s) {
             hipMalloc(&sl->dsk, B * s->words * sizeof(uint64_t)) == hipSuccess &&
             hipMalloc(&sl->did, B * s->sp.k * sizeof(int32_t)) == hipSuccess &&
             hipMalloc(&sl->dsc, B * s->sp.k * sizeof(float)) == hipSuccess &&
+            hipMalloc(&sl->dcell, B * std::max(1, s->ctx->nparts) * sizeof(int32_t)) == hipSuccess &&
             hipHostMalloc(&sl->hx, B * s->d * sizeof(float), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc(&sl->hid, B * s->sp.k * sizeof(int32_t), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc(&sl->hsc, B * s->sp.k * sizeof(float), hipHostMallocDefault) == hipSuccess;
@@ -457,17 +472,24 @@ Slot* new_slot(knng_server* s) {
 }
 
 // queue one search launch on `stream`: device queries dx (float32) -> ids/scores
+// dcell: nparts * nq int32 scratch for the query's cell in each partition
 int launch_search(knng_server* s, hipStream_t stream, const float* dx, int8_t* dq, uint64_t* dsk,
-                  int32_t nq, int beam, int k, int rerank, int32_t* did, float* dsc) {
+                  int32_t* dcell, int32_t nq, int beam, int k, int rerank, int32_t* did, float* dsc) {
   knng_ctx* c = s->ctx;
   if (knng_host::launch_quantize(stream, dx, nq, s->d, dq, dsk, s->num_cu))
     return sfail(s, KNNG_EHIP, "quantise launch failed");
+  for (int g = 0; g < c->nparts; ++g)   // route the queries to cells
+    if (knng_host::launch_assign(stream, dq, nq, s->d, c->cz + size_t(g) * c->ncells * s->d,
+                                 c->cw + size_t(g) * c->ncells, c->ncells, dcell + size_t(g) * nq,
+                                 s->num_cu))
+      return sfail(s, KNNG_EHIP, "cell routing launch failed");
   const int grid = std::max(1, std::min<int>(nq, s->num_cu * 8));
   // at most 4 trees: their leaves (<= 64 rows each) fill the candidate buffer
   const int et = std::min(std::min<int>(s->sp.entry_trees, c->p.num_trees), kCandCap / 64);
   hipLaunchKernelGGL(k_search, dim3(grid), dim3(256), 0, stream, s->q, s->x, s->d, s->gid, s->K,
-                     c->hier, c->hier_count, c->leaves, c->perm, s->n_forest, et, dq, dx, nq, beam, k,
-                     rerank, s->sp.max_steps, did, dsc);
+                     c->hier, c->hier_count, c->leaves, c->perm, s->n_forest, c->cell_root, dcell,
+                     c->ncells, std::max(1, c->nparts), et, dq, dx, nq, beam, k, rerank,
+                     s->sp.max_steps, did, dsc);
   SCHK(s, hipGetLastError());
   return 0;
 }
@@ -483,12 +505,14 @@ int do_insert(knng_server* s, Request* r) {
                          hipMemcpyHostToDevice, st));
   int32_t* did = nullptr;
   float* dsc = nullptr;
+  int32_t* dcell = nullptr;
   SCHK(s, hipMalloc(&did, size_t(m) * K * sizeof(int32_t)));
   SCHK(s, hipMalloc(&dsc, size_t(m) * K * sizeof(float)));
+  SCHK(s, hipMalloc(&dcell, size_t(m) * std::max(1, s->ctx->nparts) * sizeof(int32_t)));
   // neighbours of the new rows among the current ones (int8 scores, no rerank)
   const int beam = std::max(s->sp.beam, K);
-  int rc = launch_search(s, st, s->x + base * s->d, s->q + base * s->d, s->sk + base * s->words, m,
-                         std::min(beam, kBeamMax), K, 0, did, dsc);
+  int rc = launch_search(s, st, s->x + base * s->d, s->q + base * s->d, s->sk + base * s->words,
+                         dcell, m, std::min(beam, kBeamMax), K, 0, did, dsc);
   std::vector<int32_t> ids(size_t(m) * K);
   std::vector<float> sc(size_t(m) * K);
   if (!rc) {
@@ -498,6 +522,7 @@ int do_insert(knng_server* s, Request* r) {
   }
   (void)hipFree(did);
   (void)hipFree(dsc);
+  (void)hipFree(dcell);
   if (rc) return rc;
   // lists of the new rows; reverse proposals (target, new id, score); a new
   // row is forced into the lists of its kForced nearest neighbours
@@ -597,8 +622,8 @@ void dispatch(knng_server* s, Request* r, Slot* sl) {
   memcpy(sl->hx, r->rows.data() + size_t(r->next_row) * s->d, size_t(m) * s->d * sizeof(float));
   bool ok = hipMemcpyAsync(sl->dx, sl->hx, size_t(m) * s->d * sizeof(float), hipMemcpyHostToDevice,
                            sl->stream) == hipSuccess;
-  ok = ok && !launch_search(s, sl->stream, sl->dx, sl->dq, sl->dsk, m, s->sp.beam, s->sp.k,
-                            s->sp.rerank, sl->did, sl->dsc);
+  ok = ok && !launch_search(s, sl->stream, sl->dx, sl->dq, sl->dsk, sl->dcell, m, s->sp.beam,
+                            s->sp.k, s->sp.rerank, sl->did, sl->dsc);
   ok = ok && hipMemcpyAsync(sl->hid, sl->did, size_t(m) * s->sp.k * sizeof(int32_t),
                             hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
   ok = ok && hipMemcpyAsync(sl->hsc, sl->dsc, size_t(m) * s->sp.k * sizeof(float),

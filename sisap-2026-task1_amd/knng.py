@@ -52,7 +52,8 @@ class Params(C.Structure):
         ("max_iters", C.c_int32), ("early_stop_rate", C.c_float), ("rate_diff_tol", C.c_float),
         ("seed", C.c_uint64), ("keep_fp32", C.c_int32), ("rerank_width", C.c_int32),
         ("stop_rule", C.c_int32), ("insert_stop_rate", C.c_float),
-        ("incremental", C.c_int32), ("pad0", C.c_int32),
+        ("incremental", C.c_int32), ("cells", C.c_int32), ("kmeans_iters", C.c_int32),
+        ("cell_partitions", C.c_int32), ("pad0", C.c_int32),
     ]
 
 
@@ -63,7 +64,8 @@ class Stats(C.Structure):
         ("updates", C.c_int64 * 32), ("rates", C.c_float * 32),
         ("ms_ingest", C.c_float), ("ms_forest", C.c_float), ("ms_init", C.c_float),
         ("ms_join", C.c_float), ("ms_join_iter", C.c_float * 32), ("ms_export", C.c_float),
-        ("ms_build", C.c_float), ("join_bytes", C.c_double * 32),
+        ("ms_build", C.c_float), ("ms_kmeans", C.c_float), ("cells", C.c_int32),
+        ("join_bytes", C.c_double * 32),
         ("join_candidates", C.c_int64 * 32), ("join_rescored", C.c_int64 * 32),
         ("join_inserted", C.c_int64 * 32), ("insert_rates", C.c_float * 32),
     ]
@@ -77,6 +79,7 @@ class Stats(C.Structure):
             "ms_ingest": self.ms_ingest, "ms_forest": self.ms_forest, "ms_init": self.ms_init,
             "ms_join": self.ms_join, "ms_join_iter": [float(v) for v in self.ms_join_iter[:it]],
             "ms_export": self.ms_export, "ms_build": self.ms_build,
+            "ms_kmeans": self.ms_kmeans, "cells": self.cells,
             "join_bytes": list(self.join_bytes[:it]),
             "join_candidates": list(self.join_candidates[:it]),
             "join_rescored": list(self.join_rescored[:it]),

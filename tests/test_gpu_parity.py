@@ -101,16 +101,26 @@ def test_score_pairs_bitexact(torch_cuda):
 
 # ------------------------------------------------------------- K2-K4
 
-@pytest.mark.parametrize("n,d,T,leaf", [(3000, 1024, 16, 64), (5000, 128, 3, 8), (700, 384, 5, 33)])
-def test_forest_and_init_bitexact(torch_cuda, n, d, T, leaf):
-    say("forest+init", n, d, T, leaf)
+@pytest.mark.parametrize("n,d,T,leaf,cells,parts", [
+    (3000, 1024, 16, 64, -1, 2),    # auto: n < 3072, no cells
+    (5000, 128, 3, 8, -1, 2),       # auto: 3 cells, 2 partitions
+    (700, 384, 5, 33, -1, 2),
+    (3000, 1024, 16, 64, 12, 2),
+    (2000, 256, 5, 16, 40, 3),      # many small cells: leaves straight from cells
+    (1500, 64, 2, 64, 700, 1),      # cells of 0-3 rows, empty cells
+])
+def test_forest_and_init_bitexact(torch_cuda, n, d, T, leaf, cells, parts):
+    say("forest+init", n, d, T, leaf, cells, parts)
     x = synth.gaussian_mixture(n + d, n, d, 24)
     q, sk = oracle.quantize(x)
     K = 50 if n >= 1000 else 20
-    b = _builder(x, num_trees=T, leaf_size=leaf, k_build=K, k_eval=15)
+    b = _builder(x, num_trees=T, leaf_size=leaf, k_build=K, k_eval=15, cells=cells,
+                 cell_partitions=parts)
     b.build_forest()
     perm, leaves = b.forest()
-    o_perm, o_leaves = oracle.forest_device(q, T, leaf, 1337)
+    plan = oracle.cells_device(q, T, leaf, cells, 6, parts)
+    assert b.stats()["cells"] == (0 if plan is None else plan[0])
+    o_perm, o_leaves = oracle.forest_cells_device(q, T, leaf, plan, 1337)
     assert np.array_equal(perm, o_perm)
     assert np.array_equal(leaves, o_leaves)
     b.build_init()
@@ -183,8 +193,26 @@ def test_incremental_full_build_bitexact(torch_cuda):
     b = _builder(x, incremental=1)
     b.build()
     ids, sc = b.graph()
+    assert b.stats()["cells"] == 2   # n // 1536
     assert np.array_equal(ids, o_ids) and np.array_equal(sc, o_sc)
     assert b.stats()["iterations"] == len(o_rates)
+
+
+@pytest.mark.parametrize("cells,parts,T", [(24, 2, 8), (9, 3, 3)])
+def test_cells_full_build_bitexact(torch_cuda, cells, parts, T):
+    """k-means cells seed the forest; the whole build stays bit-exact."""
+    say("cells build", cells, parts, T)
+    x = synth.gaussian_mixture(19 + cells, 3500, 512, 30)
+    q, sk = oracle.quantize(x)
+    o_ids, o_sc, o_rates = oracle.build_device_semantics(q, sk, trees=T, cells=cells,
+                                                         partitions=parts)
+    b = _builder(x, num_trees=T, cells=cells, cell_partitions=parts)
+    b.build()
+    ids, sc = b.graph()
+    st = b.stats()
+    assert st["cells"] == cells and st["forest_levels"] >= 2
+    assert np.array_equal(ids, o_ids) and np.array_equal(sc, o_sc)
+    assert st["iterations"] == len(o_rates)
 
 
 @pytest.mark.parametrize("K,mult", [(64, 8), (10, 3)])

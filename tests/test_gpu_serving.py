@@ -37,8 +37,10 @@ def test_search_int8_bitexact_vs_oracle(torch_cuda):
     qq, _ = oracle.quantize(queries)
     with knng.Server(b, beam=32, k=10, rerank=0, entry_trees=4, max_batch=64, max_slots=3) as s:
         g_ids, g_sc = s.search(queries)
+    plan = oracle.cells_device(q, 4, 32)   # n = 4000: two cells, two partitions
+    assert plan is not None and b.stats()["cells"] == plan[0]
     o_ids, o_sc = oracle.search_device(q, ids, qq, T=4, leaf=32, entry_trees=4, beam=32, k=10,
-                                       max_steps=128)
+                                       max_steps=128, plan=plan)
     assert np.array_equal(g_ids, o_ids)
     assert np.array_equal(g_sc, o_sc.astype(np.float32))
 

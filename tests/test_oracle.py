@@ -135,12 +135,18 @@ def test_pivot_hash():
         assert oracle.pivots(*args) == _pivots(*args)
 
 
-def _forest_bruteforce(q, T, leaf, seed):
+def _forest_bruteforce(q, T, leaf, seed, cell_assign=None):
+    """Recursive two-pivot forest; with cell_assign ([G, n]) tree t starts
+    from partition t % G's cells (rows by (cell, id)) at depth 1."""
     Q = q.astype(np.int64)
     n = len(q)
     perms, leaves = [], []
     for t in range(T):
-        perm = list(range(n))
+        if cell_assign is None:
+            perm = list(range(n))
+        else:
+            a = cell_assign[t % len(cell_assign)]
+            perm = sorted(range(n), key=lambda i: (a[i], i))
 
         def rec(start, size, depth):
             if size <= leaf or depth >= 64:   # depth cap: cut into leaves of <= leaf
@@ -156,7 +162,15 @@ def _forest_bruteforce(q, T, leaf, seed):
             rec(start, len(lo), depth + 1)
             rec(start + len(lo), size - len(lo), depth + 1)
 
-        rec(0, n, 0)
+        if cell_assign is None:
+            rec(0, n, 0)
+        else:
+            sizes = np.bincount(a, minlength=int(a.max()) + 1)
+            start = 0
+            for z in sizes:
+                if z:
+                    rec(start, int(z), 1)
+                start += int(z)
         perms.append(perm)
     return np.array(perms, np.int32), np.array(sorted(leaves), np.int64)
 
@@ -191,6 +205,109 @@ def test_forest_device_duplicate_rows_terminate():
     bp, bl = _forest_bruteforce(q, 2, 8, 5)
     assert np.array_equal(perm, bp) and np.array_equal(leaves, bl)
     assert leaves[:, 2].max() <= 8 and leaves[:, 2].sum() == 2 * 300
+
+
+# ------------------------------------------------------------------ cells
+
+def _assign_np(q, z, w):
+    """argmax_c float32(<q, z_c>) * w_c with the first maximum, in numpy."""
+    dots = (q.astype(np.int64) @ z.astype(np.int64).T).astype(np.float32)
+    return np.argmax(dots * w[None, :].astype(np.float32), axis=1).astype(np.int32)
+
+
+def test_assign_device_bruteforce():
+    rng = np.random.default_rng(5)
+    q = rng.integers(-128, 128, size=(500, 192), dtype=np.int8)
+    z = rng.integers(-127, 128, size=(37, 192), dtype=np.int8)
+    w = (1.0 / np.sqrt((z.astype(np.float64) ** 2).sum(1))).astype(np.float32)
+    assert np.array_equal(oracle.assign_device(q, z, w), _assign_np(q, z, w))
+    # exact ties (duplicate centroids) go to the smaller cell
+    z[20] = z[3]
+    w[20] = w[3]
+    a = oracle.assign_device(q, z, w)
+    assert not np.any(a == 20)
+    assert np.array_equal(a, _assign_np(q, z, w))
+    # all-zero centroids score 0 and never beat a positive score
+    z[:10] = 0
+    w[:10] = 0
+    assert np.array_equal(oracle.assign_device(q, z, w), _assign_np(q, z, w))
+
+
+def test_kmeans_seeds_are_quantised_rows():
+    """iters = 0: centroid c is row ((2c+1)n/(2C) + 7919 g) mod n scaled so its
+    largest component is +-127 (truncating), and the rows go to their argmax."""
+    x, q, sk = _small(n=400, d=64)
+    for g in (0, 1):
+        z, w, a = oracle.kmeans_device(q, 9, g, iters=0)
+        for c in range(9):
+            row = q[((2 * c + 1) * 400 // 18 + 7919 * g) % 400].astype(np.int64)
+            M = np.abs(row).max()
+            exp = np.sign(row) * ((np.abs(row) * 127) // M)
+            assert np.array_equal(z[c].astype(np.int64), exp)
+            assert np.abs(z[c]).max() == 127
+            assert w[c] == np.float32(1.0 / np.sqrt((exp.astype(np.float64) ** 2).sum()))
+        assert np.array_equal(a, _assign_np(q, z, w))
+
+
+def test_kmeans_recovers_separated_clusters():
+    """Eight sign-pattern clusters with small noise, rows ordered by cluster so
+    each seed lands in its own cluster: the cells are exactly the clusters."""
+    rng = np.random.default_rng(11)
+    C, per, d = 8, 60, 128
+    centers = rng.choice([-1.0, 1.0], size=(C, d)).astype(np.float32) * 0.5
+    x = np.repeat(centers, per, axis=0) + rng.normal(0, 0.05, size=(C * per, d)).astype(np.float32)
+    q, sk = oracle.quantize(x.astype(np.float32))
+    z, w, a = oracle.kmeans_device(q, C, 0, iters=4)
+    assert np.array_equal(a, np.repeat(np.arange(C, dtype=np.int32), per))
+
+
+def test_kmeans_update_is_requantised_cell_sum():
+    """Centroids after `it` rounds are the requantised exact sums of the cells
+    the (it-1)-round run ends with; empty cells keep their previous centroid."""
+    x, q, sk = _small(seed=8, n=600, d=128, clusters=10)
+    C = 16
+    for it in (1, 3):
+        _, _, a_prev = oracle.kmeans_device(q, C, 1, iters=it - 1)
+        z_prev, w_prev, _ = oracle.kmeans_device(q, C, 1, iters=it - 1)
+        z, w, a = oracle.kmeans_device(q, C, 1, iters=it)
+        for c in range(C):
+            m = a_prev == c
+            if not m.any():
+                assert np.array_equal(z[c], z_prev[c]) and w[c] == w_prev[c]
+                continue
+            S = q[m].astype(np.int64).sum(0)
+            M = np.abs(S).max()
+            assert np.array_equal(z[c].astype(np.int64), np.sign(S) * ((np.abs(S) * 127) // M))
+        assert np.array_equal(a, _assign_np(q, z, w))
+
+
+def test_forest_cells_bruteforce():
+    x, q, sk = _small(seed=4, n=500, d=128, clusters=8)
+    for C, G, T, leaf in [(6, 2, 3, 16), (40, 3, 4, 8), (250, 1, 2, 4)]:
+        A = np.stack([oracle.kmeans_device(q, C, g, iters=3)[2] for g in range(G)])
+        plan = (C, G, None, None, A)
+        perm, leaves = oracle.forest_cells_device(q, T, leaf, plan, seed=1337)
+        bp, bl = _forest_bruteforce(q, T, leaf, 1337, cell_assign=A)
+        assert np.array_equal(perm, bp) and np.array_equal(leaves, bl)
+        for t in range(T):
+            # each cell is one contiguous run of the tree's order, cells ascending
+            cell_of = A[t % G][perm[t]]
+            assert np.all(np.diff(cell_of) >= 0)
+            lt = leaves[leaves[:, 0] == t]
+            assert lt[:, 2].sum() == 500 and lt[:, 2].max() <= leaf
+            for _, s0, m in lt:   # no leaf straddles two cells
+                assert len(set(cell_of[s0:s0 + m].tolist())) == 1
+
+
+def test_cells_plan_rule():
+    assert oracle.cells_for(3071) == 0 and oracle.cells_for(3072) == 2
+    assert oracle.cells_for(6_350_000) == 4134 and oracle.cells_for(10, 5) == 5
+    x, q, sk = _small(n=300)
+    assert oracle.cells_device(q, 4, 64) is None          # auto: too few rows
+    C, G, z, w, A = oracle.cells_device(q, 4, 64, cells=5, partitions=3)
+    assert (C, G) == (5, 3) and A.shape == (3, 300) and z.shape == (3, 5, 128)
+    assert oracle.cells_device(q, 4, 400, cells=5) is None   # the whole set is one leaf
+    assert oracle.cells_device(q, 1, 64, cells=5, partitions=2)[1] == 1   # G <= trees
 
 
 def _topk_unique(cands, K):
