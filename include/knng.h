@@ -38,7 +38,8 @@ typedef struct knng_params {
   int32_t k_build;         /* K_BUILD = 50 (<= 64) */
   int32_t num_trees;       /* NUM_TREES = 16 (<= 64) */
   int32_t rerank_mult;     /* RERANK_MULTIPLIER = 8: sketch prefilter keeps k_build*8 */
-  int32_t leaf_size;       /* LEAF_SIZE = 64 (2..64) */
+  int32_t leaf_size;       /* LEAF_SIZE = 64 (2..2048; leaves above 64 rows keep each
+                              member's 64 best mates) */
   int32_t max_iters;       /* MAX_ITERATIONS = 30 */
   float early_stop_rate;   /* EARLY_STOP_RATE = 0.04 */
   float rate_diff_tol;     /* RATE_DIFF_TOLERANCE = 0.01 */
@@ -49,10 +50,17 @@ typedef struct knng_params {
   float insert_stop_rate;  /* KNNG_STOP_INSERTED: stop once fewer than this share of slots got a new id */
   int32_t incremental;     /* 1: a 2-hop pair is a candidate only if one of its edges is new
                               since the previous iteration (NN-Descent's new/old rule) */
-  int32_t cells;           /* k-means cells that seed every tree (0: plain RP trees over all
-                              rows, as the reference; -1: n / 1536) */
-  int32_t kmeans_iters;    /* Lloyd iterations per cell partition */
+  int32_t cells;           /* target cell count of the k-means partition that seeds every
+                              tree (0: plain RP trees over all rows, as the reference;
+                              -1: n / 1536) */
+  int32_t kmeans_iters;    /* Lloyd iterations on the row sample */
   int32_t cell_partitions; /* independently seeded partitions; tree t uses t % partitions */
+  int32_t cell_oversample; /* seeds = cells * cell_oversample before merging */
+  int32_t cell_sample;     /* Lloyd and merging run on every cell_sample-th row */
+  float cell_merge;        /* merge mutually nearest cells whose centroid cosine exceeds this */
+  int32_t cell_full_iters; /* Lloyd iterations over all rows after merging */
+  float cell_prune;        /* after merging, drop cells holding fewer sample rows than
+                              cell_prune * the median cell (their rows go to the others) */
   int32_t pad0;
 } knng_params;
 
@@ -79,7 +87,7 @@ typedef struct knng_stats {
   float ms_export;
   float ms_build;              /* forest + init + join, the reference's buildtime span */
   float ms_kmeans;             /* cell k-means (part of ms_forest) */
-  int32_t cells;               /* cells per partition (0: none) */
+  int32_t cells;               /* cells over all partitions after merging (0: none) */
   double join_bytes[32];       /* algorithmic HBM bytes per join iteration */
   int64_t join_candidates[32]; /* unique 2-hop candidates scored by sketch */
   int64_t join_rescored[32];   /* candidates re-scored with int8 dots */
@@ -142,6 +150,12 @@ int knng_copy_quantized(knng_ctx* ctx, int8_t* q, uint64_t* sketch);   /* n*dim,
  * sorted by (tree,start); *nleaf receives the count (<= cap). */
 int knng_copy_forest(knng_ctx* ctx, int32_t* perm, int64_t* leaves, int64_t cap, int64_t* nleaf);
 int knng_copy_graph(knng_ctx* ctx, int32_t* ids, int32_t* scores);      /* n*k_build each */
+/* Cells of partition `part` after knng_build_forest: *count cells, assign[n]
+ * (cell of every row), z[count*dim] int8 centroid directions, w[count] = 1/|z|.
+ * Any output may be NULL (count alone sizes the buffers); *count = 0 when the
+ * build used no cells. */
+int knng_copy_cells(knng_ctx* ctx, int32_t part, int32_t* count, int32_t* assign, int8_t* z,
+                    float* w);
 int knng_load_graph(knng_ctx* ctx, const int32_t* ids, const int32_t* scores);
 
 /* Stand-alone primitives on device buffers (synchronous, device 0 stream):

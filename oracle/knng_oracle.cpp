@@ -370,10 +370,12 @@ inline uint64_t node_key(int32_t tree, int32_t depth, int64_t start) {
 }
 
 // Cell seeding of the forest: `assign` holds G partitions' cell of every row
-// ([G][n], cells 0..C-1); tree t starts from partition t % G.
+// ([G][n]; partition g has cells 0..counts[g]-1); tree t starts from
+// partition t % G.
 struct CellSeed {
   const int32_t* assign;
-  int32_t C, G;
+  const int32_t* counts;
+  int32_t G;
 };
 
 // stable counting sort of rows by cell; offs has C + 1 entries
@@ -393,14 +395,15 @@ int64_t forest_impl(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t le
   std::vector<int32_t> lo, hi;
   struct Node { int64_t start, size; int32_t depth; };
   // cells replace the root split (as the library does) unless the whole set is one leaf
-  const bool cells = cs && cs->C >= 2 && n > leaf_size;
+  const bool cells = cs && n > leaf_size;
   for (int t = 0; t < T; ++t) {
     int32_t* P = perm + int64_t(t) * n;
     std::vector<Node> stack;
     if (cells) {
       std::vector<int64_t> offs;
-      cell_order(cs->assign + int64_t(t % cs->G) * n, n, cs->C, P, &offs);
-      for (int32_t c = cs->C - 1; c >= 0; --c)
+      const int32_t C = cs->counts[t % cs->G];
+      cell_order(cs->assign + int64_t(t % cs->G) * n, n, C, P, &offs);
+      for (int32_t c = C - 1; c >= 0; --c)
         stack.push_back(Node{offs[size_t(c)], offs[size_t(c) + 1] - offs[size_t(c)], 1});
     } else {
       for (int64_t i = 0; i < n; ++i) P[i] = int32_t(i);
@@ -466,21 +469,37 @@ void quantize_centroid(const int64_t* S, int32_t d, int8_t* z, float* w) {
   *w = float(1.0 / std::sqrt(ss));
 }
 
-// cell of every row: argmax_c float(<q_i, z_c>) * w_c, the first (smallest c) maximum
-void assign_rows(const int8_t* q, int64_t n, int32_t d, const int8_t* z, const float* w, int32_t C,
-                 int32_t* out) {
-  for (int64_t i = 0; i < n; ++i) {
-    const int8_t* v = q + i * d;
+// cell of rows off + j * stride: argmax_c float(<q_r, z_c>) * w_c, the first
+// (smallest c) maximum; excl skips c == j; score[j] (if given) = the maximum
+void assign_rows(const int8_t* q, int64_t nrows, int64_t off, int64_t stride, int32_t d,
+                 const int8_t* z, const float* w, int32_t C, bool excl, int32_t* out, float* score) {
+  for (int64_t j = 0; j < nrows; ++j) {
+    const int8_t* v = q + (off + j * stride) * d;
     float best = -std::numeric_limits<float>::infinity();
     int32_t bi = std::numeric_limits<int32_t>::max();
     for (int32_t c = 0; c < C; ++c) {
+      if (excl && c == j) continue;
       const float s = float(dot_i8(v, z + int64_t(c) * d, d)) * w[c];
       if (s > best) {
         best = s;
         bi = c;
       }
     }
-    out[i] = bi;
+    out[j] = bi;
+    if (score) score[j] = best;
+  }
+}
+
+// exact member sums per cell of rows off + j * stride with cells a[j]
+void cell_sums(const int8_t* q, int32_t d, const int32_t* a, int64_t nrows, int64_t off, int64_t stride,
+               int32_t C, std::vector<int64_t>* S, std::vector<int64_t>* cnt) {
+  S->assign(size_t(C) * d, 0);
+  cnt->assign(size_t(C), 0);
+  for (int64_t j = 0; j < nrows; ++j) {
+    int64_t* s = &(*S)[size_t(a[j]) * d];
+    const int8_t* v = q + (off + j * stride) * d;
+    for (int32_t k = 0; k < d; ++k) s[k] += v[k];
+    ++(*cnt)[size_t(a[j])];
   }
 }
 
@@ -496,53 +515,128 @@ int64_t orc_forest_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int3
 int32_t orc_assign_device(const int8_t* q, int64_t n, int32_t d, const int8_t* z, const float* w,
                           int32_t C, int32_t* out) {
   if (C < 1) return -1;
-  assign_rows(q, n, d, z, w, C, out);
+  assign_rows(q, n, 0, 1, d, z, w, C, false, out, nullptr);
   return 0;
 }
 
-// Spherical k-means on the int8 rows (partition g): seeds are rows
-// ((2c + 1) n / (2C) + 7919 g) mod n; then `iters` rounds of (assign, exact
-// integer member sums, requantised centroid; an empty cell keeps its
-// centroid) and a final assignment with the last centroids.
-int32_t orc_kmeans_device(const int8_t* q, int64_t n, int32_t d, int32_t C, int32_t g,
-                          int32_t iters, int8_t* z, float* w, int32_t* assign) {
-  if (C < 1 || n < 1) return -1;
-  std::vector<int64_t> S(size_t(C) * d);
-  for (int32_t c = 0; c < C; ++c) {
-    const int64_t id = ((2 * int64_t(c) + 1) * n / (2 * int64_t(C)) + int64_t(g) * 7919) % n;
-    for (int32_t k = 0; k < d; ++k) S[size_t(c) * d + k] = q[id * d + k];
-    quantize_centroid(&S[size_t(c) * d], d, z + int64_t(c) * d, w + c);
-  }
-  std::vector<int64_t> cnt(C);
-  for (int32_t it = 0;; ++it) {
-    assign_rows(q, n, d, z, w, C, assign);
-    if (it == iters) break;
-    std::fill(S.begin(), S.end(), 0);
-    std::fill(cnt.begin(), cnt.end(), 0);
-    for (int64_t i = 0; i < n; ++i) {
-      int64_t* s = &S[size_t(assign[i]) * d];
-      for (int32_t k = 0; k < d; ++k) s[k] += q[i * d + k];
-      ++cnt[size_t(assign[i])];
+// Cells of partition g (device semantics of knng.hip's run_kmeans):
+//  seeds: Cs = min(Ct * over, ns) rows ((2c + 1) n / (2 Cs) + 7919 g) mod n,
+//    ns = sample rows off + j * sample, off = g mod sample;
+//  iters Lloyd rounds on the sample (an empty cell keeps its centroid), a
+//    last sample assignment and its exact sums;
+//  merging: live (non-empty) cells, centroids requantised from their sums;
+//    per round each live cell j finds k = argmax_{k != j} float(<z_j, z_k>) *
+//    w_k (value b_j); mutual pairs j < k with b_j * w_j > merge fold k into j
+//    (sums and counts added, centroid requantised); rounds until none merges
+//    (<= 64);
+//  pruning: cells with float(count) < prune * float(upper-median count) go;
+//  full_iters Lloyd rounds over all rows with the live cells in index order,
+//    then the final assignment.
+// Returns the cell count C_g (z / w hold C_g centroids; room for Cs needed).
+int32_t orc_kmeans_device(const int8_t* q, int64_t n, int32_t d, int32_t Ct, int32_t g,
+                          int32_t iters, int32_t over, int32_t sample, float merge,
+                          int32_t full_iters, float prune, int8_t* zf, float* wf,
+                          int32_t* assign) {
+  if (Ct < 1 || n < 1 || over < 1 || sample < 1) return -1;
+  const int64_t stride = sample;
+  const int64_t off = int64_t(g) % stride < n ? int64_t(g) % stride : 0;
+  const int64_t ns = (n - off + stride - 1) / stride;
+  const int32_t Cs = int32_t(std::max<int64_t>(1, std::min<int64_t>(int64_t(Ct) * over, ns)));
+  std::vector<int8_t> z(size_t(Cs) * d);
+  std::vector<float> w(static_cast<size_t>(Cs));
+  {
+    std::vector<int64_t> S(static_cast<size_t>(d));
+    for (int32_t c = 0; c < Cs; ++c) {
+      const int64_t id = ((2 * int64_t(c) + 1) * n / (2 * int64_t(Cs)) + int64_t(g) * 7919) % n;
+      for (int32_t k = 0; k < d; ++k) S[size_t(k)] = q[id * d + k];
+      quantize_centroid(S.data(), d, &z[size_t(c) * d], &w[size_t(c)]);
     }
-    for (int32_t c = 0; c < C; ++c)
-      if (cnt[size_t(c)]) quantize_centroid(&S[size_t(c) * d], d, z + int64_t(c) * d, w + c);
   }
-  return 0;
+  std::vector<int32_t> a(static_cast<size_t>(ns));
+  std::vector<int64_t> S, cnt;
+  for (int32_t it = 0;; ++it) {
+    assign_rows(q, ns, off, stride, d, z.data(), w.data(), Cs, false, a.data(), nullptr);
+    cell_sums(q, d, a.data(), ns, off, stride, Cs, &S, &cnt);
+    if (it == iters) break;
+    for (int32_t c = 0; c < Cs; ++c)
+      if (cnt[size_t(c)]) quantize_centroid(&S[size_t(c) * d], d, &z[size_t(c) * d], &w[size_t(c)]);
+  }
+  std::vector<int32_t> live;
+  for (int32_t c = 0; c < Cs; ++c)
+    if (cnt[size_t(c)]) {
+      live.push_back(c);
+      quantize_centroid(&S[size_t(c) * d], d, &z[size_t(c) * d], &w[size_t(c)]);
+    }
+  for (int round = 0; round < 64 && live.size() >= 2; ++round) {
+    const int32_t m = int32_t(live.size());
+    std::vector<int8_t> zl(size_t(m) * d);
+    std::vector<float> wl(static_cast<size_t>(m)), bval(static_cast<size_t>(m));
+    std::vector<int32_t> best(static_cast<size_t>(m));
+    for (int32_t j = 0; j < m; ++j) {
+      std::copy_n(&z[size_t(live[size_t(j)]) * d], d, &zl[size_t(j) * d]);
+      wl[size_t(j)] = w[size_t(live[size_t(j)])];
+    }
+    assign_rows(zl.data(), m, 0, 1, d, zl.data(), wl.data(), m, true, best.data(), bval.data());
+    std::vector<char> gone(size_t(m), 0);
+    int merged = 0;
+    for (int32_t j = 0; j < m; ++j) {
+      const int32_t k = best[size_t(j)];
+      if (k <= j || k >= m || best[size_t(k)] != j) continue;
+      if (!(bval[size_t(j)] * wl[size_t(j)] > merge)) continue;
+      int64_t* sj = &S[size_t(live[size_t(j)]) * d];
+      const int64_t* sk = &S[size_t(live[size_t(k)]) * d];
+      for (int32_t t = 0; t < d; ++t) sj[t] += sk[t];
+      cnt[size_t(live[size_t(j)])] += cnt[size_t(live[size_t(k)])];
+      quantize_centroid(sj, d, &z[size_t(live[size_t(j)]) * d], &w[size_t(live[size_t(j)])]);
+      gone[size_t(k)] = 1;
+      ++merged;
+    }
+    if (!merged) break;
+    std::vector<int32_t> next;
+    for (int32_t j = 0; j < m; ++j)
+      if (!gone[size_t(j)]) next.push_back(live[size_t(j)]);
+    live.swap(next);
+  }
+  {
+    std::vector<int64_t> sz;
+    for (int32_t k : live) sz.push_back(cnt[size_t(k)]);
+    std::sort(sz.begin(), sz.end());
+    const float cut = prune * float(sz[sz.size() / 2]);
+    std::vector<int32_t> keep;
+    for (int32_t k : live)
+      if (!(float(cnt[size_t(k)]) < cut)) keep.push_back(k);
+    live.swap(keep);
+  }
+  const int32_t C = int32_t(live.size());
+  for (int32_t j = 0; j < C; ++j) {
+    std::copy_n(&z[size_t(live[size_t(j)]) * d], d, zf + int64_t(j) * d);
+    wf[j] = w[size_t(live[size_t(j)])];
+  }
+  for (int32_t it = 0;; ++it) {
+    assign_rows(q, n, 0, 1, d, zf, wf, C, false, assign, nullptr);
+    if (it == full_iters) break;
+    cell_sums(q, d, assign, n, 0, 1, C, &S, &cnt);
+    for (int32_t c = 0; c < C; ++c)
+      if (cnt[size_t(c)]) quantize_centroid(&S[size_t(c) * d], d, zf + int64_t(c) * d, wf + c);
+  }
+  return C;
 }
 
-// Forest whose trees start from cells: assign = [G][n] cells of the rows.
+// Forest whose trees start from cells: assign = [G][n] cells of the rows,
+// counts = cells per partition.
 int64_t orc_forest_cells_device(const int8_t* q, int64_t n, int32_t d, int32_t T, int32_t leaf_size,
-                                uint64_t seed, const int32_t* assign, int32_t C, int32_t G,
-                                int32_t* perm, int64_t* leaf_tri, int64_t leaf_cap) {
+                                uint64_t seed, const int32_t* assign, const int32_t* counts,
+                                int32_t G, int32_t* perm, int64_t* leaf_tri, int64_t leaf_cap) {
   if (G < 1) return -1;
-  const CellSeed cs{assign, C, G};
+  const CellSeed cs{assign, counts, G};
   return forest_impl(q, n, d, T, leaf_size, seed, perm, leaf_tri, leaf_cap, nullptr, &cs);
 }
 
 // Beam search over a K-list graph (device semantics of the serving kernel):
 // entry points = members of the leaf reached by descending each of the first
 // min(entry_trees, T, 4) trees (left iff <q,p1> - <q,p2> > 0; a depth-capped
-// node enters its first <= leaf_size rows); then, per step, merge the scored
+// node enters its first <= leaf_size rows, and any leaf at most its first 64;
+// with cells, tree t starts at the query's cell of partition t % G); then, per step, merge the scored
 // candidates into the beam (best `beam` by (score desc, id asc), expanded
 // flags kept), expand the first 4 unexpanded beam entries in beam order (at
 // most max_steps times), taking their neighbours that are not in the visited
@@ -552,18 +646,18 @@ int64_t orc_forest_cells_device(const int8_t* q, int64_t n, int32_t d, int32_t T
 int orc_search_device(const int8_t* q, int64_t n, int32_t d, const int32_t* gid, int32_t K,
                       int32_t T, int32_t leaf_size, uint64_t seed, int32_t entry_trees,
                       int32_t beam, int32_t k, int32_t max_steps, const int8_t* qq, int64_t nq,
-                      const int32_t* assign, int32_t C, int32_t G, const int32_t* qcell,
-                      int32_t* out_ids, int32_t* out_sc) {
+                      const int32_t* assign, const int32_t* counts, int32_t G,
+                      const int32_t* qcell, int32_t* out_ids, int32_t* out_sc) {
   std::vector<int32_t> perm(size_t(T) * n);
   std::unordered_map<uint64_t, Split> rec;
-  const CellSeed cs{assign, C, G};
-  const bool cells = assign && C >= 2 && G >= 1 && n > leaf_size;
+  const CellSeed cs{assign, counts, G};
+  const bool cells = assign && G >= 1 && n > leaf_size;
   if (forest_impl(q, n, d, T, leaf_size, seed, perm.data(), nullptr, 0, &rec, cells ? &cs : nullptr) < 0)
     return -1;
   std::vector<std::vector<int64_t>> coffs(cells ? G : 0);
   for (int32_t g = 0; g < int32_t(coffs.size()); ++g) {
     std::vector<int32_t> rows(n);
-    cell_order(assign + int64_t(g) * n, n, C, rows.data(), &coffs[g]);
+    cell_order(assign + int64_t(g) * n, n, counts[g], rows.data(), &coffs[g]);
   }
   const int et = std::min(std::min(entry_trees, T), 4);
   constexpr int kExpand = 4;
@@ -594,7 +688,7 @@ int orc_search_device(const int8_t* q, int64_t n, int32_t d, const int32_t* gid,
         }
         ++depth;
       }
-      const int64_t m = std::min<int64_t>(size, leaf_size);
+      const int64_t m = std::min<int64_t>(std::min<int64_t>(size, leaf_size), 64);
       for (int64_t j = 0; j < m; ++j) {
         const int32_t id = perm[size_t(t) * n + start + j];
         if (vis.insert(id).second) {

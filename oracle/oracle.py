@@ -56,10 +56,11 @@ def lib() -> C.CDLL:
             "orc_join_device": ([V, V, i64, i32, i32, i32, V, V, V, V, C.POINTER(i64), V, V], i64),
             "orc_rerank_device": ([V, i64, i32, i32, i32, i32, V, V, V], i32),
             "orc_search_device": ([V, i64, i32, V, i32, i32, i32, u64, i32, i32, i32, i32, V, i64,
-                                   V, i32, i32, V, V, V], i32),
+                                   V, V, i32, V, V, V], i32),
             "orc_assign_device": ([V, i64, i32, V, V, i32, V], i32),
-            "orc_kmeans_device": ([V, i64, i32, i32, i32, i32, V, V, V], i32),
-            "orc_forest_cells_device": ([V, i64, i32, i32, i32, u64, V, i32, i32, V, V, i64], i64),
+            "orc_kmeans_device": ([V, i64, i32, i32, i32, i32, i32, i32, C.c_float, i32,
+                                   C.c_float, V, V, V], i32),
+            "orc_forest_cells_device": ([V, i64, i32, i32, i32, u64, V, V, i32, V, V, i64], i64),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -156,41 +157,51 @@ def assign_device(q: np.ndarray, z: np.ndarray, w: np.ndarray) -> np.ndarray:
     return out
 
 
-def kmeans_device(q: np.ndarray, C: int, g: int = 0, iters: int = 6):
-    """Spherical int8 k-means of partition g -> (z [C, d] int8, w [C] f32, assign [n] int32)."""
+def kmeans_device(q: np.ndarray, Ct: int, g: int = 0, iters: int = 2, over: int = 4,
+                  sample: int = 8, merge: float = 0.6, full_iters: int = 1, prune: float = 0.25):
+    """Cells of partition g (over-seeded spherical int8 k-means with merging)
+    -> (z [C, d] int8, w [C] f32, assign [n] int32)."""
     n, d = q.shape
-    z = np.empty((C, d), np.int8)
-    w = np.empty(C, np.float32)
+    off = g % sample if g % sample < n else 0
+    ns = (n - off + sample - 1) // sample
+    cap = max(1, min(Ct * over, ns))
+    z = np.empty((cap, d), np.int8)
+    w = np.empty(cap, np.float32)
     a = np.empty(n, np.int32)
-    assert lib().orc_kmeans_device(_p(np.ascontiguousarray(q)), n, d, C, g, iters, _p(z), _p(w),
-                                   _p(a)) == 0
-    return z, w, a
+    C = lib().orc_kmeans_device(_p(np.ascontiguousarray(q)), n, d, Ct, g, iters, over, sample,
+                                merge, full_iters, prune, _p(z), _p(w), _p(a))
+    assert C >= 1
+    return z[:C].copy(), w[:C].copy(), a
 
 
-def cells_device(q: np.ndarray, trees: int, leaf: int, cells: int = -1, iters: int = 6,
-                 partitions: int = 2):
+def cells_device(q: np.ndarray, trees: int, leaf: int, cells: int = -1, iters: int = 2,
+                 partitions: int = 2, over: int = 4, sample: int = 8, merge: float = 0.6,
+                 full_iters: int = 1, prune: float = 0.25):
     """The library's cell plan for a build: None when no cells are used, else
-    (C, G, z [G, C, d], w [G, C], assign [G, n])."""
+    (counts [G], G, [z_g], [w_g], assign [G, n])."""
     n = q.shape[0]
-    C = min(cells_for(n, cells), n)
-    if C < 2 or n <= leaf:
+    Ct = min(cells_for(n, cells), n)
+    if Ct < 2 or n <= leaf:
         return None
     G = max(1, min(partitions, trees))
-    zs, ws, As = zip(*(kmeans_device(q, C, g, iters) for g in range(G)))
-    return C, G, np.stack(zs), np.stack(ws), np.stack(As)
+    zs, ws, As = zip(*(kmeans_device(q, Ct, g, iters, over, sample, merge, full_iters, prune)
+                       for g in range(G)))
+    counts = np.array([len(w) for w in ws], np.int32)
+    return counts, G, list(zs), list(ws), np.stack(As)
 
 
 def forest_cells_device(q: np.ndarray, trees: int, leaf: int, plan, seed: int = 1337):
     """Forest seeded by a cell plan (from cells_device); plan None = plain forest."""
     if plan is None:
         return forest_device(q, trees, leaf, seed)
-    C, G, _, _, A = plan
+    counts, G, _, _, A = plan
     n, d = q.shape
     perm = np.empty((trees, n), np.int32)
     cap = trees * n + 16
     leaves = np.empty((cap, 3), np.int64)
     nl = lib().orc_forest_cells_device(_p(q), n, d, trees, leaf, seed,
-                                       _p(np.ascontiguousarray(A, np.int32)), C, G, _p(perm),
+                                       _p(np.ascontiguousarray(A, np.int32)),
+                                       _p(np.ascontiguousarray(counts, np.int32)), G, _p(perm),
                                        _p(leaves), cap)
     assert nl >= 0
     return perm, leaves[:nl].copy()
@@ -250,14 +261,16 @@ STOP_REFERENCE, STOP_INSERTED, STOP_NONE = 0, 1, 2
 
 def build_device_semantics(q, sk, trees=16, leaf=64, K=50, mult=8, max_iters=30,
                            stop=0.04, tol=0.01, seed=1337, stop_rule=STOP_INSERTED,
-                           insert_stop=0.01, incremental=False, cells=-1, kmeans_iters=6,
-                           partitions=2):
+                           insert_stop=0.01, incremental=False, cells=-1, kmeans_iters=2,
+                           partitions=2, over=4, sample=8, merge=0.6, full_iters=1,
+                           prune=0.25):
     """Full device-mode build.  Stop rules: STOP_REFERENCE is src/main.cpp:432-439
     on the rate of slots whose id changed; STOP_INSERTED stops once fewer than
     insert_stop of all slots received a new id; STOP_NONE runs max_iters.
     cells / kmeans_iters / partitions select the cell seeding of the forest
     with the library's rule (cells=-1: n // 1536 when >= 2, 0: none)."""
-    plan = cells_device(q, trees, leaf, cells, kmeans_iters, partitions)
+    plan = cells_device(q, trees, leaf, cells, kmeans_iters, partitions, over, sample, merge,
+                        full_iters, prune)
     perm, leaves = forest_cells_device(q, trees, leaf, plan, seed)
     ids, sc = init_device(q, perm, leaves, K)
     n = q.shape[0]
@@ -307,15 +320,17 @@ def search_device(q, gid, queries_q, T=16, leaf=64, seed=1337, entry_trees=4, be
     nq = queries_q.shape[0]
     ids = np.empty((nq, k), np.int32)
     sc = np.empty((nq, k), np.int32)
-    A = qc = None
-    C = G = 0
+    A = qc = counts = None
+    G = 0
     if plan is not None:
-        C, G, Z, W, A = plan
+        counts, G, Z, W, A = plan
         qc = np.ascontiguousarray(np.stack([assign_device(queries_q, Z[g], W[g]) for g in range(G)]))
         A = np.ascontiguousarray(A, np.int32)
+        counts = np.ascontiguousarray(counts, np.int32)
     rc = lib().orc_search_device(_p(np.ascontiguousarray(q)), n, d, _p(np.ascontiguousarray(gid)), K,
                                  T, leaf, seed, entry_trees, beam, k, max_steps,
-                                 _p(queries_q), nq, None if A is None else _p(A), C, G,
+                                 _p(queries_q), nq, None if A is None else _p(A),
+                                 None if counts is None else _p(counts), G,
                                  None if qc is None else _p(qc), _p(ids), _p(sc))
     assert rc == 0
     return ids, sc

@@ -324,21 +324,26 @@ __global__ void k_hier_append(const Node* __restrict__ nodes, int nnodes, Node* 
 
 // ---------------------------------------------------------------------------
 // Cells (spherical k-means on the int8 rows), used to seed the forest.
-// k_assign: a[i] = argmax_c float(<q_i, Z_c>) * w[c], ties to the smaller c.
+// k_assign: a[j] = argmax_c float(<q_r, Z_c>) * w[c] for the rows
+// r = row_off + j * row_stride, ties to the smaller c; EXCL skips c == j
+// (rows and centroids are the same set); score[j] (if given) = the best value.
 // One block = 128 rows, wave v owns rows 32v..32v+31 and keeps their A
-// fragments for the whole pass in registers; 64-cell centroid tiles are
+// fragments for the whole pass in registers; 32-cell centroid tiles are
 // staged in LDS and shared by the four waves; each wave runs
-// v_mfma_i32_32x32x32_i8 over the two 32-cell halves of a tile and keeps a
-// running best per accumulator slot, reduced across lanes once at the end.
+// v_mfma_i32_32x32x32_i8 over the tile and keeps a running best per
+// accumulator slot, reduced across lanes once at the end.
 // ---------------------------------------------------------------------------
 
 constexpr int kAssignRows = 128;
 constexpr int kAssignTile = 32;
 
-__global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, int64_t nrows, int d,
+template <bool EXCL>
+__global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, int64_t nrows,
+                                                int64_t row_off, int64_t row_stride, int d,
                                                 const int8_t* __restrict__ Z,
                                                 const float* __restrict__ w, int C,
-                                                int32_t* __restrict__ out) {
+                                                int32_t* __restrict__ out,
+                                                float* __restrict__ score) {
   __shared__ __attribute__((aligned(16))) int8_t ztile[kAssignTile * (kMaxDim + 16)];
   const int stride = d + 16;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -348,12 +353,12 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
     // A fragments: row r0 + 32*wv + (lane&31), bytes 32*ks + 16*(lane>>5)
     const int64_t arow = r0 + 32 * wv + (lane & 31);
     const bool arow_ok = arow < nrows;
+    const int8_t* abase = q + (row_off + arow * row_stride) * d + 16 * (lane >> 5);
     v4i a[32];
 #pragma unroll
     for (int ks = 0; ks < 32; ++ks)
-      a[ks] = (arow_ok && ks < ksteps)
-                  ? *reinterpret_cast<const v4i*>(q + arow * d + 32 * ks + 16 * (lane >> 5))
-                  : v4i{0, 0, 0, 0};
+      a[ks] = (arow_ok && ks < ksteps) ? *reinterpret_cast<const v4i*>(abase + 32 * ks)
+                                       : v4i{0, 0, 0, 0};
     float best[16];
     int32_t bidx[16];
 #pragma unroll
@@ -387,7 +392,9 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const float s = float(acc[j]) * wc;
-          if (ok && s > best[j]) {   // cells arrive in increasing order: ties keep the smaller
+          // acc element j of lane l holds row 8*(j/4) + 4*(l/32) + (j%4) of the wave's 32
+          const bool self = EXCL && (r0 + 32 * wv + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3)) == cell;
+          if (ok && !self && s > best[j]) {   // cells arrive in increasing order: ties keep the smaller
             best[j] = s;
             bidx[j] = cell;
           }
@@ -406,10 +413,26 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
           bidx[j] = oi;
         }
       }
-      // acc element j of lane l holds row 8*(j/4) + 4*(l/32) + (j%4) of the wave's 32
       const int64_t row = r0 + 32 * wv + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-      if ((lane & 31) == 0 && row < nrows) out[row] = bidx[j];
+      if ((lane & 31) == 0 && row < nrows) {
+        out[row] = bidx[j];
+        if (score) score[row] = best[j];
+      }
     }
+  }
+}
+
+// out[j] = row ids[j] (d bytes each)
+__global__ __launch_bounds__(256) void k_gather_rows(const int8_t* __restrict__ q, int d,
+                                                     const int64_t* __restrict__ ids, int64_t m,
+                                                     int8_t* __restrict__ out) {
+  const int per = d >> 4;
+  for (int64_t u = int64_t(blockIdx.x) * 256 + threadIdx.x; u < m * per;
+       u += int64_t(gridDim.x) * 256) {
+    const int64_t j = u / per;
+    const int c = int(u - j * per);
+    *reinterpret_cast<v4i*>(out + j * d + 16 * c) =
+        *reinterpret_cast<const v4i*>(q + ids[j] * d + 16 * c);
   }
 }
 
@@ -477,6 +500,7 @@ __global__ __launch_bounds__(256) void k_leaf_gram(const int8_t* __restrict__ q,
   for (int64_t L = blockIdx.x; L < nleaves; L += gridDim.x) {
     const int4 lf = leaves[L];
     const int t = lf.x, start = lf.y, size = lf.z;
+    if (size > kLeafSlots) continue;   // big leaves: k_leaf_big (uniform per block)
     if (tid < 64) mid[tid] = tid < size ? perm[int64_t(t) * n + start + tid] : -1;
     __syncthreads();
     const bool rows_ok = mq * 32 < size, cols_ok = nq * 32 < size;
@@ -516,6 +540,150 @@ __global__ __launch_bounds__(256) void k_leaf_gram(const int8_t* __restrict__ q,
       }
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K3b: leaves of 65..kMaxLeaf members.  A work item is (leaf, 128 members);
+// wave v owns members r0+32v .. r0+32v+31 and holds their A fragments in
+// registers; the leaf's members stream through LDS as 32-row B tiles shared
+// by the four waves; v_mfma_i32_32x32x32_i8 scores go to the block's scratch
+// rows.  Then each wave keeps, per member, the 64 best mates by (score desc,
+// id asc), self excluded: a bisection finds the 64th score, and a second
+// bisection over ids settles ties at that score.  The keys fill the member's
+// 64 slots for this tree (order irrelevant: K4 sorts them).
+// ---------------------------------------------------------------------------
+
+constexpr int kBigRows = 128;
+
+__global__ __launch_bounds__(256) void k_leaf_big(const int8_t* __restrict__ q, int d, int64_t n, int T,
+                                                  const int32_t* __restrict__ perm,
+                                                  const int4* __restrict__ items, int64_t nitems,
+                                                  int32_t* __restrict__ scratch, int mpad,
+                                                  uint64_t* __restrict__ cand) {
+  __shared__ int32_t mid[kMaxLeaf];
+  __shared__ __attribute__((aligned(16))) int8_t tile[32 * (kMaxDim + 16)];
+  constexpr int kPer = kMaxLeaf / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int stride = d + 16, ksteps = d >> 5, upr = d >> 4;
+  int32_t* scr = scratch + int64_t(blockIdx.x) * kBigRows * mpad;
+  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int4 item = items[it];   // {tree, start, size, first member}
+    const int t = item.x, start = item.y, size = item.z, r0 = item.w;
+    __syncthreads();   // the previous item is done with mid[] and tile[]
+    for (int j = tid; j < size; j += 256) mid[j] = perm[int64_t(t) * n + start + j];
+    __syncthreads();
+    const bool wave_on = r0 + 32 * wv < size;
+    v4i a[32];
+    {
+      const int arow = r0 + 32 * wv + (lane & 31);
+      const bool ok = arow < size;
+      const int8_t* abase = q + int64_t(ok ? mid[arow] : 0) * d + 16 * (lane >> 5);
+#pragma unroll
+      for (int ks = 0; ks < 32; ++ks)
+        a[ks] = (ok && ks < ksteps) ? *reinterpret_cast<const v4i*>(abase + 32 * ks) : v4i{0, 0, 0, 0};
+    }
+    for (int c0 = 0; c0 < size; c0 += 32) {
+      const int nc = (size - c0) < 32 ? (size - c0) : 32;
+      __syncthreads();   // tile free
+      for (int u = tid; u < nc * upr; u += 256) {
+        const int r = u / upr, cc = u - r * upr;
+        *reinterpret_cast<v4i*>(tile + r * stride + 16 * cc) =
+            *reinterpret_cast<const v4i*>(q + int64_t(mid[c0 + r]) * d + 16 * cc);
+      }
+      __syncthreads();
+      if (wave_on) {
+        const int cl = lane & 31;
+        const int8_t* rb = tile + (cl < nc ? cl : 0) * stride + 16 * (lane >> 5);
+        v16i acc;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[j] = 0;
+#pragma unroll
+        for (int ks = 0; ks < 32; ++ks)
+          if (ks < ksteps)
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], *reinterpret_cast<const v4i*>(rb + 32 * ks),
+                                                        acc, 0, 0, 0);
+        if (cl < nc) {
+#pragma unroll
+          for (int j = 0; j < 16; ++j) {
+            // acc element j of lane l: row 8*(j/4) + 4*(l/32) + (j%4) of the wave's 32, column l%32
+            const int rl = 32 * wv + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+            scr[int64_t(rl) * mpad + c0 + cl] = acc[j];
+          }
+        }
+      }
+    }
+    __syncthreads();   // scores written (each wave reads back only its own rows)
+    if (!wave_on) continue;
+    const int R = (size - 1) < kLeafSlots ? (size - 1) : kLeafSlots;
+    for (int rr = 0; rr < 32; ++rr) {
+      const int m = r0 + 32 * wv + rr;
+      if (m >= size) break;
+      const int32_t* row = scr + int64_t(32 * wv + rr) * mpad;
+      int32_t v[kPer];
+      int lmin = INT32_MAX, lmax = INT32_MIN;
+#pragma unroll
+      for (int e = 0; e < kPer; ++e) {
+        const int col = lane + 64 * e;
+        v[e] = (col < size && col != m) ? row[col] : INT32_MIN;   // scores are > -2^25
+        if (v[e] != INT32_MIN) {
+          lmin = min(lmin, v[e]);
+          lmax = max(lmax, v[e]);
+        }
+      }
+      // largest theta with |{v >= theta}| >= R
+      int lo = wave_min_i(lmin), hi = wave_max_i(lmax);
+      while (lo < hi) {
+        const int md = lo + (hi - lo + 1) / 2;
+        int cnt = 0;
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) cnt += v[e] >= md ? 1 : 0;
+        if (wave_sum_i(cnt) >= R)
+          lo = md;
+        else
+          hi = md - 1;
+      }
+      const int theta = lo;
+      int cgt = 0, ceq = 0;
+#pragma unroll
+      for (int e = 0; e < kPer; ++e) {
+        cgt += v[e] > theta ? 1 : 0;
+        ceq += v[e] == theta ? 1 : 0;
+      }
+      const int need = R - wave_sum_i(cgt);
+      int idcut = INT32_MAX;
+      if (wave_sum_i(ceq) > need) {
+        // smallest idcut with |{v == theta, id <= idcut}| >= need
+        int ilo = 0, ihi = int(n - 1);
+        while (ilo < ihi) {
+          const int md = ilo + (ihi - ilo) / 2;
+          int cnt = 0;
+#pragma unroll
+          for (int e = 0; e < kPer; ++e) {
+            const int col = lane + 64 * e;
+            cnt += (v[e] == theta && mid[col < size ? col : 0] <= md) ? 1 : 0;
+          }
+          if (wave_sum_i(cnt) >= need)
+            ihi = md;
+          else
+            ilo = md + 1;
+        }
+        idcut = ilo;
+      }
+      uint64_t* dst = cand + int64_t(mid[m]) * (T * kLeafSlots) + t * kLeafSlots;
+      int base = 0;
+#pragma unroll
+      for (int e = 0; e < kPer; ++e) {
+        const int col = lane + 64 * e;
+        const int32_t id = mid[col < size ? col : 0];
+        const bool sel = v[e] > theta || (v[e] == theta && id <= idcut);
+        const uint64_t bal = __ballot(sel);
+        const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+        if (sel) dst[pos] = pack_key(v[e], id);
+        base += __popcll(bal);
+      }
+      if (lane >= R) dst[lane] = 0ull;
+    }
   }
 }
 
@@ -1082,12 +1250,18 @@ int launch_quantize(hipStream_t stream, const float* dx, int64_t nrows, int d, i
 }
 
 int launch_assign(hipStream_t stream, const int8_t* q, int64_t nrows, int d, const int8_t* Z,
-                  const float* w, int C, int32_t* out, int num_cu) {
+                  const float* w, int C, int32_t* out, int num_cu, int64_t row_off,
+                  int64_t row_stride, bool exclude_self, float* score) {
   if (nrows <= 0) return 0;
   int64_t g = (nrows + kAssignRows - 1) / kAssignRows;
   const int64_t cap = int64_t(num_cu) * 8;
   if (g > cap) g = cap;
-  hipLaunchKernelGGL(k_assign, dim3(g), dim3(256), 0, stream, q, nrows, d, Z, w, C, out);
+  if (exclude_self)
+    hipLaunchKernelGGL(k_assign<true>, dim3(g), dim3(256), 0, stream, q, nrows, row_off, row_stride,
+                       d, Z, w, C, out, score);
+  else
+    hipLaunchKernelGGL(k_assign<false>, dim3(g), dim3(256), 0, stream, q, nrows, row_off,
+                       row_stride, d, Z, w, C, out, score);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1108,8 +1282,24 @@ int quantize_rows(knng_ctx* c, const float* dx, int64_t row0, int64_t nrows) {
 // ---------------------------------------------------------------------------
 // Cells.  A centroid is stored as an int8 direction z (largest component
 // +-127) with w = 1/|z|; a row goes to argmax_c float(<q, z_c>) * w_c.
-// The update is exact integer arithmetic on the host: S_c = sum of the int8
-// member rows, z_k = trunc(S_k * 127 / max_k |S_k|), w = float(1/sqrt(sum z^2)).
+// Centroids come from exact integer sums S on the host:
+// z_k = trunc(S_k * 127 / max_k |S_k|), w = float(1/sqrt(sum z^2)).
+//
+// One partition g:
+//  1. seeds: cells * cell_oversample rows ((2c+1) n / (2C') + 7919 g) mod n,
+//     capped at the sample size;
+//  2. kmeans_iters Lloyd rounds on the sample rows off + j * cell_sample
+//     (off = g mod cell_sample), then a last sample assignment and its sums;
+//  3. merging rounds: every live cell finds its nearest other live cell
+//     (argmax_k float(<z_j, z_k>) * w_k); mutual pairs whose cosine
+//     (that value * w_j) exceeds cell_merge merge into the lower index,
+//     summing their sample sums and counts; repeat until no pair merges;
+//  4. pruning: cells with fewer sample rows than cell_prune * the median
+//     cell's (upper median) are dropped;
+//  5. cell_full_iters Lloyd rounds over all rows, then the final assignment.
+// Over-seeding makes every dense region start with a seed; merging joins
+// the pieces a region was split into; pruning removes the small cells that
+// collect stray rows of many regions.
 // ---------------------------------------------------------------------------
 
 int32_t cells_for(const knng_params& p, int64_t n) {
@@ -1131,114 +1321,272 @@ void quantize_centroid(const int64_t* S, int d, int8_t* z, float* w) {
     z[k] = int8_t((S[k] * 127) / M);
     ss += double(z[k]) * double(z[k]);
   }
-  *w = ss > 0.0 ? float(1.0 / std::sqrt(ss)) : 0.0f;
+  *w = float(1.0 / std::sqrt(ss));
 }
 
-// stable counting sort of rows by cell: members[offs[c]..offs[c+1]) ascending ids
-void cell_lists(const std::vector<int32_t>& a, int C, std::vector<int32_t>* members,
-                std::vector<int64_t>* offs) {
+// stable counting sort of the listed rows by cell: members[offs[c]..offs[c+1])
+// in list order; row j of the list is id off + j * stride
+void cell_lists(const std::vector<int32_t>& a, int C, int64_t off, int64_t stride,
+                std::vector<int32_t>* members, std::vector<int64_t>* offs) {
   offs->assign(size_t(C) + 1, 0);
   for (int32_t c : a) (*offs)[size_t(c) + 1]++;
   for (int c = 0; c < C; ++c) (*offs)[size_t(c) + 1] += (*offs)[size_t(c)];
   std::vector<int64_t> cur(offs->begin(), offs->end() - 1);
   members->resize(a.size());
-  for (size_t i = 0; i < a.size(); ++i) (*members)[size_t(cur[size_t(a[i])]++)] = int32_t(i);
+  for (size_t j = 0; j < a.size(); ++j)
+    (*members)[size_t(cur[size_t(a[j])]++)] = int32_t(off + int64_t(j) * stride);
 }
 
-// spherical k-means on the int8 rows for partition g; leaves the final
-// assignment in c->cassign + g*n and the centroids in c->cz / c->cw
-int run_kmeans(knng_ctx* c, int g, int32_t* d_members, int64_t* d_offs, int64_t* d_sums) {
+struct KmeansBufs {
+  int8_t* z = nullptr;       // centroids / merge rows [cap][d]
+  float* w = nullptr;        // [cap]
+  int32_t* a = nullptr;      // sample assignment [ns] / merge partner [cap]
+  float* sc = nullptr;       // merge best value [cap]
+  int32_t* members = nullptr;
+  int64_t* offs = nullptr;
+  int64_t* sums = nullptr;   // [cap][d]
+  int64_t* ids = nullptr;    // seed row ids [cap]
+};
+
+// sums of the listed rows per cell (device), downloaded into S [C][d]
+int cell_sums(knng_ctx* c, KmeansBufs& B, const std::vector<int32_t>& a, int C, int64_t off,
+              int64_t stride, std::vector<int64_t>* S, std::vector<int64_t>* cnt) {
+  std::vector<int32_t> members;
+  std::vector<int64_t> offs;
+  cell_lists(a, C, off, stride, &members, &offs);
+  HIPCHK(c, hipMemcpyAsync(B.members, members.data(), members.size() * 4, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(B.offs, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_cell_sum, dim3(std::max(1, std::min(C, c->num_cu * 8))), dim3(256), 0,
+                     c->stream, c->q, c->d, B.members, B.offs, C, B.sums);
+  HIPCHK(c, hipGetLastError());
+  S->resize(size_t(C) * c->d);
+  HIPCHK(c, hipMemcpyAsync(S->data(), B.sums, S->size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  cnt->resize(size_t(C));
+  for (int k = 0; k < C; ++k) (*cnt)[size_t(k)] = offs[size_t(k) + 1] - offs[size_t(k)];
+  return 0;
+}
+
+int assign_rows_dev(knng_ctx* c, const int8_t* rows, int64_t nrows, int64_t off, int64_t stride,
+                    const std::vector<int8_t>& z, const std::vector<float>& w, int C, int32_t* out,
+                    KmeansBufs& B, bool excl, float* score) {
+  HIPCHK(c, hipMemcpyAsync(B.z, z.data(), size_t(C) * c->d, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(B.w, w.data(), size_t(C) * 4, hipMemcpyHostToDevice, c->stream));
+  if (launch_assign(c->stream, rows, nrows, c->d, B.z, B.w, C, out, c->num_cu, off, stride, excl,
+                    score))
+    return fail(c, KNNG_EHIP, std::string("k_assign launch: ") + hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+// k-means of partition g; leaves the final assignment of every row in
+// c->cassign + g*n and the centroids in *zf / *wf (C_g of them)
+int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* merge_rows,
+               std::vector<int8_t>* zf, std::vector<float>* wf) {
   const int64_t n = c->n;
-  const int d = c->d, C = c->ncells;
-  std::vector<int8_t> z(size_t(C) * d);
-  std::vector<float> w(C);
-  // seed rows: evenly spaced ids, shifted per partition
+  const int d = c->d;
+  const int64_t stride = std::max(1, c->p.cell_sample);
+  const int64_t off = int64_t(g) % stride < n ? int64_t(g) % stride : 0;
+  const int64_t ns = (n - off + stride - 1) / stride;
+  const int Cs = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(Ct) * c->p.cell_oversample, ns)));
+  int rc;
+  // 1. seeds
+  std::vector<int64_t> ids(Cs);
+  for (int k = 0; k < Cs; ++k)
+    ids[size_t(k)] = ((2 * int64_t(k) + 1) * n / (2 * int64_t(Cs)) + int64_t(g) * 7919) % n;
+  HIPCHK(c, hipMemcpyAsync(B.ids, ids.data(), ids.size() * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(c, int64_t(Cs) * (d >> 4), 256, 8)), dim3(256), 0,
+                     c->stream, c->q, d, B.ids, int64_t(Cs), B.z);
+  HIPCHK(c, hipGetLastError());
+  std::vector<int8_t> z(size_t(Cs) * d);
+  std::vector<float> w(Cs);
+  HIPCHK(c, hipMemcpyAsync(z.data(), B.z, z.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   {
-    std::vector<int8_t> row(d);
     std::vector<int64_t> S(d);
-    for (int cc = 0; cc < C; ++cc) {
-      const int64_t id = ((2 * int64_t(cc) + 1) * n / (2 * int64_t(C)) + int64_t(g) * 7919) % n;
-      HIPCHK(c, hipMemcpy(row.data(), c->q + id * d, d, hipMemcpyDeviceToHost));
-      for (int k = 0; k < d; ++k) S[k] = row[k];
-      quantize_centroid(S.data(), d, &z[size_t(cc) * d], &w[cc]);
+    for (int k = 0; k < Cs; ++k) {
+      for (int j = 0; j < d; ++j) S[size_t(j)] = z[size_t(k) * d + j];
+      quantize_centroid(S.data(), d, &z[size_t(k) * d], &w[size_t(k)]);
     }
   }
-  int8_t* dz = c->cz + size_t(g) * C * d;
-  float* dw = c->cw + size_t(g) * C;
-  int32_t* da = c->cassign + size_t(g) * n;
-  std::vector<int32_t> a(n), members;
-  std::vector<int64_t> offs;
-  std::vector<int64_t> sums(size_t(C) * d);
-  const int agrid = grid_for(c, (n + kAssignRows - 1) / kAssignRows, 1, 8);
+  // 2. Lloyd on the sample
+  std::vector<int32_t> a(ns);
+  std::vector<int64_t> S, cnt;
   for (int it = 0;; ++it) {
-    HIPCHK(c, hipMemcpyAsync(dz, z.data(), z.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(dw, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_assign, dim3(agrid), dim3(256), 0, c->stream, c->q, n, d, dz, dw, C, da);
-    HIPCHK(c, hipGetLastError());
+    if ((rc = assign_rows_dev(c, c->q, ns, off, stride, z, w, Cs, B.a, B, false, nullptr))) return rc;
+    HIPCHK(c, hipMemcpyAsync(a.data(), B.a, size_t(ns) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = cell_sums(c, B, a, Cs, off, stride, &S, &cnt))) return rc;
     if (it == c->p.kmeans_iters) break;
-    HIPCHK(c, hipMemcpyAsync(a.data(), da, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    cell_lists(a, C, &members, &offs);
-    HIPCHK(c, hipMemcpyAsync(d_members, members.data(), size_t(n) * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(d_offs, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_cell_sum, dim3(std::min(C, c->num_cu * 8)), dim3(256), 0, c->stream, c->q, d,
-                       d_members, d_offs, C, d_sums);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(sums.data(), d_sums, sums.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int cc = 0; cc < C; ++cc) {
-      if (offs[size_t(cc) + 1] == offs[size_t(cc)]) continue;   // empty: keep the centroid
-      quantize_centroid(&sums[size_t(cc) * d], d, &z[size_t(cc) * d], &w[cc]);
+    for (int k = 0; k < Cs; ++k)
+      if (cnt[size_t(k)]) quantize_centroid(&S[size_t(k) * d], d, &z[size_t(k) * d], &w[size_t(k)]);
+  }
+  // 3. merging
+  std::vector<int32_t> live;
+  for (int k = 0; k < Cs; ++k)
+    if (cnt[size_t(k)]) {
+      live.push_back(k);
+      quantize_centroid(&S[size_t(k) * d], d, &z[size_t(k) * d], &w[size_t(k)]);
     }
+  std::vector<int8_t> zl;
+  std::vector<float> wl;
+  std::vector<int32_t> best;
+  std::vector<float> bval;
+  for (int round = 0; round < 64 && live.size() >= 2; ++round) {
+    const int m = int(live.size());
+    zl.resize(size_t(m) * d);
+    wl.resize(size_t(m));
+    for (int j = 0; j < m; ++j) {
+      std::copy_n(&z[size_t(live[size_t(j)]) * d], d, &zl[size_t(j) * d]);
+      wl[size_t(j)] = w[size_t(live[size_t(j)])];
+    }
+    HIPCHK(c, hipMemcpyAsync(merge_rows, zl.data(), zl.size(), hipMemcpyHostToDevice, c->stream));
+    if ((rc = assign_rows_dev(c, merge_rows, m, 0, 1, zl, wl, m, B.a, B, true, B.sc))) return rc;
+    best.resize(size_t(m));
+    bval.resize(size_t(m));
+    HIPCHK(c, hipMemcpyAsync(best.data(), B.a, size_t(m) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(bval.data(), B.sc, size_t(m) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<char> gone(size_t(m), 0);
+    int merged = 0;
+    for (int j = 0; j < m; ++j) {
+      const int32_t k = best[size_t(j)];
+      if (k <= j || k >= m || best[size_t(k)] != j) continue;
+      if (!(bval[size_t(j)] * wl[size_t(j)] > c->p.cell_merge)) continue;
+      int64_t* sj = &S[size_t(live[size_t(j)]) * d];
+      const int64_t* sk = &S[size_t(live[size_t(k)]) * d];
+      for (int t = 0; t < d; ++t) sj[t] += sk[t];
+      cnt[size_t(live[size_t(j)])] += cnt[size_t(live[size_t(k)])];
+      quantize_centroid(sj, d, &z[size_t(live[size_t(j)]) * d], &w[size_t(live[size_t(j)])]);
+      gone[size_t(k)] = 1;
+      ++merged;
+    }
+    if (!merged) break;
+    std::vector<int32_t> next;
+    for (int j = 0; j < m; ++j)
+      if (!gone[size_t(j)]) next.push_back(live[size_t(j)]);
+    live.swap(next);
+  }
+  // 4. pruning
+  {
+    std::vector<int64_t> sz;
+    for (int32_t k : live) sz.push_back(cnt[size_t(k)]);
+    std::sort(sz.begin(), sz.end());
+    const float cut = c->p.cell_prune * float(sz[sz.size() / 2]);
+    std::vector<int32_t> keep;
+    for (int32_t k : live)
+      if (!(float(cnt[size_t(k)]) < cut)) keep.push_back(k);
+    live.swap(keep);
+  }
+  // 5. Lloyd over all rows with the merged cells
+  const int C = int(live.size());
+  zf->resize(size_t(C) * d);
+  wf->resize(size_t(C));
+  for (int j = 0; j < C; ++j) {
+    std::copy_n(&z[size_t(live[size_t(j)]) * d], d, &(*zf)[size_t(j) * d]);
+    (*wf)[size_t(j)] = w[size_t(live[size_t(j)])];
+  }
+  int32_t* da = c->cassign + size_t(g) * n;
+  std::vector<int32_t> af;
+  for (int it = 0;; ++it) {
+    if ((rc = assign_rows_dev(c, c->q, n, 0, 1, *zf, *wf, C, da, B, false, nullptr))) return rc;
+    if (it == c->p.cell_full_iters) break;
+    af.resize(size_t(n));
+    HIPCHK(c, hipMemcpyAsync(af.data(), da, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = cell_sums(c, B, af, C, 0, 1, &S, &cnt))) return rc;
+    for (int k = 0; k < C; ++k)
+      if (cnt[size_t(k)])
+        quantize_centroid(&S[size_t(k) * d], d, &(*zf)[size_t(k) * d], &(*wf)[size_t(k)]);
   }
   return 0;
+}
+
+void free_kmeans(KmeansBufs& B, int8_t** merge_rows) {
+  dfree(&B.z);
+  dfree(&B.w);
+  dfree(&B.a);
+  dfree(&B.sc);
+  dfree(&B.members);
+  dfree(&B.offs);
+  dfree(&B.sums);
+  dfree(&B.ids);
+  dfree(merge_rows);
 }
 
 // Seed every tree with its partition's cells: perm = rows by (cell, id);
 // cells larger than a leaf become the first level's nodes (depth 1), the
 // others leaves.  Returns the node count in *nnodes.
-int seed_from_cells(knng_ctx* c, int* nnodes) {
+int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
   const int64_t n = c->n, T = c->p.num_trees;
   const int d = c->d, leaf = c->p.leaf_size;
-  const int C = c->ncells, G = c->nparts;
+  const int G = c->nparts;
+  const int64_t stride = std::max(1, c->p.cell_sample);
+  const int64_t cap = std::max<int64_t>(
+      2, std::min<int64_t>(int64_t(Ct) * c->p.cell_oversample, (n + stride - 1) / stride));
   int rc;
-  if ((rc = dalloc(c, &c->cz, size_t(G) * C * d, "centroids")) ||
-      (rc = dalloc(c, &c->cw, size_t(G) * C, "centroid norms")) ||
-      (rc = dalloc(c, &c->cassign, size_t(G) * n, "cell assignment")) ||
-      (rc = dalloc(c, &c->cell_root, size_t(T) * C, "cell roots")))
+  dfree(&c->cassign);
+  if ((rc = dalloc(c, &c->cassign, size_t(G) * n, "cell assignment"))) return rc;
+  KmeansBufs B;
+  int8_t* merge_rows = nullptr;
+  if ((rc = dalloc(c, &B.z, size_t(cap) * d, "centroids")) ||
+      (rc = dalloc(c, &B.w, size_t(cap), "centroid norms")) ||
+      (rc = dalloc(c, &B.a, size_t(std::max<int64_t>(cap, (n + stride - 1) / stride)), "assignment")) ||
+      (rc = dalloc(c, &B.sc, size_t(cap), "merge values")) ||
+      (rc = dalloc(c, &B.members, size_t(n), "cell members")) ||
+      (rc = dalloc(c, &B.offs, size_t(cap) + 1, "cell offsets")) ||
+      (rc = dalloc(c, &B.sums, size_t(cap) * d, "cell sums")) ||
+      (rc = dalloc(c, &B.ids, size_t(cap), "seed ids")) ||
+      (rc = dalloc(c, &merge_rows, size_t(cap) * d, "merge rows"))) {
+    free_kmeans(B, &merge_rows);
     return rc;
-  int32_t* d_members = nullptr;
-  int64_t* d_offs = nullptr;
-  int64_t* d_sums = nullptr;
-  if ((rc = dalloc(c, &d_members, size_t(n), "cell members")) ||
-      (rc = dalloc(c, &d_offs, size_t(C) + 1, "cell offsets")) ||
-      (rc = dalloc(c, &d_sums, size_t(C) * d, "cell sums")))
-    return rc;
-  for (int g = 0; g < G && !rc; ++g) rc = run_kmeans(c, g, d_members, d_offs, d_sums);
-  dfree(&d_members);
-  dfree(&d_offs);
-  dfree(&d_sums);
+  }
+  std::vector<std::vector<int8_t>> zs(G);
+  std::vector<std::vector<float>> ws(G);
+  c->cell_count.assign(size_t(G), 0);
+  for (int g = 0; g < G && !rc; ++g) {
+    rc = run_kmeans(c, g, Ct, B, merge_rows, &zs[size_t(g)], &ws[size_t(g)]);
+    if (!rc) c->cell_count[size_t(g)] = int32_t(ws[size_t(g)].size());
+  }
+  free_kmeans(B, &merge_rows);
   if (rc) return rc;
+  const int Cm = *std::max_element(c->cell_count.begin(), c->cell_count.end());
+  c->ncells = Cm;
+  dfree(&c->cz);
+  dfree(&c->cw);
+  dfree(&c->cell_root);
+  if ((rc = dalloc(c, &c->cz, size_t(G) * Cm * d, "centroids")) ||
+      (rc = dalloc(c, &c->cw, size_t(G) * Cm, "centroid norms")) ||
+      (rc = dalloc(c, &c->cell_root, size_t(T) * Cm, "cell roots")))
+    return rc;
+  for (int g = 0; g < G; ++g) {
+    HIPCHK(c, hipMemcpyAsync(c->cz + size_t(g) * Cm * d, zs[size_t(g)].data(), zs[size_t(g)].size(),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cw + size_t(g) * Cm, ws[size_t(g)].data(), ws[size_t(g)].size() * 4,
+                             hipMemcpyHostToDevice, c->stream));
+  }
   // host build of perm, nodepos, first-level nodes, small-cell leaves, cell roots
   std::vector<std::vector<int32_t>> mem(G);
   std::vector<std::vector<int64_t>> off(G);
   std::vector<int32_t> a(n);
   for (int g = 0; g < G; ++g) {
-    HIPCHK(c, hipMemcpy(a.data(), c->cassign + size_t(g) * n, size_t(n) * 4, hipMemcpyDeviceToHost));
-    cell_lists(a, C, &mem[g], &off[g]);
+    HIPCHK(c, hipMemcpyAsync(a.data(), c->cassign + size_t(g) * n, size_t(n) * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    cell_lists(a, c->cell_count[size_t(g)], 0, 1, &mem[size_t(g)], &off[size_t(g)]);
   }
-  std::vector<int32_t> perm(size_t(T) * n), npos(size_t(T) * n, -1), roots(size_t(T) * C);
+  std::vector<int32_t> perm(size_t(T) * n), npos(size_t(T) * n, -1), roots(size_t(T) * Cm, INT32_MIN);
   std::vector<Node> nodes;
   std::vector<int4> leaves;
   for (int t = 0; t < T; ++t) {
     const int g = t % G;
-    std::copy(mem[g].begin(), mem[g].end(), perm.begin() + size_t(t) * n);
-    for (int cc = 0; cc < C; ++cc) {
-      const int64_t s = off[g][size_t(cc)], z = off[g][size_t(cc) + 1] - s;
+    std::copy(mem[size_t(g)].begin(), mem[size_t(g)].end(), perm.begin() + size_t(t) * n);
+    for (int cc = 0; cc < c->cell_count[size_t(g)]; ++cc) {
+      const int64_t s = off[size_t(g)][size_t(cc)], z = off[size_t(g)][size_t(cc) + 1] - s;
       if (z == 0) {
-        roots[size_t(t) * C + cc] = INT32_MIN;
+        roots[size_t(t) * Cm + cc] = INT32_MIN;
       } else if (z <= leaf) {
-        roots[size_t(t) * C + cc] = -int32_t(leaves.size() + 1);
+        roots[size_t(t) * Cm + cc] = -int32_t(leaves.size() + 1);
         leaves.push_back(make_int4(t, int(s), int(z), 0));
       } else {
         Node nd;
@@ -1249,7 +1597,7 @@ int seed_from_cells(knng_ctx* c, int* nnodes) {
         nd.left = nd.right = -1;
         nd.nleft = 0;
         nd.p1id = nd.p2id = -1;
-        roots[size_t(t) * C + cc] = int32_t(nodes.size());
+        roots[size_t(t) * Cm + cc] = int32_t(nodes.size());
         for (int64_t j = s; j < s + z; ++j) npos[size_t(t) * n + j] = int32_t(nodes.size());
         nodes.push_back(nd);
       }
@@ -1267,8 +1615,8 @@ int seed_from_cells(knng_ctx* c, int* nnodes) {
                              hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->cell_root, roots.data(), roots.size() * 4, hipMemcpyHostToDevice,
                            c->stream));
-  int32_t cnt[2] = {0, int32_t(leaves.size())};
-  HIPCHK(c, hipMemcpyAsync(c->dcount, cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
+  int32_t cnt2[2] = {0, int32_t(leaves.size())};
+  HIPCHK(c, hipMemcpyAsync(c->dcount, cnt2, sizeof(cnt2), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *nnodes = int(nodes.size());
   return 0;
@@ -1280,7 +1628,7 @@ int check_params(const knng_params* p, int64_t n, std::string* why) {
   if (p->k_build < 1 || p->k_build > kMaxK) return *why = "k_build must be in [1, 64]", KNNG_ERANGE;
   if (p->k_eval < 1 || p->k_eval > p->k_build) return *why = "k_eval must be in [1, k_build]", KNNG_EINVAL;
   if (p->num_trees < 1 || p->num_trees > kMaxTrees) return *why = "num_trees must be in [1, 64]", KNNG_ERANGE;
-  if (p->leaf_size < 2 || p->leaf_size > kLeafSlots) return *why = "leaf_size must be in [2, 64]", KNNG_ERANGE;
+  if (p->leaf_size < 2 || p->leaf_size > kMaxLeaf) return *why = "leaf_size must be in [2, 2048]", KNNG_ERANGE;
   if (p->rerank_mult < 1) return *why = "rerank_mult must be >= 1", KNNG_EINVAL;
   if (p->max_iters < 0 || p->max_iters > 32) return *why = "max_iters must be in [0, 32]", KNNG_ERANGE;
   if (p->rerank_width < 0 || p->rerank_width > p->k_build) return *why = "rerank_width must be in [0, k_build]", KNNG_EINVAL;
@@ -1289,6 +1637,11 @@ int check_params(const knng_params* p, int64_t n, std::string* why) {
   if (p->cells < -1) return *why = "cells must be -1 (auto), 0 (none) or a count", KNNG_EINVAL;
   if (p->kmeans_iters < 0 || p->kmeans_iters > 100) return *why = "kmeans_iters must be in [0, 100]", KNNG_EINVAL;
   if (p->cell_partitions < 1) return *why = "cell_partitions must be >= 1", KNNG_EINVAL;
+  if (p->cell_oversample < 1 || p->cell_oversample > 64) return *why = "cell_oversample must be in [1, 64]", KNNG_EINVAL;
+  if (p->cell_sample < 1) return *why = "cell_sample must be >= 1", KNNG_EINVAL;
+  if (p->cell_full_iters < 0 || p->cell_full_iters > 100) return *why = "cell_full_iters must be in [0, 100]", KNNG_EINVAL;
+  if (!(p->cell_merge == p->cell_merge)) return *why = "cell_merge must be a number", KNNG_EINVAL;
+  if (!(p->cell_prune >= 0.0f && p->cell_prune <= 1.0f)) return *why = "cell_prune must be in [0, 1]", KNNG_EINVAL;
   if (int64_t(p->num_trees) * n >= (int64_t(1) << 31) - 1) return *why = "num_trees * n must fit in int32", KNNG_ERANGE;
   if (n >= (int64_t(1) << 31) - 1) return *why = "n must fit in int32", KNNG_ERANGE;
   return 0;
@@ -1316,8 +1669,13 @@ int knng_default_params(knng_params* p) {
   p->insert_stop_rate = 0.01f;
   p->incremental = 0;
   p->cells = -1;
-  p->kmeans_iters = 6;
+  p->kmeans_iters = 2;
   p->cell_partitions = 2;
+  p->cell_oversample = 4;
+  p->cell_sample = 8;
+  p->cell_merge = 0.6f;
+  p->cell_full_iters = 1;
+  p->cell_prune = 0.25f;
   p->pad0 = 0;
   return 0;
 }
@@ -1523,7 +1881,8 @@ int knng_build_forest(knng_ctx* c) {
   const int leaf = c->p.leaf_size;
   c->node_cap = T * (n / (leaf + 1) + 1) + 16;
   // every non-empty cell below the leaf size is a leaf of its own
-  c->leaf_cap = total / 4 + T + 1024 + T * int64_t(std::min<int64_t>(cells_for(c->p, n), n));
+  c->leaf_cap = total / 4 + T + 1024 +
+                T * std::min<int64_t>(int64_t(cells_for(c->p, n)) * std::max(1, c->p.cell_oversample), n);
   int rc;
   if ((rc = dalloc(c, &c->perm, total, "perm")) || (rc = dalloc(c, &c->perm2, total, "perm2")) ||
       (rc = dalloc(c, &c->nodepos, total, "nodepos")) ||
@@ -1555,17 +1914,17 @@ int knng_build_forest(knng_ctx* c) {
   }
   c->ncells = 0;
   c->nparts = 0;
+  c->cell_count.clear();
   c->st.ms_kmeans = 0;
   const int32_t ncells = cells_for(c->p, n);
   if (nnodes > 0 && ncells >= 2) {
     // k-means cells replace the root split of every tree
-    c->ncells = std::min<int32_t>(ncells, int32_t(n));
     c->nparts = std::max(1, std::min<int>(c->p.cell_partitions, int(T)));
     hipEvent_t k0, k1;
     HIPCHK(c, hipEventCreate(&k0));
     HIPCHK(c, hipEventCreate(&k1));
     HIPCHK(c, hipEventRecord(k0, c->stream));
-    rc = seed_from_cells(c, &nnodes);
+    rc = seed_from_cells(c, int(std::min<int64_t>(ncells, n)), &nnodes);
     (void)hipEventRecord(k1, c->stream);
     (void)hipEventSynchronize(k1);
     (void)hipEventElapsedTime(&c->st.ms_kmeans, k0, k1);
@@ -1574,7 +1933,8 @@ int knng_build_forest(knng_ctx* c) {
     if (rc) return rc;
     levels = 1;
   }
-  c->st.cells = c->ncells;
+  c->st.cells = 0;
+  for (int32_t v : c->cell_count) c->st.cells += v;
   const int nch = c->d / 16;
   while (nnodes > 0) {
     const int run = 16;
@@ -1641,6 +2001,45 @@ int knng_build_init(knng_ctx* c) {
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   hipLaunchKernelGGL(k_leaf_gram, dim3(grid_for(c, c->nleaves, 1, 64)), dim3(256), 0, c->stream, c->q,
                      c->d, n, T, c->perm, c->leaves, c->nleaves, cand);
+  if (c->p.leaf_size > kLeafSlots) {
+    // leaves above 64 rows: work items of 128 members each
+    std::vector<int4> lv(size_t(c->nleaves));
+    HIPCHK(c, hipMemcpyAsync(lv.data(), c->leaves, lv.size() * sizeof(int4), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<int4> items;
+    int mmax = 0;
+    for (const int4& l : lv)
+      if (l.z > kLeafSlots) {
+        mmax = std::max(mmax, int(l.z));
+        for (int r0 = 0; r0 < l.z; r0 += kBigRows) items.push_back(make_int4(l.x, l.y, l.z, r0));
+      }
+    if (mmax > kMaxLeaf) return fail(c, KNNG_ERANGE, "leaf larger than the big-leaf kernel allows");
+    if (!items.empty()) {
+      const int mpad = (mmax + 63) & ~63;
+      const int grid = int(std::min<int64_t>(int64_t(items.size()), int64_t(c->num_cu) * 2));
+      int4* ditems = nullptr;
+      int32_t* scratch = nullptr;
+      if ((rc = dalloc(c, &ditems, items.size(), "big-leaf items")) ||
+          (rc = dalloc(c, &scratch, size_t(grid) * kBigRows * mpad, "big-leaf scores"))) {
+        dfree(&ditems);
+        dfree(&cand);
+        return rc;
+      }
+      HIPCHK(c, hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
+                               c->stream));
+      hipLaunchKernelGGL(k_leaf_big, dim3(grid), dim3(256), 0, c->stream, c->q, c->d, n, T, c->perm,
+                         ditems, int64_t(items.size()), scratch, mpad, cand);
+      const hipError_t e = hipGetLastError();
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      dfree(&ditems);
+      dfree(&scratch);
+      if (e != hipSuccess) {
+        dfree(&cand);
+        return fail(c, KNNG_EHIP, std::string("k_leaf_big launch: ") + hipGetErrorString(e));
+      }
+    }
+  }
   const int slots = T * kLeafSlots;
   const int mgrid = grid_for(c, n, 1, 32);
   c->cur = 0;
@@ -1835,6 +2234,31 @@ int knng_copy_forest(knng_ctx* c, int32_t* perm, int64_t* leaves, int64_t cap, i
       leaves[3 * l + 2] = lv[l].z;
     }
   }
+  return 0;
+}
+
+int knng_copy_cells(knng_ctx* c, int32_t part, int32_t* count, int32_t* assign, int8_t* z,
+                    float* w) {
+  if (!c) return KNNG_EINVAL;
+  if (!c->have_forest) return fail(c, KNNG_ESTATE, "knng_copy_cells: no forest");
+  if (c->nparts == 0) {
+    if (count) *count = 0;
+    return 0;
+  }
+  if (part < 0 || part >= c->nparts) return fail(c, KNNG_EINVAL, "knng_copy_cells: no such partition");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int32_t C = c->cell_count[size_t(part)];
+  if (count) *count = C;
+  if (assign)
+    HIPCHK(c, hipMemcpyAsync(assign, c->cassign + size_t(part) * c->n, size_t(c->n) * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  if (z)
+    HIPCHK(c, hipMemcpyAsync(z, c->cz + size_t(part) * c->ncells * c->d, size_t(C) * c->d,
+                             hipMemcpyDeviceToHost, c->stream));
+  if (w)
+    HIPCHK(c, hipMemcpyAsync(w, c->cw + size_t(part) * c->ncells, size_t(C) * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 

@@ -16,6 +16,7 @@ constexpr int kMaxDepth = 64;           // forest depth at which a node is cut i
 constexpr int kMaxK = 64;
 constexpr int kMaxTrees = 64;
 constexpr int kLeafSlots = 64;          // candidate slots per (point, tree)
+constexpr int kMaxLeaf = 2048;          // largest leaf (leaves above kLeafSlots keep 64 mates)
 constexpr int kNumCounters = 1024;      // spread of the per-block atomic counters
 constexpr float kFltMin = 1.17549435e-38f;
 
@@ -26,6 +27,18 @@ constexpr float kFltMin = 1.17549435e-38f;
 __device__ __forceinline__ int wave_sum_i(int v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m));
+  return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m));
   return v;
 }
 

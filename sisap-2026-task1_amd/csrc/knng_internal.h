@@ -37,7 +37,8 @@ struct knng_ctx {
   Node* hier = nullptr;        // every level's nodes, global child links
   int64_t hier_count = 0;
   // cells: spherical k-means partitions of the int8 rows that seed the trees
-  int32_t ncells = 0, nparts = 0;
+  int32_t ncells = 0, nparts = 0;   // ncells: the largest partition's cell count (the stride)
+  std::vector<int32_t> cell_count;  // cells of each partition
   int8_t* cz = nullptr;        // [nparts][ncells][d] int8 centroid directions
   float* cw = nullptr;         // [nparts][ncells] 1 / |centroid|
   int32_t* cassign = nullptr;  // [nparts][n] cell of every row
@@ -112,8 +113,10 @@ inline int grid_for(const knng_ctx* c, int64_t work, int per_block, int blocks_p
 // K1 on an arbitrary stream (defined in knng.hip)
 int launch_quantize(hipStream_t stream, const float* dx, int64_t nrows, int d, int8_t* q,
                     uint64_t* sk, int num_cu);
-// route int8 rows to cells: out[i] = argmax_c float(<q_i, Z_c>) * w[c] (knng.hip)
+// route int8 rows to cells: out[j] = argmax_c float(<q_r, Z_c>) * w[c] for the rows
+// r = row_off + j * row_stride (knng.hip); exclude_self skips c == j, score gets the best value
 int launch_assign(hipStream_t stream, const int8_t* q, int64_t nrows, int d, const int8_t* Z,
-                  const float* w, int C, int32_t* out, int num_cu);
+                  const float* w, int C, int32_t* out, int num_cu, int64_t row_off = 0,
+                  int64_t row_stride = 1, bool exclude_self = false, float* score = nullptr);
 
 }  // namespace knng_host

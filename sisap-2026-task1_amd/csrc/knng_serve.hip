@@ -125,7 +125,8 @@ __global__ __launch_bounds__(256) void k_search(
       }
       if (leaf_idx >= 0) {
         const int4 lf = leaves[leaf_idx];
-        for (int j = lane; j < lf.z; j += 64) {
+        const int m = lf.z < kLeafSlots ? lf.z : kLeafSlots;   // a big leaf enters its first 64 rows
+        for (int j = lane; j < m; j += 64) {
           const int32_t id = perm[int64_t(lf.x) * n_forest + lf.y + j];
           if (hs_insert(hs, kVisitBits, id)) {
             atomicAdd(&cnt[2], 1);
@@ -480,7 +481,8 @@ int launch_search(knng_server* s, hipStream_t stream, const float* dx, int8_t* d
     return sfail(s, KNNG_EHIP, "quantise launch failed");
   for (int g = 0; g < c->nparts; ++g)   // route the queries to cells
     if (knng_host::launch_assign(stream, dq, nq, s->d, c->cz + size_t(g) * c->ncells * s->d,
-                                 c->cw + size_t(g) * c->ncells, c->ncells, dcell + size_t(g) * nq,
+                                 c->cw + size_t(g) * c->ncells, c->cell_count[size_t(g)],
+                                 dcell + size_t(g) * nq,
                                  s->num_cu))
       return sfail(s, KNNG_EHIP, "cell routing launch failed");
   const int grid = std::max(1, std::min<int>(nq, s->num_cu * 8));

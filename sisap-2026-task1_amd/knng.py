@@ -23,7 +23,8 @@ KNNG_SYMBOLS = [
     "knng_host_alloc", "knng_host_free",
     "knng_build", "knng_build_forest", "knng_build_init", "knng_join_iteration",
     "knng_export", "knng_export_device", "knng_get_stats", "knng_synchronize",
-    "knng_copy_quantized", "knng_copy_forest", "knng_copy_graph", "knng_load_graph",
+    "knng_copy_quantized", "knng_copy_forest", "knng_copy_graph", "knng_copy_cells",
+    "knng_load_graph",
     "knng_quantize_device", "knng_score_pairs_device",
     "knng_server_default_params", "knng_server_create", "knng_server_destroy",
     "knng_server_last_error", "knng_server_submit", "knng_server_poll", "knng_server_wait",
@@ -53,7 +54,9 @@ class Params(C.Structure):
         ("seed", C.c_uint64), ("keep_fp32", C.c_int32), ("rerank_width", C.c_int32),
         ("stop_rule", C.c_int32), ("insert_stop_rate", C.c_float),
         ("incremental", C.c_int32), ("cells", C.c_int32), ("kmeans_iters", C.c_int32),
-        ("cell_partitions", C.c_int32), ("pad0", C.c_int32),
+        ("cell_partitions", C.c_int32), ("cell_oversample", C.c_int32),
+        ("cell_sample", C.c_int32), ("cell_merge", C.c_float), ("cell_full_iters", C.c_int32),
+        ("cell_prune", C.c_float), ("pad0", C.c_int32),
     ]
 
 
@@ -127,6 +130,7 @@ def lib() -> C.CDLL:
             "knng_copy_quantized": ([V, V, V], i32),
             "knng_copy_forest": ([V, V, V, i64, P(i64)], i32),
             "knng_copy_graph": ([V, V, V], i32),
+            "knng_copy_cells": ([V, i32, P(i32), V, V, V], i32),
             "knng_load_graph": ([V, V, V], i32),
             "knng_quantize_device": ([V, i64, i32, V, V], i32),
             "knng_score_pairs_device": ([V, V, i32, V, V, i64, V, V], i32),
@@ -281,6 +285,19 @@ class Builder:
         self._ck(self.L.knng_copy_forest(self.ctx, None, _ptr(leaves), leaves.shape[0], C.byref(nl)),
                  "copy_forest")
         return perm, leaves[: nl.value]
+
+    def cells(self, part: int = 0):
+        """(z [C, d] int8, w [C] f32, assign [n] int32) of a cell partition, or None."""
+        cnt = C.c_int32()
+        self._ck(self.L.knng_copy_cells(self.ctx, part, C.byref(cnt), None, None, None), "copy_cells")
+        if cnt.value == 0:
+            return None
+        z = np.empty((cnt.value, self.p.dim), np.int8)
+        w = np.empty(cnt.value, np.float32)
+        a = np.empty(self.n, np.int32)
+        self._ck(self.L.knng_copy_cells(self.ctx, part, C.byref(cnt), _ptr(a), _ptr(z), _ptr(w)),
+                 "copy_cells")
+        return z, w, a
 
     def graph(self):
         K = self.p.k_build

@@ -27,19 +27,20 @@ def _built(x, **kw):
     return b
 
 
-def test_search_int8_bitexact_vs_oracle(torch_cuda):
-    say("search bit-exact")
+@pytest.mark.parametrize("T,leaf", [(4, 32), (2, 2048)])
+def test_search_int8_bitexact_vs_oracle(torch_cuda, T, leaf):
+    say("search bit-exact", T, leaf)
     x = synth.gaussian_mixture(41, 4000, 256, 16)
-    b = _built(x, num_trees=4, leaf_size=32, k_build=20, k_eval=15)
+    b = _built(x, num_trees=T, leaf_size=leaf, k_build=20, k_eval=15)
     ids, _ = b.graph()
     q, _ = oracle.quantize(x)
     queries = synth.gaussian_mixture(42, 300, 256, 16)
     qq, _ = oracle.quantize(queries)
     with knng.Server(b, beam=32, k=10, rerank=0, entry_trees=4, max_batch=64, max_slots=3) as s:
         g_ids, g_sc = s.search(queries)
-    plan = oracle.cells_device(q, 4, 32)   # n = 4000: two cells, two partitions
-    assert plan is not None and b.stats()["cells"] == plan[0]
-    o_ids, o_sc = oracle.search_device(q, ids, qq, T=4, leaf=32, entry_trees=4, beam=32, k=10,
+    plan = oracle.cells_device(q, T, leaf)   # n = 4000: a 2-cell target, two partitions
+    assert plan is not None and b.stats()["cells"] == int(plan[0].sum())
+    o_ids, o_sc = oracle.search_device(q, ids, qq, T=T, leaf=leaf, entry_trees=4, beam=32, k=10,
                                        max_steps=128, plan=plan)
     assert np.array_equal(g_ids, o_ids)
     assert np.array_equal(g_sc, o_sc.astype(np.float32))

@@ -233,66 +233,119 @@ def test_assign_device_bruteforce():
     assert np.array_equal(oracle.assign_device(q, z, w), _assign_np(q, z, w))
 
 
-def test_kmeans_seeds_are_quantised_rows():
-    """iters = 0: centroid c is row ((2c+1)n/(2C) + 7919 g) mod n scaled so its
-    largest component is +-127 (truncating), and the rows go to their argmax."""
-    x, q, sk = _small(n=400, d=64)
-    for g in (0, 1):
-        z, w, a = oracle.kmeans_device(q, 9, g, iters=0)
-        for c in range(9):
-            row = q[((2 * c + 1) * 400 // 18 + 7919 * g) % 400].astype(np.int64)
-            M = np.abs(row).max()
-            exp = np.sign(row) * ((np.abs(row) * 127) // M)
-            assert np.array_equal(z[c].astype(np.int64), exp)
-            assert np.abs(z[c]).max() == 127
-            assert w[c] == np.float32(1.0 / np.sqrt((exp.astype(np.float64) ** 2).sum()))
-        assert np.array_equal(a, _assign_np(q, z, w))
+def _quant_centroid_np(S):
+    """int8 direction of an exact integer sum: trunc(S * 127 / max|S|), w = 1/|z|."""
+    S = S.astype(np.int64)
+    M = np.abs(S).max()
+    if M == 0:
+        return np.zeros(S.shape, np.int8), np.float32(0)
+    z = (np.sign(S) * ((np.abs(S) * 127) // M)).astype(np.int8)
+    return z, np.float32(1.0 / np.sqrt((z.astype(np.float64) ** 2).sum()))
+
+
+def _kmeans_np(q, Ct, g, iters, over, sample, merge, full_iters, prune):
+    """numpy restatement of the cell k-means: seeds, sample Lloyd, mutual-best
+    merging, pruning, full Lloyd, final assignment."""
+    n, d = q.shape
+    off = g % sample if g % sample < n else 0
+    rows = np.arange(off, n, sample)
+    Cs = max(1, min(Ct * over, len(rows)))
+    seeds = ((2 * np.arange(Cs, dtype=np.int64) + 1) * n // (2 * Cs) + 7919 * g) % n
+    Z = np.zeros((Cs, d), np.int8)
+    W = np.zeros(Cs, np.float32)
+    for c in range(Cs):
+        Z[c], W[c] = _quant_centroid_np(q[seeds[c]])
+
+    def sums(idx, a, C):
+        S = np.zeros((C, d), np.int64)
+        np.add.at(S, a, q[idx].astype(np.int64))
+        return S, np.bincount(a, minlength=C)
+
+    for it in range(iters + 1):
+        a = _assign_np(q[rows], Z, W)
+        S, cnt = sums(rows, a, Cs)
+        if it == iters:
+            break
+        for c in np.nonzero(cnt)[0]:
+            Z[c], W[c] = _quant_centroid_np(S[c])
+    live = [c for c in range(Cs) if cnt[c]]
+    for c in live:
+        Z[c], W[c] = _quant_centroid_np(S[c])
+    for _ in range(64):
+        if len(live) < 2:
+            break
+        zl, wl = Z[live], W[live]
+        dots = (zl.astype(np.int64) @ zl.astype(np.int64).T).astype(np.float32) * wl[None, :]
+        np.fill_diagonal(dots, -np.inf)
+        best = np.argmax(dots, 1)
+        val = dots[np.arange(len(live)), best]
+        gone = set()
+        for j in range(len(live)):
+            k = best[j]
+            if k > j and best[k] == j and np.float32(val[j] * wl[j]) > np.float32(merge):
+                S[live[j]] += S[live[k]]
+                cnt[live[j]] += cnt[live[k]]
+                Z[live[j]], W[live[j]] = _quant_centroid_np(S[live[j]])
+                gone.add(k)
+        if not gone:
+            break
+        live = [c for j, c in enumerate(live) if j not in gone]
+    med = np.sort(cnt[live])[len(live) // 2]
+    live = [c for c in live if not np.float32(cnt[c]) < np.float32(prune) * np.float32(med)]
+    Zf, Wf = Z[live].copy(), W[live].copy()
+    allr = np.arange(n)
+    for it in range(full_iters + 1):
+        a = _assign_np(q, Zf, Wf)
+        if it == full_iters:
+            break
+        S, cnt = sums(allr, a, len(live))
+        for c in np.nonzero(cnt)[0]:
+            Zf[c], Wf[c] = _quant_centroid_np(S[c])
+    return Zf, Wf, a
+
+
+@pytest.mark.parametrize("Ct,g,iters,over,sample,merge,full,prune", [
+    (6, 0, 2, 4, 3, 0.6, 1, 0.25), (6, 1, 0, 1, 1, 2.0, 0, 0.0), (9, 1, 1, 3, 2, 0.3, 2, 0.5),
+    (40, 0, 2, 4, 8, 0.6, 1, 0.25), (30, 1, 2, 4, 2, 0.6, 1, 1.0),
+])
+def test_kmeans_device_numpy_restatement(Ct, g, iters, over, sample, merge, full, prune):
+    x, q, sk = _small(seed=8, n=600, d=128, clusters=10)
+    z, w, a = oracle.kmeans_device(q, Ct, g, iters, over, sample, merge, full, prune)
+    ez, ew, ea = _kmeans_np(q, Ct, g, iters, over, sample, merge, full, prune)
+    assert np.array_equal(z, ez) and np.array_equal(w, ew) and np.array_equal(a, ea)
 
 
 def test_kmeans_recovers_separated_clusters():
-    """Eight sign-pattern clusters with small noise, rows ordered by cluster so
-    each seed lands in its own cluster: the cells are exactly the clusters."""
+    """Eight sign-pattern clusters with small noise: four seeds per cluster
+    are merged back, and the cells are exactly the clusters."""
     rng = np.random.default_rng(11)
     C, per, d = 8, 60, 128
     centers = rng.choice([-1.0, 1.0], size=(C, d)).astype(np.float32) * 0.5
-    x = np.repeat(centers, per, axis=0) + rng.normal(0, 0.05, size=(C * per, d)).astype(np.float32)
+    lab = rng.integers(0, C, C * per)
+    x = centers[lab] + rng.normal(0, 0.05, size=(C * per, d)).astype(np.float32)
     q, sk = oracle.quantize(x.astype(np.float32))
-    z, w, a = oracle.kmeans_device(q, C, 0, iters=4)
-    assert np.array_equal(a, np.repeat(np.arange(C, dtype=np.int32), per))
-
-
-def test_kmeans_update_is_requantised_cell_sum():
-    """Centroids after `it` rounds are the requantised exact sums of the cells
-    the (it-1)-round run ends with; empty cells keep their previous centroid."""
-    x, q, sk = _small(seed=8, n=600, d=128, clusters=10)
-    C = 16
-    for it in (1, 3):
-        _, _, a_prev = oracle.kmeans_device(q, C, 1, iters=it - 1)
-        z_prev, w_prev, _ = oracle.kmeans_device(q, C, 1, iters=it - 1)
-        z, w, a = oracle.kmeans_device(q, C, 1, iters=it)
-        for c in range(C):
-            m = a_prev == c
-            if not m.any():
-                assert np.array_equal(z[c], z_prev[c]) and w[c] == w_prev[c]
-                continue
-            S = q[m].astype(np.int64).sum(0)
-            M = np.abs(S).max()
-            assert np.array_equal(z[c].astype(np.int64), np.sign(S) * ((np.abs(S) * 127) // M))
-        assert np.array_equal(a, _assign_np(q, z, w))
+    z, w, a = oracle.kmeans_device(q, C, 0, iters=2, over=4, sample=2, merge=0.6, full_iters=1)
+    assert len(w) == C
+    # a bijection between cells and clusters
+    pairs = set(zip(a.tolist(), lab.tolist()))
+    assert len(pairs) == C and len({p[0] for p in pairs}) == C
+    # without merging the pieces stay apart
+    z2, w2, a2 = oracle.kmeans_device(q, C, 0, iters=2, over=4, sample=2, merge=2.0, full_iters=1)
+    assert len(w2) > C
 
 
 def test_forest_cells_bruteforce():
     x, q, sk = _small(seed=4, n=500, d=128, clusters=8)
-    for C, G, T, leaf in [(6, 2, 3, 16), (40, 3, 4, 8), (250, 1, 2, 4)]:
-        A = np.stack([oracle.kmeans_device(q, C, g, iters=3)[2] for g in range(G)])
-        plan = (C, G, None, None, A)
+    for Ct, G, T, leaf in [(6, 2, 3, 16), (40, 3, 4, 8), (125, 1, 2, 4), (4, 2, 2, 300)]:
+        plan = oracle.cells_device(q, T, leaf, cells=Ct, partitions=G, sample=2)
+        counts, G, _, _, A = plan
         perm, leaves = oracle.forest_cells_device(q, T, leaf, plan, seed=1337)
         bp, bl = _forest_bruteforce(q, T, leaf, 1337, cell_assign=A)
         assert np.array_equal(perm, bp) and np.array_equal(leaves, bl)
         for t in range(T):
             # each cell is one contiguous run of the tree's order, cells ascending
             cell_of = A[t % G][perm[t]]
-            assert np.all(np.diff(cell_of) >= 0)
+            assert np.all(np.diff(cell_of) >= 0) and cell_of.max() < counts[t % G]
             lt = leaves[leaves[:, 0] == t]
             assert lt[:, 2].sum() == 500 and lt[:, 2].max() <= leaf
             for _, s0, m in lt:   # no leaf straddles two cells
@@ -304,10 +357,32 @@ def test_cells_plan_rule():
     assert oracle.cells_for(6_350_000) == 4134 and oracle.cells_for(10, 5) == 5
     x, q, sk = _small(n=300)
     assert oracle.cells_device(q, 4, 64) is None          # auto: too few rows
-    C, G, z, w, A = oracle.cells_device(q, 4, 64, cells=5, partitions=3)
-    assert (C, G) == (5, 3) and A.shape == (3, 300) and z.shape == (3, 5, 128)
+    counts, G, z, w, A = oracle.cells_device(q, 4, 64, cells=5, partitions=3, sample=2)
+    assert G == 3 and A.shape == (3, 300) and len(z) == 3 and z[0].shape[1] == 128
+    assert all(1 <= c <= 20 for c in counts)
     assert oracle.cells_device(q, 4, 400, cells=5) is None   # the whole set is one leaf
     assert oracle.cells_device(q, 1, 64, cells=5, partitions=2)[1] == 1   # G <= trees
+
+
+def test_init_big_leaves_bruteforce():
+    """Leaves above 64 rows: the init equals the brute-force top-K over all
+    leaf mates (the device keeps each member's 64 best per tree)."""
+    x, q, sk = _small(seed=6, n=700, d=128, clusters=5)
+    plan = oracle.cells_device(q, 2, 300, cells=4, partitions=2, sample=2)
+    perm, leaves = oracle.forest_cells_device(q, 2, 300, plan)
+    assert leaves[:, 2].max() > 64
+    ids, sc = oracle.init_device(q, perm, leaves, 50)
+    Q = q.astype(np.int64)
+    for i in range(0, 700, 37):
+        mates = set()
+        for t, s0, m in leaves:
+            mem = perm[t, s0:s0 + m]
+            if i in mem:
+                mates |= set(mem.tolist())
+        mates.discard(i)
+        cand = sorted(((int(Q[i] @ Q[j]), j) for j in mates), key=lambda e: (-e[0], e[1]))[:50]
+        assert ids[i].tolist() == [j for _, j in cand]
+        assert sc[i].tolist() == [s for s, _ in cand]
 
 
 def _topk_unique(cands, K):
