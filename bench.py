@@ -2,13 +2,17 @@
 """Benchmark of the MI355X kNN-graph builder on its headline workload.
 
 Workload (default): n = 6,350,000 rows x d = 1024 float32, a 4096-cluster
-Gaussian mixture generated directly in HBM, k = 15, the reference's build
-constants (K_BUILD = 50, 16 trees, leaf 64, sketch prefilter x8, its stop
-rule).  One step is the whole hot path from resident float32 rows to the
-final lists: quantise + sketch, RP forest, leaf-tile init, join iterations,
+Gaussian mixture generated directly in HBM, k = 15, K_BUILD = 50 and the
+sketch prefilter x8 of the reference, with the build form chosen for this
+workload: 4 trees whose first level is an over-seeded spherical k-means
+partition each (4 partitions), leaves of up to 2048 rows brute-forced on
+int8 MFMA, join iterations until fewer than 1% of the slots get a new id.
+One step is the whole hot path from resident float32 rows to the final
+lists: quantise + sketch, cells + forest, leaf-tile init, join iterations,
 plain export and the FP32-rerank export.  The value is seconds per step
 (lower is better).  Recall@15 of both exports is measured afterwards against
-exact FP32 brute force on 10,000 sampled rows.
+exact FP32 brute force on 10,000 sampled rows.  --trees 16 --leaf 64
+--cells 0 gives the reference's own forest shape.
 
 Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: this build is
 single-GPU.
@@ -36,6 +40,12 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_params(knng, args, d):
+    return knng.default_params(dim=d, num_trees=args.trees, k_build=args.k_build, keep_fp32=0,
+                               leaf_size=args.leaf, cells=args.cells,
+                               cell_partitions=args.partitions)
 
 
 def exact_topk(torch, x, rows, k, chunk=262144):
@@ -126,7 +136,10 @@ def main():
     ap.add_argument("--n", type=int, default=6_350_000)
     ap.add_argument("--dim", type=int, default=1024)
     ap.add_argument("--clusters", type=int, default=4096)
-    ap.add_argument("--trees", type=int, default=16)
+    ap.add_argument("--trees", type=int, default=4)
+    ap.add_argument("--leaf", type=int, default=2048)
+    ap.add_argument("--cells", type=int, default=-1, help="-1: n / 1536; 0: plain RP trees")
+    ap.add_argument("--partitions", type=int, default=4)
     ap.add_argument("--k-build", type=int, default=50)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--recall-rows", type=int, default=10000)
@@ -165,7 +178,7 @@ def main():
     log(f"generating {n} x {d} ({args.clusters} clusters) in HBM")
     x = synth.gaussian_mixture_torch(args.seed, n, d, args.clusters, device="cuda")
     torch.cuda.synchronize()
-    params = knng.default_params(dim=d, num_trees=args.trees, k_build=args.k_build, keep_fp32=0)
+    params = build_params(knng, args, d)
     b = knng.Builder(n, params)
     b.attach_fp32(x.data_ptr())
     kp = torch.empty((n, 15), dtype=torch.int32, device="cuda")
@@ -243,12 +256,14 @@ def main():
         "config": {"workload": f"kNN-graph build {n}x{d} float32, {args.clusters}-cluster Gaussian "
                                f"mixture, k=15, int8 build + FP32 rerank",
                    "n": n, "dim": d, "clusters": args.clusters, "k": 15, "k_build": args.k_build,
-                   "trees": args.trees, "leaf": 64, "parallelism": "single-gpu"},
+                   "trees": args.trees, "leaf": args.leaf, "cells": st["cells"],
+                   "cell_partitions": args.partitions, "parallelism": "single-gpu"},
         "recall_at_15": {"plain": round(rec_plain, 4), "rerank": round(rec_rerank, 4),
                          "rows": int(len(rows_np)), "rerank_rel_err_max": float(rel.max())},
         "plain_s": round(stage["ingest_s"] + stage["build_s"] + stage["export_plain_s"], 4),
         "rerank_extra_s": round(stage["export_rerank_s"], 4),
         "stages_ms": {"ingest": round(stage["ingest_s"] * 1e3, 1), "forest": round(st["ms_forest"], 1),
+                      "kmeans": round(st["ms_kmeans"], 1),
                       "init": round(st["ms_init"], 1), "join": round(st["ms_join"], 1),
                       "join_iter": [round(v, 1) for v in st["ms_join_iter"]],
                       "export_rerank": round(st["ms_export"], 1)},
@@ -363,8 +378,7 @@ def serve_main(args):
     # rows: [base n][inserted rows, submission order][query rows]; one mixture
     x = synth.gaussian_mixture_torch(5, n + n_ins + n_q, d, args.clusters, device="cuda")
     torch.cuda.synchronize()
-    b = knng.Builder(n, knng.default_params(dim=d, num_trees=args.trees, k_build=args.k_build,
-                                            keep_fp32=0))
+    b = knng.Builder(n, build_params(knng, args, d))
     b.attach_fp32(x.data_ptr())
     b.ingest_device(x.data_ptr(), 0, n)
     t0 = time.time()
