@@ -328,14 +328,32 @@ __global__ void k_hier_append(const Node* __restrict__ nodes, int nnodes, Node* 
 // r = row_off + j * row_stride, ties to the smaller c; EXCL skips c == j
 // (rows and centroids are the same set); score[j] (if given) = the best value.
 // One block = 128 rows, wave v owns rows 32v..32v+31 and keeps their A
-// fragments for the whole pass in registers; 32-cell centroid tiles are
-// staged in LDS and shared by the four waves; each wave runs
-// v_mfma_i32_32x32x32_i8 over the tile and keeps a running best per
-// accumulator slot, reduced across lanes once at the end.
+// fragments for the whole pass in registers.  32-cell centroid tiles are
+// double-buffered in LDS: the next tile streams in with direct global->LDS
+// loads (16 bytes per lane, 64 lanes = 1 KB contiguous) while the MFMAs run
+// on the current one.  Row r of a tile holds its 16-byte chunk c at slot
+// (c - r) mod (d/16), so the 32 rows a B fragment read touches fall on
+// different banks.  Even and odd k-steps accumulate into two chains
+// (exact int32 sums, added at the end) to keep two MFMAs in flight.
 // ---------------------------------------------------------------------------
 
 constexpr int kAssignRows = 128;
 constexpr int kAssignTile = 32;
+
+// issue the direct loads of centroid tile [c0, c0 + nc) into `buf` (this wave's share)
+__device__ __forceinline__ void assign_issue_tile(const int8_t* __restrict__ Z, int d, int cpr, int c0,
+                                                  int nc, int8_t* buf, int wv, int lane) {
+  const int chunks = nc * cpr;
+  for (int j = wv; j * 64 < chunks; j += 4) {
+    const int f = j * 64 + lane;
+    const int r = f / cpr, p = f - r * cpr;
+    if (f < chunks) {
+      int c = p + r % cpr;
+      if (c >= cpr) c -= cpr;
+      __builtin_amdgcn_global_load_lds(Z + int64_t(c0 + r) * d + 16 * c, buf + 1024 * j, 16, 0, 0);
+    }
+  }
+}
 
 template <bool EXCL>
 __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, int64_t nrows,
@@ -344,16 +362,18 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
                                                 const float* __restrict__ w, int C,
                                                 int32_t* __restrict__ out,
                                                 float* __restrict__ score) {
-  __shared__ __attribute__((aligned(16))) int8_t ztile[kAssignTile * (kMaxDim + 16)];
-  const int stride = d + 16;
+  __shared__ __attribute__((aligned(1024))) int8_t ztile[2][kAssignTile * kMaxDim];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int ksteps = d >> 5;
+  const int ksteps = d >> 5, cpr = d >> 4;
+  const int ntiles = (C + kAssignTile - 1) / kAssignTile;
+  const int cl = lane & 31, hi = lane >> 5;
+  const int rrot = cl % cpr;   // rotation of the B row this lane reads
   for (int64_t r0 = int64_t(blockIdx.x) * kAssignRows; r0 < nrows;
        r0 += int64_t(gridDim.x) * kAssignRows) {
     // A fragments: row r0 + 32*wv + (lane&31), bytes 32*ks + 16*(lane>>5)
-    const int64_t arow = r0 + 32 * wv + (lane & 31);
+    const int64_t arow = r0 + 32 * wv + cl;
     const bool arow_ok = arow < nrows;
-    const int8_t* abase = q + (row_off + arow * row_stride) * d + 16 * (lane >> 5);
+    const int8_t* abase = q + (row_off + (arow_ok ? arow : 0) * row_stride) * d + 16 * hi;
     v4i a[32];
 #pragma unroll
     for (int ks = 0; ks < 32; ++ks)
@@ -366,38 +386,49 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
       best[j] = -INFINITY;
       bidx[j] = 0x7FFFFFFF;
     }
-    for (int c0 = 0; c0 < C; c0 += kAssignTile) {
+    __syncthreads();   // both buffers free (previous row tile done)
+    assign_issue_tile(Z, d, cpr, 0, C < kAssignTile ? C : kAssignTile, ztile[0], wv, lane);
+    for (int t = 0; t < ntiles; ++t) {
+      const int c0 = t * kAssignTile;
       const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
-      __syncthreads();   // previous tile fully consumed
-      for (int u = tid; u < nc * (d >> 4); u += 256) {
-        const int r = u / (d >> 4), c = u - r * (d >> 4);
-        *reinterpret_cast<v4i*>(ztile + r * stride + 16 * c) =
-            *reinterpret_cast<const v4i*>(Z + int64_t(c0 + r) * d + 16 * c);
+      __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of tile t have landed
+      __syncthreads();                 // all of tile t visible; tile t-1's buffer free
+      if (t + 1 < ntiles) {
+        const int c1 = c0 + kAssignTile;
+        assign_issue_tile(Z, d, cpr, c1, (C - c1) < kAssignTile ? (C - c1) : kAssignTile,
+                          ztile[(t + 1) & 1], wv, lane);
       }
-      __syncthreads();
-      {
-        const int cl = lane & 31;   // B column (cell) of this lane
-        const int8_t* rb = ztile + (cl < nc ? cl : 0) * stride + 16 * (lane >> 5);
-        v16i acc;
+      const int8_t* rb = ztile[t & 1] + cl * d;
+      v16i acc0, acc1;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) acc[j] = 0;
+      for (int j = 0; j < 16; ++j) {
+        acc0[j] = 0;
+        acc1[j] = 0;
+      }
 #pragma unroll
-        for (int ks = 0; ks < 32; ++ks)
-          if (ks < ksteps)
-            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], *reinterpret_cast<const v4i*>(rb + 32 * ks),
-                                                        acc, 0, 0, 0);
-        const int cell = c0 + cl;
-        const bool ok = cl < nc;
-        const float wc = ok ? w[cell] : 0.0f;
+      for (int ks = 0; ks < 32; ks += 2) {
+        if (ks < ksteps) {
+          int p0 = 2 * ks + hi - rrot;
+          if (p0 < 0) p0 += cpr;
+          int p1 = 2 * ks + 2 + hi - rrot;
+          if (p1 < 0) p1 += cpr;
+          acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], *reinterpret_cast<const v4i*>(rb + 16 * p0),
+                                                       acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks + 1], *reinterpret_cast<const v4i*>(rb + 16 * p1),
+                                                       acc1, 0, 0, 0);
+        }
+      }
+      const int cell = c0 + cl;
+      const bool ok = cl < nc;
+      const float wc = ok ? w[cell] : 0.0f;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          const float s = float(acc[j]) * wc;
-          // acc element j of lane l holds row 8*(j/4) + 4*(l/32) + (j%4) of the wave's 32
-          const bool self = EXCL && (r0 + 32 * wv + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3)) == cell;
-          if (ok && !self && s > best[j]) {   // cells arrive in increasing order: ties keep the smaller
-            best[j] = s;
-            bidx[j] = cell;
-          }
+      for (int j = 0; j < 16; ++j) {
+        const float s = float(acc0[j] + acc1[j]) * wc;
+        // acc element j of lane l holds row 8*(j/4) + 4*(l/32) + (j%4) of the wave's 32
+        const bool self = EXCL && (r0 + 32 * wv + 8 * (j >> 2) + 4 * hi + (j & 3)) == cell;
+        if (ok && !self && s > best[j]) {   // cells arrive in increasing order: ties keep the smaller
+          best[j] = s;
+          bidx[j] = cell;
         }
       }
     }
@@ -413,8 +444,8 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
           bidx[j] = oi;
         }
       }
-      const int64_t row = r0 + 32 * wv + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-      if ((lane & 31) == 0 && row < nrows) {
+      const int64_t row = r0 + 32 * wv + 8 * (j >> 2) + 4 * hi + (j & 3);
+      if (cl == 0 && row < nrows) {
         out[row] = bidx[j];
         if (score) score[row] = best[j];
       }
