@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -453,6 +454,88 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
   }
 }
 
+// out[ids[j]] = rows[j] (d bytes each)
+__global__ __launch_bounds__(256) void k_scatter_rows(const int8_t* __restrict__ rows, int d,
+                                                      const int64_t* __restrict__ ids, int64_t m,
+                                                      int8_t* __restrict__ out) {
+  const int per = d >> 4;
+  for (int64_t u = int64_t(blockIdx.x) * 256 + threadIdx.x; u < m * per;
+       u += int64_t(gridDim.x) * 256) {
+    const int64_t j = u / per;
+    const int c = int(u - j * per);
+    *reinterpret_cast<v4i*>(out + ids[j] * d + 16 * c) =
+        *reinterpret_cast<const v4i*>(rows + j * d + 16 * c);
+  }
+}
+
+// Centroid of cell c from its exact sum S_c (int8 rows or int64 sums):
+// z_k = trunc(S_k * 127 / max_k |S_k|), w = float(1 / sqrt(sum z_k^2)); an
+// all-zero sum gives z = 0, w = 0.  One block per listed cell; in place is
+// allowed (every S value is read before any z value is written).
+template <typename TS>
+__global__ __launch_bounds__(256) void k_quant_centroids(const TS* S, int d, const int32_t* __restrict__ cells,
+                                                         int m, int8_t* z, float* __restrict__ w) {
+  __shared__ long long red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = blockIdx.x; i < m; i += gridDim.x) {
+    const int64_t cc = cells ? cells[i] : i;
+    int64_t v[4];
+    long long mx = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = tid + 256 * u;
+      v[u] = k < d ? int64_t(S[cc * d + k]) : 0;
+      const long long av = v[u] < 0 ? -v[u] : v[u];
+      mx = av > mx ? av : mx;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const long long ot = __shfl_xor(mx, o);
+      mx = ot > mx ? ot : mx;
+    }
+    if (lane == 0) red[wv] = mx;
+    __syncthreads();
+    long long M = red[0];
+    for (int u = 1; u < 4; ++u) M = red[u] > M ? red[u] : M;
+    __syncthreads();
+    long long ss = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = tid + 256 * u;
+      if (k < d) {
+        const int8_t zk = M ? int8_t((v[u] * 127) / M) : int8_t(0);
+        z[cc * d + k] = zk;
+        ss += int(zk) * int(zk);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
+    if (lane == 0) red[wv] = ss;
+    __syncthreads();
+    if (tid == 0) {
+      const long long t = red[0] + red[1] + red[2] + red[3];
+      w[cc] = t > 0 ? float(1.0 / sqrt(double(t))) : 0.0f;
+    }
+    __syncthreads();
+  }
+}
+
+// S[pairs[2i]] += S[pairs[2i+1]] (int64 rows of d)
+__global__ __launch_bounds__(256) void k_merge_sums(int64_t* __restrict__ S, int d,
+                                                    const int32_t* __restrict__ pairs, int m) {
+  for (int i = blockIdx.x; i < m; i += gridDim.x) {
+    int64_t* dst = S + int64_t(pairs[2 * i]) * d;
+    const int64_t* src = S + int64_t(pairs[2 * i + 1]) * d;
+    for (int k = threadIdx.x; k < d; k += 256) dst[k] += src[k];
+  }
+}
+
+// w[cells[i]] = NaN: a dead cell never wins an argmax
+__global__ void k_kill_cells(float* __restrict__ w, const int32_t* __restrict__ cells, int m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) w[cells[i]] = __builtin_nanf("");
+}
+
 // out[j] = row ids[j] (d bytes each)
 __global__ __launch_bounds__(256) void k_gather_rows(const int8_t* __restrict__ q, int d,
                                                      const int64_t* __restrict__ ids, int64_t m,
@@ -578,14 +661,33 @@ __global__ __launch_bounds__(256) void k_leaf_gram(const int8_t* __restrict__ q,
 // K3b: leaves of 65..kMaxLeaf members.  A work item is (leaf, 128 members);
 // wave v owns members r0+32v .. r0+32v+31 and holds their A fragments in
 // registers; the leaf's members stream through LDS as 32-row B tiles shared
-// by the four waves; v_mfma_i32_32x32x32_i8 scores go to the block's scratch
-// rows.  Then each wave keeps, per member, the 64 best mates by (score desc,
-// id asc), self excluded: a bisection finds the 64th score, and a second
-// bisection over ids settles ties at that score.  The keys fill the member's
-// 64 slots for this tree (order irrelevant: K4 sorts them).
+// by the four waves, double-buffered with direct global->LDS loads (rows in
+// the rotated chunk layout of k_assign); v_mfma_i32_32x32x32_i8 scores go
+// to the block's scratch rows.  Then each wave keeps, per member, the 64
+// best mates by (score desc, id asc), self excluded: a bisection finds the
+// 64th score, and a second bisection over ids settles ties at that score;
+// counts are ballot popcounts.  The keys fill the member's 64 slots for this
+// tree (order irrelevant: K4 sorts them).
 // ---------------------------------------------------------------------------
 
 constexpr int kBigRows = 128;
+
+__device__ __forceinline__ void big_issue_tile(const int8_t* __restrict__ q, int d, int cpr,
+                                               const int32_t* mid, int c0, int nc, int8_t* buf,
+                                               int wv, int lane) {
+  const int chunks = nc * cpr;
+  for (int j = wv; j * 64 < chunks; j += 4) {
+    const int f = j * 64 + lane;
+    const int r = f / cpr, p = f - r * cpr;
+    if (f < chunks) {
+      int c = p + r % cpr;
+      if (c >= cpr) c -= cpr;
+      __builtin_amdgcn_global_load_lds(q + int64_t(mid[c0 + r]) * d + 16 * c, buf + 1024 * j, 16, 0, 0);
+    }
+  }
+}
+
+__device__ __forceinline__ int ballot_count(bool p) { return __popcll(__ballot(p)); }
 
 __global__ __launch_bounds__(256) void k_leaf_big(const int8_t* __restrict__ q, int d, int64_t n, int T,
                                                   const int32_t* __restrict__ perm,
@@ -593,10 +695,12 @@ __global__ __launch_bounds__(256) void k_leaf_big(const int8_t* __restrict__ q, 
                                                   int32_t* __restrict__ scratch, int mpad,
                                                   uint64_t* __restrict__ cand) {
   __shared__ int32_t mid[kMaxLeaf];
-  __shared__ __attribute__((aligned(16))) int8_t tile[32 * (kMaxDim + 16)];
+  __shared__ __attribute__((aligned(1024))) int8_t tile[2][32 * kMaxDim];
   constexpr int kPer = kMaxLeaf / 64;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int stride = d + 16, ksteps = d >> 5, upr = d >> 4;
+  const int ksteps = d >> 5, cpr = d >> 4;
+  const int cl = lane & 31, hi = lane >> 5;
+  const int rrot = cl % cpr;
   int32_t* scr = scratch + int64_t(blockIdx.x) * kBigRows * mpad;
   for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int4 item = items[it];   // {tree, start, size, first member}
@@ -607,39 +711,52 @@ __global__ __launch_bounds__(256) void k_leaf_big(const int8_t* __restrict__ q, 
     const bool wave_on = r0 + 32 * wv < size;
     v4i a[32];
     {
-      const int arow = r0 + 32 * wv + (lane & 31);
+      const int arow = r0 + 32 * wv + cl;
       const bool ok = arow < size;
-      const int8_t* abase = q + int64_t(ok ? mid[arow] : 0) * d + 16 * (lane >> 5);
+      const int8_t* abase = q + int64_t(ok ? mid[arow] : 0) * d + 16 * hi;
 #pragma unroll
       for (int ks = 0; ks < 32; ++ks)
         a[ks] = (ok && ks < ksteps) ? *reinterpret_cast<const v4i*>(abase + 32 * ks) : v4i{0, 0, 0, 0};
     }
-    for (int c0 = 0; c0 < size; c0 += 32) {
+    const int ntiles = (size + 31) >> 5;
+    big_issue_tile(q, d, cpr, mid, 0, size < 32 ? size : 32, tile[0], wv, lane);
+    for (int tt = 0; tt < ntiles; ++tt) {
+      const int c0 = tt * 32;
       const int nc = (size - c0) < 32 ? (size - c0) : 32;
-      __syncthreads();   // tile free
-      for (int u = tid; u < nc * upr; u += 256) {
-        const int r = u / upr, cc = u - r * upr;
-        *reinterpret_cast<v4i*>(tile + r * stride + 16 * cc) =
-            *reinterpret_cast<const v4i*>(q + int64_t(mid[c0 + r]) * d + 16 * cc);
+      __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of tile tt have landed
+      __syncthreads();                 // tile tt visible; tile tt-1's buffer free
+      if (tt + 1 < ntiles) {
+        const int c1 = c0 + 32;
+        big_issue_tile(q, d, cpr, mid, c1, (size - c1) < 32 ? (size - c1) : 32, tile[(tt + 1) & 1], wv,
+                       lane);
       }
-      __syncthreads();
       if (wave_on) {
-        const int cl = lane & 31;
-        const int8_t* rb = tile + (cl < nc ? cl : 0) * stride + 16 * (lane >> 5);
-        v16i acc;
+        const int8_t* rb = tile[tt & 1] + cl * d;
+        v16i acc0, acc1;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) acc[j] = 0;
+        for (int j = 0; j < 16; ++j) {
+          acc0[j] = 0;
+          acc1[j] = 0;
+        }
 #pragma unroll
-        for (int ks = 0; ks < 32; ++ks)
-          if (ks < ksteps)
-            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], *reinterpret_cast<const v4i*>(rb + 32 * ks),
-                                                        acc, 0, 0, 0);
+        for (int ks = 0; ks < 32; ks += 2)
+          if (ks < ksteps) {
+            int p0 = 2 * ks + hi - rrot;
+            if (p0 < 0) p0 += cpr;
+            int p1 = 2 * ks + 2 + hi - rrot;
+            if (p1 < 0) p1 += cpr;
+            acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], *reinterpret_cast<const v4i*>(rb + 16 * p0),
+                                                         acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks + 1],
+                                                         *reinterpret_cast<const v4i*>(rb + 16 * p1), acc1,
+                                                         0, 0, 0);
+          }
         if (cl < nc) {
 #pragma unroll
           for (int j = 0; j < 16; ++j) {
             // acc element j of lane l: row 8*(j/4) + 4*(l/32) + (j%4) of the wave's 32, column l%32
-            const int rl = 32 * wv + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-            scr[int64_t(rl) * mpad + c0 + cl] = acc[j];
+            const int rl = 32 * wv + 8 * (j >> 2) + 4 * hi + (j & 3);
+            scr[int64_t(rl) * mpad + c0 + cl] = acc0[j] + acc1[j];
           }
         }
       }
@@ -652,38 +769,38 @@ __global__ __launch_bounds__(256) void k_leaf_big(const int8_t* __restrict__ q, 
       if (m >= size) break;
       const int32_t* row = scr + int64_t(32 * wv + rr) * mpad;
       int32_t v[kPer];
-      int lmin = INT32_MAX, lmax = INT32_MIN;
+      int lmax = INT32_MIN;
 #pragma unroll
       for (int e = 0; e < kPer; ++e) {
         const int col = lane + 64 * e;
         v[e] = (col < size && col != m) ? row[col] : INT32_MIN;   // scores are > -2^25
-        if (v[e] != INT32_MIN) {
-          lmin = min(lmin, v[e]);
-          lmax = max(lmax, v[e]);
-        }
+        lmax = max(lmax, v[e]);
       }
-      // largest theta with |{v >= theta}| >= R
-      int lo = wave_min_i(lmin), hi = wave_max_i(lmax);
-      while (lo < hi) {
-        const int md = lo + (hi - lo + 1) / 2;
+      // largest theta with |{v >= theta}| >= R.  When every lane holds a valid
+      // entry, the 64 lane maxima are distinct entries, so their minimum has at
+      // least 64 >= R entries at or above it; otherwise search the full range.
+      int lo = wave_min_i(lmax), hi2 = wave_max_i(lmax);
+      if (ballot_count(lmax != INT32_MIN) < 64) lo = -(1 << 25);
+      while (lo < hi2) {
+        const int md = lo + (hi2 - lo + 1) / 2;
         int cnt = 0;
 #pragma unroll
-        for (int e = 0; e < kPer; ++e) cnt += v[e] >= md ? 1 : 0;
-        if (wave_sum_i(cnt) >= R)
+        for (int e = 0; e < kPer; ++e) cnt += ballot_count(v[e] >= md);
+        if (cnt >= R)
           lo = md;
         else
-          hi = md - 1;
+          hi2 = md - 1;
       }
       const int theta = lo;
       int cgt = 0, ceq = 0;
 #pragma unroll
       for (int e = 0; e < kPer; ++e) {
-        cgt += v[e] > theta ? 1 : 0;
-        ceq += v[e] == theta ? 1 : 0;
+        cgt += ballot_count(v[e] > theta);
+        ceq += ballot_count(v[e] == theta);
       }
-      const int need = R - wave_sum_i(cgt);
+      const int need = R - cgt;
       int idcut = INT32_MAX;
-      if (wave_sum_i(ceq) > need) {
+      if (ceq > need) {
         // smallest idcut with |{v == theta, id <= idcut}| >= need
         int ilo = 0, ihi = int(n - 1);
         while (ilo < ihi) {
@@ -692,9 +809,9 @@ __global__ __launch_bounds__(256) void k_leaf_big(const int8_t* __restrict__ q, 
 #pragma unroll
           for (int e = 0; e < kPer; ++e) {
             const int col = lane + 64 * e;
-            cnt += (v[e] == theta && mid[col < size ? col : 0] <= md) ? 1 : 0;
+            cnt += ballot_count(v[e] == theta && mid[col < size ? col : 0] <= md);
           }
-          if (wave_sum_i(cnt) >= need)
+          if (cnt >= need)
             ihi = md;
           else
             ilo = md + 1;
@@ -1313,7 +1430,7 @@ int quantize_rows(knng_ctx* c, const float* dx, int64_t row0, int64_t nrows) {
 // ---------------------------------------------------------------------------
 // Cells.  A centroid is stored as an int8 direction z (largest component
 // +-127) with w = 1/|z|; a row goes to argmax_c float(<q, z_c>) * w_c.
-// Centroids come from exact integer sums S on the host:
+// Centroids come from exact integer sums S (k_cell_sum, k_quant_centroids):
 // z_k = trunc(S_k * 127 / max_k |S_k|), w = float(1/sqrt(sum z^2)).
 //
 // One partition g:
@@ -1337,22 +1454,6 @@ int32_t cells_for(const knng_params& p, int64_t n) {
   if (p.cells >= 0) return p.cells;
   const int64_t c = n / 1536;
   return c >= 2 ? int32_t(c) : 0;
-}
-
-void quantize_centroid(const int64_t* S, int d, int8_t* z, float* w) {
-  int64_t M = 0;
-  for (int k = 0; k < d; ++k) M = std::max<int64_t>(M, S[k] < 0 ? -S[k] : S[k]);
-  if (M == 0) {
-    std::fill(z, z + d, int8_t(0));
-    *w = 0.0f;
-    return;
-  }
-  double ss = 0.0;
-  for (int k = 0; k < d; ++k) {
-    z[k] = int8_t((S[k] * 127) / M);
-    ss += double(z[k]) * double(z[k]);
-  }
-  *w = float(1.0 / std::sqrt(ss));
 }
 
 // stable counting sort of the listed rows by cell: members[offs[c]..offs[c+1])
@@ -1379,9 +1480,9 @@ struct KmeansBufs {
   int64_t* ids = nullptr;    // seed row ids [cap]
 };
 
-// sums of the listed rows per cell (device), downloaded into S [C][d]
+// exact sums of the listed rows per cell into B.sums (device); counts on the host
 int cell_sums(knng_ctx* c, KmeansBufs& B, const std::vector<int32_t>& a, int C, int64_t off,
-              int64_t stride, std::vector<int64_t>* S, std::vector<int64_t>* cnt) {
+              int64_t stride, std::vector<int64_t>* cnt) {
   std::vector<int32_t> members;
   std::vector<int64_t> offs;
   cell_lists(a, C, off, stride, &members, &offs);
@@ -1391,28 +1492,35 @@ int cell_sums(knng_ctx* c, KmeansBufs& B, const std::vector<int32_t>& a, int C, 
   hipLaunchKernelGGL(k_cell_sum, dim3(std::max(1, std::min(C, c->num_cu * 8))), dim3(256), 0,
                      c->stream, c->q, c->d, B.members, B.offs, C, B.sums);
   HIPCHK(c, hipGetLastError());
-  S->resize(size_t(C) * c->d);
-  HIPCHK(c, hipMemcpyAsync(S->data(), B.sums, S->size() * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
   cnt->resize(size_t(C));
   for (int k = 0; k < C; ++k) (*cnt)[size_t(k)] = offs[size_t(k) + 1] - offs[size_t(k)];
+  HIPCHK(c, hipStreamSynchronize(c->stream));   // members / offs are host temporaries
   return 0;
 }
 
-int assign_rows_dev(knng_ctx* c, const int8_t* rows, int64_t nrows, int64_t off, int64_t stride,
-                    const std::vector<int8_t>& z, const std::vector<float>& w, int C, int32_t* out,
-                    KmeansBufs& B, bool excl, float* score) {
-  HIPCHK(c, hipMemcpyAsync(B.z, z.data(), size_t(C) * c->d, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(B.w, w.data(), size_t(C) * 4, hipMemcpyHostToDevice, c->stream));
-  if (launch_assign(c->stream, rows, nrows, c->d, B.z, B.w, C, out, c->num_cu, off, stride, excl,
-                    score))
+// requantise the listed cells' centroids (B.z / B.w rows) from B.sums
+int requantize(knng_ctx* c, KmeansBufs& B, const std::vector<int32_t>& cells, int8_t* z, float* w) {
+  if (cells.empty()) return 0;
+  HIPCHK(c, hipMemcpyAsync(B.a, cells.data(), cells.size() * 4, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_quant_centroids<int64_t>, dim3(std::min<int>(int(cells.size()), c->num_cu * 16)),
+                     dim3(256), 0, c->stream, B.sums, c->d, B.a, int(cells.size()), z, w);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));   // `cells` is a host temporary
+  return 0;
+}
+
+int assign_launch(knng_ctx* c, const int8_t* rows, int64_t nrows, int64_t off, int64_t stride,
+                  const int8_t* z, const float* w, int C, int32_t* out, bool excl, float* score) {
+  if (launch_assign(c->stream, rows, nrows, c->d, z, w, C, out, c->num_cu, off, stride, excl, score))
     return fail(c, KNNG_EHIP, std::string("k_assign launch: ") + hipGetErrorString(hipGetLastError()));
   return 0;
 }
 
 // k-means of partition g; leaves the final assignment of every row in
-// c->cassign + g*n and the centroids in *zf / *wf (C_g of them)
-int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* merge_rows,
+// c->cassign + g*n and the centroids in *zf / *wf (C_g of them).  Centroids,
+// sums and merges stay on the device; the host holds assignments, counts and
+// the merge decisions.
+int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* zfin,
                std::vector<int8_t>* zf, std::vector<float>* wf) {
   const int64_t n = c->n;
   const int d = c->d;
@@ -1421,83 +1529,88 @@ int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* merge_rows,
   const int64_t ns = (n - off + stride - 1) / stride;
   const int Cs = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(Ct) * c->p.cell_oversample, ns)));
   int rc;
-  // 1. seeds
+  // 1. seeds: gathered rows, quantised in place
   std::vector<int64_t> ids(Cs);
   for (int k = 0; k < Cs; ++k)
     ids[size_t(k)] = ((2 * int64_t(k) + 1) * n / (2 * int64_t(Cs)) + int64_t(g) * 7919) % n;
   HIPCHK(c, hipMemcpyAsync(B.ids, ids.data(), ids.size() * 8, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(c, int64_t(Cs) * (d >> 4), 256, 8)), dim3(256), 0,
                      c->stream, c->q, d, B.ids, int64_t(Cs), B.z);
+  hipLaunchKernelGGL(k_quant_centroids<int8_t>, dim3(std::min(Cs, c->num_cu * 16)), dim3(256), 0,
+                     c->stream, B.z, d, nullptr, Cs, B.z, B.w);
   HIPCHK(c, hipGetLastError());
-  std::vector<int8_t> z(size_t(Cs) * d);
-  std::vector<float> w(Cs);
-  HIPCHK(c, hipMemcpyAsync(z.data(), B.z, z.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  {
-    std::vector<int64_t> S(d);
-    for (int k = 0; k < Cs; ++k) {
-      for (int j = 0; j < d; ++j) S[size_t(j)] = z[size_t(k) * d + j];
-      quantize_centroid(S.data(), d, &z[size_t(k) * d], &w[size_t(k)]);
-    }
-  }
+  trace_mark(c, "kmeans seeds");
   // 2. Lloyd on the sample
-  std::vector<int32_t> a(ns);
-  std::vector<int64_t> S, cnt;
+  std::vector<int32_t> a(ns), upd;
+  std::vector<int64_t> cnt;
   for (int it = 0;; ++it) {
-    if ((rc = assign_rows_dev(c, c->q, ns, off, stride, z, w, Cs, B.a, B, false, nullptr))) return rc;
+    if ((rc = assign_launch(c, c->q, ns, off, stride, B.z, B.w, Cs, B.a, false, nullptr))) return rc;
     HIPCHK(c, hipMemcpyAsync(a.data(), B.a, size_t(ns) * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if ((rc = cell_sums(c, B, a, Cs, off, stride, &S, &cnt))) return rc;
+    trace_mark(c, "kmeans sample assign");
+    if ((rc = cell_sums(c, B, a, Cs, off, stride, &cnt))) return rc;
     if (it == c->p.kmeans_iters) break;
+    upd.clear();
     for (int k = 0; k < Cs; ++k)
-      if (cnt[size_t(k)]) quantize_centroid(&S[size_t(k) * d], d, &z[size_t(k) * d], &w[size_t(k)]);
+      if (cnt[size_t(k)]) upd.push_back(k);   // an empty cell keeps its centroid
+    if ((rc = requantize(c, B, upd, B.z, B.w))) return rc;
+    trace_mark(c, "kmeans sample update");
   }
-  // 3. merging
-  std::vector<int32_t> live;
-  for (int k = 0; k < Cs; ++k)
-    if (cnt[size_t(k)]) {
-      live.push_back(k);
-      quantize_centroid(&S[size_t(k) * d], d, &z[size_t(k) * d], &w[size_t(k)]);
-    }
-  std::vector<int8_t> zl;
-  std::vector<float> wl;
-  std::vector<int32_t> best;
-  std::vector<float> bval;
+  // 3. merging.  Live cells get centroids from their final sample sums; a dead
+  // cell's weight is NaN on the device, so it never wins an argmax (float
+  // compares with NaN are false) and each round's result equals the argmax
+  // over the live cells alone.
+  std::vector<int32_t> live, dead;
+  for (int k = 0; k < Cs; ++k) (cnt[size_t(k)] ? live : dead).push_back(k);
+  if ((rc = requantize(c, B, live, B.z, B.w))) return rc;
+  if (!dead.empty()) {
+    HIPCHK(c, hipMemcpyAsync(B.a, dead.data(), dead.size() * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_kill_cells, dim3((int(dead.size()) + 255) / 256), dim3(256), 0, c->stream, B.w,
+                       B.a, int(dead.size()));
+    HIPCHK(c, hipGetLastError());
+  }
+  std::vector<int32_t> best(static_cast<size_t>(Cs)), pairs, merged_into, killed;
+  std::vector<float> bval(static_cast<size_t>(Cs)), w(static_cast<size_t>(Cs));
+  std::vector<char> gone(static_cast<size_t>(Cs), 0);
   for (int round = 0; round < 64 && live.size() >= 2; ++round) {
-    const int m = int(live.size());
-    zl.resize(size_t(m) * d);
-    wl.resize(size_t(m));
-    for (int j = 0; j < m; ++j) {
-      std::copy_n(&z[size_t(live[size_t(j)]) * d], d, &zl[size_t(j) * d]);
-      wl[size_t(j)] = w[size_t(live[size_t(j)])];
-    }
-    HIPCHK(c, hipMemcpyAsync(merge_rows, zl.data(), zl.size(), hipMemcpyHostToDevice, c->stream));
-    if ((rc = assign_rows_dev(c, merge_rows, m, 0, 1, zl, wl, m, B.a, B, true, B.sc))) return rc;
-    best.resize(size_t(m));
-    bval.resize(size_t(m));
-    HIPCHK(c, hipMemcpyAsync(best.data(), B.a, size_t(m) * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(bval.data(), B.sc, size_t(m) * 4, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = assign_launch(c, B.z, Cs, 0, 1, B.z, B.w, Cs, B.a, true, B.sc))) return rc;
+    HIPCHK(c, hipMemcpyAsync(best.data(), B.a, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(bval.data(), B.sc, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(w.data(), B.w, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::vector<char> gone(size_t(m), 0);
-    int merged = 0;
-    for (int j = 0; j < m; ++j) {
+    pairs.clear();
+    merged_into.clear();
+    killed.clear();
+    for (const int32_t j : live) {
       const int32_t k = best[size_t(j)];
-      if (k <= j || k >= m || best[size_t(k)] != j) continue;
-      if (!(bval[size_t(j)] * wl[size_t(j)] > c->p.cell_merge)) continue;
-      int64_t* sj = &S[size_t(live[size_t(j)]) * d];
-      const int64_t* sk = &S[size_t(live[size_t(k)]) * d];
-      for (int t = 0; t < d; ++t) sj[t] += sk[t];
-      cnt[size_t(live[size_t(j)])] += cnt[size_t(live[size_t(k)])];
-      quantize_centroid(sj, d, &z[size_t(live[size_t(j)]) * d], &w[size_t(live[size_t(j)])]);
+      if (k <= j || k >= Cs || best[size_t(k)] != j) continue;
+      if (!(bval[size_t(j)] * w[size_t(j)] > c->p.cell_merge)) continue;
+      pairs.push_back(j);
+      pairs.push_back(k);
+      merged_into.push_back(j);
+      killed.push_back(k);
+      cnt[size_t(j)] += cnt[size_t(k)];
       gone[size_t(k)] = 1;
-      ++merged;
     }
-    if (!merged) break;
+    if (pairs.empty()) break;
     std::vector<int32_t> next;
-    for (int j = 0; j < m; ++j)
-      if (!gone[size_t(j)]) next.push_back(live[size_t(j)]);
+    for (const int32_t j : live)
+      if (!gone[size_t(j)]) next.push_back(j);
     live.swap(next);
+    const int m = int(killed.size());
+    HIPCHK(c, hipMemcpyAsync(B.a, pairs.data(), pairs.size() * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_merge_sums, dim3(std::min(m, c->num_cu * 16)), dim3(256), 0, c->stream, B.sums,
+                       d, B.a, m);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = requantize(c, B, merged_into, B.z, B.w))) return rc;
+    HIPCHK(c, hipMemcpyAsync(B.a, killed.data(), killed.size() * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_kill_cells, dim3((m + 255) / 256), dim3(256), 0, c->stream, B.w, B.a, m);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
   }
+  trace_mark(c, "kmeans merging");
   // 4. pruning
   {
     std::vector<int64_t> sz;
@@ -1509,27 +1622,44 @@ int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* merge_rows,
       if (!(float(cnt[size_t(k)]) < cut)) keep.push_back(k);
     live.swap(keep);
   }
-  // 5. Lloyd over all rows with the merged cells
+  // 5. Lloyd over all rows with the live cells, compacted in index order
   const int C = int(live.size());
-  zf->resize(size_t(C) * d);
-  wf->resize(size_t(C));
-  for (int j = 0; j < C; ++j) {
-    std::copy_n(&z[size_t(live[size_t(j)]) * d], d, &(*zf)[size_t(j) * d]);
-    (*wf)[size_t(j)] = w[size_t(live[size_t(j)])];
+  float* wfin = B.sc;   // C <= Cs floats
+  {
+    std::vector<int64_t> lids(live.begin(), live.end());
+    HIPCHK(c, hipMemcpyAsync(B.ids, lids.data(), lids.size() * 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(c, int64_t(C) * (d >> 4), 256, 8)), dim3(256), 0,
+                       c->stream, B.z, d, B.ids, int64_t(C), zfin);
+    HIPCHK(c, hipGetLastError());
+    std::vector<float> wl(static_cast<size_t>(C));
+    // current weights (cells merged in the last round were requantised after its download)
+    HIPCHK(c, hipMemcpyAsync(w.data(), B.w, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int j = 0; j < C; ++j) wl[size_t(j)] = w[size_t(live[size_t(j)])];
+    HIPCHK(c, hipMemcpyAsync(wfin, wl.data(), wl.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   int32_t* da = c->cassign + size_t(g) * n;
   std::vector<int32_t> af;
   for (int it = 0;; ++it) {
-    if ((rc = assign_rows_dev(c, c->q, n, 0, 1, *zf, *wf, C, da, B, false, nullptr))) return rc;
+    if ((rc = assign_launch(c, c->q, n, 0, 1, zfin, wfin, C, da, false, nullptr))) return rc;
+    trace_mark(c, "kmeans full assign");
     if (it == c->p.cell_full_iters) break;
     af.resize(size_t(n));
     HIPCHK(c, hipMemcpyAsync(af.data(), da, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if ((rc = cell_sums(c, B, af, C, 0, 1, &S, &cnt))) return rc;
+    if ((rc = cell_sums(c, B, af, C, 0, 1, &cnt))) return rc;
+    upd.clear();
     for (int k = 0; k < C; ++k)
-      if (cnt[size_t(k)])
-        quantize_centroid(&S[size_t(k) * d], d, &(*zf)[size_t(k) * d], &(*wf)[size_t(k)]);
+      if (cnt[size_t(k)]) upd.push_back(k);
+    if ((rc = requantize(c, B, upd, zfin, wfin))) return rc;
+    trace_mark(c, "kmeans full update");
   }
+  zf->resize(size_t(C) * d);
+  wf->resize(size_t(C));
+  HIPCHK(c, hipMemcpyAsync(zf->data(), zfin, zf->size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(wf->data(), wfin, wf->size() * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -1596,6 +1726,7 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
     HIPCHK(c, hipMemcpyAsync(c->cw + size_t(g) * Cm, ws[size_t(g)].data(), ws[size_t(g)].size() * 4,
                              hipMemcpyHostToDevice, c->stream));
   }
+  trace_mark(c, "cells uploaded");
   // host build of perm, nodepos, first-level nodes, small-cell leaves, cell roots
   std::vector<std::vector<int32_t>> mem(G);
   std::vector<std::vector<int64_t>> off(G);
@@ -1649,6 +1780,7 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
   int32_t cnt2[2] = {0, int32_t(leaves.size())};
   HIPCHK(c, hipMemcpyAsync(c->dcount, cnt2, sizeof(cnt2), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  trace_mark(c, "cell forest level");
   *nnodes = int(nodes.size());
   return 0;
 }
@@ -1955,6 +2087,7 @@ int knng_build_forest(knng_ctx* c) {
     HIPCHK(c, hipEventCreate(&k0));
     HIPCHK(c, hipEventCreate(&k1));
     HIPCHK(c, hipEventRecord(k0, c->stream));
+    trace_mark(c, "forest start");
     rc = seed_from_cells(c, int(std::min<int64_t>(ncells, n)), &nnodes);
     (void)hipEventRecord(k1, c->stream);
     (void)hipEventSynchronize(k1);
@@ -1999,6 +2132,7 @@ int knng_build_forest(knng_ctx* c) {
     ++levels;
     if (levels > 4096) return fail(c, KNNG_ERANGE, "forest did not terminate");
   }
+  trace_mark(c, "forest levels");
   int32_t nl = 0;
   HIPCHK(c, hipMemcpyAsync(&nl, c->dcount + 1, sizeof(nl), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));

@@ -4,6 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -100,6 +103,35 @@ template <typename T>
 void dfree(T** ptr) {
   if (*ptr) hipFree(reinterpret_cast<void*>(const_cast<typename std::remove_const<T>::type*>(*ptr)));
   *ptr = nullptr;
+}
+
+// Stage tracing: with KNNG_TRACE=1 in the environment, trace_mark(c, label)
+// waits for the context's stream and prints the wall time since the previous
+// mark to stderr.  Off (and free) otherwise.
+struct TraceClock {
+  bool on = false;
+  std::chrono::steady_clock::time_point last;
+};
+
+inline TraceClock& trace_clock() {
+  static TraceClock tc = [] {
+    TraceClock t;
+    const char* e = std::getenv("KNNG_TRACE");
+    t.on = e && e[0] == '1';
+    t.last = std::chrono::steady_clock::now();
+    return t;
+  }();
+  return tc;
+}
+
+inline void trace_mark(knng_ctx* c, const char* label) {
+  TraceClock& tc = trace_clock();
+  if (!tc.on) return;
+  (void)hipStreamSynchronize(c->stream);
+  const auto now = std::chrono::steady_clock::now();
+  std::fprintf(stderr, "[knng] %-28s %9.2f ms\n", label,
+               std::chrono::duration<double, std::milli>(now - tc.last).count());
+  tc.last = now;
 }
 
 inline int grid_for(const knng_ctx* c, int64_t work, int per_block, int blocks_per_cu) {
