@@ -158,7 +158,7 @@ def assign_device(q: np.ndarray, z: np.ndarray, w: np.ndarray) -> np.ndarray:
 
 
 def kmeans_device(q: np.ndarray, Ct: int, g: int = 0, iters: int = 2, over: int = 4,
-                  sample: int = 8, merge: float = 0.6, full_iters: int = 1, prune: float = 0.25):
+                  sample: int = 8, merge: float = 0.6, full_iters: int = 0, prune: float = 0.25):
     """Cells of partition g (over-seeded spherical int8 k-means with merging)
     -> (z [C, d] int8, w [C] f32, assign [n] int32)."""
     n, d = q.shape
@@ -176,7 +176,7 @@ def kmeans_device(q: np.ndarray, Ct: int, g: int = 0, iters: int = 2, over: int 
 
 def cells_device(q: np.ndarray, trees: int, leaf: int, cells: int = -1, iters: int = 2,
                  partitions: int = 2, over: int = 4, sample: int = 8, merge: float = 0.6,
-                 full_iters: int = 1, prune: float = 0.25):
+                 full_iters: int = 0, prune: float = 0.25):
     """The library's cell plan for a build: None when no cells are used, else
     (counts [G], G, [z_g], [w_g], assign [G, n])."""
     n = q.shape[0]
@@ -262,7 +262,7 @@ STOP_REFERENCE, STOP_INSERTED, STOP_NONE = 0, 1, 2
 def build_device_semantics(q, sk, trees=16, leaf=64, K=50, mult=8, max_iters=30,
                            stop=0.04, tol=0.01, seed=1337, stop_rule=STOP_INSERTED,
                            insert_stop=0.01, incremental=False, cells=-1, kmeans_iters=2,
-                           partitions=2, over=4, sample=8, merge=0.6, full_iters=1,
+                           partitions=2, over=4, sample=8, merge=0.6, full_iters=0,
                            prune=0.25):
     """Full device-mode build.  Stop rules: STOP_REFERENCE is src/main.cpp:432-439
     on the rate of slots whose id changed; STOP_INSERTED stops once fewer than

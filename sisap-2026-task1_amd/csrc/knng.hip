@@ -1837,7 +1837,7 @@ int knng_default_params(knng_params* p) {
   p->cell_oversample = 4;
   p->cell_sample = 8;
   p->cell_merge = 0.6f;
-  p->cell_full_iters = 1;
+  p->cell_full_iters = 0;
   p->cell_prune = 0.25f;
   p->pad0 = 0;
   return 0;
