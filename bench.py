@@ -42,6 +42,24 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+PMC_PROFILE = "profiles/k_join_c3_pmc_traffic.md"
+
+
+def measured_join_traffic(args):
+    """HBM bytes per k_join launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (calibrated against k_quantize), valid for the default workload only."""
+    default = (args.n == 6_350_000 and args.dim == 1024 and args.clusters == 4096 and args.trees == 4
+               and args.leaf == 2048 and args.cells == -1 and args.partitions == 4
+               and args.k_build == 50 and args.seed == 3)
+    path = os.path.join(REPO, PMC_PROFILE)
+    if not default or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def build_params(knng, args, d):
     return knng.default_params(dim=d, num_trees=args.trees, k_build=args.k_build, keep_fp32=0,
                                leaf_size=args.leaf, cells=args.cells,
@@ -237,8 +255,10 @@ def main():
     jb = sum(st["join_bytes"][:it])
     jms = sum(st["ms_join_iter"][:it])
     achieved = jb / (jms / 1e3) / 1e9 if jms > 0 else 0.0
+    traffic = measured_join_traffic(args)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": PMC_PROFILE if traffic is not None else None,
                 "kernel": "k_join", "launches": it,
                 "algorithmic_bytes_per_launch": round(jb / max(it, 1)),
                 "avg_launch_ms": round(jms / max(it, 1), 3)}
