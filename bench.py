@@ -171,6 +171,7 @@ def main():
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--rate", type=float, default=2000.0, help="Poisson request rate (req/s)")
+    ap.add_argument("--insert-frac", type=float, default=0.1, help="share of requests that insert rows")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -388,7 +389,7 @@ def serve_main(args):
         t += rng.exponential(1.0 / args.rate)
         if t > args.serve_seconds:
             break
-        if rng.random() < 0.1:
+        if rng.random() < args.insert_frac:
             reqs.append([t, 1, int(rng.integers(1, 1025))])
         else:
             reqs.append([t, 0, int(np.exp(rng.uniform(0.0, np.log(4096.0))))])
@@ -462,7 +463,7 @@ def serve_main(args):
         "metric": "query p50/p99 ms (serving a 6.35M x 1024 graph)", "value": round(float(np.percentile(small, 99)), 3) if len(small) else None,
         "unit": "ms (p99, batches <= 64)", "higher_is_better": False, "n_gpus": 1,
         "config": {"workload": f"Poisson {args.rate:.0f} req/s for {args.serve_seconds}s over a {n}x{d} "
-                               f"graph; query batches log-uniform [1,4096], 10% inserts of 1-1024 rows; "
+                               f"graph; query batches log-uniform [1,4096], {args.insert_frac:.0%} inserts of 1-1024 rows; "
                                f"beam 64, k 15, FP32 rerank", "n": n, "dim": d,
                    "clusters": args.clusters},
         "requests": len(plan), "queries": n_q, "inserted_rows": n_ins, "wall_s": round(wall, 3),

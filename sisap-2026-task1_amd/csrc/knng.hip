@@ -986,6 +986,7 @@ __global__ __launch_bounds__(256) void k_init_merge(const int8_t* __restrict__ q
 // ---------------------------------------------------------------------------
 
 struct JoinScratch {
+  uint64_t own_key[kMaxK];    // pack_key(own score, own id)
   int32_t own_id[kMaxK];
   int32_t own_sc[kMaxK];
   int32_t new_id[kMaxK];
@@ -1001,7 +1002,7 @@ struct JoinScratch {
 
 
 template <int HSBITS, int CAP>
-__global__ __launch_bounds__(256) void k_join(
+__global__ __launch_bounds__(256, 3) void k_join(
     const int8_t* __restrict__ q, const uint64_t* __restrict__ sk, int d, int64_t n, int K, int M,
     const int32_t* __restrict__ ids_in, const int32_t* __restrict__ sc_in,
     int32_t* __restrict__ ids_out, int32_t* __restrict__ sc_out,
@@ -1013,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_join(
   constexpr int HS = 1 << HSBITS;
   static_assert(HS * 4 >= CAP * 2 + CAP * 4, "selection arrays must fit in the hash area");
   __shared__ __attribute__((aligned(16))) int32_t hs[HS];
-  __shared__ int32_t cid[CAP];
+  __shared__ __attribute__((aligned(16))) int32_t cid[CAP];
   __shared__ uint16_t ch[CAP];
   __shared__ __attribute__((aligned(16))) int8_t own_row[kMaxDim];
   __shared__ JoinScratch S;
@@ -1042,8 +1043,10 @@ __global__ __launch_bounds__(256) void k_join(
       *reinterpret_cast<v4i*>(own_row + 16 * c) = *reinterpret_cast<const v4i*>(q + i * d + 16 * c);
     if (tid < words) S.own_sk[tid] = sk[i * words + tid];
     if (tid < K) {
-      S.own_id[tid] = ids_in[i * K + tid];
-      S.own_sc[tid] = sc_in[i * K + tid];
+      const int32_t oid = ids_in[i * K + tid], osc = sc_in[i * K + tid];
+      S.own_id[tid] = oid;
+      S.own_sc[tid] = osc;
+      S.own_key[tid] = pack_key(osc, oid);
     }
     if (tid == 0) S.own_new = fresh_in ? fresh_in[i] : ~0ull;
     __syncthreads();
@@ -1066,24 +1069,26 @@ __global__ __launch_bounds__(256) void k_join(
         }
       }
 #pragma unroll
-      for (int r = 0; r < kRowsPerWave; ++r)
-        if (c[r] >= 0 && hs_insert(hs, HSBITS, c[r])) {
-          const int slot = atomicAdd(&S.cnt[0], 1);
-          cid[slot] = c[r];
-        }
+      for (int r = 0; r < kRowsPerWave; ++r) {
+        const bool fresh_id = c[r] >= 0 && hs_insert(hs, HSBITS, c[r]);
+        const int slot = wave_append(fresh_id, &S.cnt[0]);
+        if (fresh_id) cid[slot] = c[r];
+      }
     }
     __syncthreads();
     const int U = S.cnt[0];
-    // ---- 3: sketch similarity, 8 lanes per candidate, 4 candidates per group
+    // ---- 3: sketch similarity, 8 lanes per candidate, 8 candidates per group
+    //         in flight (16 B per lane each)
     {
+      constexpr int kSkPer = 8;
       const int sub = tid & 7, grp = tid >> 3;
       uint64_t o0 = 0, o1 = 0;
       if (2 * sub < words) o0 = S.own_sk[2 * sub];
       if (2 * sub + 1 < words) o1 = S.own_sk[2 * sub + 1];
-      for (int u0 = 0; u0 < U; u0 += 128) {
-        uint64_t a[4], b[4];
+      for (int u0 = 0; u0 < U; u0 += 32 * kSkPer) {
+        uint64_t a[kSkPer], b[kSkPer];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < kSkPer; ++j) {
           const int u = u0 + grp + 32 * j;
           a[j] = b[j] = 0;
           if (u < U) {
@@ -1093,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_join(
           }
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < kSkPer; ++j) {
           const int u = u0 + grp + 32 * j;
           int pc = __popcll(a[j]) + __popcll(b[j]);
           pc += __shfl_xor(pc, 1);
@@ -1146,11 +1151,12 @@ __global__ __launch_bounds__(256) void k_join(
     // ---- selection
     {
       const int thr = S.cnt[3];
-      for (int u = tid; u < U; u += 256)
-        if (ch[u] >= thr) {
-          const int s = atomicAdd(&S.cnt[1], 1);
-          sel_idx[s] = uint16_t(u);
-        }
+      for (int u0 = 0; u0 < U; u0 += 256) {
+        const int u = u0 + tid;
+        const bool take = u < U && ch[u] >= thr;
+        const int s = wave_append(take, &S.cnt[1]);
+        if (take) sel_idx[s] = uint16_t(u);
+      }
     }
     __syncthreads();
     const int NS = S.cnt[1];
@@ -1163,48 +1169,73 @@ __global__ __launch_bounds__(256) void k_join(
         const int chn = s16 + 16 * c;
         own[c] = chn < nch ? *reinterpret_cast<const v4i*>(own_row + 16 * chn) : v4i{0, 0, 0, 0};
       }
-      for (int s0 = 0; s0 < NS; s0 += 32) {
-        const int sa = s0 + w * 4 + g;        // first candidate of this group
-        const int sb = sa + 16;               // second, issued together
-        const int8_t* ra = sa < NS ? q + int64_t(cid[sel_idx[sa]]) * d : nullptr;
-        const int8_t* rb = sb < NS ? q + int64_t(cid[sel_idx[sb]]) * d : nullptr;
-        v4i va[4], vb[4];
+      // four candidates per 16-lane group in flight: 64 B per lane each
+      constexpr int kDotPer = 4;
+      for (int s0 = 0; s0 < NS; s0 += 16 * kDotPer) {
+        v4i v[kDotPer][4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int chn = s16 + 16 * c;
-          va[c] = (ra && chn < nch) ? *reinterpret_cast<const v4i*>(ra + 16 * chn) : v4i{0, 0, 0, 0};
-          vb[c] = (rb && chn < nch) ? *reinterpret_cast<const v4i*>(rb + 16 * chn) : v4i{0, 0, 0, 0};
-        }
-        int da = 0, db = 0;
+        for (int j = 0; j < kDotPer; ++j) {
+          const int sj = s0 + w * 4 + g + 16 * j;
+          const int8_t* rj = sj < NS ? q + int64_t(cid[sel_idx[sj]]) * d : nullptr;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          da = dot16_i8(va[c], own[c], da);
-          db = dot16_i8(vb[c], own[c], db);
+          for (int c = 0; c < 4; ++c) {
+            const int chn = s16 + 16 * c;
+            v[j][c] = (rj && chn < nch) ? *reinterpret_cast<const v4i*>(rj + 16 * chn) : v4i{0, 0, 0, 0};
+          }
         }
 #pragma unroll
-        for (int m = 8; m >= 1; m >>= 1) {
-          da += __shfl_xor(da, m);
-          db += __shfl_xor(db, m);
-        }
-        if (s16 == 0) {
-          if (sa < NS) sel_sc[sa] = da;
-          if (sb < NS) sel_sc[sb] = db;
+        for (int j = 0; j < kDotPer; ++j) {
+          int dj = 0;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) dj = dot16_i8(v[j][c], own[c], dj);
+#pragma unroll
+          for (int m = 8; m >= 1; m >>= 1) dj += __shfl_xor(dj, m);
+          const int sj = s0 + w * 4 + g + 16 * j;
+          if (s16 == 0 && sj < NS) sel_sc[sj] = dj;
         }
       }
     }
     __syncthreads();
-    // ---- 6a: survivors (fewer than K own entries precede them)
-    for (int s = tid; s < NS; s += 256) {
-      const int32_t sc = sel_sc[s], id = cid[sel_idx[s]];
-      int c = 0;
-      for (int k = 0; k < K; ++k) c += before(S.own_sc[k], S.own_id[k], k, sc, id, K) ? 1 : 0;
-      if (c < K) {
-        const int p = atomicAdd(&S.cnt[2], 1);
-        surv[p] = uint16_t(s);
+    // ---- 6a: survivors (fewer than K own entries precede them).  Keys order
+    // (score desc, id asc); a candidate's id is never an own id, and own ids
+    // are distinct, so key comparisons decide every pair the slot rule would.
+    for (int s0 = 0; s0 < NS; s0 += 256) {
+      const int s = s0 + tid;
+      bool keep = false;
+      if (s < NS) {
+        const uint64_t key = pack_key(sel_sc[s], cid[sel_idx[s]]);
+        int c = 0;
+        for (int k = 0; k < K; ++k) c += S.own_key[k] > key ? 1 : 0;
+        keep = c < K;
       }
+      const int p = wave_append(keep, &S.cnt[2]);
+      if (keep) surv[p] = uint16_t(s);
     }
     __syncthreads();
     const int P = S.cnt[2];
+    // survivors' keys packed over cid[] (no longer needed once read) when they fit
+    constexpr int kPackCap = CAP / 2, kPackPer = (kPackCap + 255) / 256;
+    uint64_t* pkey = reinterpret_cast<uint64_t*>(cid);
+    const bool packed = P <= kPackCap;
+    if (packed) {
+      uint64_t kk[kPackPer];
+#pragma unroll
+      for (int j = 0; j < kPackPer; ++j) {
+        const int p = tid + 256 * j;
+        kk[j] = 0;
+        if (p < P) {
+          const int sv = surv[p];
+          kk[j] = pack_key(sel_sc[sv], cid[sel_idx[sv]]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kPackPer; ++j) {
+        const int p = tid + 256 * j;
+        if (p < P) pkey[p] = kk[j];
+      }
+      __syncthreads();
+    }
     // ---- 6b: ranks
     for (int e = tid; e < K + P; e += 256) {
       int32_t sc, id, sl;
@@ -1212,6 +1243,11 @@ __global__ __launch_bounds__(256) void k_join(
         sc = S.own_sc[e];
         id = S.own_id[e];
         sl = e;
+      } else if (packed) {
+        const uint64_t key = pkey[e - K];
+        sc = key_score(key);
+        id = key_id(key);
+        sl = K;
       } else {
         const int s = surv[e - K];
         sc = sel_sc[s];
@@ -1220,9 +1256,14 @@ __global__ __launch_bounds__(256) void k_join(
       }
       int r = 0;
       for (int k = 0; k < K; ++k) r += before(S.own_sc[k], S.own_id[k], k, sc, id, sl) ? 1 : 0;
-      for (int p = 0; p < P && r < K; ++p) {
-        const int s = surv[p];
-        r += before(sel_sc[s], cid[sel_idx[s]], K, sc, id, sl) ? 1 : 0;
+      if (packed) {
+        const uint64_t key = pack_key(sc, id);
+        for (int p = 0; p < P && r < K; ++p) r += pkey[p] > key ? 1 : 0;
+      } else {
+        for (int p = 0; p < P && r < K; ++p) {
+          const int s = surv[p];
+          r += before(sel_sc[s], cid[sel_idx[s]], K, sc, id, sl) ? 1 : 0;
+        }
       }
       if (r < K) {
         S.new_id[r] = id;

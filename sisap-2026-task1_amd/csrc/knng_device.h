@@ -167,6 +167,19 @@ __device__ __forceinline__ bool hs_insert(int32_t* hs, int bits, int32_t key) {
   }
 }
 
+// Wave-aggregated append: lanes with pred get consecutive slots from *counter
+// (one LDS atomic per wave).  Every lane of the wave must call it.
+__device__ __forceinline__ int wave_append(bool pred, int* counter) {
+  const unsigned long long m = __ballot(pred);
+  if (m == 0ull) return -1;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll(m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader);
+  return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
 // a before b in (score desc, id asc, slot asc)
 __device__ __forceinline__ bool before(int32_t sa, int32_t ia, int32_t la, int32_t sb, int32_t ib,
                                        int32_t lb) {
