@@ -260,6 +260,9 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": PMC_PROFILE if traffic is not None else None,
+                # HBM bytes actually moved (rocprofv3 FETCH_SIZE + WRITE_SIZE) per second
+                "traffic_GBps": round(traffic / (jms / max(it, 1) / 1e3) / 1e9, 1)
+                if traffic is not None and jms > 0 else None,
                 "kernel": "k_join", "launches": it,
                 "algorithmic_bytes_per_launch": round(jb / max(it, 1)),
                 "avg_launch_ms": round(jms / max(it, 1), 3)}

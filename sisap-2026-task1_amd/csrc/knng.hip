@@ -1001,12 +1001,13 @@ struct JoinScratch {
 
 
 
-template <int HSBITS, int CAP>
-__global__ __launch_bounds__(256, 3) void k_join(
+template <int HSBITS, int CAP, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_join(
     const int8_t* __restrict__ q, const uint64_t* __restrict__ sk, int d, int64_t n, int K, int M,
     const int32_t* __restrict__ ids_in, const int32_t* __restrict__ sc_in,
     int32_t* __restrict__ ids_out, int32_t* __restrict__ sc_out,
-    unsigned long long* __restrict__ counters, const int32_t* __restrict__ order,
+    unsigned long long* __restrict__ counters, unsigned long long* __restrict__ rowq,
+    const int32_t* __restrict__ order,
     const uint64_t* __restrict__ fresh_in, uint64_t* __restrict__ fresh_out) {
   // fresh_in != nullptr: incremental join -- the pair (i -> n1 -> c) is a
   // candidate only if n1 is fresh in i's list or c is fresh in n1's list
@@ -1028,12 +1029,22 @@ __global__ __launch_bounds__(256, 3) void k_join(
 
   // Schedule: blocks are dealt to the 8 XCDs round-robin, so XCD x = block % 8
   // takes the contiguous slice [x*n/8, (x+1)*n/8) of the processing order and
-  // its blocks walk it together.  With `order` = tree 0's leaf order,
-  // concurrently processed rows share most neighbours, which then hit that
-  // XCD's L2.  The result does not depend on the order (Jacobi iteration).
-  const int xcd = blockIdx.x & 7, per_xcd = gridDim.x >> 3;
+  // its blocks walk it together, each taking its next row from the slice's
+  // queue counter rowq[32x] (one atomic per row, issued a row ahead).  With
+  // `order` = tree 0's leaf order, concurrently processed rows share most
+  // neighbours, which then hit that XCD's L2.  Blocks that start late (the
+  // grid may exceed what is resident) or draw cheap rows simply take more or
+  // fewer rows, so the slices drain together.  The result does not depend on
+  // the order (Jacobi iteration).
+  __shared__ int64_t s_row;
+  const int xcd = blockIdx.x & 7;
   const int64_t lo = n * xcd / 8, hi = n * (xcd + 1) / 8;
-  for (int64_t j = lo + (blockIdx.x >> 3); j < hi; j += per_xcd) {
+  unsigned long long* const qctr = rowq + 32 * xcd;
+  if (tid == 0) s_row = lo + int64_t(atomicAdd(qctr, 1ull));
+  __syncthreads();
+  for (int64_t j = s_row; j < hi; j = s_row) {
+    unsigned long long next_row = 0;
+    if (tid == 0) next_row = atomicAdd(qctr, 1ull);
     const int64_t i = order ? int64_t(order[j]) : j;
     // ---- 0/1: reset, own list
     for (int j = tid; j < HS; j += 256) hs[j] = -1;
@@ -1169,8 +1180,9 @@ __global__ __launch_bounds__(256, 3) void k_join(
         const int chn = s16 + 16 * c;
         own[c] = chn < nch ? *reinterpret_cast<const v4i*>(own_row + 16 * chn) : v4i{0, 0, 0, 0};
       }
-      // four candidates per 16-lane group in flight: 64 B per lane each
-      constexpr int kDotPer = 4;
+      // two candidates per 16-lane group in flight (64 B per lane each): the
+      // kernel then fits 128 VGPRs, i.e. four blocks per CU
+      constexpr int kDotPer = 2;
       for (int s0 = 0; s0 < NS; s0 += 16 * kDotPer) {
         v4i v[kDotPer][4];
 #pragma unroll
@@ -1292,6 +1304,7 @@ __global__ __launch_bounds__(256, 3) void k_join(
         my_ins += S.cnt[4];
       }
     }
+    if (tid == 0) s_row = lo + int64_t(next_row);
     __syncthreads();
   }
   if (tid == 0) {
@@ -1303,12 +1316,19 @@ __global__ __launch_bounds__(256, 3) void k_join(
   }
 }
 
-__global__ void k_sum_counters(const unsigned long long* __restrict__ c,
+// Sums the per-block counters of a k_join launch into out[0..3] and leaves
+// the counters and the row-queue counters behind them at zero for the next
+// launch (no memset node, so the sequence can be captured in a graph).
+__global__ void k_sum_counters(unsigned long long* __restrict__ c,
                                unsigned long long* __restrict__ out) {
   __shared__ unsigned long long part[4][256];
   unsigned long long a[4] = {0, 0, 0, 0};
   for (int j = threadIdx.x; j < kNumCounters; j += 256)
-    for (int f = 0; f < 4; ++f) a[f] += c[4 * j + f];
+    for (int f = 0; f < 4; ++f) {
+      a[f] += c[4 * j + f];
+      c[4 * j + f] = 0;
+    }
+  if (threadIdx.x < kRowQueueWords) c[4 * kNumCounters + threadIdx.x] = 0;
   for (int f = 0; f < 4; ++f) part[f][threadIdx.x] = a[f];
   __syncthreads();
   if (threadIdx.x < 4) {
@@ -1919,7 +1939,11 @@ int knng_create(knng_ctx** out, int32_t device, const knng_params* p, int64_t n)
   if ((rc = dalloc(c, &c->q, size_t(n) * c->d, "int8 rows"))) return bail(rc);
   if ((rc = dalloc(c, &c->sk, size_t(n) * c->words, "sketches"))) return bail(rc);
   if (p->keep_fp32 && (rc = dalloc(c, &c->x, size_t(n) * c->d, "float32 rows"))) return bail(rc);
-  if ((rc = dalloc(c, &c->counters, 4 * kNumCounters, "counters"))) return bail(rc);
+  if ((rc = dalloc(c, &c->counters, 4 * kNumCounters + kRowQueueWords, "counters"))) return bail(rc);
+  // zero once; k_sum_counters re-zeroes after every join launch
+  if (hipMemset(c->counters, 0, (4 * kNumCounters + kRowQueueWords) * sizeof(unsigned long long)) !=
+      hipSuccess)
+    return bail(fail(c, KNNG_EHIP, "hipMemset of the join counters failed"));
   if ((rc = dalloc(c, &c->csum, 4, "counter sums"))) return bail(rc);
   c->st.n = n;
   c->st.dim = c->d;
@@ -2281,19 +2305,21 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   HIPCHK(c, hipSetDevice(c->device));
   const int64_t n = c->n;
   const int K = c->p.k_build, M = K * c->p.rerank_mult;
-  HIPCHK(c, hipMemsetAsync(c->counters, 0, 4 * kNumCounters * sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   const int a = c->cur, b = c->cur ^ 1;
-  // grid: a multiple of the 8 XCDs (the kernel's schedule needs it)
+  // grid: a multiple of the 8 XCDs (the kernel's schedule needs it), four
+  // blocks per CU (what fits at K <= 50; larger K leaves extra blocks queued,
+  // which the row queues make harmless)
   const int grid = std::max(8, (c->num_cu * 4) / 8 * 8);
+  unsigned long long* rowq = c->counters + 4 * kNumCounters;
   const int32_t* order = c->have_forest ? c->perm : nullptr;   // tree 0's leaf order
   if (K <= 50)
-    hipLaunchKernelGGL((k_join<12, 2560>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n, K,
-                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, order,
+    hipLaunchKernelGGL((k_join<12, 2560, 4>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n,
+                       K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order,
                        c->p.incremental ? c->fresh[a] : nullptr, c->fresh[b]);
   else
-    hipLaunchKernelGGL((k_join<13, 4096>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n, K,
-                       M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, order,
+    hipLaunchKernelGGL((k_join<13, 4096, 2>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n,
+                       K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order,
                        c->p.incremental ? c->fresh[a] : nullptr, c->fresh[b]);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
