@@ -169,9 +169,15 @@ def main():
                          "graph; batch: many uneven independent builds")
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--batch-cells", type=int, default=0,
+                    help="batch jobs' cells parameter (0: plain RP trees, packed; -1: n/1536 cells)")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--rate", type=float, default=2000.0, help="Poisson request rate (req/s)")
     ap.add_argument("--insert-frac", type=float, default=0.1, help="share of requests that insert rows")
+    ap.add_argument("--serve-batch", type=int, default=2048, help="queries per bulk launch")
+    ap.add_argument("--serve-slots", type=int, default=4, help="concurrent bulk launches")
+    ap.add_argument("--express-slots", type=int, default=2, help="express-lane slots (batches <= 64)")
+    ap.add_argument("--express-cus", type=int, default=0, help="CUs reserved for the express lane")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -320,7 +326,8 @@ def batch_main(args):
         k = int(rng.choice([10, 15, 32]))
         x = synth.gaussian_mixture_torch(4 * (1 << 20) + j, n, d, cl, device="cuda")
         data.append(x)
-        p = knng.default_params(dim=d, k_eval=k, k_build=max(50, k), num_trees=8, keep_fp32=0)
+        p = knng.default_params(dim=d, k_eval=k, k_build=max(50, k), num_trees=8, keep_fp32=0,
+                                cells=args.batch_cells)
         jobs.append({"rows": x.data_ptr(), "n": n, "params": p, "rerank": 1, "k": k, "d": d})
     torch.cuda.synchronize()
     total_rows = sum(j["n"] for j in jobs)
@@ -328,6 +335,7 @@ def batch_main(args):
     t0 = time.time()
     rc = knng.build_batch(jobs, 0, args.workers)
     wall = time.time() - t0
+    packed = os.environ.get("KNNG_BATCH_NO_PACK", "0") != "1" and args.batch_cells == 0
     assert rc == 0, [j["status"] for j in jobs if j["status"]][:5]
     recs = []
     for j, job in zip(range(len(jobs)), jobs):
@@ -342,7 +350,11 @@ def batch_main(args):
         "unit": "builds/s", "higher_is_better": True, "n_gpus": 1, "wall_s": round(wall, 3),
         "config": {"workload": f"{len(jobs)} independent builds, n log-uniform [2000,200000], d in "
                                f"{{384,768,1024}}, 16-512 clusters, k in {{10,15,32}}, 8 trees, FP32 "
-                               f"rerank export, {args.workers} workers", "total_rows": total_rows},
+                               f"rerank export", "total_rows": total_rows,
+                   "form": "packed per dimension (one context each: forest, init, graph-replayed "
+                           "join iterations over a device work list)" if packed else
+                           f"whole builds on {args.workers} worker threads",
+                   "cells": args.batch_cells},
         "recall_at_k": {"mean": round(float(recs.mean()), 4), "min": round(float(recs.min()), 4),
                         "p10": round(float(np.percentile(recs, 10)), 4)},
         "iterations_mean": round(float(np.mean([j["iterations"] for j in jobs])), 2),
@@ -378,6 +390,10 @@ def serve_main(args):
     submission to completion."""
     import threading
 
+    # one hardware queue per stream (bulk slots, express slots, inserts, the
+    # build context): with HIP's default of 4, express launches would queue
+    # behind bulk launches that share their hardware queue
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
     import numpy as np
     import torch
 
@@ -411,8 +427,10 @@ def serve_main(args):
     log(f"graph built in {time.time() - t0:.2f}s")
     xh = x[n:].cpu().numpy()
     ins_rows, q_rows = xh[:n_ins], xh[n_ins:]
-    srv = knng.Server(b, beam=64, k=15, rerank=1, entry_trees=4, max_batch=1024, max_slots=8,
-                      insert_capacity=n_ins + 16)
+    sp = dict(beam=64, k=15, rerank=1, entry_trees=4, max_batch=args.serve_batch,
+              max_slots=args.serve_slots, express_rows=64, express_slots=args.express_slots,
+              express_cus=args.express_cus)
+    srv = knng.Server(b, insert_capacity=n_ins + 16, **sp)
     srv.search(q_rows[:2048])   # warm-up (code objects, slots)
     qoff = ioff = 0
     plan = []
@@ -440,23 +458,26 @@ def serve_main(args):
     for j, (tt, kind, off, sz) in enumerate(plan):
         while tickets[j] is None:
             time.sleep(0.0002)
-        ids, sc, lat = srv.wait(tickets[j], sz, insert=bool(kind))
-        results.append((kind, off, sz, ids, lat))
+        ids, sc, lat, seen = srv.wait_info(tickets[j], sz, insert=bool(kind))
+        results.append((kind, off, sz, ids, lat, seen))
     th.join()
     wall = time.time() - start
+    lanes = srv.info()
     srv.close()
     qlat = np.array([r[4] for r in results if r[0] == 0])
+    behind = np.array([n + sum(rr[2] for rr in results[:j] if rr[0] == 1) - r[5]
+                       for j, r in enumerate(results) if r[0] == 0])
     small = np.array([r[4] for r in results if r[0] == 0 and r[2] <= 64])
     ilat = np.array([r[4] for r in results if r[0] == 1])
-    # recall: sampled query rows against brute force over base + the inserts
-    # submitted before the query (the server applies inserts in order)
-    samp, seen_ins = [], 0
-    for kind, off, sz, ids, lat in results:
+    # recall: sampled query rows against brute force over the rows the query
+    # was dispatched against (base + the inserts finished by then; the server
+    # gives inserted rows ids in submission order, the order of x)
+    samp = []
+    for kind, off, sz, ids, lat, seen in results:
         if kind == 1:
-            seen_ins += sz
             continue
         for r in range(0, sz, max(1, sz // 4)):
-            samp.append((off + r, n + seen_ins, ids[r]))
+            samp.append((off + r, seen, ids[r]))
     pick = np.random.default_rng(1).choice(len(samp), min(2000, len(samp)), replace=False)
     qv = torch.from_numpy(np.stack([q_rows[samp[p][0]] for p in pick])).cuda()
     lim = torch.tensor([samp[p][1] for p in pick], device="cuda")
@@ -468,7 +489,8 @@ def serve_main(args):
         "config": {"workload": f"Poisson {args.rate:.0f} req/s for {args.serve_seconds}s over a {n}x{d} "
                                f"graph; query batches log-uniform [1,4096], {args.insert_frac:.0%} inserts of 1-1024 rows; "
                                f"beam 64, k 15, FP32 rerank", "n": n, "dim": d,
-                   "clusters": args.clusters},
+                   "clusters": args.clusters, "server": sp, "lanes": lanes,
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "requests": len(plan), "queries": n_q, "inserted_rows": n_ins, "wall_s": round(wall, 3),
         "queries_per_s": round(n_q / wall, 1),
         "latency_ms": {"p50": round(float(np.percentile(qlat, 50)), 3),
@@ -478,6 +500,9 @@ def serve_main(args):
                        "insert_p50": round(float(np.percentile(ilat, 50)), 3) if len(ilat) else None,
                        "insert_p99": round(float(np.percentile(ilat, 99)), 3) if len(ilat) else None},
         "query_recall_at_15": round(float(rec), 4), "recall_rows": int(len(pick)),
+        # inserted rows submitted before a query but not yet applied when it ran
+        "insert_lag_rows": {"p50": float(np.percentile(behind, 50)),
+                            "max": float(behind.max()) if len(behind) else 0.0},
         "data": "synthetic",
     }
     b.close()

@@ -93,6 +93,8 @@ typedef struct knng_stats {
   int64_t join_rescored[32];   /* candidates re-scored with int8 dots */
   int64_t join_inserted[32];   /* ids that newly entered a list */
   float insert_rates[32];      /* join_inserted / (n * k_build) */
+  int32_t join_variant;        /* k_join shape the autotuner chose (0..2) */
+  float ms_tune;               /* time spent timing k_join shapes (first build of a shape) */
 } knng_stats;
 
 typedef struct knng_ctx knng_ctx;
@@ -211,6 +213,15 @@ typedef struct knng_server_params {
   int32_t pad0;
   int64_t insert_capacity; /* rows that later inserts may add */
   int64_t reserve_bytes;   /* HBM left free when admitting a new slot */
+  int32_t express_rows;    /* query requests of at most this many rows take the express
+                              lane (0: no express lane) */
+  int32_t express_slots;   /* express slots, created with the server */
+  int32_t express_cus;     /* CUs reserved for the express slots' streams through
+                              hipExtStreamCreateWithCUMask (0, the default: none; where the
+                              runtime refuses CU masks the server runs without them) */
+  int32_t strict_order;    /* 0 (default): inserts run beside searches on their own thread and
+                              stream; 1: an insert waits for the queries submitted before it and
+                              a query for the inserts submitted before it */
 } knng_server_params;
 
 int knng_server_default_params(knng_server_params* sp);
@@ -227,9 +238,15 @@ const char* knng_server_last_error(const knng_server* srv);
  * kind 1 = insert of nrows new vectors (ids n, n+1, ... in submission order).
  * Returns at once with a ticket.  Queries are split into max_batch pieces and
  * admitted as slots and HBM free up; pieces of different requests run
- * concurrently on different streams.  An insert waits until the queries
- * admitted before it have finished, then updates the graph; queries admitted
- * after it see the new rows. */
+ * concurrently on different streams.  Query requests of at most express_rows
+ * rows bypass the bulk queue: they run on the express slots, whose streams
+ * (with express_cus > 0) own a reserved set of CUs that bulk launches never
+ * occupy, so their latency does not grow with the bulk backlog.  Inserts run
+ * on their own thread and stream beside the searches (pending ones together):
+ * a query sees every insert finished before it was dispatched, and may see
+ * one that overlaps it.  With strict_order = 1 an insert instead waits until
+ * the queries submitted before it have finished, and queries submitted after
+ * it (either lane) wait for it. */
 int knng_server_submit(knng_server* srv, int32_t kind, const float* rows, int32_t nrows,
                        int64_t* ticket);
 /* state: 0 queued, 1 running, 2 done. */
@@ -239,6 +256,18 @@ int knng_server_poll(knng_server* srv, int64_t ticket, int32_t* state);
  * ticket.  ids/scores may be NULL. */
 int knng_server_wait(knng_server* srv, int64_t ticket, int32_t* ids, float* scores,
                      double* latency_ms);
+/* knng_server_wait plus *index_rows: for a query, the index size when it was
+ * dispatched (it saw at least those rows); for an insert, the size just after
+ * it.  May be NULL. */
+int knng_server_wait_info(knng_server* srv, int64_t ticket, int32_t* ids, float* scores,
+                          double* latency_ms, int64_t* index_rows);
+/* Lanes in force: *cu_masked = 1 when the express streams own reserved CUs,
+ * and the bulk / express slot counts.  Any output may be NULL. */
+int knng_server_info(const knng_server* srv, int32_t* cu_masked, int32_t* bulk_slots,
+                     int32_t* express_slots);
+/* Why the runtime refused the express CU masks ("" when they are in force or
+ * were not asked for). */
+const char* knng_server_mask_error(const knng_server* srv);
 /* Rows currently in the index. */
 int64_t knng_server_size(const knng_server* srv);
 

@@ -21,6 +21,9 @@
 
 #include <algorithm>
 #include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -101,8 +104,26 @@ __global__ void k_forest_seed(int64_t n, int T, int leaf, uint64_t seed, int32_t
       node_pivots(seed, t, 0, 0, n, &nd.p1, &nd.p2);
       nd.left = nd.right = -1;
       nd.nleft = 0;
+      nd.base = 0;
       nodes[t] = nd;
     }
+  }
+}
+
+// Independent builds packed into one forest (segments [seg_base[s],
+// seg_base[s+1]) of the rows): perm = identity and every position points at
+// its segment's root node in its tree, segnode[t * nseg + s] (-1 where the
+// segment is a single leaf).
+__global__ void k_forest_seed_segs(int64_t n, int T, const int64_t* __restrict__ seg_base, int nseg,
+                                   const int32_t* __restrict__ segnode, int32_t* __restrict__ perm,
+                                   int32_t* __restrict__ nodepos) {
+  const int64_t total = int64_t(T) * n;
+  for (int64_t gp = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; gp < total;
+       gp += int64_t(gridDim.x) * blockDim.x) {
+    const int t = int(gp / n);
+    const int64_t r = gp - int64_t(t) * n;
+    perm[gp] = int32_t(r);
+    nodepos[gp] = segnode[int64_t(t) * nseg + seg_of(seg_base, nseg, r)];
   }
 }
 
@@ -262,9 +283,10 @@ __global__ void k_forest_nodes(int64_t n, int leaf, uint64_t seed, int depth, No
       ch.tree = nd.tree;
       ch.start = cs[c];
       ch.size = cz[c];
-      node_pivots(seed, nd.tree, depth + 1, cs[c], cz[c], &ch.p1, &ch.p2);
+      node_pivots(seed, nd.tree, depth + 1, cs[c] - nd.base, cz[c], &ch.p1, &ch.p2);
       ch.left = ch.right = -1;
       ch.nleft = 0;
+      ch.base = nd.base;
       next[id] = ch;
       child[c] = id;
     } else {
@@ -336,6 +358,11 @@ __global__ void k_hier_append(const Node* __restrict__ nodes, int nnodes, Node* 
 // (c - r) mod (d/16), so the 32 rows a B fragment read touches fall on
 // different banks.  Even and odd k-steps accumulate into two chains
 // (exact int32 sums, added at the end) to keep two MFMAs in flight.
+// With gridDim.y = S > 1 the cell tiles are split into S consecutive slices,
+// block (x, y) scans slice y only and writes its per-row best to the partial
+// arrays pbest/pidx [S][nrows]; k_assign_reduce then takes the best over the
+// slices (larger value, then smaller cell), which is the single-pass result.
+// Few rows (queries, centroid merging) thus still fill the GPU.
 // ---------------------------------------------------------------------------
 
 constexpr int kAssignRows = 128;
@@ -362,11 +389,16 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
                                                 const int8_t* __restrict__ Z,
                                                 const float* __restrict__ w, int C,
                                                 int32_t* __restrict__ out,
-                                                float* __restrict__ score) {
+                                                float* __restrict__ score,
+                                                float* __restrict__ pbest,
+                                                int32_t* __restrict__ pidx) {
   __shared__ __attribute__((aligned(1024))) int8_t ztile[2][kAssignTile * kMaxDim];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int ksteps = d >> 5, cpr = d >> 4;
-  const int ntiles = (C + kAssignTile - 1) / kAssignTile;
+  const int ntiles_all = (C + kAssignTile - 1) / kAssignTile;
+  const int per_slice = (ntiles_all + int(gridDim.y) - 1) / int(gridDim.y);
+  const int t_lo = int(blockIdx.y) * per_slice;
+  const int ntiles = min(ntiles_all, t_lo + per_slice);   // this block scans tiles [t_lo, ntiles)
   const int cl = lane & 31, hi = lane >> 5;
   const int rrot = cl % cpr;   // rotation of the B row this lane reads
   for (int64_t r0 = int64_t(blockIdx.x) * kAssignRows; r0 < nrows;
@@ -388,8 +420,12 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
       bidx[j] = 0x7FFFFFFF;
     }
     __syncthreads();   // both buffers free (previous row tile done)
-    assign_issue_tile(Z, d, cpr, 0, C < kAssignTile ? C : kAssignTile, ztile[0], wv, lane);
-    for (int t = 0; t < ntiles; ++t) {
+    if (t_lo < ntiles) {
+      const int c0 = t_lo * kAssignTile;
+      assign_issue_tile(Z, d, cpr, c0, (C - c0) < kAssignTile ? (C - c0) : kAssignTile,
+                        ztile[t_lo & 1], wv, lane);
+    }
+    for (int t = t_lo; t < ntiles; ++t) {
       const int c0 = t * kAssignTile;
       const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
       __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of tile t have landed
@@ -447,10 +483,37 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
       }
       const int64_t row = r0 + 32 * wv + 8 * (j >> 2) + 4 * hi + (j & 3);
       if (cl == 0 && row < nrows) {
-        out[row] = bidx[j];
-        if (score) score[row] = best[j];
+        if (gridDim.y > 1) {
+          pbest[int64_t(blockIdx.y) * nrows + row] = best[j];
+          pidx[int64_t(blockIdx.y) * nrows + row] = bidx[j];
+        } else {
+          out[row] = bidx[j];
+          if (score) score[row] = best[j];
+        }
       }
     }
+  }
+}
+
+// best over the S cell slices of a split k_assign: larger value, then smaller cell
+__global__ __launch_bounds__(256) void k_assign_reduce(const float* __restrict__ pbest,
+                                                       const int32_t* __restrict__ pidx, int S,
+                                                       int64_t nrows, int32_t* __restrict__ out,
+                                                       float* __restrict__ score) {
+  for (int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x; r < nrows; r += int64_t(gridDim.x) * 256) {
+    float b = pbest[r];
+    int32_t bi = pidx[r];
+#pragma unroll 8
+    for (int y = 1; y < S; ++y) {
+      const float v = pbest[int64_t(y) * nrows + r];
+      const int32_t vi = pidx[int64_t(y) * nrows + r];
+      if (v > b || (v == b && vi < bi)) {
+        b = v;
+        bi = vi;
+      }
+    }
+    out[r] = bi;
+    if (score) score[r] = b;
   }
 }
 
@@ -847,7 +910,8 @@ template <int S2>
 __global__ __launch_bounds__(256) void k_init_merge(const int8_t* __restrict__ q, int d, int64_t n,
                                                     int T, int K, const uint64_t* __restrict__ cand,
                                                     int32_t* __restrict__ gid,
-                                                    int32_t* __restrict__ gsc) {
+                                                    int32_t* __restrict__ gsc,
+                                                    const int64_t* __restrict__ seg_base, int nseg) {
   __shared__ uint64_t keys[S2];
   __shared__ int lds[8];
   __shared__ int s_unique;
@@ -905,13 +969,20 @@ __global__ __launch_bounds__(256) void k_init_merge(const int8_t* __restrict__ q
     __syncthreads();
     const int U0 = s_unique;
     if (U0 < K) {   // uniform across the block
-      // padding ids: src/main.cpp:271-286
+      // the row's own build: rows [sb, sb + sn) (packed builds), else all n
+      int64_t sb = 0, sn = n;
+      if (nseg > 0) {
+        const int sg = seg_of(seg_base, nseg, i);
+        sb = seg_base[sg];
+        sn = seg_base[sg + 1] - sb;
+      }
+      // padding ids: src/main.cpp:271-286, in the build's own row numbers
       if (tid == 0) {
-        uint32_t s = uint32_t(1337 + i);
+        uint32_t s = uint32_t(1337 + (i - sb));
         int np = 0;
         while (U0 + np < K) {
           s = xorshift32(s);
-          const int32_t r = int32_t(s % uint64_t(n));
+          const int32_t r = int32_t(sb + int64_t(s % uint64_t(sn)));
           if (r == i) continue;
           pad_id[np++] = r;
         }
@@ -961,7 +1032,7 @@ __global__ __launch_bounds__(256) void k_init_merge(const int8_t* __restrict__ q
         uint32_t s = state;
         while (kept < K) {   // src/main.cpp:303-312
           s = xorshift32(s);
-          const int32_t r = int32_t(s % uint64_t(n));
+          const int32_t r = int32_t(sb + int64_t(s % uint64_t(sn)));
           if (r != i) {
             gid[i * K + kept] = r;
             gsc[i * K + kept] = INT32_MIN;
@@ -1001,14 +1072,21 @@ struct JoinScratch {
 
 
 
-template <int HSBITS, int CAP, int OCC>
+template <int HSBITS, int CAP, int OCC, int DOTPER, int SKPER>
 __global__ __launch_bounds__(256, OCC) void k_join(
     const int8_t* __restrict__ q, const uint64_t* __restrict__ sk, int d, int64_t n, int K, int M,
     const int32_t* __restrict__ ids_in, const int32_t* __restrict__ sc_in,
     int32_t* __restrict__ ids_out, int32_t* __restrict__ sc_out,
     unsigned long long* __restrict__ counters, unsigned long long* __restrict__ rowq,
     const int32_t* __restrict__ order,
-    const uint64_t* __restrict__ fresh_in, uint64_t* __restrict__ fresh_out) {
+    const uint64_t* __restrict__ fresh_in, uint64_t* __restrict__ fresh_out,
+    const int64_t* __restrict__ nwork_dev, const int64_t* __restrict__ seg_base, int nseg,
+    unsigned long long* __restrict__ segc) {
+  // order[0 .. nwork) lists the rows to process (nwork = *nwork_dev if given,
+  // else n).  With segments (independent builds packed together, nseg > 0),
+  // each row's changed-slot and new-id counts are also added to its build's
+  // pair segc[2s], segc[2s+1]; a block accumulates while consecutive rows
+  // stay in one build and flushes on a change.
   // fresh_in != nullptr: incremental join -- the pair (i -> n1 -> c) is a
   // candidate only if n1 is fresh in i's list or c is fresh in n1's list
   // (otherwise the previous iteration already considered it)
@@ -1038,7 +1116,16 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   // the order (Jacobi iteration).
   __shared__ int64_t s_row;
   const int xcd = blockIdx.x & 7;
-  const int64_t lo = n * xcd / 8, hi = n * (xcd + 1) / 8;
+  const int64_t nwork = nwork_dev ? *nwork_dev : n;
+  const int64_t lo = nwork * xcd / 8, hi = nwork * (xcd + 1) / 8;
+  // per-build counts of the current run of rows (thread 0 only; LDS keeps
+  // them out of the register budget)
+  __shared__ int seg_cur;
+  __shared__ unsigned long long seg_upd, seg_ins;
+  if (tid == 0) {
+    seg_cur = -1;
+    seg_upd = seg_ins = 0;
+  }
   unsigned long long* const qctr = rowq + 32 * xcd;
   if (tid == 0) s_row = lo + int64_t(atomicAdd(qctr, 1ull));
   __syncthreads();
@@ -1091,7 +1178,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     // ---- 3: sketch similarity, 8 lanes per candidate, 8 candidates per group
     //         in flight (16 B per lane each)
     {
-      constexpr int kSkPer = 8;
+      constexpr int kSkPer = SKPER;
       const int sub = tid & 7, grp = tid >> 3;
       uint64_t o0 = 0, o1 = 0;
       if (2 * sub < words) o0 = S.own_sk[2 * sub];
@@ -1180,9 +1267,9 @@ __global__ __launch_bounds__(256, OCC) void k_join(
         const int chn = s16 + 16 * c;
         own[c] = chn < nch ? *reinterpret_cast<const v4i*>(own_row + 16 * chn) : v4i{0, 0, 0, 0};
       }
-      // two candidates per 16-lane group in flight (64 B per lane each): the
-      // kernel then fits 128 VGPRs, i.e. four blocks per CU
-      constexpr int kDotPer = 2;
+      // DOTPER candidates per 16-lane group in flight (64 B per lane each);
+      // at 2 the kernel fits 128 VGPRs, i.e. four blocks per CU
+      constexpr int kDotPer = DOTPER;
       for (int s0 = 0; s0 < NS; s0 += 16 * kDotPer) {
         v4i v[kDotPer][4];
 #pragma unroll
@@ -1302,10 +1389,27 @@ __global__ __launch_bounds__(256, OCC) void k_join(
         my_cand += U;
         my_sel += NS;
         my_ins += S.cnt[4];
+        if (segc && tid == 0) {
+          const int sg = seg_of(seg_base, nseg, i);
+          if (sg != seg_cur) {
+            if (seg_cur >= 0 && (seg_upd | seg_ins)) {
+              atomicAdd(&segc[2 * seg_cur], seg_upd);
+              atomicAdd(&segc[2 * seg_cur + 1], seg_ins);
+            }
+            seg_cur = sg;
+            seg_upd = seg_ins = 0;
+          }
+          seg_upd += __popcll(m);
+          seg_ins += S.cnt[4];
+        }
       }
     }
     if (tid == 0) s_row = lo + int64_t(next_row);
     __syncthreads();
+  }
+  if (tid == 0 && segc && seg_cur >= 0 && (seg_upd | seg_ins)) {
+    atomicAdd(&segc[2 * seg_cur], seg_upd);
+    atomicAdd(&segc[2 * seg_cur + 1], seg_ins);
   }
   if (tid == 0) {
     unsigned long long* c = counters + 4 * (blockIdx.x % kNumCounters);
@@ -1338,20 +1442,78 @@ __global__ void k_sum_counters(unsigned long long* __restrict__ c,
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Packed builds: the join's work list is tree 0's leaf order restricted to the
+// segments still iterating.  Segment s occupies positions [seg_base[s],
+// seg_base[s+1]) of every tree's perm (no node crosses a segment), so the
+// list is a concatenation of those ranges.
+// ---------------------------------------------------------------------------
+
+// dst[s] = first list position of segment s (active ones), *nwork = list length
+__global__ __launch_bounds__(1024) void k_seg_scan(const int32_t* __restrict__ act,
+                                                   const int64_t* __restrict__ seg_base, int nseg,
+                                                   int64_t* __restrict__ dst, int64_t* __restrict__ nwork) {
+  __shared__ int64_t part[1024];
+  const int tid = threadIdx.x;
+  // each thread owns a contiguous run of segments
+  const int per = (nseg + 1023) / 1024, s0 = tid * per, s1 = min(nseg, s0 + per);
+  int64_t sum = 0;
+  for (int s = s0; s < s1; ++s) sum += act[s] ? seg_base[s + 1] - seg_base[s] : 0;
+  part[tid] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {   // inclusive scan of the runs
+    const int64_t v = tid >= o ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int64_t run = part[tid] - sum;
+  for (int s = s0; s < s1; ++s) {
+    dst[s] = run;
+    run += act[s] ? seg_base[s + 1] - seg_base[s] : 0;
+  }
+  if (tid == 1023) *nwork = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_seg_gather(const int32_t* __restrict__ perm0, int64_t n,
+                                                    const int64_t* __restrict__ seg_base, int nseg,
+                                                    const int32_t* __restrict__ act,
+                                                    const int64_t* __restrict__ dst,
+                                                    int32_t* __restrict__ list) {
+  for (int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x; p < n; p += int64_t(gridDim.x) * 256) {
+    const int s = seg_of(seg_base, nseg, p);
+    if (act[s]) list[dst[s] + (p - seg_base[s])] = perm0[p];
+  }
+}
+
+// snap[j] = c[j]; c[j] = 0 (per-segment join counters of one iteration)
+__global__ __launch_bounds__(256) void k_seg_snapshot(unsigned long long* __restrict__ c,
+                                                      unsigned long long* __restrict__ snap, int m) {
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) {
+    snap[j] = c[j];
+    c[j] = 0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // K8: export.  Plain: src/main.cpp:449-457.  Rerank: FP32 dots of the first R
 // list entries, one wave per point, best k by (score desc, id asc, slot).
 // ---------------------------------------------------------------------------
 
+// rows [row_base, row_base + n) of the graph; ids are written relative to
+// row_base (a packed build's own numbering; 0 for a single build)
 __global__ void k_export_plain(const int32_t* __restrict__ gid, const int32_t* __restrict__ gsc,
                                int64_t n, int K, int k, int32_t* __restrict__ knns,
-                               float* __restrict__ dists) {
+                               float* __restrict__ dists, int64_t row_base) {
   const int64_t total = n * k;
+  gid += row_base * K;
+  gsc += row_base * K;
   for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
        e += int64_t(gridDim.x) * blockDim.x) {
     const int64_t i = e / k;
     const int j = int(e - i * k);
-    knns[i * k + (k - 1 - j)] = gid[i * K + j] + 1;
+    knns[i * k + (k - 1 - j)] = int32_t(gid[i * K + j] - row_base) + 1;
     dists[i * k + (k - 1 - j)] = float(gsc[i * K + j]);
   }
 }
@@ -1360,7 +1522,7 @@ __global__ __launch_bounds__(256) void k_export_rerank(const float* __restrict__
                                                        int64_t n, int K, int R, int k,
                                                        const int32_t* __restrict__ gid,
                                                        int32_t* __restrict__ knns,
-                                                       float* __restrict__ dists) {
+                                                       float* __restrict__ dists, int64_t row_base) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nch = d >> 2;   // float4 chunks
   for (int64_t i = int64_t(blockIdx.x) * 4 + w; i < n; i += int64_t(gridDim.x) * 4) {
@@ -1368,11 +1530,11 @@ __global__ __launch_bounds__(256) void k_export_rerank(const float* __restrict__
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int chn = lane + 64 * c;
-      own[c] = chn < nch ? reinterpret_cast<const float4*>(x + i * d)[chn]
+      own[c] = chn < nch ? reinterpret_cast<const float4*>(x + (row_base + i) * d)[chn]
                          : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     // lane r ends up holding candidate r's id and FP32 score
-    const int32_t my_id = lane < R ? gid[i * K + lane] : -1;
+    const int32_t my_id = lane < R ? gid[(row_base + i) * K + lane] : -1;
     float my_s = 0.f;
     for (int r = 0; r < R; r += 2) {
       const int32_t ia = __shfl(my_id, r);
@@ -1408,7 +1570,7 @@ __global__ __launch_bounds__(256) void k_export_rerank(const float* __restrict__
       rank += (t > my_s || (t == my_s && (tid < my_id || (tid == my_id && r < lane)))) ? 1 : 0;
     }
     if (lane < R && rank < k) {
-      knns[i * k + (k - 1 - rank)] = my_id + 1;
+      knns[i * k + (k - 1 - rank)] = int32_t(my_id - row_base) + 1;
       dists[i * k + (k - 1 - rank)] = my_s;
     }
   }
@@ -1458,19 +1620,34 @@ int launch_quantize(hipStream_t stream, const float* dx, int64_t nrows, int d, i
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+int assign_splits(int64_t nrows, int C, int num_cu, int64_t part_cap) {
+  const int64_t g = (nrows + kAssignRows - 1) / kAssignRows;
+  const int64_t ntiles = (C + kAssignTile - 1) / kAssignTile;
+  // aim at two blocks per CU; every slice keeps at least two tiles
+  int64_t S = (2 * int64_t(num_cu) + g - 1) / g;
+  S = std::min<int64_t>(S, ntiles / 2);
+  if (nrows > 0) S = std::min<int64_t>(S, part_cap / nrows);
+  return int(std::max<int64_t>(1, std::min<int64_t>(S, 1024)));
+}
+
 int launch_assign(hipStream_t stream, const int8_t* q, int64_t nrows, int d, const int8_t* Z,
                   const float* w, int C, int32_t* out, int num_cu, int64_t row_off,
-                  int64_t row_stride, bool exclude_self, float* score) {
+                  int64_t row_stride, bool exclude_self, float* score, float* pbest,
+                  int32_t* pidx, int64_t part_cap) {
   if (nrows <= 0) return 0;
   int64_t g = (nrows + kAssignRows - 1) / kAssignRows;
   const int64_t cap = int64_t(num_cu) * 8;
   if (g > cap) g = cap;
+  const int S = (pbest && pidx) ? assign_splits(nrows, C, num_cu, part_cap) : 1;
   if (exclude_self)
-    hipLaunchKernelGGL(k_assign<true>, dim3(g), dim3(256), 0, stream, q, nrows, row_off, row_stride,
-                       d, Z, w, C, out, score);
+    hipLaunchKernelGGL(k_assign<true>, dim3(g, S), dim3(256), 0, stream, q, nrows, row_off, row_stride,
+                       d, Z, w, C, out, score, pbest, pidx);
   else
-    hipLaunchKernelGGL(k_assign<false>, dim3(g), dim3(256), 0, stream, q, nrows, row_off,
-                       row_stride, d, Z, w, C, out, score);
+    hipLaunchKernelGGL(k_assign<false>, dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
+                       row_stride, d, Z, w, C, out, score, pbest, pidx);
+  if (S > 1)
+    hipLaunchKernelGGL(k_assign_reduce, dim3(int(std::min<int64_t>((nrows + 255) / 256, 4 * num_cu))),
+                       dim3(256), 0, stream, pbest, pidx, S, nrows, out, score);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1511,11 +1688,6 @@ int quantize_rows(knng_ctx* c, const float* dx, int64_t row0, int64_t nrows) {
 // collect stray rows of many regions.
 // ---------------------------------------------------------------------------
 
-int32_t cells_for(const knng_params& p, int64_t n) {
-  if (p.cells >= 0) return p.cells;
-  const int64_t c = n / 1536;
-  return c >= 2 ? int32_t(c) : 0;
-}
 
 // stable counting sort of the listed rows by cell: members[offs[c]..offs[c+1])
 // in list order; row j of the list is id off + j * stride
@@ -1539,6 +1711,9 @@ struct KmeansBufs {
   int64_t* offs = nullptr;
   int64_t* sums = nullptr;   // [cap][d]
   int64_t* ids = nullptr;    // seed row ids [cap]
+  float* pbest = nullptr;    // split k_assign partials [pcap]
+  int32_t* pidx = nullptr;
+  int64_t pcap = 0;
 };
 
 // exact sums of the listed rows per cell into B.sums (device); counts on the host
@@ -1570,9 +1745,11 @@ int requantize(knng_ctx* c, KmeansBufs& B, const std::vector<int32_t>& cells, in
   return 0;
 }
 
-int assign_launch(knng_ctx* c, const int8_t* rows, int64_t nrows, int64_t off, int64_t stride,
-                  const int8_t* z, const float* w, int C, int32_t* out, bool excl, float* score) {
-  if (launch_assign(c->stream, rows, nrows, c->d, z, w, C, out, c->num_cu, off, stride, excl, score))
+int assign_launch(knng_ctx* c, KmeansBufs& B, const int8_t* rows, int64_t nrows, int64_t off,
+                  int64_t stride, const int8_t* z, const float* w, int C, int32_t* out, bool excl,
+                  float* score) {
+  if (launch_assign(c->stream, rows, nrows, c->d, z, w, C, out, c->num_cu, off, stride, excl, score,
+                    B.pbest, B.pidx, B.pcap))
     return fail(c, KNNG_EHIP, std::string("k_assign launch: ") + hipGetErrorString(hipGetLastError()));
   return 0;
 }
@@ -1606,7 +1783,7 @@ int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* zfin,
   std::vector<int32_t> a(ns), upd;
   std::vector<int64_t> cnt;
   for (int it = 0;; ++it) {
-    if ((rc = assign_launch(c, c->q, ns, off, stride, B.z, B.w, Cs, B.a, false, nullptr))) return rc;
+    if ((rc = assign_launch(c, B, c->q, ns, off, stride, B.z, B.w, Cs, B.a, false, nullptr))) return rc;
     HIPCHK(c, hipMemcpyAsync(a.data(), B.a, size_t(ns) * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     trace_mark(c, "kmeans sample assign");
@@ -1635,7 +1812,7 @@ int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* zfin,
   std::vector<float> bval(static_cast<size_t>(Cs)), w(static_cast<size_t>(Cs));
   std::vector<char> gone(static_cast<size_t>(Cs), 0);
   for (int round = 0; round < 64 && live.size() >= 2; ++round) {
-    if ((rc = assign_launch(c, B.z, Cs, 0, 1, B.z, B.w, Cs, B.a, true, B.sc))) return rc;
+    if ((rc = assign_launch(c, B, B.z, Cs, 0, 1, B.z, B.w, Cs, B.a, true, B.sc))) return rc;
     HIPCHK(c, hipMemcpyAsync(best.data(), B.a, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(bval.data(), B.sc, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(w.data(), B.w, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1703,7 +1880,7 @@ int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* zfin,
   int32_t* da = c->cassign + size_t(g) * n;
   std::vector<int32_t> af;
   for (int it = 0;; ++it) {
-    if ((rc = assign_launch(c, c->q, n, 0, 1, zfin, wfin, C, da, false, nullptr))) return rc;
+    if ((rc = assign_launch(c, B, c->q, n, 0, 1, zfin, wfin, C, da, false, nullptr))) return rc;
     trace_mark(c, "kmeans full assign");
     if (it == c->p.cell_full_iters) break;
     af.resize(size_t(n));
@@ -1733,6 +1910,8 @@ void free_kmeans(KmeansBufs& B, int8_t** merge_rows) {
   dfree(&B.offs);
   dfree(&B.sums);
   dfree(&B.ids);
+  dfree(&B.pbest);
+  dfree(&B.pidx);
   dfree(merge_rows);
 }
 
@@ -1750,6 +1929,7 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
   dfree(&c->cassign);
   if ((rc = dalloc(c, &c->cassign, size_t(G) * n, "cell assignment"))) return rc;
   KmeansBufs B;
+  B.pcap = assign_part_entries(cap, c->num_cu);
   int8_t* merge_rows = nullptr;
   if ((rc = dalloc(c, &B.z, size_t(cap) * d, "centroids")) ||
       (rc = dalloc(c, &B.w, size_t(cap), "centroid norms")) ||
@@ -1759,6 +1939,8 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
       (rc = dalloc(c, &B.offs, size_t(cap) + 1, "cell offsets")) ||
       (rc = dalloc(c, &B.sums, size_t(cap) * d, "cell sums")) ||
       (rc = dalloc(c, &B.ids, size_t(cap), "seed ids")) ||
+      (rc = dalloc(c, &B.pbest, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
+      (rc = dalloc(c, &B.pidx, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
       (rc = dalloc(c, &merge_rows, size_t(cap) * d, "merge rows"))) {
     free_kmeans(B, &merge_rows);
     return rc;
@@ -1820,6 +2002,7 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
         nd.left = nd.right = -1;
         nd.nleft = 0;
         nd.p1id = nd.p2id = -1;
+        nd.base = 0;
         roots[size_t(t) * Cm + cc] = int32_t(nodes.size());
         for (int64_t j = s; j < s + z; ++j) npos[size_t(t) * n + j] = int32_t(nodes.size());
         nodes.push_back(nd);
@@ -1868,6 +2051,112 @@ int check_params(const knng_params* p, int64_t n, std::string* why) {
   if (!(p->cell_prune >= 0.0f && p->cell_prune <= 1.0f)) return *why = "cell_prune must be in [0, 1]", KNNG_EINVAL;
   if (int64_t(p->num_trees) * n >= (int64_t(1) << 31) - 1) return *why = "num_trees * n must fit in int32", KNNG_ERANGE;
   if (n >= (int64_t(1) << 31) - 1) return *why = "n must fit in int32", KNNG_ERANGE;
+  return 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// k_join shapes and the autotuner.  Every shape computes the same lists (the
+// iteration is a deterministic function of the previous graph); they differ
+// in blocks per CU and in how many gathers each lane keeps in flight:
+//   0: 4 blocks/CU, 2 rows + 8 sketches in flight per lane group (128 VGPRs)
+//   1: 3 blocks/CU, 4 rows + 8 sketches
+//   2: 4 blocks/CU, 2 rows + 4 sketches
+// k_build > 50 has one shape (its 64 KB of LDS allows 2 blocks per CU).
+// The first join iteration of a (dim, k_build, rule) shape in a process times
+// each shape on a prefix of the work list (same rows, same outputs, so the
+// result is unchanged) and keeps the fastest; KNNG_JOIN_VARIANT=v forces v.
+// ---------------------------------------------------------------------------
+
+constexpr int kJoinVariants = 3;
+
+int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const int64_t* nwork_dev,
+                const int64_t* seg_base, int nseg, unsigned long long* segc) {
+  const int64_t n = c->n;
+  const int K = c->p.k_build, M = K * c->p.rerank_mult;
+  // grid: a multiple of the 8 XCDs (the schedule needs it), four blocks per
+  // CU; blocks beyond what is resident just find their row queue drained
+  const int grid = std::max(8, (c->num_cu * 4) / 8 * 8);
+  unsigned long long* rowq = c->counters + 4 * kNumCounters;
+  const uint64_t* fin = c->p.incremental ? c->fresh[a] : nullptr;
+#define KNNG_JOIN_ARGS                                                                          \
+  c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order, \
+      fin, c->fresh[b], nwork_dev, seg_base, nseg, segc
+  if (K > 50)
+    hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+  else if (v == 1)
+    hipLaunchKernelGGL((k_join<12, 2560, 3, 4, 8>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+  else if (v == 2)
+    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 4>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+  else
+    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 8>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+#undef KNNG_JOIN_ARGS
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+struct TuneCache {
+  std::mutex mu;
+  std::map<std::tuple<int, int, int>, int> best;
+};
+
+TuneCache& tune_cache() {
+  static TuneCache tc;
+  return tc;
+}
+
+// Choose c->join_variant (see above) before the first launch of iteration
+// a -> b over order[0 .. nwork).  The timing launches also add to segc
+// (packed builds), which the caller clears afterwards.
+int ensure_join_variant(knng_ctx* c, int a, int b, const int32_t* order, int64_t nwork,
+                        const int64_t* seg_base, int nseg, unsigned long long* segc) {
+  if (c->join_variant >= 0) return 0;
+  c->st.ms_tune = 0;
+  if (c->p.k_build > 50) return c->join_variant = 0, 0;
+  if (const char* e = std::getenv("KNNG_JOIN_VARIANT")) {
+    const int v = std::atoi(e);
+    if (v >= 0 && v < kJoinVariants) return c->join_variant = v, 0;
+  }
+  const auto key = std::make_tuple(c->d, c->p.k_build, c->p.incremental);
+  {
+    std::lock_guard<std::mutex> lk(tune_cache().mu);
+    auto it = tune_cache().best.find(key);
+    if (it != tune_cache().best.end()) return c->join_variant = it->second, 0;
+  }
+  // too few rows to time reliably: the default shape, not cached
+  const int64_t rows = std::max<int64_t>(65536, nwork / 32);
+  if (nwork < 2 * rows) return c->join_variant = 0, 0;
+  int rc;
+  if ((rc = dalloc(c, &c->tune_n, 1, "tuning size"))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->tune_n, &rows, sizeof(rows), hipMemcpyHostToDevice, c->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  float best_ms = 0, total = 0;
+  int best = 0;
+  for (int round = 0; round < 2; ++round)
+    for (int v = 0; v < kJoinVariants; ++v) {
+      (void)hipEventRecord(e0, c->stream);
+      if ((rc = launch_join(c, v, a, b, order, c->tune_n, seg_base, nseg, segc))) break;
+      (void)hipEventRecord(e1, c->stream);
+      hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(256), 0, c->stream, c->counters, c->csum);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      total += ms;
+      if (round == 1 && (v == 0 || ms < best_ms)) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  c->st.ms_tune = total;
+  c->join_variant = best;
+  std::lock_guard<std::mutex> lk(tune_cache().mu);
+  tune_cache().best[key] = best;
   return 0;
 }
 
@@ -1984,6 +2273,8 @@ int knng_destroy(knng_ctx* c) {
   }
   dfree(&c->counters);
   dfree(&c->csum);
+  dfree(&c->seg_base);
+  dfree(&c->tune_n);
   dfree(&c->ex_knns);
   dfree(&c->ex_dists);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -2109,7 +2400,7 @@ int knng_build_forest(knng_ctx* c) {
   const int leaf = c->p.leaf_size;
   c->node_cap = T * (n / (leaf + 1) + 1) + 16;
   // every non-empty cell below the leaf size is a leaf of its own
-  c->leaf_cap = total / 4 + T + 1024 +
+  c->leaf_cap = total / 4 + T + 1024 + T * int64_t(c->nseg) +
                 T * std::min<int64_t>(int64_t(cells_for(c->p, n)) * std::max(1, c->p.cell_oversample), n);
   int rc;
   if ((rc = dalloc(c, &c->perm, total, "perm")) || (rc = dalloc(c, &c->perm2, total, "perm2")) ||
@@ -2130,7 +2421,53 @@ int knng_build_forest(knng_ctx* c) {
   HIPCHK(c, hipGetLastError());
   int nnodes = int(T);
   int levels = 0;
-  if (n <= leaf) {
+  if (c->nseg > 0) {
+    // one root per (tree, segment) above the leaf size, else a leaf; pivots
+    // as in the segment's own build (start 0, base = its first row)
+    const int S = c->nseg;
+    std::vector<Node> roots;
+    std::vector<int4> lv;
+    std::vector<int32_t> segnode(size_t(T) * S, -1);
+    for (int t = 0; t < T; ++t)
+      for (int sg = 0; sg < S; ++sg) {
+        const int64_t b = c->seg_host[size_t(sg)], z = c->seg_host[size_t(sg) + 1] - b;
+        if (z > leaf) {
+          Node nd;
+          nd.tree = t;
+          nd.start = int32_t(b);
+          nd.size = int32_t(z);
+          node_pivots(c->p.seed, t, 0, 0, z, &nd.p1, &nd.p2);
+          nd.left = nd.right = -1;
+          nd.nleft = 0;
+          nd.p1id = nd.p2id = -1;
+          nd.base = int32_t(b);
+          segnode[size_t(t) * S + sg] = int32_t(roots.size());
+          roots.push_back(nd);
+        } else {
+          lv.push_back(make_int4(t, int(b), int(z), 0));
+        }
+      }
+    if (int64_t(roots.size()) > c->node_cap || int64_t(lv.size()) > c->leaf_cap)
+      return fail(c, KNNG_ERANGE, "packed builds exceed the node/leaf tables");
+    int32_t* dsegnode = nullptr;
+    if ((rc = dalloc(c, &dsegnode, segnode.size(), "segment roots"))) return rc;
+    HIPCHK(c, hipMemcpyAsync(dsegnode, segnode.data(), segnode.size() * 4, hipMemcpyHostToDevice,
+                             c->stream));
+    if (!roots.empty())
+      HIPCHK(c, hipMemcpyAsync(c->nodes, roots.data(), roots.size() * sizeof(Node), hipMemcpyHostToDevice,
+                               c->stream));
+    if (!lv.empty())
+      HIPCHK(c, hipMemcpyAsync(c->leaves, lv.data(), lv.size() * sizeof(int4), hipMemcpyHostToDevice,
+                               c->stream));
+    int32_t cnt[2] = {0, int32_t(lv.size())};
+    HIPCHK(c, hipMemcpyAsync(c->dcount, cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_forest_seed_segs, dim3(grid_for(c, total, 256, 16)), dim3(256), 0, c->stream, n,
+                       int(T), c->seg_base, S, dsegnode, c->perm, c->nodepos);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(&dsegnode);
+    nnodes = int(roots.size());
+  } else if (n <= leaf) {
     // every tree is one leaf
     std::vector<int4> lv(T);
     for (int t = 0; t < T; ++t) lv[t] = make_int4(t, 0, int(n), 0);
@@ -2144,7 +2481,7 @@ int knng_build_forest(knng_ctx* c) {
   c->nparts = 0;
   c->cell_count.clear();
   c->st.ms_kmeans = 0;
-  const int32_t ncells = cells_for(c->p, n);
+  const int32_t ncells = c->nseg > 0 ? 0 : cells_for(c->p, n);
   if (nnodes > 0 && ncells >= 2) {
     // k-means cells replace the root split of every tree
     c->nparts = std::max(1, std::min<int>(c->p.cell_partitions, int(T)));
@@ -2273,21 +2610,22 @@ int knng_build_init(knng_ctx* c) {
   const int slots = T * kLeafSlots;
   const int mgrid = grid_for(c, n, 1, 32);
   c->cur = 0;
+  const int64_t* sb = c->nseg ? c->seg_base : nullptr;
   if (slots <= 256)
     hipLaunchKernelGGL(k_init_merge<256>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
-                       cand, c->gid[0], c->gsc[0]);
+                       cand, c->gid[0], c->gsc[0], sb, c->nseg);
   else if (slots <= 512)
     hipLaunchKernelGGL(k_init_merge<512>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
-                       cand, c->gid[0], c->gsc[0]);
+                       cand, c->gid[0], c->gsc[0], sb, c->nseg);
   else if (slots <= 1024)
     hipLaunchKernelGGL(k_init_merge<1024>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
-                       cand, c->gid[0], c->gsc[0]);
+                       cand, c->gid[0], c->gsc[0], sb, c->nseg);
   else if (slots <= 2048)
     hipLaunchKernelGGL(k_init_merge<2048>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
-                       cand, c->gid[0], c->gsc[0]);
+                       cand, c->gid[0], c->gsc[0], sb, c->nseg);
   else
     hipLaunchKernelGGL(k_init_merge<4096>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
-                       cand, c->gid[0], c->gsc[0]);
+                       cand, c->gid[0], c->gsc[0], sb, c->nseg);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -2305,23 +2643,11 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   HIPCHK(c, hipSetDevice(c->device));
   const int64_t n = c->n;
   const int K = c->p.k_build, M = K * c->p.rerank_mult;
-  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   const int a = c->cur, b = c->cur ^ 1;
-  // grid: a multiple of the 8 XCDs (the kernel's schedule needs it), four
-  // blocks per CU (what fits at K <= 50; larger K leaves extra blocks queued,
-  // which the row queues make harmless)
-  const int grid = std::max(8, (c->num_cu * 4) / 8 * 8);
-  unsigned long long* rowq = c->counters + 4 * kNumCounters;
   const int32_t* order = c->have_forest ? c->perm : nullptr;   // tree 0's leaf order
-  if (K <= 50)
-    hipLaunchKernelGGL((k_join<12, 2560, 4>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n,
-                       K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order,
-                       c->p.incremental ? c->fresh[a] : nullptr, c->fresh[b]);
-  else
-    hipLaunchKernelGGL((k_join<13, 4096, 2>), dim3(grid), dim3(256), 0, c->stream, c->q, c->sk, c->d, n,
-                       K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order,
-                       c->p.incremental ? c->fresh[a] : nullptr, c->fresh[b]);
-  HIPCHK(c, hipGetLastError());
+  if (int rc = ensure_join_variant(c, a, b, order, n, nullptr, 0, nullptr)) return rc;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  if (int rc = launch_join(c, c->join_variant, a, b, order, nullptr, nullptr, 0, nullptr)) return rc;
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(256), 0, c->stream, c->counters, c->csum);
   unsigned long long sums[4];
@@ -2352,6 +2678,7 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
 
 int knng_build(knng_ctx* c) {
   if (!c) return KNNG_EINVAL;
+  if (c->nseg > 0) return fail(c, KNNG_ESTATE, "knng_build: packed builds run through knng_build_batch");
   memset(c->st.updates, 0, sizeof(c->st.updates));
   c->st.iterations = 0;
   c->st.ms_join = 0;
@@ -2391,14 +2718,14 @@ int knng_export_device(knng_ctx* c, int32_t* dk, float* dd, int32_t k, int32_t r
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   if (!rerank) {
     hipLaunchKernelGGL(k_export_plain, dim3(grid_for(c, n * k, 256, 16)), dim3(256), 0, c->stream,
-                       c->gid[c->cur], c->gsc[c->cur], n, K, k, dk, dd);
+                       c->gid[c->cur], c->gsc[c->cur], n, K, k, dk, dd, int64_t(0));
   } else {
     const float* x = c->x_ext ? c->x_ext : c->x;
     if (!x) return fail(c, KNNG_ESTATE, "knng_export: rerank needs float32 rows (keep_fp32 or attach)");
     int R = c->p.rerank_width ? c->p.rerank_width : K;
     if (R < k) R = k;
     hipLaunchKernelGGL(k_export_rerank, dim3(grid_for(c, n, 4, 16)), dim3(256), 0, c->stream, x, c->d,
-                       n, K, R, k, c->gid[c->cur], dk, dd);
+                       n, K, R, k, c->gid[c->cur], dk, dd, int64_t(0));
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -2430,6 +2757,7 @@ int knng_export(knng_ctx* c, int32_t* knns, float* dists, int32_t k, int32_t rer
 
 int knng_get_stats(knng_ctx* c, knng_stats* st) {
   if (!c || !st) return KNNG_EINVAL;
+  c->st.join_variant = c->join_variant;
   *st = c->st;
   return 0;
 }
@@ -2551,3 +2879,208 @@ int knng_score_pairs_device(const int8_t* dq, const uint64_t* dsk, int32_t d, co
 }
 
 }  // extern "C"
+
+namespace knng_host {
+
+int set_segments(knng_ctx* c, const std::vector<int64_t>& seg) {
+  if (!c || seg.size() < 2 || seg.front() != 0 || seg.back() != c->n) return KNNG_EINVAL;
+  for (size_t s = 0; s + 1 < seg.size(); ++s)
+    if (seg[s + 1] - seg[s] < 2) return fail(c, KNNG_EINVAL, "every packed build needs >= 2 rows");
+  if (c->p.cells != 0) return fail(c, KNNG_EINVAL, "packed builds use plain RP trees (cells = 0)");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->nseg = int32_t(seg.size() - 1);
+  c->seg_host = seg;
+  dfree(&c->seg_base);
+  if (int rc = dalloc(c, &c->seg_base, seg.size(), "segment bases")) return rc;
+  HIPCHK(c, hipMemcpy(c->seg_base, seg.data(), seg.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  return 0;
+}
+
+namespace {
+
+// buffers of one packed join loop
+struct PackedLoop {
+  int32_t* act = nullptr;          // device: segment still iterating
+  int64_t* dst = nullptr;          // device: list position of each segment
+  int64_t* nwork = nullptr;        // device: list length
+  int32_t* list = nullptr;         // device: rows to process this iteration
+  unsigned long long* segc = nullptr;   // device: per-segment (changed, new) counts
+  unsigned long long* snap = nullptr;   // device: their snapshot
+  int32_t* act_h = nullptr;        // pinned host copies
+  unsigned long long* snap_h = nullptr;
+  unsigned long long* csum_h = nullptr;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+void free_loop(PackedLoop& L) {
+  dfree(&L.act);
+  dfree(&L.dst);
+  dfree(&L.nwork);
+  dfree(&L.list);
+  dfree(&L.segc);
+  dfree(&L.snap);
+  if (L.act_h) (void)hipHostFree(L.act_h);
+  if (L.snap_h) (void)hipHostFree(L.snap_h);
+  if (L.csum_h) (void)hipHostFree(L.csum_h);
+  for (auto& e : L.exec)
+    if (e) (void)hipGraphExecDestroy(e);
+}
+
+// enqueue one join iteration a -> b of the packed build on c->stream
+int enqueue_packed_iteration(knng_ctx* c, PackedLoop& L, int a, int b) {
+  const int S = c->nseg;
+  HIPCHK(c, hipMemcpyAsync(L.act, L.act_h, size_t(S) * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, c->stream, L.act, c->seg_base, S, L.dst, L.nwork);
+  hipLaunchKernelGGL(k_seg_gather, dim3(grid_for(c, c->n, 256, 8)), dim3(256), 0, c->stream, c->perm, c->n,
+                     c->seg_base, S, L.act, L.dst, L.list);
+  if (int rc = launch_join(c, c->join_variant, a, b, L.list, L.nwork, c->seg_base, S, L.segc)) return rc;
+  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(256), 0, c->stream, c->counters, c->csum);
+  hipLaunchKernelGGL(k_seg_snapshot, dim3(std::max(1, std::min(1024, (2 * S + 255) / 256))), dim3(256), 0,
+                     c->stream, L.segc, L.snap, 2 * S);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(L.snap_h, L.snap, size_t(2 * S) * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(L.csum_h, c->csum, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           c->stream));
+  return 0;
+}
+
+}  // namespace
+
+int build_packed(knng_ctx* c, std::vector<int32_t>* iters, std::vector<int32_t>* cur, bool use_graph,
+                 bool* graph_used) {
+  if (!c || c->nseg < 1) return KNNG_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int S = c->nseg;
+  const int K = c->p.k_build;
+  memset(c->st.updates, 0, sizeof(c->st.updates));
+  c->st.iterations = 0;
+  c->st.ms_join = 0;
+  int rc;
+  if ((rc = knng_build_forest(c))) return rc;
+  if ((rc = knng_build_init(c))) return rc;
+  PackedLoop L;
+  auto bail = [&](int code) {
+    free_loop(L);
+    return code;
+  };
+  if ((rc = dalloc(c, &L.act, size_t(S), "segment flags")) || (rc = dalloc(c, &L.dst, size_t(S), "segment offsets")) ||
+      (rc = dalloc(c, &L.nwork, 1, "work size")) || (rc = dalloc(c, &L.list, size_t(c->n), "work list")) ||
+      (rc = dalloc(c, &L.segc, size_t(2 * S), "segment counters")) ||
+      (rc = dalloc(c, &L.snap, size_t(2 * S), "segment counters")))
+    return bail(rc);
+  if (hipHostMalloc(&L.act_h, size_t(S) * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&L.snap_h, size_t(2 * S) * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&L.csum_h, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+    return bail(fail(c, KNNG_ENOMEM, "hipHostMalloc failed for the packed join loop"));
+  HIPCHK(c, hipMemsetAsync(L.segc, 0, size_t(2 * S) * sizeof(unsigned long long), c->stream));
+  for (int s = 0; s < S; ++s) L.act_h[s] = 1;
+  iters->assign(size_t(S), 0);
+  cur->assign(size_t(S), 0);
+  std::vector<float> prev(size_t(S), 1.0f);
+  c->cur = 0;
+  // autotune on the first iteration's full work list (every row of tree 0's order)
+  if ((rc = ensure_join_variant(c, 0, 1, c->perm, c->n, c->seg_base, S, L.segc))) return bail(rc);
+  HIPCHK(c, hipMemsetAsync(L.segc, 0, size_t(2 * S) * sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // one graph per buffer direction
+  bool graphs = use_graph;
+  for (int dir = 0; dir < 2 && graphs; ++dir) {
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      graphs = false;
+      break;
+    }
+    const int erc = enqueue_packed_iteration(c, L, dir, dir ^ 1);
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+    if (erc || ec != hipSuccess || !g ||
+        hipGraphInstantiate(&L.exec[dir], g, nullptr, nullptr, 0) != hipSuccess)
+      graphs = false;
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+  }
+  if (!graphs)
+    for (auto& e : L.exec)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+  *graph_used = graphs;
+  int active = S;
+  for (int it = 0; it < c->p.max_iters && active > 0; ++it) {
+    const int a = c->cur, b = a ^ 1;
+    int64_t rows = 0;   // rows this iteration processes
+    for (int s = 0; s < S; ++s)
+      if (L.act_h[s]) rows += c->seg_host[size_t(s) + 1] - c->seg_host[size_t(s)];
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (graphs) {
+      HIPCHK(c, hipGraphLaunch(L.exec[a], c->stream));
+    } else if ((rc = enqueue_packed_iteration(c, L, a, b))) {
+      return bail(rc);
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->cur = b;
+    if (it < 32) {
+      c->st.updates[it] = int64_t(L.csum_h[0]);
+      c->st.rates[it] = float(L.csum_h[0]) / float(size_t(c->n) * size_t(K));
+      c->st.ms_join_iter[it] = ms;
+      c->st.join_candidates[it] = int64_t(L.csum_h[1]);
+      c->st.join_rescored[it] = int64_t(L.csum_h[2]);
+      c->st.join_inserted[it] = int64_t(L.csum_h[3]);
+      c->st.insert_rates[it] = float(L.csum_h[3]) / float(size_t(c->n) * size_t(K));
+      // algorithmic bytes as knng_join_iteration counts them, over the rows processed
+      c->st.join_bytes[it] = double(rows) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) +
+                             double(L.csum_h[1]) * 8.0 * c->words + double(L.csum_h[2]) * c->d;
+    }
+    c->st.iterations = it + 1;
+    c->st.ms_join += ms;
+    // each segment's stop rule, exactly as knng_build applies it to a build alone
+    for (int s = 0; s < S; ++s) {
+      if (!L.act_h[s]) continue;
+      const int64_t ns = c->seg_host[size_t(s) + 1] - c->seg_host[size_t(s)];
+      const int j = (*iters)[size_t(s)];
+      (*iters)[size_t(s)] = j + 1;
+      (*cur)[size_t(s)] = b;
+      bool stop = false;
+      if (c->p.stop_rule == KNNG_STOP_INSERTED) {
+        stop = float(L.snap_h[2 * s + 1]) / float(size_t(ns) * size_t(K)) < c->p.insert_stop_rate;
+      } else if (c->p.stop_rule == KNNG_STOP_REFERENCE) {
+        const float rate = float(L.snap_h[2 * s]) / float(size_t(ns) * size_t(K));
+        stop = rate < c->p.early_stop_rate || (j > 0 && (prev[size_t(s)] - rate) < c->p.rate_diff_tol);
+        prev[size_t(s)] = rate;
+      }
+      if (stop) {
+        L.act_h[s] = 0;
+        --active;
+      }
+    }
+  }
+  c->st.ms_build = c->st.ms_forest + c->st.ms_init + c->st.ms_join;
+  free_loop(L);
+  return 0;
+}
+
+int export_segment(knng_ctx* c, int s, int cur, int k, int rerank, int32_t* dk, float* dd) {
+  if (!c || s < 0 || s >= c->nseg || k < 1 || k > c->p.k_build) return KNNG_EINVAL;
+  const int64_t base = c->seg_host[size_t(s)], ns = c->seg_host[size_t(s) + 1] - base;
+  const int K = c->p.k_build;
+  if (!rerank) {
+    hipLaunchKernelGGL(k_export_plain, dim3(grid_for(c, ns * k, 256, 16)), dim3(256), 0, c->stream,
+                       c->gid[cur], c->gsc[cur], ns, K, k, dk, dd, base);
+  } else {
+    const float* x = c->x_ext ? c->x_ext : c->x;
+    if (!x) return fail(c, KNNG_ESTATE, "export_segment: rerank needs float32 rows");
+    int R = c->p.rerank_width ? c->p.rerank_width : K;
+    if (R < k) R = k;
+    hipLaunchKernelGGL(k_export_rerank, dim3(grid_for(c, ns, 4, 16)), dim3(256), 0, c->stream, x, c->d,
+                       ns, K, R, k, c->gid[cur], dk, dd, base);
+  }
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace knng_host
+

@@ -134,7 +134,23 @@ struct Node {
   int32_t tree, start, size, p1, p2;   // p1/p2: pivot positions inside the segment
   int32_t left, right, nleft;
   int32_t p1id, p2id;                  // pivot row ids
+  int32_t base;   // first row of the node's build when independent builds share one forest
+                  // (pivots are drawn from start - base, as in that build alone); else 0
 };
+
+// Build (segment) of row r when independent builds are packed into one
+// context: the largest s with seg_base[s] <= r (seg_base ascending).
+__device__ __host__ __forceinline__ int seg_of(const int64_t* seg_base, int nseg, int64_t r) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg_base[mid] <= r)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
 
 __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int* total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;

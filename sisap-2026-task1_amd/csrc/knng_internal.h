@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -67,6 +68,16 @@ struct knng_ctx {
   int32_t* ex_knns = nullptr;
   float* ex_dists = nullptr;
   int64_t ex_cap = 0;
+
+  // independent builds packed into one context (knng_build_batch): segment s
+  // is rows [seg_host[s], seg_host[s+1]); trees, leaves and edges never
+  // cross a segment, and each segment's lists are exactly those of a build
+  // of its rows alone
+  int32_t nseg = 0;
+  std::vector<int64_t> seg_host;
+  int64_t* seg_base = nullptr;   // device copy of seg_host
+  int64_t* tune_n = nullptr;     // device scalar: rows of an autotuning launch
+  int join_variant = -1;         // k_join shape in use (-1: not chosen yet)
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   knng_stats st{};
@@ -142,13 +153,40 @@ inline int grid_for(const knng_ctx* c, int64_t work, int per_block, int blocks_p
   return int(g);
 }
 
+// Cells of the k-means partition for n rows (0: plain RP trees).
+inline int32_t cells_for(const knng_params& p, int64_t n) {
+  if (p.cells >= 0) return p.cells;
+  const int64_t c = n / 1536;
+  return c >= 2 ? int32_t(c) : 0;
+}
+
+// Packed independent builds (knng.hip).  set_segments: the segment bases of
+// a fresh context (seg[0] = 0, seg[nseg] = n, every segment >= 2 rows, no
+// cells).  build_packed: forest + init + join for every segment at once, each
+// with its own stop rule; iters[s] = the segment's join iterations, cur[s] =
+// the graph buffer with its final lists.  use_graph: each join iteration is
+// one hipGraph launch (captured once per buffer direction).
+// export_segment: the best k of segment s into device buffers, ids relative
+// to the segment.
+int set_segments(knng_ctx* c, const std::vector<int64_t>& seg);
+int build_packed(knng_ctx* c, std::vector<int32_t>* iters, std::vector<int32_t>* cur, bool use_graph,
+                 bool* graph_used);
+int export_segment(knng_ctx* c, int s, int cur, int k, int rerank, int32_t* dk, float* dd);
+
 // K1 on an arbitrary stream (defined in knng.hip)
 int launch_quantize(hipStream_t stream, const float* dx, int64_t nrows, int d, int8_t* q,
                     uint64_t* sk, int num_cu);
 // route int8 rows to cells: out[j] = argmax_c float(<q_r, Z_c>) * w[c] for the rows
-// r = row_off + j * row_stride (knng.hip); exclude_self skips c == j, score gets the best value
+// r = row_off + j * row_stride (knng.hip); exclude_self skips c == j, score gets the best value.
+// With partial arrays pbest/pidx of part_cap entries each, a launch with few rows splits the
+// cells over several blocks per row tile (same result).
 int launch_assign(hipStream_t stream, const int8_t* q, int64_t nrows, int d, const int8_t* Z,
                   const float* w, int C, int32_t* out, int num_cu, int64_t row_off = 0,
-                  int64_t row_stride = 1, bool exclude_self = false, float* score = nullptr);
+                  int64_t row_stride = 1, bool exclude_self = false, float* score = nullptr,
+                  float* pbest = nullptr, int32_t* pidx = nullptr, int64_t part_cap = 0);
+// partial-array entries a split launch_assign over nrows rows and C cells can use
+inline int64_t assign_part_entries(int64_t nrows, int num_cu) {
+  return nrows * std::max<int64_t>(1, (2 * int64_t(num_cu) * 128 + nrows - 1) / nrows);
+}
 
 }  // namespace knng_host

@@ -16,11 +16,24 @@
 // Results depend only on the index contents, never on what else is running.
 //
 // Scheduler (one host thread): requests are split into max_batch pieces;
-// each piece runs on a "slot" (HIP stream + buffers).  Slots are created on
-// demand while fewer than max_slots exist and hipMemGetInfo leaves
+// each piece runs on a "slot" (HIP stream + buffers).  Bulk slots are created
+// on demand while fewer than max_slots exist and hipMemGetInfo leaves
 // reserve_bytes free; pieces of different requests overlap on the device.
-// Inserts drain the running queries, then append rows, search their
-// neighbours, and merge reverse edges into the neighbours' lists.
+// Query requests of at most express_rows rows go to a second queue served by
+// express slots, created with the server; with express_cus > 0 their streams
+// carry a CU mask of express_cus CUs spread over the chip and the bulk
+// streams carry the complement, so a small request's blocks start at once
+// instead of waiting for bulk blocks to drain from the CUs.
+// Inserts append rows, search their neighbours and merge reverse edges into
+// the neighbours' lists.  By default they run on an insert thread and stream
+// of their own beside the searches (pending inserts are applied together, up
+// to the insert scratch size): a query sees the inserts finished before it
+// was dispatched (knng_server_wait_info reports that index size), and one
+// that overlaps an insert may or may not see its rows -- every list word is
+// an aligned 4-byte store, so a search reads old or new ids, always valid
+// rows.  With strict_order = 1 an insert instead waits until the queries
+// submitted before it have finished and a query runs only after the inserts
+// submitted before it (its epoch), whichever lane it takes.
 
 #include <hip/hip_runtime.h>
 
@@ -370,12 +383,22 @@ struct Slot {
   int32_t* did = nullptr;
   float* dsc = nullptr;
   int32_t* dcell = nullptr;   // query cells, one row of max_batch per partition
+  float* pbest = nullptr;     // split cell-routing partials
+  int32_t* pidx = nullptr;
+  int64_t pcap = 0;
   float* hx = nullptr;        // pinned staging
   int32_t* hid = nullptr;
   float* hsc = nullptr;
   bool busy = false;
-  int64_t ticket = -1;
-  int32_t row0 = 0, nrows = 0;
+  bool express = false;
+  // the rows of this launch: part j is rows [row0, row0 + nrows) of request
+  // `ticket`, staged at launch row `off` (bulk: one part; express: every
+  // small request that was waiting, packed into one launch)
+  struct Part {
+    int64_t ticket;
+    int32_t row0, nrows, off;
+  };
+  std::vector<Part> parts;
 };
 
 struct Request {
@@ -389,6 +412,9 @@ struct Request {
   int32_t running = 0;
   int state = 0;   // 0 queued, 1 running, 2 done
   int error = 0;
+  int64_t epoch = 0;   // inserts submitted before this request
+  int64_t rows_seen = -1;   // index rows when the first piece was dispatched
+  int64_t first_id = 0;     // inserts: id of the first new row
   clk::time_point t_submit, t_done;
 };
 
@@ -409,11 +435,23 @@ struct knng_server {
   std::vector<Slot*> slots;
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<Request*> queue;
+  std::deque<Request*> queue;     // bulk queries and inserts, in submission order
+  std::deque<Request*> express;   // query requests of at most express_rows rows
+  int64_t inserts_submitted = 0, inserts_done = 0;
+  bool cu_masked = false;         // the express / bulk CU split is in force
+  int bulk_blocks = 0;            // search blocks per bulk launch
+  std::string mask_err;           // why the runtime refused the CU masks, if it did
+  // concurrent inserts: queue, thread and preallocated scratch for ins_cap rows
+  std::deque<Request*> ins_queue;
+  std::thread ins_th;
+  int32_t ins_cap = 0;
+  int32_t *ins_did = nullptr, *ins_dcell = nullptr, *ins_pidx = nullptr;
+  float *ins_dsc = nullptr, *ins_pbest = nullptr;
+  int32_t *ins_t = nullptr, *ins_off = nullptr, *ins_pi = nullptr, *ins_ps = nullptr, *ins_pf = nullptr;
   std::map<int64_t, Request*> reqs;
   int64_t next_ticket = 1;
   bool stop = false;
-  std::thread th;
+  std::thread th, th_express;   // bulk-lane and express-lane schedulers
   std::string err;
 };
 
@@ -441,7 +479,8 @@ void free_slot(Slot* sl) {
   if (sl->stream) (void)hipStreamSynchronize(sl->stream);
   for (void* p : {static_cast<void*>(sl->dx), static_cast<void*>(sl->dq),
                   static_cast<void*>(sl->dsk), static_cast<void*>(sl->did),
-                  static_cast<void*>(sl->dsc), static_cast<void*>(sl->dcell)})
+                  static_cast<void*>(sl->dsc), static_cast<void*>(sl->dcell),
+                  static_cast<void*>(sl->pbest), static_cast<void*>(sl->pidx)})
     if (p) (void)hipFree(p);
   for (void* p : {static_cast<void*>(sl->hx), static_cast<void*>(sl->hid), static_cast<void*>(sl->hsc)})
     if (p) (void)hipHostFree(p);
@@ -450,10 +489,38 @@ void free_slot(Slot* sl) {
   delete sl;
 }
 
-Slot* new_slot(knng_server* s) {
+// CU mask of the express streams (express = true) or of the bulk streams:
+// express_cus CUs at even spacing over the chip (so each XCD gives up a few)
+std::vector<uint32_t> cu_mask(const knng_server* s, bool express) {
+  const int ncu = s->num_cu;
+  std::vector<uint32_t> m(size_t((ncu + 31) / 32), 0u);
+  const int e = std::min(s->sp.express_cus, ncu - 1);
+  std::vector<char> res(size_t(ncu), 0);
+  for (int j = 0; j < e; ++j) res[size_t(int64_t(j) * ncu / e)] = 1;
+  for (int c = 0; c < ncu; ++c)
+    if (bool(res[size_t(c)]) == express) m[size_t(c / 32)] |= 1u << (c % 32);
+  return m;
+}
+
+// A slot stream.  Where the runtime refuses CU masks (checked on the first
+// express stream, before any bulk stream exists) the server runs unmasked.
+hipError_t make_stream(knng_server* s, bool express, hipStream_t* st) {
+  if (s->cu_masked) {
+    const std::vector<uint32_t> m = cu_mask(s, express);
+    const hipError_t e = hipExtStreamCreateWithCUMask(st, uint32_t(m.size()), m.data());
+    if (e == hipSuccess || s->slots.size() > 0) return e;
+    (void)hipGetLastError();
+    s->cu_masked = false;
+    s->mask_err = hipGetErrorString(e);
+  }
+  return hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+}
+
+Slot* new_slot(knng_server* s, bool express) {
   auto* sl = new Slot();
+  sl->express = express;
   const size_t B = size_t(s->sp.max_batch);
-  bool ok = hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) == hipSuccess &&
+  bool ok = make_stream(s, express, &sl->stream) == hipSuccess &&
             hipEventCreateWithFlags(&sl->done, hipEventDisableTiming) == hipSuccess &&
             hipMalloc(&sl->dx, B * s->d * sizeof(float)) == hipSuccess &&
             hipMalloc(&sl->dq, B * s->d) == hipSuccess &&
@@ -461,6 +528,10 @@ Slot* new_slot(knng_server* s) {
             hipMalloc(&sl->did, B * s->sp.k * sizeof(int32_t)) == hipSuccess &&
             hipMalloc(&sl->dsc, B * s->sp.k * sizeof(float)) == hipSuccess &&
             hipMalloc(&sl->dcell, B * std::max(1, s->ctx->nparts) * sizeof(int32_t)) == hipSuccess &&
+            hipMalloc(&sl->pbest, size_t(knng_host::assign_part_entries(int64_t(B), s->num_cu)) *
+                                      sizeof(float)) == hipSuccess &&
+            hipMalloc(&sl->pidx, size_t(knng_host::assign_part_entries(int64_t(B), s->num_cu)) *
+                                     sizeof(int32_t)) == hipSuccess &&
             hipHostMalloc(&sl->hx, B * s->d * sizeof(float), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc(&sl->hid, B * s->sp.k * sizeof(int32_t), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc(&sl->hsc, B * s->sp.k * sizeof(float), hipHostMallocDefault) == hipSuccess;
@@ -469,23 +540,28 @@ Slot* new_slot(knng_server* s) {
     free_slot(sl);
     return nullptr;
   }
+  sl->pcap = knng_host::assign_part_entries(int64_t(B), s->num_cu);
   return sl;
 }
 
 // queue one search launch on `stream`: device queries dx (float32) -> ids/scores
-// dcell: nparts * nq int32 scratch for the query's cell in each partition
+// dcell: nparts * nq int32 scratch for the query's cell in each partition;
+// pbest/pidx: pcap-entry partials that let the routing of a few queries use
+// many blocks (cells split into slices)
+// max_blocks caps the search grid (blocks then take several queries each).
 int launch_search(knng_server* s, hipStream_t stream, const float* dx, int8_t* dq, uint64_t* dsk,
-                  int32_t* dcell, int32_t nq, int beam, int k, int rerank, int32_t* did, float* dsc) {
+                  int32_t* dcell, float* pbest, int32_t* pidx, int64_t pcap, int32_t nq, int beam,
+                  int k, int rerank, int32_t* did, float* dsc, int max_blocks) {
   knng_ctx* c = s->ctx;
   if (knng_host::launch_quantize(stream, dx, nq, s->d, dq, dsk, s->num_cu))
     return sfail(s, KNNG_EHIP, "quantise launch failed");
   for (int g = 0; g < c->nparts; ++g)   // route the queries to cells
     if (knng_host::launch_assign(stream, dq, nq, s->d, c->cz + size_t(g) * c->ncells * s->d,
                                  c->cw + size_t(g) * c->ncells, c->cell_count[size_t(g)],
-                                 dcell + size_t(g) * nq,
-                                 s->num_cu))
+                                 dcell + size_t(g) * nq, s->num_cu, 0, 1, false, nullptr,
+                                 pbest, pidx, pcap))
       return sfail(s, KNNG_EHIP, "cell routing launch failed");
-  const int grid = std::max(1, std::min<int>(nq, s->num_cu * 8));
+  const int grid = std::max(1, std::min<int>(nq, max_blocks));
   // at most 4 trees: their leaves (<= 64 rows each) fill the candidate buffer
   const int et = std::min(std::min<int>(s->sp.entry_trees, c->p.num_trees), kCandCap / 64);
   hipLaunchKernelGGL(k_search, dim3(grid), dim3(256), 0, stream, s->q, s->x, s->d, s->gid, s->K,
@@ -496,36 +572,31 @@ int launch_search(knng_server* s, hipStream_t stream, const float* dx, int8_t* d
   return 0;
 }
 
-// insert: runs on the scheduler thread with every slot idle
-int do_insert(knng_server* s, Request* r) {
-  const int32_t m = r->nrows;
+// Append m rows (host float32) to the index as ids base .. base+m-1, base =
+// the current size: copy and quantise them, find their K nearest rows with
+// the search kernel (int8 scores, no rerank), write their lists, and merge
+// reverse edges into those neighbours' lists.  Runs on ins_stream with the
+// preallocated scratch (m <= ins_cap); only one insert runs at a time.
+int do_insert_rows(knng_server* s, const float* rows, int32_t m, int64_t* first_id) {
   const int K = s->K;
   const int64_t base = s->n.load();
+  if (m > s->ins_cap) return sfail(s, KNNG_ERANGE, "insert batch exceeds the insert scratch");
   if (base + m > s->cap) return sfail(s, KNNG_ERANGE, "insert exceeds insert_capacity");
   hipStream_t st = s->ins_stream;
-  SCHK(s, hipMemcpyAsync(s->x + base * s->d, r->rows.data(), size_t(m) * s->d * sizeof(float),
+  SCHK(s, hipMemcpyAsync(s->x + base * s->d, rows, size_t(m) * s->d * sizeof(float),
                          hipMemcpyHostToDevice, st));
-  int32_t* did = nullptr;
-  float* dsc = nullptr;
-  int32_t* dcell = nullptr;
-  SCHK(s, hipMalloc(&did, size_t(m) * K * sizeof(int32_t)));
-  SCHK(s, hipMalloc(&dsc, size_t(m) * K * sizeof(float)));
-  SCHK(s, hipMalloc(&dcell, size_t(m) * std::max(1, s->ctx->nparts) * sizeof(int32_t)));
   // neighbours of the new rows among the current ones (int8 scores, no rerank)
   const int beam = std::max(s->sp.beam, K);
   int rc = launch_search(s, st, s->x + base * s->d, s->q + base * s->d, s->sk + base * s->words,
-                         dcell, m, std::min(beam, kBeamMax), K, 0, did, dsc);
+                         s->ins_dcell, s->ins_pbest, s->ins_pidx,
+                         knng_host::assign_part_entries(s->ins_cap, s->num_cu), m,
+                         std::min(beam, kBeamMax), K, 0, s->ins_did, s->ins_dsc, s->num_cu * 8);
+  if (rc) return rc;
   std::vector<int32_t> ids(size_t(m) * K);
   std::vector<float> sc(size_t(m) * K);
-  if (!rc) {
-    SCHK(s, hipMemcpyAsync(ids.data(), did, ids.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    SCHK(s, hipMemcpyAsync(sc.data(), dsc, sc.size() * sizeof(float), hipMemcpyDeviceToHost, st));
-    SCHK(s, hipStreamSynchronize(st));
-  }
-  (void)hipFree(did);
-  (void)hipFree(dsc);
-  (void)hipFree(dcell);
-  if (rc) return rc;
+  SCHK(s, hipMemcpyAsync(ids.data(), s->ins_did, ids.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  SCHK(s, hipMemcpyAsync(sc.data(), s->ins_dsc, sc.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  SCHK(s, hipStreamSynchronize(st));
   // lists of the new rows; reverse proposals (target, new id, score); a new
   // row is forced into the lists of its kForced nearest neighbours
   constexpr int kForced = 16;
@@ -571,145 +642,259 @@ int do_insert(knng_server* s, Request* r) {
   }
   offs.push_back(int32_t(pid.size()));
   if (!targets.empty()) {
-    int32_t *dt = nullptr, *doff = nullptr, *dpi = nullptr, *dps = nullptr, *dpf = nullptr;
-    SCHK(s, hipMalloc(&dt, targets.size() * 4));
-    SCHK(s, hipMalloc(&doff, offs.size() * 4));
-    SCHK(s, hipMalloc(&dpi, pid.size() * 4));
-    SCHK(s, hipMalloc(&dps, psc.size() * 4));
-    SCHK(s, hipMalloc(&dpf, pfo.size() * 4));
-    SCHK(s, hipMemcpyAsync(dt, targets.data(), targets.size() * 4, hipMemcpyHostToDevice, st));
-    SCHK(s, hipMemcpyAsync(doff, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, st));
-    SCHK(s, hipMemcpyAsync(dpi, pid.data(), pid.size() * 4, hipMemcpyHostToDevice, st));
-    SCHK(s, hipMemcpyAsync(dps, psc.data(), psc.size() * 4, hipMemcpyHostToDevice, st));
-    SCHK(s, hipMemcpyAsync(dpf, pfo.data(), pfo.size() * 4, hipMemcpyHostToDevice, st));
+    SCHK(s, hipMemcpyAsync(s->ins_t, targets.data(), targets.size() * 4, hipMemcpyHostToDevice, st));
+    SCHK(s, hipMemcpyAsync(s->ins_off, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, st));
+    SCHK(s, hipMemcpyAsync(s->ins_pi, pid.data(), pid.size() * 4, hipMemcpyHostToDevice, st));
+    SCHK(s, hipMemcpyAsync(s->ins_ps, psc.data(), psc.size() * 4, hipMemcpyHostToDevice, st));
+    SCHK(s, hipMemcpyAsync(s->ins_pf, pfo.data(), pfo.size() * 4, hipMemcpyHostToDevice, st));
     const int grid = std::max(1, std::min<int>(int(targets.size()), s->num_cu * 16));
-    hipLaunchKernelGGL(k_merge_reverse, dim3(grid), dim3(64), 0, st, s->gid, s->gsc, K, dt, doff, dpi,
-                       dps, dpf, int(targets.size()));
+    hipLaunchKernelGGL(k_merge_reverse, dim3(grid), dim3(64), 0, st, s->gid, s->gsc, K, s->ins_t,
+                       s->ins_off, s->ins_pi, s->ins_ps, s->ins_pf, int(targets.size()));
     SCHK(s, hipGetLastError());
-    SCHK(s, hipStreamSynchronize(st));
-    for (void* ptr : {static_cast<void*>(dt), static_cast<void*>(doff), static_cast<void*>(dpi),
-                      static_cast<void*>(dps), static_cast<void*>(dpf)})
-      (void)hipFree(ptr);
   }
-  SCHK(s, hipStreamSynchronize(st));
+  SCHK(s, hipStreamSynchronize(st));   // host vectors above are read by the copies
   s->n.store(base + m);
-  r->ids.resize(m);
-  for (int32_t j = 0; j < m; ++j) r->ids[j] = int32_t(base + j);
+  *first_id = base;
   return 0;
 }
 
-// A free slot, or a new one while fewer than max_slots exist and HBM beyond
-// reserve_bytes is free.  The first slot ignores the reserve (some progress is
-// always possible); *fatal is set when even that cannot be allocated.
-Slot* acquire_slot(knng_server* s, bool* fatal) {
+// An insert request of any size, in pieces of at most ins_cap rows.
+int insert_rows(knng_server* s, const float* rows, int32_t m, int64_t* first_id) {
+  for (int32_t r = 0; r < m; r += s->ins_cap) {
+    int64_t f = 0;
+    const int rc = do_insert_rows(s, rows + size_t(r) * s->d, std::min(s->ins_cap, m - r), &f);
+    if (rc) return rc;
+    if (r == 0) *first_id = f;
+  }
+  return 0;
+}
+
+// Insert thread (concurrent mode): applies pending inserts in submission
+// order, several requests per pass while their rows fit the scratch.
+void inserter(knng_server* s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  std::vector<float> rows;
+  while (true) {
+    s->cv.wait(lk, [&] { return s->stop || !s->ins_queue.empty(); });
+    if (s->ins_queue.empty()) break;   // stop requested and nothing pending
+    std::vector<Request*> batch;
+    int32_t m = 0;
+    while (!s->ins_queue.empty()) {
+      Request* r = s->ins_queue.front();
+      if (!batch.empty() && m + r->nrows > s->ins_cap) break;
+      batch.push_back(r);
+      m += r->nrows;
+      r->state = 1;
+      s->ins_queue.pop_front();
+    }
+    lk.unlock();
+    int rc = 0;
+    int64_t first = 0;
+    if (batch.size() == 1) {
+      rc = insert_rows(s, batch[0]->rows.data(), m, &first);
+    } else {
+      rows.clear();
+      for (Request* r : batch) rows.insert(rows.end(), r->rows.begin(), r->rows.end());
+      rc = insert_rows(s, rows.data(), m, &first);
+    }
+    lk.lock();
+    for (Request* r : batch) {
+      r->error = rc;
+      r->first_id = first;
+      first += r->nrows;
+      r->state = 2;
+      r->t_done = clk::now();
+    }
+    s->inserts_done += int64_t(batch.size());
+    s->cv.notify_all();
+  }
+}
+
+int count_slots(const knng_server* s, bool express) {
+  int k = 0;
+  for (const Slot* sl : s->slots) k += sl->express == express ? 1 : 0;
+  return k;
+}
+
+// A free bulk slot, or a new one while fewer than max_slots exist and HBM
+// beyond reserve_bytes is free.  The first bulk slot ignores the reserve (some
+// progress is always possible); *fatal is set when even that cannot be
+// allocated.  Express slots all exist from the start.
+Slot* acquire_slot(knng_server* s, bool express, bool* fatal) {
   *fatal = false;
   for (Slot* sl : s->slots)
-    if (!sl->busy) return sl;
-  if (int(s->slots.size()) >= s->sp.max_slots) return nullptr;
-  if (!s->slots.empty()) {
+    if (!sl->busy && sl->express == express) return sl;
+  if (express) return nullptr;
+  const int nb = count_slots(s, false);
+  if (nb >= s->sp.max_slots) return nullptr;
+  if (nb > 0) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return nullptr;
     if (free_b < slot_device_bytes(s) + size_t(s->sp.reserve_bytes)) return nullptr;   // admission
   }
-  Slot* sl = new_slot(s);
+  Slot* sl = new_slot(s, false);
   if (sl)
     s->slots.push_back(sl);
-  else if (s->slots.empty())
+  else if (nb == 0)
     *fatal = true;
   return sl;
 }
 
-void dispatch(knng_server* s, Request* r, Slot* sl) {
-  const int32_t m = std::min(s->sp.max_batch, r->nrows - r->next_row);
-  memcpy(sl->hx, r->rows.data() + size_t(r->next_row) * s->d, size_t(m) * s->d * sizeof(float));
+// Launch `parts` (request rows, see Slot::Part) on slot sl.  Called with
+// the lock held and the requests' bookkeeping already updated; the host
+// staging copy and the launches run with the lock released (the slot is
+// marked busy, so no other thread touches it), so a lane's thread never
+// holds up the other lane, the insert thread or the callers.
+void launch_parts(knng_server* s, Slot* sl, std::unique_lock<std::mutex>& lk) {
+  sl->busy = true;
+  int32_t m = 0;
+  std::vector<const float*> src;
+  for (const Slot::Part& pt : sl->parts) {
+    Request* r = s->reqs[pt.ticket];
+    src.push_back(r->rows.data() + size_t(pt.row0) * s->d);
+    m = pt.off + pt.nrows;
+  }
+  lk.unlock();
+  for (size_t j = 0; j < sl->parts.size(); ++j)
+    memcpy(sl->hx + size_t(sl->parts[j].off) * s->d, src[j],
+           size_t(sl->parts[j].nrows) * s->d * sizeof(float));
   bool ok = hipMemcpyAsync(sl->dx, sl->hx, size_t(m) * s->d * sizeof(float), hipMemcpyHostToDevice,
                            sl->stream) == hipSuccess;
-  ok = ok && !launch_search(s, sl->stream, sl->dx, sl->dq, sl->dsk, sl->dcell, m, s->sp.beam,
-                            s->sp.k, s->sp.rerank, sl->did, sl->dsc);
+  // a bulk launch leaves part of the chip's block slots free, so express
+  // launches find room at once instead of waiting for bulk blocks to drain
+  const int blocks = sl->express ? s->num_cu * 8 : s->bulk_blocks;
+  ok = ok && !launch_search(s, sl->stream, sl->dx, sl->dq, sl->dsk, sl->dcell, sl->pbest, sl->pidx,
+                            sl->pcap, m, s->sp.beam, s->sp.k, s->sp.rerank, sl->did, sl->dsc, blocks);
   ok = ok && hipMemcpyAsync(sl->hid, sl->did, size_t(m) * s->sp.k * sizeof(int32_t),
                             hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
   ok = ok && hipMemcpyAsync(sl->hsc, sl->dsc, size_t(m) * s->sp.k * sizeof(float),
                             hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
   ok = ok && hipEventRecord(sl->done, sl->stream) == hipSuccess;
-  if (!ok) r->error = KNNG_EHIP;
-  sl->busy = true;
-  sl->ticket = r->ticket;
-  sl->row0 = r->next_row;
-  sl->nrows = m;
+  lk.lock();
+  if (!ok)
+    for (const Slot::Part& pt : sl->parts) s->reqs[pt.ticket]->error = KNNG_EHIP;
+}
+
+// Add the next max_batch rows (at most) of r to slot sl's parts.
+void take_rows(knng_server* s, Request* r, Slot* sl, int32_t limit) {
+  const int32_t off = sl->parts.empty() ? 0 : sl->parts.back().off + sl->parts.back().nrows;
+  const int32_t m = std::min(limit - off, r->nrows - r->next_row);
+  if (r->rows_seen < 0) r->rows_seen = s->n.load();
+  sl->parts.push_back(Slot::Part{r->ticket, r->next_row, m, off});
   r->next_row += m;
   r->running += 1;
   r->state = 1;
 }
 
-void scheduler(knng_server* s) {
+// One thread per lane.  The express thread serves the express queue on the
+// express slots; the bulk thread serves the bulk queue (and, with
+// strict_order, the inserts in it) on the bulk slots.
+void scheduler(knng_server* s, bool express_lane) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
     bool progress = false, busy = false;
-    for (Slot* sl : s->slots) {
-      if (!sl->busy) continue;
+    for (size_t j = 0; j < s->slots.size(); ++j) {
+      Slot* sl = s->slots[j];
+      if (sl->express != express_lane || !sl->busy) continue;
       const hipError_t e = hipEventQuery(sl->done);
       if (e == hipErrorNotReady) {
         busy = true;
         continue;
       }
-      Request* r = s->reqs[sl->ticket];
-      if (e != hipSuccess) r->error = KNNG_EHIP;
       const size_t kk = size_t(s->sp.k);
-      memcpy(r->ids.data() + size_t(sl->row0) * kk, sl->hid, size_t(sl->nrows) * kk * 4);
-      memcpy(r->scores.data() + size_t(sl->row0) * kk, sl->hsc, size_t(sl->nrows) * kk * 4);
-      sl->busy = false;
-      r->running -= 1;
-      if (r->running == 0 && r->next_row == r->nrows) {
-        r->state = 2;
-        r->t_done = clk::now();
-        s->cv.notify_all();
-      }
-      progress = true;
-    }
-    while (!s->queue.empty()) {
-      Request* r = s->queue.front();
-      if (r->kind == 1) {
-        bool any = false;
-        for (Slot* sl : s->slots) any = any || sl->busy;
-        if (any) break;   // drain the queries admitted before the insert
-        r->state = 1;
-        lk.unlock();
-        const int rc = do_insert(s, r);
-        lk.lock();
-        r->error = rc;
-        r->state = 2;
-        r->t_done = clk::now();
-        s->queue.pop_front();
-        s->cv.notify_all();
-        progress = true;
-        continue;
-      }
-      bool fatal = false;
-      while (r->next_row < r->nrows) {
-        Slot* sl = acquire_slot(s, &fatal);
-        if (!sl) break;
-        dispatch(s, r, sl);
-        progress = true;
-        busy = true;
-      }
-      if (fatal) {   // not even one slot fits in HBM: fail the request
-        s->err = "cannot allocate a search slot";
-        r->error = KNNG_ENOMEM;
-        r->next_row = r->nrows;
-        if (r->running == 0) {
+      for (const Slot::Part& pt : sl->parts) {
+        Request* r = s->reqs[pt.ticket];
+        if (e != hipSuccess) r->error = KNNG_EHIP;
+        memcpy(r->ids.data() + size_t(pt.row0) * kk, sl->hid + size_t(pt.off) * kk,
+               size_t(pt.nrows) * kk * 4);
+        memcpy(r->scores.data() + size_t(pt.row0) * kk, sl->hsc + size_t(pt.off) * kk,
+               size_t(pt.nrows) * kk * 4);
+        r->running -= 1;
+        if (r->running == 0 && r->next_row == r->nrows) {
           r->state = 2;
           r->t_done = clk::now();
           s->cv.notify_all();
         }
-        s->queue.pop_front();
-        progress = true;
-        continue;
       }
-      if (r->next_row < r->nrows) break;
-      s->queue.pop_front();
+      sl->parts.clear();
+      sl->busy = false;
+      progress = true;
     }
-    if (s->stop && s->queue.empty() && !busy) break;
+    if (express_lane) {
+      // continuous batching: every waiting small request that fits goes into
+      // one launch on a free express slot
+      while (!s->express.empty()) {
+        bool fatal = false;
+        Slot* sl = acquire_slot(s, true, &fatal);
+        if (!sl) break;
+        int32_t rows = 0;
+        while (!s->express.empty()) {
+          Request* r = s->express.front();
+          if (s->sp.strict_order && r->epoch > s->inserts_done) break;   // an earlier insert is pending
+          if (rows + r->nrows > s->sp.max_batch) break;
+          s->express.pop_front();
+          take_rows(s, r, sl, s->sp.max_batch);
+          rows += r->nrows;
+        }
+        if (sl->parts.empty()) break;
+        launch_parts(s, sl, lk);
+        progress = true;
+        busy = true;
+      }
+    } else {
+      while (!s->queue.empty()) {
+        Request* r = s->queue.front();
+        if (r->kind == 1) {   // strict_order: the insert runs here, alone
+          bool any = false;
+          for (Slot* sl : s->slots) any = any || sl->busy;
+          // drain the queries submitted before the insert, in either lane
+          if (!s->express.empty() && s->express.front()->ticket < r->ticket) any = true;
+          if (any) break;
+          r->state = 1;
+          lk.unlock();
+          int64_t first = 0;
+          const int rc = insert_rows(s, r->rows.data(), r->nrows, &first);
+          lk.lock();
+          r->error = rc;
+          r->first_id = first;
+          r->state = 2;
+          r->t_done = clk::now();
+          s->inserts_done += 1;
+          s->queue.pop_front();
+          s->cv.notify_all();
+          progress = true;
+          continue;
+        }
+        bool fatal = false;
+        while (r->next_row < r->nrows) {
+          Slot* sl = acquire_slot(s, false, &fatal);
+          if (!sl) break;
+          take_rows(s, r, sl, s->sp.max_batch);
+          launch_parts(s, sl, lk);
+          progress = true;
+          busy = true;
+        }
+        if (fatal) {   // not even one slot fits in HBM: fail the request
+          s->err = "cannot allocate a search slot";
+          r->error = KNNG_ENOMEM;
+          r->next_row = r->nrows;
+          if (r->running == 0) {
+            r->state = 2;
+            r->t_done = clk::now();
+            s->cv.notify_all();
+          }
+          s->queue.pop_front();
+          progress = true;
+          continue;
+        }
+        if (r->next_row < r->nrows) break;
+        s->queue.pop_front();
+      }
+    }
+    const bool idle = express_lane ? s->express.empty() : s->queue.empty();
+    if (s->stop && idle && !busy) break;
     if (!progress) s->cv.wait_for(lk, std::chrono::microseconds(busy ? 20 : 500));
   }
 }
@@ -730,6 +915,10 @@ int knng_server_default_params(knng_server_params* sp) {
   sp->pad0 = 0;
   sp->insert_capacity = 1 << 20;
   sp->reserve_bytes = int64_t(2) << 30;
+  sp->express_rows = 64;
+  sp->express_slots = 2;
+  sp->express_cus = 0;
+  sp->strict_order = 0;
   return 0;
 }
 
@@ -740,7 +929,8 @@ int knng_server_create(knng_server** out, knng_ctx* c, const knng_server_params*
   *out = nullptr;
   if (!c->have_graph || !c->have_forest) return KNNG_ESTATE;
   if (sp->beam < 1 || sp->beam > kBeamMax || sp->k < 1 || sp->k > sp->beam || sp->entry_trees < 1 ||
-      sp->max_batch < 1 || sp->max_slots < 1 || sp->max_steps < 0 || sp->insert_capacity < 0)
+      sp->max_batch < 1 || sp->max_slots < 1 || sp->max_steps < 0 || sp->insert_capacity < 0 ||
+      sp->express_rows < 0 || sp->express_slots < 0 || sp->express_cus < 0)
     return KNNG_EINVAL;
   const float* xsrc = c->x_ext ? c->x_ext : c->x;
   if (!xsrc) return KNNG_ESTATE;   // the index keeps float32 rows (rerank, inserts)
@@ -766,6 +956,13 @@ int knng_server_create(knng_server** out, knng_ctx* c, const knng_server_params*
       hipMalloc(&s->gid, cap * s->K * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&s->gsc, cap * s->K * sizeof(int32_t)) != hipSuccess)
     return bail(KNNG_ENOMEM);
+  s->sp.express_rows = std::min(s->sp.express_rows, s->sp.max_batch);
+  // bulk launches together hold at most ~6 search blocks per CU (of the ~7
+  // that fit) while an express lane exists
+  s->bulk_blocks = s->sp.express_rows > 0 ? std::max(64, s->num_cu * 6 / std::max(1, s->sp.max_slots))
+                                          : s->num_cu * 8;
+  if (s->sp.express_slots == 0) s->sp.express_rows = 0;
+  s->cu_masked = s->sp.express_rows > 0 && s->sp.express_cus > 0 && s->sp.express_cus < s->num_cu;
   if (hipStreamCreateWithFlags(&s->ins_stream, hipStreamNonBlocking) != hipSuccess) return bail(KNNG_EHIP);
   const size_t n = size_t(c->n);
   if (hipStreamSynchronize(c->stream) != hipSuccess ||
@@ -775,7 +972,33 @@ int knng_server_create(knng_server** out, knng_ctx* c, const knng_server_params*
       hipMemcpy(s->gid, c->gid[c->cur], n * s->K * 4, hipMemcpyDeviceToDevice) != hipSuccess ||
       hipMemcpy(s->gsc, c->gsc[c->cur], n * s->K * 4, hipMemcpyDeviceToDevice) != hipSuccess)
     return bail(KNNG_EHIP);
-  s->th = std::thread(scheduler, s);
+  // insert scratch (inserts run in pieces of at most ins_cap rows)
+  s->ins_cap = std::max(s->sp.max_batch, 4096);
+  {
+    const size_t m = size_t(s->ins_cap), mk = m * size_t(s->K);
+    const size_t pe = size_t(knng_host::assign_part_entries(s->ins_cap, s->num_cu));
+    if (hipMalloc(&s->ins_did, mk * 4) != hipSuccess || hipMalloc(&s->ins_dsc, mk * 4) != hipSuccess ||
+        hipMalloc(&s->ins_dcell, m * size_t(std::max(1, c->nparts)) * 4) != hipSuccess ||
+        hipMalloc(&s->ins_pbest, pe * 4) != hipSuccess || hipMalloc(&s->ins_pidx, pe * 4) != hipSuccess ||
+        hipMalloc(&s->ins_t, mk * 4) != hipSuccess || hipMalloc(&s->ins_off, (mk + 1) * 4) != hipSuccess ||
+        hipMalloc(&s->ins_pi, mk * 4) != hipSuccess || hipMalloc(&s->ins_ps, mk * 4) != hipSuccess ||
+        hipMalloc(&s->ins_pf, mk * 4) != hipSuccess)
+      return bail(KNNG_ENOMEM);
+  }
+  // express slots up front; one bulk slot (more as requests and HBM allow)
+  for (int j = 0; j < (s->sp.express_rows > 0 ? s->sp.express_slots : 0); ++j) {
+    Slot* sl = new_slot(s, true);
+    if (!sl) return bail(KNNG_ENOMEM);
+    s->slots.push_back(sl);
+  }
+  {
+    Slot* sl = new_slot(s, false);
+    if (!sl) return bail(KNNG_ENOMEM);
+    s->slots.push_back(sl);
+  }
+  s->th = std::thread(scheduler, s, false);
+  if (s->sp.express_rows > 0) s->th_express = std::thread(scheduler, s, true);
+  if (!s->sp.strict_order) s->ins_th = std::thread(inserter, s);
   *out = s;
   return 0;
 }
@@ -789,12 +1012,20 @@ int knng_server_destroy(knng_server* s) {
     }
     s->cv.notify_all();
     s->th.join();
+    if (s->th_express.joinable()) s->th_express.join();
+    if (s->ins_th.joinable()) s->ins_th.join();
   }
   (void)hipSetDevice(s->device);
   for (Slot* sl : s->slots) free_slot(sl);
   for (auto& kv : s->reqs) delete kv.second;
   for (void* p : {static_cast<void*>(s->q), static_cast<void*>(s->sk), static_cast<void*>(s->x),
                   static_cast<void*>(s->gid), static_cast<void*>(s->gsc)})
+    if (p) (void)hipFree(p);
+  for (void* p : {static_cast<void*>(s->ins_did), static_cast<void*>(s->ins_dsc),
+                  static_cast<void*>(s->ins_dcell), static_cast<void*>(s->ins_pbest),
+                  static_cast<void*>(s->ins_pidx), static_cast<void*>(s->ins_t),
+                  static_cast<void*>(s->ins_off), static_cast<void*>(s->ins_pi),
+                  static_cast<void*>(s->ins_ps), static_cast<void*>(s->ins_pf)})
     if (p) (void)hipFree(p);
   if (s->ins_stream) (void)hipStreamDestroy(s->ins_stream);
   delete s;
@@ -816,8 +1047,15 @@ int knng_server_submit(knng_server* s, int32_t kind, const float* rows, int32_t 
   {
     std::lock_guard<std::mutex> lk(s->mu);
     r->ticket = s->next_ticket++;
+    r->epoch = s->inserts_submitted;
     s->reqs[r->ticket] = r;
-    s->queue.push_back(r);
+    if (kind == 1) s->inserts_submitted += 1;
+    if (kind == 0 && nrows <= s->sp.express_rows)
+      s->express.push_back(r);
+    else if (kind == 1 && !s->sp.strict_order)
+      s->ins_queue.push_back(r);
+    else
+      s->queue.push_back(r);
   }
   s->cv.notify_all();
   *ticket = r->ticket;
@@ -834,6 +1072,11 @@ int knng_server_poll(knng_server* s, int64_t ticket, int32_t* state) {
 }
 
 int knng_server_wait(knng_server* s, int64_t ticket, int32_t* ids, float* scores, double* latency_ms) {
+  return knng_server_wait_info(s, ticket, ids, scores, latency_ms, nullptr);
+}
+
+int knng_server_wait_info(knng_server* s, int64_t ticket, int32_t* ids, float* scores,
+                          double* latency_ms, int64_t* index_rows) {
   if (!s) return KNNG_EINVAL;
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->reqs.find(ticket);
@@ -844,15 +1087,30 @@ int knng_server_wait(knng_server* s, int64_t ticket, int32_t* ids, float* scores
   lk.unlock();
   const int rc = r->error;
   if (!rc) {
-    if (ids) memcpy(ids, r->ids.data(), r->ids.size() * sizeof(int32_t));
+    if (ids && r->kind == 0) memcpy(ids, r->ids.data(), r->ids.size() * sizeof(int32_t));
+    if (ids && r->kind == 1)
+      for (int32_t j = 0; j < r->nrows; ++j) ids[j] = int32_t(r->first_id + j);
     if (scores && r->kind == 0) memcpy(scores, r->scores.data(), r->scores.size() * sizeof(float));
   } else if (s->err.empty()) {
     s->err = "request failed";
   }
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  if (index_rows) *index_rows = r->kind == 0 ? r->rows_seen : r->first_id + r->nrows;
   delete r;
   return rc;
 }
+
+int knng_server_info(const knng_server* s, int32_t* cu_masked, int32_t* bulk_slots,
+                     int32_t* express_slots) {
+  if (!s) return KNNG_EINVAL;
+  std::lock_guard<std::mutex> lk(const_cast<knng_server*>(s)->mu);
+  if (cu_masked) *cu_masked = s->cu_masked ? 1 : 0;
+  if (bulk_slots) *bulk_slots = count_slots(s, false);
+  if (express_slots) *express_slots = count_slots(s, true);
+  return 0;
+}
+
+const char* knng_server_mask_error(const knng_server* s) { return s ? s->mask_err.c_str() : ""; }
 
 int64_t knng_server_size(const knng_server* s) {
   if (!s) return -1;

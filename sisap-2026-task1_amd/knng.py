@@ -28,7 +28,9 @@ KNNG_SYMBOLS = [
     "knng_quantize_device", "knng_score_pairs_device",
     "knng_server_default_params", "knng_server_create", "knng_server_destroy",
     "knng_server_last_error", "knng_server_submit", "knng_server_poll", "knng_server_wait",
-    "knng_server_size", "knng_server_search", "knng_build_batch",
+    "knng_server_size", "knng_server_search", "knng_server_wait_info", "knng_server_info",
+    "knng_server_mask_error",
+    "knng_build_batch",
 ]
 H5_SYMBOLS = [
     "knng_h5_open", "knng_h5_shape_of", "knng_h5_read_rows", "knng_h5_close",
@@ -71,6 +73,7 @@ class Stats(C.Structure):
         ("join_bytes", C.c_double * 32),
         ("join_candidates", C.c_int64 * 32), ("join_rescored", C.c_int64 * 32),
         ("join_inserted", C.c_int64 * 32), ("insert_rates", C.c_float * 32),
+        ("join_variant", C.c_int32), ("ms_tune", C.c_float),
     ]
 
     def as_dict(self) -> dict:
@@ -88,6 +91,7 @@ class Stats(C.Structure):
             "join_rescored": list(self.join_rescored[:it]),
             "join_inserted": list(self.join_inserted[:it]),
             "insert_rates": [float(r) for r in self.insert_rates[:it]],
+            "join_variant": self.join_variant, "ms_tune": self.ms_tune,
         }
 
 
@@ -377,6 +381,8 @@ class ServerParams(C.Structure):
         ("beam", C.c_int32), ("k", C.c_int32), ("entry_trees", C.c_int32), ("rerank", C.c_int32),
         ("max_steps", C.c_int32), ("max_batch", C.c_int32), ("max_slots", C.c_int32),
         ("pad0", C.c_int32), ("insert_capacity", C.c_int64), ("reserve_bytes", C.c_int64),
+        ("express_rows", C.c_int32), ("express_slots", C.c_int32), ("express_cus", C.c_int32),
+        ("strict_order", C.c_int32),
     ]
 
 
@@ -392,6 +398,9 @@ def _server_lib() -> C.CDLL:
             "knng_server_submit": ([V, i32, V, i32, P(i64)], i32),
             "knng_server_poll": ([V, i64, P(i32)], i32),
             "knng_server_wait": ([V, i64, V, V, P(C.c_double)], i32),
+            "knng_server_wait_info": ([V, i64, V, V, P(C.c_double), P(i64)], i32),
+            "knng_server_info": ([V, P(i32), P(i32), P(i32)], i32),
+            "knng_server_mask_error": ([V], C.c_char_p),
             "knng_server_size": ([V], i64),
             "knng_server_search": ([V, V, i32, V, V], i32),
         }
@@ -466,6 +475,21 @@ class Server:
         self._ck(self.L.knng_server_wait(self.srv, ticket, _ptr(ids), _ptr(sc), C.byref(lat)), "wait")
         return ids, sc, lat.value
 
+    def wait_info(self, ticket: int, nrows: int, insert: bool = False):
+        """-> (ids, scores, latency_ms, index_rows): index_rows is the index size a
+        query was dispatched against (an insert: the size just after it)."""
+        lat, rows = C.c_double(), C.c_int64()
+        if insert:
+            ids = np.empty(nrows, np.int32)
+            self._ck(self.L.knng_server_wait_info(self.srv, ticket, _ptr(ids), None, C.byref(lat),
+                                                  C.byref(rows)), "wait")
+            return ids, None, lat.value, rows.value
+        ids = np.empty((nrows, self.p.k), np.int32)
+        sc = np.empty((nrows, self.p.k), np.float32)
+        self._ck(self.L.knng_server_wait_info(self.srv, ticket, _ptr(ids), _ptr(sc), C.byref(lat),
+                                              C.byref(rows)), "wait")
+        return ids, sc, lat.value, rows.value
+
     def search(self, queries: np.ndarray):
         queries = np.ascontiguousarray(queries, np.float32)
         nq = queries.shape[0]
@@ -476,6 +500,12 @@ class Server:
 
     def size(self) -> int:
         return int(self.L.knng_server_size(self.srv))
+
+    def info(self) -> dict:
+        m, b, e = C.c_int32(), C.c_int32(), C.c_int32()
+        self._ck(self.L.knng_server_info(self.srv, C.byref(m), C.byref(b), C.byref(e)), "info")
+        return {"cu_masked": bool(m.value), "bulk_slots": b.value, "express_slots": e.value,
+                "mask_error": self.L.knng_server_mask_error(self.srv).decode(errors="replace")}
 
 
 # -- batched builds ------------------------------------------------------------

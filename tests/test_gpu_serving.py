@@ -114,3 +114,59 @@ def test_admission_with_tight_memory_reserve(torch_cuda):
         outs = [s.wait(t, 100)[0] for t in ts]
         ref, _ = s.search(queries)
     assert np.array_equal(np.vstack(outs), ref)
+
+
+def test_express_lane_order_and_equality(torch_cuda):
+    """Small requests take the express lane (reserved CUs) and return the same
+    answers as the bulk lane.  strict_order = 1: a query submitted after an
+    insert sees the inserted rows, one submitted before it does not, without
+    waiting in between."""
+    say("express lane, strict order")
+    x = synth.gaussian_mixture(49, 8000, 256, 16)
+    new = synth.gaussian_mixture(50, 40, 256, 16)
+    b = _built(x)
+    with knng.Server(b, beam=32, k=10, rerank=1, max_batch=256, max_slots=2, insert_capacity=64,
+                     express_rows=64, express_slots=2, express_cus=16, strict_order=1) as s:
+        big, big_sc = s.search(np.vstack([new] * 4))          # 160 rows: bulk lane
+        t_before = s.submit(new)                               # 40 rows: express lane
+        t_ins = s.submit(new, insert=True)
+        t_after = s.submit(new)
+        ids_b, sc_b, _, rows_b = s.wait_info(t_before, 40)
+        ins_ids, _, _, rows_i = s.wait_info(t_ins, 40, insert=True)
+        ids_a, _, _, rows_a = s.wait_info(t_after, 40)
+    assert np.array_equal(ids_b, big[:40]) and np.array_equal(sc_b, big_sc[:40])
+    assert ids_b.max() < 8000 and rows_b == 8000
+    assert ins_ids.tolist() == list(range(8000, 8040)) and rows_i == 8040
+    assert rows_a == 8040
+    assert (ids_a[:, 0] == np.arange(8000, 8040)).mean() > 0.95   # each finds itself first
+
+
+def test_concurrent_inserts_beside_queries(torch_cuda):
+    """Default mode: inserts run on their own thread and stream while queries
+    keep flowing; inserts are applied in submission order, a query reports
+    the index size it was dispatched against, and every inserted row becomes
+    findable."""
+    say("concurrent inserts")
+    x = synth.gaussian_mixture(51, 9000, 256, 16)
+    new = synth.gaussian_mixture(52, 600, 256, 16)
+    qs = synth.gaussian_mixture(53, 500, 256, 16)
+    b = _built(x)
+    with knng.Server(b, beam=32, k=10, rerank=1, max_batch=128, max_slots=3,
+                     insert_capacity=600) as s:
+        tickets = []
+        for j in range(6):   # interleave inserts of 100 rows with query batches
+            tickets.append(("i", s.submit(new[100 * j:100 * (j + 1)], insert=True), 100))
+            tickets.append(("q", s.submit(qs[80 * j:80 * j + 80]), 80))
+        firsts = []
+        for kind, t, m in tickets:
+            if kind == "i":
+                ids, _, _, rows = s.wait_info(t, m, insert=True)
+                assert rows == ids[-1] + 1
+                firsts.append(int(ids[0]))
+            else:
+                _, _, _, rows = s.wait_info(t, m)
+                assert 9000 <= rows <= 9600
+        assert firsts == list(range(9000, 9600, 100))   # applied in submission order
+        assert s.size() == 9600
+        got, _ = s.search(new)
+        assert (got[:, 0] == np.arange(9000, 9600)).mean() > 0.95
