@@ -393,7 +393,8 @@ def serve_main(args):
     # one hardware queue per stream (bulk slots, express slots, inserts, the
     # build context): with HIP's default of 4, express launches would queue
     # behind bulk launches that share their hardware queue
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     import numpy as np
     import torch
 

@@ -352,9 +352,11 @@ __global__ void k_hier_append(const Node* __restrict__ nodes, int nnodes, Node* 
 // (rows and centroids are the same set); score[j] (if given) = the best value.
 // One block = 128 rows, wave v owns rows 32v..32v+31 and keeps their A
 // fragments for the whole pass in registers.  32-cell centroid tiles are
-// double-buffered in LDS: the next tile streams in with direct global->LDS
-// loads (16 bytes per lane, 64 lanes = 1 KB contiguous) while the MFMAs run
-// on the current one.  Row r of a tile holds its 16-byte chunk c at slot
+// double-buffered in LDS, filled by direct global->LDS loads (16 bytes per
+// lane, 64 lanes = 1 KB contiguous) while the MFMAs run on the other one.
+// At d = 1024 the k-step count is a template constant, so the 32 MFMAs and
+// their LDS fragment reads form one straight-line block the scheduler can
+// interleave (a runtime bound puts each MFMA behind its own branch).  Row r of a tile holds its 16-byte chunk c at slot
 // (c - r) mod (d/16), so the 32 rows a B fragment read touches fall on
 // different banks.  Even and odd k-steps accumulate into two chains
 // (exact int32 sums, added at the end) to keep two MFMAs in flight.
@@ -383,7 +385,35 @@ __device__ __forceinline__ void assign_issue_tile(const int8_t* __restrict__ Z, 
   }
 }
 
-template <bool EXCL>
+// s_waitcnt vmcnt(n), other counters unconstrained (gfx9 encoding:
+// vmcnt[3:0] | expcnt 7 << 4 | lgkmcnt 15 << 8 | vmcnt[5:4] << 14)
+#define KNNG_VMCNT(n) (((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
+template <int N>
+__device__ __forceinline__ void wait_vmcnt_le(int n) {
+  if constexpr (N == 0) {
+    __builtin_amdgcn_s_waitcnt(KNNG_VMCNT(0));
+  } else {
+    if (n >= N)
+      __builtin_amdgcn_s_waitcnt(KNNG_VMCNT(N));
+    else
+      wait_vmcnt_le<N - 1>(n);
+  }
+}
+
+// this wave's direct loads for centroid tile u (16 B chunks, plus the tile's
+// 32 weights on wave 0): what assign_issue_tile issues
+__device__ __forceinline__ int assign_tile_loads(int C, int cpr, int u, int wv) {
+  const int c0 = u * kAssignTile;
+  const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
+  const int jmax = (nc * cpr + 63) / 64;
+  return (jmax > wv ? (jmax - wv + 3) / 4 : 0) + (wv == 0 ? 1 : 0);
+}
+
+constexpr int kAssignStages = 2;   // centroid tiles in LDS per block (2 x 33 KB)
+constexpr int kAssignStageW = kAssignTile * kMaxDim;             // weights' offset in a stage
+constexpr int kAssignStageBytes = kAssignStageW + 1024;          // rows + weights, 1 KB aligned
+
+template <bool EXCL, int KS>
 __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, int64_t nrows,
                                                 int64_t row_off, int64_t row_stride, int d,
                                                 const int8_t* __restrict__ Z,
@@ -392,15 +422,34 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
                                                 float* __restrict__ score,
                                                 float* __restrict__ pbest,
                                                 int32_t* __restrict__ pidx) {
-  __shared__ __attribute__((aligned(1024))) int8_t ztile[2][kAssignTile * kMaxDim];
+  // Centroid tiles stream through an LDS ring of kAssignStages buffers, each
+  // with the tile's 32 weights, so the epilogue issues no global load (one
+  // would make the next wait cover the tile being prefetched as well): the
+  // MFMAs of tile t overlap the loads of the following tiles, and each wave
+  // waits (exact vmcnt) only for its own share of tile t.  A deeper ring (4)
+  // measured slower than 2 at 6.35M x 1024.
+  // stage = the tile's rows, then its 32 weights at kAssignStageW (one array:
+  // the optimizer sees LDS that only direct loads write as unwritten and
+  // folds reads of a separate weight array away)
+  __shared__ __attribute__((aligned(1024))) int8_t ztile[kAssignStages][kAssignStageBytes];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int ksteps = d >> 5, cpr = d >> 4;
+  // KS > 0: d = 32 * KS known at compile time (straight-line MFMA chain the
+  // scheduler can feed from LDS ahead of use); KS = 0: any d
+  const int ksteps = KS > 0 ? KS : d >> 5, cpr = KS > 0 ? 2 * KS : d >> 4;
   const int ntiles_all = (C + kAssignTile - 1) / kAssignTile;
   const int per_slice = (ntiles_all + int(gridDim.y) - 1) / int(gridDim.y);
   const int t_lo = int(blockIdx.y) * per_slice;
   const int ntiles = min(ntiles_all, t_lo + per_slice);   // this block scans tiles [t_lo, ntiles)
   const int cl = lane & 31, hi = lane >> 5;
   const int rrot = cl % cpr;   // rotation of the B row this lane reads
+  auto issue = [&](int u) {
+    const int c0 = u * kAssignTile;
+    const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
+    const int buf = u % kAssignStages;
+    assign_issue_tile(Z, d, cpr, c0, nc, ztile[buf], wv, lane);
+    if (wv == 0 && lane < nc)
+      __builtin_amdgcn_global_load_lds(w + c0 + lane, ztile[buf] + kAssignStageW, 4, 0, 0);
+  };
   for (int64_t r0 = int64_t(blockIdx.x) * kAssignRows; r0 < nrows;
        r0 += int64_t(gridDim.x) * kAssignRows) {
     // A fragments: row r0 + 32*wv + (lane&31), bytes 32*ks + 16*(lane>>5)
@@ -419,23 +468,21 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
       best[j] = -INFINITY;
       bidx[j] = 0x7FFFFFFF;
     }
-    __syncthreads();   // both buffers free (previous row tile done)
-    if (t_lo < ntiles) {
-      const int c0 = t_lo * kAssignTile;
-      assign_issue_tile(Z, d, cpr, c0, (C - c0) < kAssignTile ? (C - c0) : kAssignTile,
-                        ztile[t_lo & 1], wv, lane);
-    }
+    __syncthreads();   // every ring buffer free (previous row tile done)
+    for (int u = t_lo; u < ntiles && u < t_lo + kAssignStages - 1; ++u) issue(u);
     for (int t = t_lo; t < ntiles; ++t) {
+      // loads issued after tile t's: tiles t+1 .. t+stages-2 (those that exist)
+      int later = 0;
+      for (int u = t + 1; u < ntiles && u <= t + kAssignStages - 2; ++u) later += assign_tile_loads(C, cpr, u, wv);
+      if constexpr (kAssignStages == 2)
+        __builtin_amdgcn_s_waitcnt(KNNG_VMCNT(0));   // nothing issued after tile t yet
+      else
+        wait_vmcnt_le<31>(later);
+      __syncthreads();   // tile t visible to all waves; tile t-1's buffer free
+      if (t + kAssignStages - 1 < ntiles) issue(t + kAssignStages - 1);
       const int c0 = t * kAssignTile;
       const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
-      __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of tile t have landed
-      __syncthreads();                 // all of tile t visible; tile t-1's buffer free
-      if (t + 1 < ntiles) {
-        const int c1 = c0 + kAssignTile;
-        assign_issue_tile(Z, d, cpr, c1, (C - c1) < kAssignTile ? (C - c1) : kAssignTile,
-                          ztile[(t + 1) & 1], wv, lane);
-      }
-      const int8_t* rb = ztile[t & 1] + cl * d;
+      const int8_t* rb = ztile[t % kAssignStages] + cl * d;
       v16i acc0, acc1;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
@@ -457,14 +504,15 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
       }
       const int cell = c0 + cl;
       const bool ok = cl < nc;
-      const float wc = ok ? w[cell] : 0.0f;
+      const float* wt = reinterpret_cast<const float*>(ztile[t % kAssignStages] + kAssignStageW);
+      const float wc = ok ? wt[cl] : 0.0f;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        const float s = float(acc0[j] + acc1[j]) * wc;
+        const float sc = float(acc0[j] + acc1[j]) * wc;
         // acc element j of lane l holds row 8*(j/4) + 4*(l/32) + (j%4) of the wave's 32
         const bool self = EXCL && (r0 + 32 * wv + 8 * (j >> 2) + 4 * hi + (j & 3)) == cell;
-        if (ok && !self && s > best[j]) {   // cells arrive in increasing order: ties keep the smaller
-          best[j] = s;
+        if (ok && !self && sc > best[j]) {   // cells arrive in increasing order: ties keep the smaller
+          best[j] = sc;
           bidx[j] = cell;
         }
       }
@@ -752,6 +800,7 @@ __device__ __forceinline__ void big_issue_tile(const int8_t* __restrict__ q, int
 
 __device__ __forceinline__ int ballot_count(bool p) { return __popcll(__ballot(p)); }
 
+template <int KS>   // KS > 0: d = 32 * KS at compile time (straight-line MFMA chain); 0: any d
 __global__ __launch_bounds__(256) void k_leaf_big(const int8_t* __restrict__ q, int d, int64_t n, int T,
                                                   const int32_t* __restrict__ perm,
                                                   const int4* __restrict__ items, int64_t nitems,
@@ -761,7 +810,7 @@ __global__ __launch_bounds__(256) void k_leaf_big(const int8_t* __restrict__ q, 
   __shared__ __attribute__((aligned(1024))) int8_t tile[2][32 * kMaxDim];
   constexpr int kPer = kMaxLeaf / 64;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int ksteps = d >> 5, cpr = d >> 4;
+  const int ksteps = KS > 0 ? KS : d >> 5, cpr = KS > 0 ? 2 * KS : d >> 4;
   const int cl = lane & 31, hi = lane >> 5;
   const int rrot = cl % cpr;
   int32_t* scr = scratch + int64_t(blockIdx.x) * kBigRows * mpad;
@@ -1640,11 +1689,19 @@ int launch_assign(hipStream_t stream, const int8_t* q, int64_t nrows, int d, con
   if (g > cap) g = cap;
   const int S = (pbest && pidx) ? assign_splits(nrows, C, num_cu, part_cap) : 1;
   if (exclude_self)
-    hipLaunchKernelGGL(k_assign<true>, dim3(g, S), dim3(256), 0, stream, q, nrows, row_off, row_stride,
-                       d, Z, w, C, out, score, pbest, pidx);
+    if (d == kMaxDim)
+      hipLaunchKernelGGL((k_assign<true, kMaxDim / 32>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
+                         row_stride, d, Z, w, C, out, score, pbest, pidx);
+    else
+      hipLaunchKernelGGL((k_assign<true, 0>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
+                         row_stride, d, Z, w, C, out, score, pbest, pidx);
   else
-    hipLaunchKernelGGL(k_assign<false>, dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
-                       row_stride, d, Z, w, C, out, score, pbest, pidx);
+    if (d == kMaxDim)
+      hipLaunchKernelGGL((k_assign<false, kMaxDim / 32>), dim3(g, S), dim3(256), 0, stream, q, nrows,
+                         row_off, row_stride, d, Z, w, C, out, score, pbest, pidx);
+    else
+      hipLaunchKernelGGL((k_assign<false, 0>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
+                         row_stride, d, Z, w, C, out, score, pbest, pidx);
   if (S > 1)
     hipLaunchKernelGGL(k_assign_reduce, dim3(int(std::min<int64_t>((nrows + 255) / 256, 4 * num_cu))),
                        dim3(256), 0, stream, pbest, pidx, S, nrows, out, score);
@@ -2595,8 +2652,12 @@ int knng_build_init(knng_ctx* c) {
       }
       HIPCHK(c, hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
                                c->stream));
-      hipLaunchKernelGGL(k_leaf_big, dim3(grid), dim3(256), 0, c->stream, c->q, c->d, n, T, c->perm,
-                         ditems, int64_t(items.size()), scratch, mpad, cand);
+      if (c->d == kMaxDim)
+        hipLaunchKernelGGL((k_leaf_big<kMaxDim / 32>), dim3(grid), dim3(256), 0, c->stream, c->q, c->d, n, T,
+                           c->perm, ditems, int64_t(items.size()), scratch, mpad, cand);
+      else
+        hipLaunchKernelGGL((k_leaf_big<0>), dim3(grid), dim3(256), 0, c->stream, c->q, c->d, n, T, c->perm,
+                           ditems, int64_t(items.size()), scratch, mpad, cand);
       const hipError_t e = hipGetLastError();
       HIPCHK(c, hipStreamSynchronize(c->stream));
       dfree(&ditems);
