@@ -800,23 +800,26 @@ __device__ __forceinline__ void big_issue_tile(const int8_t* __restrict__ q, int
 
 __device__ __forceinline__ int ballot_count(bool p) { return __popcll(__ballot(p)); }
 
+// Scores of item it = item0 + blockIdx.x (128 members of a big leaf against
+// all its members) into scratch[blockIdx.x][128][mpad], int32 exact.  A
+// fragments stay in registers (one wave per SIMD); member tiles are
+// double-buffered in LDS by direct loads.
 template <int KS>   // KS > 0: d = 32 * KS at compile time (straight-line MFMA chain); 0: any d
-__global__ __launch_bounds__(256) void k_leaf_big(const int8_t* __restrict__ q, int d, int64_t n, int T,
-                                                  const int32_t* __restrict__ perm,
-                                                  const int4* __restrict__ items, int64_t nitems,
-                                                  int32_t* __restrict__ scratch, int mpad,
-                                                  uint64_t* __restrict__ cand) {
+__global__ __launch_bounds__(256) void k_leaf_scores(const int8_t* __restrict__ q, int d, int64_t n,
+                                                     const int32_t* __restrict__ perm,
+                                                     const int4* __restrict__ items, int64_t item0,
+                                                     int64_t nitems, int32_t* __restrict__ scratch,
+                                                     int mpad) {
   __shared__ int32_t mid[kMaxLeaf];
   __shared__ __attribute__((aligned(1024))) int8_t tile[2][32 * kMaxDim];
-  constexpr int kPer = kMaxLeaf / 64;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int ksteps = KS > 0 ? KS : d >> 5, cpr = KS > 0 ? 2 * KS : d >> 4;
   const int cl = lane & 31, hi = lane >> 5;
   const int rrot = cl % cpr;
-  int32_t* scr = scratch + int64_t(blockIdx.x) * kBigRows * mpad;
-  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const int4 item = items[it];   // {tree, start, size, first member}
+  for (int64_t bi = blockIdx.x; bi < nitems; bi += gridDim.x) {
+    const int4 item = items[item0 + bi];   // {tree, start, size, first member}
     const int t = item.x, start = item.y, size = item.z, r0 = item.w;
+    int32_t* scr = scratch + bi * kBigRows * mpad;
     __syncthreads();   // the previous item is done with mid[] and tile[]
     for (int j = tid; j < size; j += 256) mid[j] = perm[int64_t(t) * n + start + j];
     __syncthreads();
@@ -873,77 +876,95 @@ __global__ __launch_bounds__(256) void k_leaf_big(const int8_t* __restrict__ q, 
         }
       }
     }
-    __syncthreads();   // scores written (each wave reads back only its own rows)
-    if (!wave_on) continue;
+  }
+}
+
+// Top-R mates (R = min(size - 1, 64)) of one member row of a big leaf by
+// (score desc, id asc), self excluded, from its scores row; one wave per row,
+// rows (bi, rr) of the chunk's items taken in turn.  The member's 64 slots of
+// this tree get the keys (0 beyond R).
+__global__ __launch_bounds__(256) void k_leaf_select(int64_t n, int T, const int32_t* __restrict__ perm,
+                                                     const int4* __restrict__ items, int64_t item0,
+                                                     int64_t nitems, const int32_t* __restrict__ scratch,
+                                                     int mpad, uint64_t* __restrict__ cand) {
+  constexpr int kPer = kMaxLeaf / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  for (int64_t wr = wave; wr < nitems * kBigRows; wr += nwaves) {
+    const int64_t bi = wr / kBigRows;
+    const int rr = int(wr - bi * kBigRows);
+    const int4 item = items[item0 + bi];
+    const int t = item.x, start = item.y, size = item.z;
+    const int m = item.w + rr;
+    if (m >= size) continue;
+    const int32_t* mid = perm + int64_t(t) * n + start;
     const int R = (size - 1) < kLeafSlots ? (size - 1) : kLeafSlots;
-    for (int rr = 0; rr < 32; ++rr) {
-      const int m = r0 + 32 * wv + rr;
-      if (m >= size) break;
-      const int32_t* row = scr + int64_t(32 * wv + rr) * mpad;
-      int32_t v[kPer];
-      int lmax = INT32_MIN;
+    const int32_t* row = scratch + (bi * kBigRows + rr) * mpad;
+    int32_t v[kPer];
+    int lmax = INT32_MIN;
 #pragma unroll
-      for (int e = 0; e < kPer; ++e) {
-        const int col = lane + 64 * e;
-        v[e] = (col < size && col != m) ? row[col] : INT32_MIN;   // scores are > -2^25
-        lmax = max(lmax, v[e]);
-      }
-      // largest theta with |{v >= theta}| >= R.  When every lane holds a valid
-      // entry, the 64 lane maxima are distinct entries, so their minimum has at
-      // least 64 >= R entries at or above it; otherwise search the full range.
-      int lo = wave_min_i(lmax), hi2 = wave_max_i(lmax);
-      if (ballot_count(lmax != INT32_MIN) < 64) lo = -(1 << 25);
-      while (lo < hi2) {
-        const int md = lo + (hi2 - lo + 1) / 2;
+    for (int e = 0; e < kPer; ++e) {
+      const int col = lane + 64 * e;
+      v[e] = (col < size && col != m) ? row[col] : INT32_MIN;   // scores are > -2^25
+      lmax = max(lmax, v[e]);
+    }
+    // largest theta with |{v >= theta}| >= R.  When every lane holds a valid
+    // entry, the 64 lane maxima are distinct entries, so their minimum has at
+    // least 64 >= R entries at or above it; otherwise search the full range.
+    int lo = wave_min_i(lmax), hi2 = wave_max_i(lmax);
+    if (ballot_count(lmax != INT32_MIN) < 64) lo = -(1 << 25);
+    while (lo < hi2) {
+      const int md = lo + (hi2 - lo + 1) / 2;
+      int cnt = 0;
+#pragma unroll
+      for (int e = 0; e < kPer; ++e) cnt += ballot_count(v[e] >= md);
+      if (cnt >= R)
+        lo = md;
+      else
+        hi2 = md - 1;
+    }
+    const int theta = lo;
+    int cgt = 0, ceq = 0;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      cgt += ballot_count(v[e] > theta);
+      ceq += ballot_count(v[e] == theta);
+    }
+    const int need = R - cgt;
+    int32_t ids[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const int col = lane + 64 * e;
+      ids[e] = col < size ? mid[col] : 0;
+    }
+    int idcut = INT32_MAX;
+    if (ceq > need) {
+      // smallest idcut with |{v == theta, id <= idcut}| >= need
+      int ilo = 0, ihi = int(n - 1);
+      while (ilo < ihi) {
+        const int md = ilo + (ihi - ilo) / 2;
         int cnt = 0;
 #pragma unroll
-        for (int e = 0; e < kPer; ++e) cnt += ballot_count(v[e] >= md);
-        if (cnt >= R)
-          lo = md;
+        for (int e = 0; e < kPer; ++e) cnt += ballot_count(v[e] == theta && ids[e] <= md);
+        if (cnt >= need)
+          ihi = md;
         else
-          hi2 = md - 1;
+          ilo = md + 1;
       }
-      const int theta = lo;
-      int cgt = 0, ceq = 0;
-#pragma unroll
-      for (int e = 0; e < kPer; ++e) {
-        cgt += ballot_count(v[e] > theta);
-        ceq += ballot_count(v[e] == theta);
-      }
-      const int need = R - cgt;
-      int idcut = INT32_MAX;
-      if (ceq > need) {
-        // smallest idcut with |{v == theta, id <= idcut}| >= need
-        int ilo = 0, ihi = int(n - 1);
-        while (ilo < ihi) {
-          const int md = ilo + (ihi - ilo) / 2;
-          int cnt = 0;
-#pragma unroll
-          for (int e = 0; e < kPer; ++e) {
-            const int col = lane + 64 * e;
-            cnt += ballot_count(v[e] == theta && mid[col < size ? col : 0] <= md);
-          }
-          if (cnt >= need)
-            ihi = md;
-          else
-            ilo = md + 1;
-        }
-        idcut = ilo;
-      }
-      uint64_t* dst = cand + int64_t(mid[m]) * (T * kLeafSlots) + t * kLeafSlots;
-      int base = 0;
-#pragma unroll
-      for (int e = 0; e < kPer; ++e) {
-        const int col = lane + 64 * e;
-        const int32_t id = mid[col < size ? col : 0];
-        const bool sel = v[e] > theta || (v[e] == theta && id <= idcut);
-        const uint64_t bal = __ballot(sel);
-        const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
-        if (sel) dst[pos] = pack_key(v[e], id);
-        base += __popcll(bal);
-      }
-      if (lane >= R) dst[lane] = 0ull;
+      idcut = ilo;
     }
+    uint64_t* dst = cand + int64_t(mid[m]) * (T * kLeafSlots) + t * kLeafSlots;
+    int base = 0;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const bool sel = v[e] > theta || (v[e] == theta && ids[e] <= idcut);
+      const uint64_t bal = __ballot(sel);
+      const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+      if (sel) dst[pos] = pack_key(v[e], ids[e]);
+      base += __popcll(bal);
+    }
+    if (lane >= R) dst[lane] = 0ull;
   }
 }
 
@@ -2640,31 +2661,44 @@ int knng_build_init(knng_ctx* c) {
       }
     if (mmax > kMaxLeaf) return fail(c, KNNG_ERANGE, "leaf larger than the big-leaf kernel allows");
     if (!items.empty()) {
+      // items in chunks whose score rows fit a bounded scratch (<= 4 GB):
+      // scores on MFMA (one wave per SIMD), then the top-64 selection with
+      // one wave per row at full occupancy
       const int mpad = (mmax + 63) & ~63;
-      const int grid = int(std::min<int64_t>(int64_t(items.size()), int64_t(c->num_cu) * 2));
+      const int64_t per_item = int64_t(kBigRows) * mpad;
+      const int64_t chunk = std::max<int64_t>(
+          1, std::min<int64_t>(int64_t(items.size()), (int64_t(1) << 30) / per_item));
       int4* ditems = nullptr;
       int32_t* scratch = nullptr;
       if ((rc = dalloc(c, &ditems, items.size(), "big-leaf items")) ||
-          (rc = dalloc(c, &scratch, size_t(grid) * kBigRows * mpad, "big-leaf scores"))) {
+          (rc = dalloc(c, &scratch, size_t(chunk * per_item), "big-leaf scores"))) {
         dfree(&ditems);
+        dfree(&scratch);
         dfree(&cand);
         return rc;
       }
       HIPCHK(c, hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
                                c->stream));
-      if (c->d == kMaxDim)
-        hipLaunchKernelGGL((k_leaf_big<kMaxDim / 32>), dim3(grid), dim3(256), 0, c->stream, c->q, c->d, n, T,
-                           c->perm, ditems, int64_t(items.size()), scratch, mpad, cand);
-      else
-        hipLaunchKernelGGL((k_leaf_big<0>), dim3(grid), dim3(256), 0, c->stream, c->q, c->d, n, T, c->perm,
-                           ditems, int64_t(items.size()), scratch, mpad, cand);
-      const hipError_t e = hipGetLastError();
+      hipError_t e = hipSuccess;
+      for (int64_t i0 = 0; i0 < int64_t(items.size()) && e == hipSuccess; i0 += chunk) {
+        const int64_t ni = std::min<int64_t>(chunk, int64_t(items.size()) - i0);
+        const int sgrid = int(std::min<int64_t>(ni, int64_t(c->num_cu) * 2));
+        if (c->d == kMaxDim)
+          hipLaunchKernelGGL((k_leaf_scores<kMaxDim / 32>), dim3(sgrid), dim3(256), 0, c->stream, c->q, c->d, n,
+                             c->perm, ditems, i0, ni, scratch, mpad);
+        else
+          hipLaunchKernelGGL((k_leaf_scores<0>), dim3(sgrid), dim3(256), 0, c->stream, c->q, c->d, n, c->perm,
+                             ditems, i0, ni, scratch, mpad);
+        hipLaunchKernelGGL(k_leaf_select, dim3(grid_for(c, ni * kBigRows, 4, 32)), dim3(256), 0, c->stream, n, T,
+                           c->perm, ditems, i0, ni, scratch, mpad, cand);
+        e = hipGetLastError();
+      }
       HIPCHK(c, hipStreamSynchronize(c->stream));
       dfree(&ditems);
       dfree(&scratch);
       if (e != hipSuccess) {
         dfree(&cand);
-        return fail(c, KNNG_EHIP, std::string("k_leaf_big launch: ") + hipGetErrorString(e));
+        return fail(c, KNNG_EHIP, std::string("big-leaf launch: ") + hipGetErrorString(e));
       }
     }
   }
