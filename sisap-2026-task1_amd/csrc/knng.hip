@@ -18,6 +18,7 @@
 // node index.
 
 #include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <limits>
@@ -1566,6 +1567,26 @@ __global__ __launch_bounds__(256) void k_seg_snapshot(unsigned long long* __rest
   }
 }
 
+// Cell forest level on the device: row ids, cell sizes, per-position nodes.
+__global__ __launch_bounds__(256) void k_iota(int32_t* __restrict__ v, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    v[i] = int32_t(i);
+}
+
+__global__ __launch_bounds__(256) void k_cell_count(const int32_t* __restrict__ cell, int64_t n,
+                                                    int32_t* __restrict__ cnt) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    atomicAdd(&cnt[cell[i]], 1);
+}
+
+// nodepos[p] = cellnode[cell of the row at position p] (rows sorted by cell)
+__global__ __launch_bounds__(256) void k_cell_nodepos(const int32_t* __restrict__ sorted_cell, int64_t n,
+                                                      const int32_t* __restrict__ cellnode,
+                                                      int32_t* __restrict__ nodepos) {
+  for (int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x; p < n; p += int64_t(gridDim.x) * 256)
+    nodepos[p] = cellnode[sorted_cell[p]];
+}
+
 // ---------------------------------------------------------------------------
 // K8: export.  Plain: src/main.cpp:449-457.  Rerank: FP32 dots of the first R
 // list entries, one wave per point, best k by (score desc, id asc, slot).
@@ -2048,24 +2069,60 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
                              hipMemcpyHostToDevice, c->stream));
   }
   trace_mark(c, "cells uploaded");
-  // host build of perm, nodepos, first-level nodes, small-cell leaves, cell roots
-  std::vector<std::vector<int32_t>> mem(G);
-  std::vector<std::vector<int64_t>> off(G);
-  std::vector<int32_t> a(n);
-  for (int g = 0; g < G; ++g) {
-    HIPCHK(c, hipMemcpyAsync(a.data(), c->cassign + size_t(g) * n, size_t(n) * 4, hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    cell_lists(a, c->cell_count[size_t(g)], 0, 1, &mem[size_t(g)], &off[size_t(g)]);
+  // perm and nodepos on the device: per partition, a stable radix sort of the
+  // rows by cell (inside a cell the rows keep increasing id) and the cell
+  // sizes from a histogram; the host builds only the per-cell first-level
+  // nodes, small-cell leaves and roots from the sizes
+  int32_t *skeys = nullptr, *ivals = nullptr, *svals = nullptr, *dcnt = nullptr, *cellnode = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int bits = 1;
+  while ((int64_t(1) << bits) < Cm) ++bits;
+  auto free_sort = [&] {
+    dfree(&skeys);
+    dfree(&ivals);
+    dfree(&svals);
+    dfree(&dcnt);
+    dfree(&cellnode);
+    if (tmp) (void)hipFree(tmp);
+    tmp = nullptr;
+  };
+  HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, static_cast<const int32_t*>(nullptr),
+                                               static_cast<int32_t*>(nullptr),
+                                               static_cast<const int32_t*>(nullptr),
+                                               static_cast<int32_t*>(nullptr), int(n), 0, bits, c->stream));
+  if ((rc = dalloc(c, &skeys, size_t(G) * n, "sorted cells")) || (rc = dalloc(c, &ivals, size_t(n), "row ids")) ||
+      (rc = dalloc(c, &svals, size_t(G) * n, "rows by cell")) ||
+      (rc = dalloc(c, &dcnt, size_t(G) * Cm, "cell sizes")) ||
+      (rc = dalloc(c, &cellnode, size_t(T) * Cm, "cell nodes"))) {
+    free_sort();
+    return rc;
   }
-  std::vector<int32_t> perm(size_t(T) * n), npos(size_t(T) * n, -1), roots(size_t(T) * Cm, INT32_MIN);
+  if (hipMalloc(&tmp, tmp_bytes + 256) != hipSuccess) {
+    free_sort();
+    return fail(c, KNNG_ENOMEM, "hipMalloc failed for the cell sort");
+  }
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(c, n, 256, 8)), dim3(256), 0, c->stream, ivals, n);
+  HIPCHK(c, hipMemsetAsync(dcnt, 0, size_t(G) * Cm * sizeof(int32_t), c->stream));
+  for (int g = 0; g < G; ++g) {
+    size_t tb = tmp_bytes;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, c->cassign + size_t(g) * n, skeys + size_t(g) * n,
+                                                 ivals, svals + size_t(g) * n, int(n), 0, bits, c->stream));
+    hipLaunchKernelGGL(k_cell_count, dim3(grid_for(c, n, 256, 8)), dim3(256), 0, c->stream,
+                       c->cassign + size_t(g) * n, n, dcnt + size_t(g) * Cm);
+  }
+  HIPCHK(c, hipGetLastError());
+  std::vector<int32_t> hcnt(size_t(G) * Cm);
+  HIPCHK(c, hipMemcpyAsync(hcnt.data(), dcnt, hcnt.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<int32_t> roots(size_t(T) * Cm, INT32_MIN), cnode(size_t(T) * Cm, -1);
   std::vector<Node> nodes;
   std::vector<int4> leaves;
   for (int t = 0; t < T; ++t) {
     const int g = t % G;
-    std::copy(mem[size_t(g)].begin(), mem[size_t(g)].end(), perm.begin() + size_t(t) * n);
+    int64_t s = 0;
     for (int cc = 0; cc < c->cell_count[size_t(g)]; ++cc) {
-      const int64_t s = off[size_t(g)][size_t(cc)], z = off[size_t(g)][size_t(cc) + 1] - s;
+      const int64_t z = hcnt[size_t(g) * Cm + cc];
       if (z == 0) {
         roots[size_t(t) * Cm + cc] = INT32_MIN;
       } else if (z <= leaf) {
@@ -2082,15 +2139,25 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
         nd.p1id = nd.p2id = -1;
         nd.base = 0;
         roots[size_t(t) * Cm + cc] = int32_t(nodes.size());
-        for (int64_t j = s; j < s + z; ++j) npos[size_t(t) * n + j] = int32_t(nodes.size());
+        cnode[size_t(t) * Cm + cc] = int32_t(nodes.size());
         nodes.push_back(nd);
       }
+      s += z;
     }
   }
-  if (int64_t(nodes.size()) > c->node_cap || int64_t(leaves.size()) > c->leaf_cap)
+  if (int64_t(nodes.size()) > c->node_cap || int64_t(leaves.size()) > c->leaf_cap) {
+    free_sort();
     return fail(c, KNNG_ERANGE, "cell seeding exceeds the node/leaf tables");
-  HIPCHK(c, hipMemcpyAsync(c->perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->nodepos, npos.data(), npos.size() * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipMemcpyAsync(cellnode, cnode.data(), cnode.size() * 4, hipMemcpyHostToDevice, c->stream));
+  for (int t = 0; t < T; ++t) {
+    const int g = t % G;
+    HIPCHK(c, hipMemcpyAsync(c->perm + size_t(t) * n, svals + size_t(g) * n, size_t(n) * 4,
+                             hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_cell_nodepos, dim3(grid_for(c, n, 256, 8)), dim3(256), 0, c->stream,
+                       skeys + size_t(g) * n, n, cellnode + size_t(t) * Cm, c->nodepos + size_t(t) * n);
+  }
+  HIPCHK(c, hipGetLastError());
   if (!nodes.empty())
     HIPCHK(c, hipMemcpyAsync(c->nodes, nodes.data(), nodes.size() * sizeof(Node), hipMemcpyHostToDevice,
                              c->stream));
@@ -2099,6 +2166,8 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
                              hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->cell_root, roots.data(), roots.size() * 4, hipMemcpyHostToDevice,
                            c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));   // host vectors above are sources of the copies
+  free_sort();
   int32_t cnt2[2] = {0, int32_t(leaves.size())};
   HIPCHK(c, hipMemcpyAsync(c->dcount, cnt2, sizeof(cnt2), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
