@@ -371,11 +371,11 @@ __global__ void k_hier_append(const Node* __restrict__ nodes, int nnodes, Node* 
 constexpr int kAssignRows = 128;
 constexpr int kAssignTile = 32;
 
-// issue the direct loads of centroid tile [c0, c0 + nc) into `buf` (this wave's share)
+// issue the direct loads of centroid tile [c0, c0 + nc) into `buf` (this wave's share of nw waves)
 __device__ __forceinline__ void assign_issue_tile(const int8_t* __restrict__ Z, int d, int cpr, int c0,
-                                                  int nc, int8_t* buf, int wv, int lane) {
+                                                  int nc, int8_t* buf, int wv, int lane, int nw = 4) {
   const int chunks = nc * cpr;
-  for (int j = wv; j * 64 < chunks; j += 4) {
+  for (int j = wv; j * 64 < chunks; j += nw) {
     const int f = j * 64 + lane;
     const int r = f / cpr, p = f - r * cpr;
     if (f < chunks) {
@@ -531,6 +531,146 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
         }
       }
       const int64_t row = r0 + 32 * wv + 8 * (j >> 2) + 4 * hi + (j & 3);
+      if (cl == 0 && row < nrows) {
+        if (gridDim.y > 1) {
+          pbest[int64_t(blockIdx.y) * nrows + row] = best[j];
+          pidx[int64_t(blockIdx.y) * nrows + row] = bidx[j];
+        } else {
+          out[row] = bidx[j];
+          if (score) score[row] = best[j];
+        }
+      }
+    }
+  }
+}
+
+// k_assign at d = 1024 with the k dimension split over two waves per row
+// group: 8 waves, wave (rg, h) holds the A fragments of rows 32rg..32rg+31
+// for k-steps 16h..16h+15 only (64 VGPRs instead of 128), which lets two
+// waves share each SIMD, so one wave's LDS fragment reads overlap the other's
+// MFMAs.  Per centroid tile, the h = 1 wave writes its partial sums to LDS
+// (double-buffered by tile parity) and the h = 0 wave adds them one tile
+// later, then runs the same epilogue as k_assign.  int32 sums are exact in
+// any order, so the result is k_assign's.
+constexpr int kAssign2Waves = 8;
+
+template <bool EXCL>
+__global__ __launch_bounds__(512) void k_assign_ksplit(const int8_t* __restrict__ q, int64_t nrows,
+                                                       int64_t row_off, int64_t row_stride,
+                                                       const int8_t* __restrict__ Z,
+                                                       const float* __restrict__ w, int C,
+                                                       int32_t* __restrict__ out,
+                                                       float* __restrict__ score,
+                                                       float* __restrict__ pbest,
+                                                       int32_t* __restrict__ pidx) {
+  constexpr int d = kMaxDim, cpr = kMaxDim / 16, KH = kMaxDim / 64;   // k-steps per half
+  __shared__ __attribute__((aligned(1024))) int8_t ztile[2][kAssignStageBytes];
+  __shared__ __attribute__((aligned(16))) int32_t part[2][4][16 * 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int rg = wv & 3, h = wv >> 2;
+  const int ntiles_all = (C + kAssignTile - 1) / kAssignTile;
+  const int per_slice = (ntiles_all + int(gridDim.y) - 1) / int(gridDim.y);
+  const int t_lo = int(blockIdx.y) * per_slice;
+  const int ntiles = min(ntiles_all, t_lo + per_slice);
+  const int cl = lane & 31, hi = lane >> 5;
+  const int rrot = cl % cpr;
+  auto issue = [&](int u) {
+    const int c0 = u * kAssignTile;
+    const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
+    const int buf = u & 1;
+    assign_issue_tile(Z, d, cpr, c0, nc, ztile[buf], wv, lane, kAssign2Waves);
+    if (wv == 0 && lane < nc)
+      __builtin_amdgcn_global_load_lds(w + c0 + lane, ztile[buf] + kAssignStageW, 4, 0, 0);
+  };
+  for (int64_t r0 = int64_t(blockIdx.x) * kAssignRows; r0 < nrows;
+       r0 += int64_t(gridDim.x) * kAssignRows) {
+    const int64_t arow = r0 + 32 * rg + cl;
+    const bool arow_ok = arow < nrows;
+    const int8_t* abase = q + (row_off + (arow_ok ? arow : 0) * row_stride) * d + 16 * hi + 32 * KH * h;
+    v4i a[KH];
+#pragma unroll
+    for (int ks = 0; ks < KH; ++ks)
+      a[ks] = arow_ok ? *reinterpret_cast<const v4i*>(abase + 32 * ks) : v4i{0, 0, 0, 0};
+    float best[16];
+    int32_t bidx[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      best[j] = -INFINITY;
+      bidx[j] = 0x7FFFFFFF;
+    }
+    v16i prev;          // h = 0: own half of the previous tile
+    float w_prev = 0.f;
+    // h = 0: add the other half of tile u (part[u & 1]) to prev and fold it into best/bidx
+    auto finish = [&](int u) {
+      const int c0 = u * kAssignTile;
+      const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
+      const int cell = c0 + cl;
+      const bool ok = cl < nc;
+      const int32_t* pp = part[u & 1][rg];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const float sc = float(prev[j] + pp[j * 64 + lane]) * w_prev;
+        const bool self = EXCL && (r0 + 32 * rg + 8 * (j >> 2) + 4 * hi + (j & 3)) == cell;
+        if (ok && !self && sc > best[j]) {   // cells arrive in increasing order: ties keep the smaller
+          best[j] = sc;
+          bidx[j] = cell;
+        }
+      }
+    };
+    __syncthreads();   // both stages and both partial buffers free (previous row tile done)
+    if (t_lo < ntiles) issue(t_lo);
+    for (int t = t_lo; t < ntiles; ++t) {
+      __builtin_amdgcn_s_waitcnt(KNNG_VMCNT(0));   // this wave's loads of tile t have landed
+      __syncthreads();   // tile t and part[(t-1) & 1] visible; stage (t+1) & 1 no longer read
+      if (t + 1 < ntiles) issue(t + 1);
+      if (h == 0 && t > t_lo) finish(t - 1);
+      const int8_t* rb = ztile[t & 1] + cl * d;
+      v16i acc0, acc1;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        acc0[j] = 0;
+        acc1[j] = 0;
+      }
+#pragma unroll
+      for (int ks = 0; ks < KH; ks += 2) {
+        const int kg = ks + KH * h;   // global k-step
+        int p0 = 2 * kg + hi - rrot;
+        if (p0 < 0) p0 += cpr;
+        if (p0 >= cpr) p0 -= cpr;
+        int p1 = p0 + 2;
+        if (p1 >= cpr) p1 -= cpr;
+        acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], *reinterpret_cast<const v4i*>(rb + 16 * p0), acc0,
+                                                     0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks + 1], *reinterpret_cast<const v4i*>(rb + 16 * p1),
+                                                     acc1, 0, 0, 0);
+      }
+      if (h == 1) {
+        int32_t* pp = part[t & 1][rg];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) pp[j * 64 + lane] = acc0[j] + acc1[j];
+      } else {
+        const float* wt = reinterpret_cast<const float*>(ztile[t & 1] + kAssignStageW);
+        w_prev = cl < ((C - t * kAssignTile) < kAssignTile ? (C - t * kAssignTile) : kAssignTile) ? wt[cl] : 0.f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) prev[j] = acc0[j] + acc1[j];
+      }
+    }
+    __syncthreads();   // the last tile's partial sums visible
+    if (h == 1) continue;
+    if (ntiles > t_lo) finish(ntiles - 1);
+    // reduce over the 32 lanes that share the accumulator rows
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+#pragma unroll
+      for (int m = 16; m >= 1; m >>= 1) {
+        const float ov = __shfl_xor(best[j], m);
+        const int32_t oi = __shfl_xor(bidx[j], m);
+        if (ov > best[j] || (ov == best[j] && oi < bidx[j])) {
+          best[j] = ov;
+          bidx[j] = oi;
+        }
+      }
+      const int64_t row = r0 + 32 * rg + 8 * (j >> 2) + 4 * hi + (j & 3);
       if (cl == 0 && row < nrows) {
         if (gridDim.y > 1) {
           pbest[int64_t(blockIdx.y) * nrows + row] = best[j];
@@ -1732,15 +1872,15 @@ int launch_assign(hipStream_t stream, const int8_t* q, int64_t nrows, int d, con
   const int S = (pbest && pidx) ? assign_splits(nrows, C, num_cu, part_cap) : 1;
   if (exclude_self)
     if (d == kMaxDim)
-      hipLaunchKernelGGL((k_assign<true, kMaxDim / 32>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
-                         row_stride, d, Z, w, C, out, score, pbest, pidx);
+      hipLaunchKernelGGL((k_assign_ksplit<true>), dim3(g, S), dim3(512), 0, stream, q, nrows, row_off,
+                         row_stride, Z, w, C, out, score, pbest, pidx);
     else
       hipLaunchKernelGGL((k_assign<true, 0>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
                          row_stride, d, Z, w, C, out, score, pbest, pidx);
   else
     if (d == kMaxDim)
-      hipLaunchKernelGGL((k_assign<false, kMaxDim / 32>), dim3(g, S), dim3(256), 0, stream, q, nrows,
-                         row_off, row_stride, d, Z, w, C, out, score, pbest, pidx);
+      hipLaunchKernelGGL((k_assign_ksplit<false>), dim3(g, S), dim3(512), 0, stream, q, nrows, row_off,
+                         row_stride, Z, w, C, out, score, pbest, pidx);
     else
       hipLaunchKernelGGL((k_assign<false, 0>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
                          row_stride, d, Z, w, C, out, score, pbest, pidx);
