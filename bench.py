@@ -269,7 +269,7 @@ def main():
                 # HBM bytes actually moved (rocprofv3 FETCH_SIZE + WRITE_SIZE) per second
                 "traffic_GBps": round(traffic / (jms / max(it, 1) / 1e3) / 1e9, 1)
                 if traffic is not None and jms > 0 else None,
-                "kernel": "k_join", "launches": it,
+                "kernel": "k_join", "join_shape": st["join_variant"], "launches": it,
                 "algorithmic_bytes_per_launch": round(jb / max(it, 1)),
                 "avg_launch_ms": round(jms / max(it, 1), 3)}
 

@@ -926,9 +926,9 @@ constexpr int kBigRows = 128;
 
 __device__ __forceinline__ void big_issue_tile(const int8_t* __restrict__ q, int d, int cpr,
                                                const int32_t* mid, int c0, int nc, int8_t* buf,
-                                               int wv, int lane) {
+                                               int wv, int lane, int nw = 4) {
   const int chunks = nc * cpr;
-  for (int j = wv; j * 64 < chunks; j += 4) {
+  for (int j = wv; j * 64 < chunks; j += nw) {
     const int f = j * 64 + lane;
     const int r = f / cpr, p = f - r * cpr;
     if (f < chunks) {
@@ -1017,6 +1017,100 @@ __global__ __launch_bounds__(256) void k_leaf_scores(const int8_t* __restrict__ 
         }
       }
     }
+  }
+}
+
+// k_leaf_scores at d = 1024 with the k dimension split over two waves per
+// row group (8 waves, two per SIMD), as k_assign_ksplit: wave (rg, h) holds
+// the A fragments of its 32 rows for k-steps 16h..16h+15; the h = 1 wave
+// leaves its partial sums in LDS and the h = 0 wave adds them one tile later
+// and writes the exact int32 scores.
+__global__ __launch_bounds__(512) void k_leaf_scores_ksplit(const int8_t* __restrict__ q, int64_t n,
+                                                            const int32_t* __restrict__ perm,
+                                                            const int4* __restrict__ items, int64_t item0,
+                                                            int64_t nitems, int32_t* __restrict__ scratch,
+                                                            int mpad) {
+  constexpr int d = kMaxDim, cpr = kMaxDim / 16, KH = kMaxDim / 64;
+  __shared__ int32_t mid[kMaxLeaf];
+  __shared__ __attribute__((aligned(1024))) int8_t tile[2][32 * kMaxDim];
+  __shared__ __attribute__((aligned(16))) int32_t part[2][4][16 * 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int rg = wv & 3, h = wv >> 2;
+  const int cl = lane & 31, hi = lane >> 5;
+  const int rrot = cl % cpr;
+  for (int64_t bi = blockIdx.x; bi < nitems; bi += gridDim.x) {
+    const int4 item = items[item0 + bi];   // {tree, start, size, first member}
+    const int t = item.x, start = item.y, size = item.z, r0 = item.w;
+    int32_t* scr = scratch + bi * kBigRows * mpad;
+    __syncthreads();   // the previous item is done with mid[], tile[] and part[]
+    for (int j = tid; j < size; j += 512) mid[j] = perm[int64_t(t) * n + start + j];
+    __syncthreads();
+    const bool wave_on = r0 + 32 * rg < size;
+    v4i a[KH];
+    {
+      const int arow = r0 + 32 * rg + cl;
+      const bool ok = arow < size;
+      const int8_t* abase = q + int64_t(ok ? mid[arow] : 0) * d + 16 * hi + 32 * KH * h;
+#pragma unroll
+      for (int ks = 0; ks < KH; ++ks)
+        a[ks] = ok ? *reinterpret_cast<const v4i*>(abase + 32 * ks) : v4i{0, 0, 0, 0};
+    }
+    v16i prev;
+    // h = 0: scores of tile u = own half (prev) + the other half (part[u & 1])
+    auto finish = [&](int u) {
+      const int c0 = u * 32;
+      const int nc = (size - c0) < 32 ? (size - c0) : 32;
+      if (cl >= nc) return;
+      const int32_t* pp = part[u & 1][rg];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int rl = 32 * rg + 8 * (j >> 2) + 4 * hi + (j & 3);
+        scr[int64_t(rl) * mpad + c0 + cl] = prev[j] + pp[j * 64 + lane];
+      }
+    };
+    const int ntiles = (size + 31) >> 5;
+    big_issue_tile(q, d, cpr, mid, 0, size < 32 ? size : 32, tile[0], wv, lane, kAssign2Waves);
+    for (int tt = 0; tt < ntiles; ++tt) {
+      __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of tile tt have landed
+      __syncthreads();                 // tile tt and part[(tt-1) & 1] visible; tile tt-1's buffer free
+      if (tt + 1 < ntiles) {
+        const int c1 = (tt + 1) * 32;
+        big_issue_tile(q, d, cpr, mid, c1, (size - c1) < 32 ? (size - c1) : 32, tile[(tt + 1) & 1], wv, lane,
+                       kAssign2Waves);
+      }
+      if (!wave_on) continue;
+      if (h == 0 && tt > 0) finish(tt - 1);
+      const int8_t* rb = tile[tt & 1] + cl * d;
+      v16i acc0, acc1;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        acc0[j] = 0;
+        acc1[j] = 0;
+      }
+#pragma unroll
+      for (int ks = 0; ks < KH; ks += 2) {
+        const int kg = ks + KH * h;
+        int p0 = 2 * kg + hi - rrot;
+        if (p0 < 0) p0 += cpr;
+        if (p0 >= cpr) p0 -= cpr;
+        int p1 = p0 + 2;
+        if (p1 >= cpr) p1 -= cpr;
+        acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], *reinterpret_cast<const v4i*>(rb + 16 * p0), acc0,
+                                                     0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks + 1], *reinterpret_cast<const v4i*>(rb + 16 * p1),
+                                                     acc1, 0, 0, 0);
+      }
+      if (h == 1) {
+        int32_t* pp = part[tt & 1][rg];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) pp[j * 64 + lane] = acc0[j] + acc1[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) prev[j] = acc0[j] + acc1[j];
+      }
+    }
+    __syncthreads();   // the last tile's partial sums visible
+    if (wave_on && h == 0 && ntiles > 0) finish(ntiles - 1);
   }
 }
 
@@ -1283,7 +1377,7 @@ struct JoinScratch {
 
 
 
-template <int HSBITS, int CAP, int OCC, int DOTPER, int SKPER>
+template <int HSBITS, int CAP, int OCC, int DOTPER, int SKPER, int DIM>
 __global__ __launch_bounds__(256, OCC) void k_join(
     const int8_t* __restrict__ q, const uint64_t* __restrict__ sk, int d, int64_t n, int K, int M,
     const int32_t* __restrict__ ids_in, const int32_t* __restrict__ sc_in,
@@ -1313,7 +1407,10 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   uint16_t* surv = ch;                                            // [CAP] after selection
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int words = d >> 6, nch = d >> 4;
+  // DIM > 0: the row length is a compile-time constant (d = DIM), which turns
+  // the per-chunk bounds of the sketch and row gathers into straight-line code
+  if (DIM > 0) d = DIM;
+  const int words = DIM > 0 ? DIM / 64 : d >> 6, nch = DIM > 0 ? DIM / 16 : d >> 4;
   unsigned long long my_upd = 0, my_cand = 0, my_sel = 0, my_ins = 0;
 
   // Schedule: blocks are dealt to the 8 XCDs round-robin, so XCD x = block % 8
@@ -2369,14 +2466,23 @@ int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const in
 #define KNNG_JOIN_ARGS                                                                          \
   c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order, \
       fin, c->fresh[b], nwork_dev, seg_base, nseg, segc
-  if (K > 50)
-    hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+  const bool d1024 = c->d == kMaxDim;   // the rows' length as a compile-time constant
+  if (K > 50 && d1024)
+    hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+  else if (K > 50)
+    hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, 0>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+  else if (v == 1 && d1024)
+    hipLaunchKernelGGL((k_join<12, 2560, 3, 4, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
   else if (v == 1)
-    hipLaunchKernelGGL((k_join<12, 2560, 3, 4, 8>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+    hipLaunchKernelGGL((k_join<12, 2560, 3, 4, 8, 0>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+  else if (v == 2 && d1024)
+    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 4, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
   else if (v == 2)
-    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 4>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 4, 0>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+  else if (d1024)
+    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
   else
-    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 8>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 8, 0>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
 #undef KNNG_JOIN_ARGS
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -2893,8 +2999,8 @@ int knng_build_init(knng_ctx* c) {
         const int64_t ni = std::min<int64_t>(chunk, int64_t(items.size()) - i0);
         const int sgrid = int(std::min<int64_t>(ni, int64_t(c->num_cu) * 2));
         if (c->d == kMaxDim)
-          hipLaunchKernelGGL((k_leaf_scores<kMaxDim / 32>), dim3(sgrid), dim3(256), 0, c->stream, c->q, c->d, n,
-                             c->perm, ditems, i0, ni, scratch, mpad);
+          hipLaunchKernelGGL(k_leaf_scores_ksplit, dim3(sgrid), dim3(512), 0, c->stream, c->q, n, c->perm, ditems,
+                             i0, ni, scratch, mpad);
         else
           hipLaunchKernelGGL((k_leaf_scores<0>), dim3(sgrid), dim3(256), 0, c->stream, c->q, c->d, n, c->perm,
                              ditems, i0, ni, scratch, mpad);
