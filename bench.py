@@ -6,10 +6,11 @@ Gaussian mixture generated directly in HBM, k = 15, K_BUILD = 50 and the
 sketch prefilter x8 of the reference, with the build form chosen for this
 workload: 4 trees whose first level is an over-seeded spherical k-means
 partition each (4 partitions), leaves of up to 2048 rows brute-forced on
-int8 MFMA, join iterations until fewer than 1% of the slots get a new id.
-One step is the whole hot path from resident float32 rows to the final
-lists: quantise + sketch, cells + forest, leaf-tile init, join iterations,
-plain export and the FP32-rerank export.  The value is seconds per step
+int8 MFMA, join iterations until fewer than 1% of the slots get a new id,
+and an FP32 rerank of each row's 32 best list entries.  One step is the
+whole hot path from resident float32 rows to the final lists: quantise +
+sketch, cells + forest, leaf-tile init, join iterations, plain export and
+the FP32-rerank export.  The value is seconds per step
 (lower is better).  Recall@15 of both exports is measured afterwards against
 exact FP32 brute force on 10,000 sampled rows.  --trees 16 --leaf 64
 --cells 0 gives the reference's own forest shape.
@@ -63,7 +64,7 @@ def measured_join_traffic(args):
 def build_params(knng, args, d):
     return knng.default_params(dim=d, num_trees=args.trees, k_build=args.k_build, keep_fp32=0,
                                leaf_size=args.leaf, cells=args.cells,
-                               cell_partitions=args.partitions)
+                               cell_partitions=args.partitions, rerank_width=args.rerank_width)
 
 
 def exact_topk(torch, x, rows, k, chunk=262144):
@@ -159,6 +160,9 @@ def main():
     ap.add_argument("--cells", type=int, default=-1, help="-1: n / 1536; 0: plain RP trees")
     ap.add_argument("--partitions", type=int, default=4)
     ap.add_argument("--k-build", type=int, default=50)
+    ap.add_argument("--rerank-width", type=int, default=32,
+                    help="list entries the FP32 rerank re-scores (0: all k_build); 32 keeps the rerank "
+                         "under 10%% of the build at 6.35M rows")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--recall-rows", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=50000)
@@ -287,6 +291,7 @@ def main():
                                f"mixture, k=15, int8 build + FP32 rerank",
                    "n": n, "dim": d, "clusters": args.clusters, "k": 15, "k_build": args.k_build,
                    "trees": args.trees, "leaf": args.leaf, "cells": st["cells"],
+                   "rerank_width": args.rerank_width or args.k_build,
                    "cell_partitions": args.partitions, "parallelism": "single-gpu"},
         "recall_at_15": {"plain": round(rec_plain, 4), "rerank": round(rec_rerank, 4),
                          "rows": int(len(rows_np)), "rerank_rel_err_max": float(rel.max())},

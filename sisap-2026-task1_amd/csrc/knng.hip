@@ -1861,34 +1861,35 @@ __global__ __launch_bounds__(256) void k_export_rerank(const float* __restrict__
       own[c] = chn < nch ? reinterpret_cast<const float4*>(x + (row_base + i) * d)[chn]
                          : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    // lane r ends up holding candidate r's id and FP32 score
+    // lane r ends up holding candidate r's id and FP32 score; four candidate
+    // rows (4 x 4 KB) in flight per wave
     const int32_t my_id = lane < R ? gid[(row_base + i) * K + lane] : -1;
     float my_s = 0.f;
-    for (int r = 0; r < R; r += 2) {
-      const int32_t ia = __shfl(my_id, r);
-      const int32_t ib = __shfl(my_id, r + 1 < R ? r + 1 : r);
-      const float4* pa = reinterpret_cast<const float4*>(x + int64_t(ia) * d);
-      const float4* pb = reinterpret_cast<const float4*>(x + int64_t(ib) * d);
-      float sa = 0.f, sb = 0.f;
+    for (int r = 0; r < R; r += 4) {
+      float4 v[4][4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int chn = lane + 64 * c;
-        if (chn < nch) {
-          const float4 a = pa[chn], b = pb[chn];
-          sa = fmaf(own[c].x, a.x, sa);
-          sa = fmaf(own[c].y, a.y, sa);
-          sa = fmaf(own[c].z, a.z, sa);
-          sa = fmaf(own[c].w, a.w, sa);
-          sb = fmaf(own[c].x, b.x, sb);
-          sb = fmaf(own[c].y, b.y, sb);
-          sb = fmaf(own[c].z, b.z, sb);
-          sb = fmaf(own[c].w, b.w, sb);
+      for (int u = 0; u < 4; ++u) {
+        const int32_t id = __shfl(my_id, r + u < R ? r + u : r);
+        const float4* pr = reinterpret_cast<const float4*>(x + int64_t(id) * d);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int chn = lane + 64 * c;
+          v[u][c] = chn < nch ? pr[chn] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
       }
-      sa = wave_sum_f(sa);
-      sb = wave_sum_f(sb);
-      if (lane == r) my_s = sa;
-      if (lane == r + 1) my_s = sb;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        float sa = 0.f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          sa = fmaf(own[c].x, v[u][c].x, sa);
+          sa = fmaf(own[c].y, v[u][c].y, sa);
+          sa = fmaf(own[c].z, v[u][c].z, sa);
+          sa = fmaf(own[c].w, v[u][c].w, sa);
+        }
+        sa = wave_sum_f(sa);
+        if (lane == r + u) my_s = sa;
+      }
     }
     // rank among the R by (score desc, id asc, slot asc)
     int rank = 0;
