@@ -64,7 +64,9 @@ def measured_join_traffic(args):
 def build_params(knng, args, d):
     return knng.default_params(dim=d, num_trees=args.trees, k_build=args.k_build, keep_fp32=0,
                                leaf_size=args.leaf, cells=args.cells,
-                               cell_partitions=args.partitions, rerank_width=args.rerank_width)
+                               cell_partitions=args.partitions, rerank_width=args.rerank_width,
+                               incremental=args.incremental, cell_oversample=args.oversample,
+                               max_iters=args.max_iters)
 
 
 def exact_topk(torch, x, rows, k, chunk=262144):
@@ -160,6 +162,10 @@ def main():
     ap.add_argument("--cells", type=int, default=-1, help="-1: n / 1536; 0: plain RP trees")
     ap.add_argument("--partitions", type=int, default=4)
     ap.add_argument("--k-build", type=int, default=50)
+    ap.add_argument("--oversample", type=int, default=4, help="k-means seeds per target cell")
+    ap.add_argument("--max-iters", type=int, default=30, help="join iteration cap")
+    ap.add_argument("--incremental", type=int, default=0,
+                    help="1: NN-Descent's new/old rule in the join (a 2-hop pair needs a fresh edge)")
     ap.add_argument("--rerank-width", type=int, default=32,
                     help="list entries the FP32 rerank re-scores (0: all k_build); 32 keeps the rerank "
                          "under 10%% of the build at 6.35M rows")
@@ -182,6 +188,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4, help="concurrent bulk launches")
     ap.add_argument("--express-slots", type=int, default=2, help="express-lane slots (batches <= 64)")
     ap.add_argument("--express-cus", type=int, default=0, help="CUs reserved for the express lane")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for the serve workload (0: leave the environment's)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -291,7 +299,8 @@ def main():
                                f"mixture, k=15, int8 build + FP32 rerank",
                    "n": n, "dim": d, "clusters": args.clusters, "k": 15, "k_build": args.k_build,
                    "trees": args.trees, "leaf": args.leaf, "cells": st["cells"],
-                   "rerank_width": args.rerank_width or args.k_build,
+                   "rerank_width": args.rerank_width or args.k_build, "incremental": args.incremental,
+                   "cell_oversample": args.oversample, "max_iters": args.max_iters,
                    "cell_partitions": args.partitions, "parallelism": "single-gpu"},
         "recall_at_15": {"plain": round(rec_plain, 4), "rerank": round(rec_rerank, 4),
                          "rows": int(len(rows_np)), "rerank_rel_err_max": float(rel.max())},
@@ -395,11 +404,10 @@ def serve_main(args):
     submission to completion."""
     import threading
 
-    # one hardware queue per stream (bulk slots, express slots, inserts, the
-    # build context): with HIP's default of 4, express launches would queue
-    # behind bulk launches that share their hardware queue
-    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
-        os.environ["GPU_MAX_HW_QUEUES"] = "16"
+    # HIP's hardware queues per process (4 by default); more give every slot
+    # stream its own queue (measured: better express p50, lower bulk throughput)
+    if args.hw_queues > 0:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     import numpy as np
     import torch
 

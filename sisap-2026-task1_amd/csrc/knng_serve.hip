@@ -61,6 +61,7 @@ constexpr int kBeamMax = 64;
 constexpr int kExpand = 4;        // beam entries expanded per step (one per wave)
 constexpr int kCandCap = 256;     // >= kExpand * kMaxK and >= 4 entry leaves of 64
 constexpr int kVisitBits = 12;    // visited set: 4096 slots, reset when 3/4 full
+constexpr int kExpressRouteCUs = 16;   // express routing: ~32 blocks per partition
 
 __device__ __forceinline__ bool ahead(int32_t sa, int32_t ia, int32_t sb, int32_t ib) {
   return sa != sb ? sa > sb : ia < ib;   // (score desc, id asc)
@@ -548,17 +549,20 @@ Slot* new_slot(knng_server* s, bool express) {
 // dcell: nparts * nq int32 scratch for the query's cell in each partition;
 // pbest/pidx: pcap-entry partials that let the routing of a few queries use
 // many blocks (cells split into slices)
-// max_blocks caps the search grid (blocks then take several queries each).
+// max_blocks caps the search grid (blocks then take several queries each);
+// route_cu sizes the cell routing as if the chip had that many CUs (two
+// blocks per CU per partition), so a small launch's routing fits the block
+// slots that bulk launches leave free.
 int launch_search(knng_server* s, hipStream_t stream, const float* dx, int8_t* dq, uint64_t* dsk,
                   int32_t* dcell, float* pbest, int32_t* pidx, int64_t pcap, int32_t nq, int beam,
-                  int k, int rerank, int32_t* did, float* dsc, int max_blocks) {
+                  int k, int rerank, int32_t* did, float* dsc, int max_blocks, int route_cu) {
   knng_ctx* c = s->ctx;
   if (knng_host::launch_quantize(stream, dx, nq, s->d, dq, dsk, s->num_cu))
     return sfail(s, KNNG_EHIP, "quantise launch failed");
   for (int g = 0; g < c->nparts; ++g)   // route the queries to cells
     if (knng_host::launch_assign(stream, dq, nq, s->d, c->cz + size_t(g) * c->ncells * s->d,
                                  c->cw + size_t(g) * c->ncells, c->cell_count[size_t(g)],
-                                 dcell + size_t(g) * nq, s->num_cu, 0, 1, false, nullptr,
+                                 dcell + size_t(g) * nq, route_cu, 0, 1, false, nullptr,
                                  pbest, pidx, pcap))
       return sfail(s, KNNG_EHIP, "cell routing launch failed");
   const int grid = std::max(1, std::min<int>(nq, max_blocks));
@@ -590,7 +594,7 @@ int do_insert_rows(knng_server* s, const float* rows, int32_t m, int64_t* first_
   int rc = launch_search(s, st, s->x + base * s->d, s->q + base * s->d, s->sk + base * s->words,
                          s->ins_dcell, s->ins_pbest, s->ins_pidx,
                          knng_host::assign_part_entries(s->ins_cap, s->num_cu), m,
-                         std::min(beam, kBeamMax), K, 0, s->ins_did, s->ins_dsc, s->num_cu * 8);
+                         std::min(beam, kBeamMax), K, 0, s->ins_did, s->ins_dsc, s->num_cu * 8, s->num_cu);
   if (rc) return rc;
   std::vector<int32_t> ids(size_t(m) * K);
   std::vector<float> sc(size_t(m) * K);
@@ -761,11 +765,11 @@ void launch_parts(knng_server* s, Slot* sl, std::unique_lock<std::mutex>& lk) {
            size_t(sl->parts[j].nrows) * s->d * sizeof(float));
   bool ok = hipMemcpyAsync(sl->dx, sl->hx, size_t(m) * s->d * sizeof(float), hipMemcpyHostToDevice,
                            sl->stream) == hipSuccess;
-  // a bulk launch leaves part of the chip's block slots free, so express
-  // launches find room at once instead of waiting for bulk blocks to drain
   const int blocks = sl->express ? s->num_cu * 8 : s->bulk_blocks;
+  const int route_cu = sl->express ? kExpressRouteCUs : s->num_cu;
   ok = ok && !launch_search(s, sl->stream, sl->dx, sl->dq, sl->dsk, sl->dcell, sl->pbest, sl->pidx,
-                            sl->pcap, m, s->sp.beam, s->sp.k, s->sp.rerank, sl->did, sl->dsc, blocks);
+                            sl->pcap, m, s->sp.beam, s->sp.k, s->sp.rerank, sl->did, sl->dsc, blocks,
+                            route_cu);
   ok = ok && hipMemcpyAsync(sl->hid, sl->did, size_t(m) * s->sp.k * sizeof(int32_t),
                             hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
   ok = ok && hipMemcpyAsync(sl->hsc, sl->dsc, size_t(m) * s->sp.k * sizeof(float),
@@ -957,10 +961,10 @@ int knng_server_create(knng_server** out, knng_ctx* c, const knng_server_params*
       hipMalloc(&s->gsc, cap * s->K * sizeof(int32_t)) != hipSuccess)
     return bail(KNNG_ENOMEM);
   s->sp.express_rows = std::min(s->sp.express_rows, s->sp.max_batch);
-  // bulk launches together hold at most ~6 search blocks per CU (of the ~7
-  // that fit) while an express lane exists
-  s->bulk_blocks = s->sp.express_rows > 0 ? std::max(64, s->num_cu * 6 / std::max(1, s->sp.max_slots))
-                                          : s->num_cu * 8;
+  // bulk launches are not capped below the chip's block slots: a cap that left
+  // room for express blocks (~6 of the ~7 search blocks per CU) cost more
+  // bulk throughput than it saved express latency at 880K queries/s
+  s->bulk_blocks = s->num_cu * 8;
   if (s->sp.express_slots == 0) s->sp.express_rows = 0;
   s->cu_masked = s->sp.express_rows > 0 && s->sp.express_cus > 0 && s->sp.express_cus < s->num_cu;
   if (hipStreamCreateWithFlags(&s->ins_stream, hipStreamNonBlocking) != hipSuccess) return bail(KNNG_EHIP);
