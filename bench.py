@@ -162,9 +162,9 @@ def main():
     ap.add_argument("--cells", type=int, default=-1, help="-1: n / 1536; 0: plain RP trees")
     ap.add_argument("--partitions", type=int, default=4)
     ap.add_argument("--k-build", type=int, default=50)
-    ap.add_argument("--oversample", type=int, default=4, help="k-means seeds per target cell")
-    ap.add_argument("--max-iters", type=int, default=30, help="join iteration cap")
-    ap.add_argument("--incremental", type=int, default=0,
+    ap.add_argument("--oversample", type=int, default=2, help="k-means seeds per target cell")
+    ap.add_argument("--max-iters", type=int, default=2, help="join iteration cap")
+    ap.add_argument("--incremental", type=int, default=1,
                     help="1: NN-Descent's new/old rule in the join (a 2-hop pair needs a fresh edge)")
     ap.add_argument("--rerank-width", type=int, default=32,
                     help="list entries the FP32 rerank re-scores (0: all k_build); 32 keeps the rerank "

@@ -548,11 +548,19 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
 // group: 8 waves, wave (rg, h) holds the A fragments of rows 32rg..32rg+31
 // for k-steps 16h..16h+15 only (64 VGPRs instead of 128), which lets two
 // waves share each SIMD, so one wave's LDS fragment reads overlap the other's
-// MFMAs.  Per centroid tile, the h = 1 wave writes its partial sums to LDS
-// (double-buffered by tile parity) and the h = 0 wave adds them one tile
-// later, then runs the same epilogue as k_assign.  int32 sums are exact in
-// any order, so the result is k_assign's.
+// MFMAs.  Per centroid tile, each wave of the pair hands the other, through
+// LDS (double-buffered by tile parity), its k-half sums of the 8 accumulator
+// elements the other finishes, and one tile later adds the other's half to
+// its own 8 and runs k_assign's epilogue on them: the pair splits the
+// epilogue evenly.  int32 sums are exact in any order, so the result is
+// k_assign's.
 constexpr int kAssign2Waves = 8;
+// centroid rows in LDS with a 1040-byte pitch: one direct 1 KB load per row,
+// and the 16 B column reads of 8 consecutive rows cover all 32 banks, so
+// every lane's fragment address is one base register plus an immediate
+constexpr int kAssignPitch = kMaxDim + 16;
+constexpr int kAssign2StageW = kAssignTile * kAssignPitch;        // weights' offset in a stage
+constexpr int kAssign2StageBytes = kAssign2StageW + 256;
 
 template <bool EXCL>
 __global__ __launch_bounds__(512) void k_assign_ksplit(const int8_t* __restrict__ q, int64_t nrows,
@@ -563,33 +571,184 @@ __global__ __launch_bounds__(512) void k_assign_ksplit(const int8_t* __restrict_
                                                        float* __restrict__ score,
                                                        float* __restrict__ pbest,
                                                        int32_t* __restrict__ pidx) {
-  constexpr int d = kMaxDim, cpr = kMaxDim / 16, KH = kMaxDim / 64;   // k-steps per half
-  __shared__ __attribute__((aligned(1024))) int8_t ztile[2][kAssignStageBytes];
-  __shared__ __attribute__((aligned(16))) int32_t part[2][4][16 * 64];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int d = kMaxDim, KH = kMaxDim / 64;   // k-steps per half
+  __shared__ __attribute__((aligned(1024))) int8_t ztile[2][kAssign2StageBytes];
+  // part[parity][rg][hd]: the other wave's k-half sums of the 8 accumulator
+  // elements that wave (rg, hd) finishes
+  __shared__ __attribute__((aligned(16))) int32_t part[2][4][2][8 * 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int rg = wv & 3, h = wv >> 2;
   const int ntiles_all = (C + kAssignTile - 1) / kAssignTile;
   const int per_slice = (ntiles_all + int(gridDim.y) - 1) / int(gridDim.y);
   const int t_lo = int(blockIdx.y) * per_slice;
   const int ntiles = min(ntiles_all, t_lo + per_slice);
   const int cl = lane & 31, hi = lane >> 5;
-  const int rrot = cl % cpr;
+  // wave h = 1 takes its 32 rows in the order cl ^ 16, so accumulator element
+  // j of either wave is row 16h + 8(j/4) + 4(lane/32) + j%4 of the group for
+  // j < 8, and elements 8..15 of one wave are rows that elements 0..7 of the
+  // other hold: each wave finishes its elements 0..7 and hands over 8..15
+  const int rsw = 16 * h;
   auto issue = [&](int u) {
     const int c0 = u * kAssignTile;
     const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
-    const int buf = u & 1;
-    assign_issue_tile(Z, d, cpr, c0, nc, ztile[buf], wv, lane, kAssign2Waves);
+    int8_t* buf = ztile[u & 1];
+    for (int r = wv; r < nc; r += kAssign2Waves)   // one 1 KB row per load
+      __builtin_amdgcn_global_load_lds(Z + int64_t(c0 + r) * d + 16 * lane, buf + r * kAssignPitch, 16, 0, 0);
     if (wv == 0 && lane < nc)
-      __builtin_amdgcn_global_load_lds(w + c0 + lane, ztile[buf] + kAssignStageW, 4, 0, 0);
+      __builtin_amdgcn_global_load_lds(w + c0 + lane, buf + kAssign2StageW, 4, 0, 0);
   };
   for (int64_t r0 = int64_t(blockIdx.x) * kAssignRows; r0 < nrows;
        r0 += int64_t(gridDim.x) * kAssignRows) {
-    const int64_t arow = r0 + 32 * rg + cl;
+    const int64_t arow = r0 + 32 * rg + (cl ^ rsw);
     const bool arow_ok = arow < nrows;
     const int8_t* abase = q + (row_off + (arow_ok ? arow : 0) * row_stride) * d + 16 * hi + 32 * KH * h;
     v4i a[KH];
 #pragma unroll
     for (int ks = 0; ks < KH; ++ks)
+      a[ks] = arow_ok ? *reinterpret_cast<const v4i*>(abase + 32 * ks) : v4i{0, 0, 0, 0};
+    float best[8];
+    int32_t bidx[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      best[j] = -INFINITY;
+      bidx[j] = 0x7FFFFFFF;
+    }
+    int32_t prev[8];    // own k-half sums of elements 0..7 of the previous tile
+    float w_prev = 0.f;
+    // add the other wave's k half of tile u (part[u & 1]) to prev and fold the
+    // 8 elements into best/bidx with selects; on a partial tile (FULL false)
+    // a column past its cells scores -inf, which never wins
+    auto fold = [&](int u, auto full) {
+      constexpr bool FULL = decltype(full)::value;
+      const int c0 = u * kAssignTile;
+      const int cell = c0 + cl;
+      const bool ok = FULL || cl < C - c0;
+      const int32_t* pp = part[u & 1][rg][h];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        float sc = float(prev[jj] + pp[jj * 64 + lane]) * w_prev;
+        if constexpr (EXCL) {
+          const bool self = (r0 + 32 * rg + rsw + 8 * (jj >> 2) + 4 * hi + (jj & 3)) == cell;
+          sc = (ok && !self) ? sc : -INFINITY;
+        } else if constexpr (!FULL) {
+          sc = ok ? sc : -INFINITY;
+        }
+        const bool take = sc > best[jj];   // cells arrive in increasing order: ties keep the smaller
+        best[jj] = take ? sc : best[jj];
+        bidx[jj] = take ? cell : bidx[jj];
+      }
+    };
+    auto finish = [&](int u) {
+      if (C - u * kAssignTile >= kAssignTile)
+        fold(u, std::true_type{});
+      else
+        fold(u, std::false_type{});
+    };
+    __syncthreads();   // both stages and both partial buffers free (previous row tile done)
+    if (t_lo < ntiles) issue(t_lo);
+    for (int t = t_lo; t < ntiles; ++t) {
+      __builtin_amdgcn_s_waitcnt(KNNG_VMCNT(0));   // this wave's loads of tile t have landed
+      __syncthreads();   // tile t and part[(t-1) & 1] visible; stage (t+1) & 1 no longer read
+      if (t + 1 < ntiles) issue(t + 1);
+      if (t > t_lo) finish(t - 1);
+      // B fragments of this wave's k half: column cl, bytes 32 kg + 16 hi
+      const int8_t* rb = ztile[t & 1] + cl * kAssignPitch + 16 * hi + 32 * KH * h;
+      v4i b[KH];
+#pragma unroll
+      for (int ks = 0; ks < KH; ++ks) b[ks] = *reinterpret_cast<const v4i*>(rb + 32 * ks);
+      v16i acc;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) acc[j] = 0;
+#pragma unroll
+      for (int ks = 0; ks < KH; ++ks) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], b[ks], acc, 0, 0, 0);
+      // keep kAhead fragment reads in flight ahead of the MFMAs (left alone,
+      // the scheduler puts each read right before its MFMA and the wave waits
+      // out the LDS latency every 32 cycles)
+      constexpr int kAhead = 6;
+      __builtin_amdgcn_sched_group_barrier(0x100, kAhead, 0);   // DS reads
+#pragma unroll
+      for (int i = 0; i < KH - kAhead; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);      // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);      // DS read
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, kAhead, 0);
+      // hand the other wave elements 8..15, keep 0..7
+      int32_t* pp = part[t & 1][rg][1 - h];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        pp[jj * 64 + lane] = acc[8 + jj];
+        prev[jj] = acc[jj];
+      }
+      const float* wt = reinterpret_cast<const float*>(ztile[t & 1] + kAssign2StageW);
+      w_prev = cl < ((C - t * kAssignTile) < kAssignTile ? (C - t * kAssignTile) : kAssignTile) ? wt[cl] : 0.f;
+    }
+    __syncthreads();   // the last tile's partial sums visible
+    if (ntiles > t_lo) finish(ntiles - 1);
+    // reduce over the 32 lanes that share the accumulator rows
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+#pragma unroll
+      for (int m = 16; m >= 1; m >>= 1) {
+        const float ov = __shfl_xor(best[jj], m);
+        const int32_t oi = __shfl_xor(bidx[jj], m);
+        if (ov > best[jj] || (ov == best[jj] && oi < bidx[jj])) {
+          best[jj] = ov;
+          bidx[jj] = oi;
+        }
+      }
+      const int64_t row = r0 + 32 * rg + rsw + 8 * (jj >> 2) + 4 * hi + (jj & 3);
+      if (cl == 0 && row < nrows) {
+        if (gridDim.y > 1) {
+          pbest[int64_t(blockIdx.y) * nrows + row] = best[jj];
+          pidx[int64_t(blockIdx.y) * nrows + row] = bidx[jj];
+        } else {
+          out[row] = bidx[jj];
+          if (score) score[row] = best[jj];
+        }
+      }
+    }
+  }
+}
+
+// k_assign at d = 1024 with two blocks per CU: 4 waves per block, each with
+// the A fragments of 32 rows over all of k (128 VGPRs), centroid tiles
+// double-buffered in LDS in the padded layout (67 KB per block).  The two
+// blocks on a CU wait at independent barriers, so on each SIMD one block's
+// epilogue overlaps the other's MFMAs.
+template <bool EXCL>
+__global__ __launch_bounds__(256, 2) void k_assign_wide(const int8_t* __restrict__ q, int64_t nrows,
+                                                        int64_t row_off, int64_t row_stride,
+                                                        const int8_t* __restrict__ Z,
+                                                        const float* __restrict__ w, int C,
+                                                        int32_t* __restrict__ out,
+                                                        float* __restrict__ score,
+                                                        float* __restrict__ pbest,
+                                                        int32_t* __restrict__ pidx) {
+  constexpr int d = kMaxDim, KS = kMaxDim / 32;
+  __shared__ __attribute__((aligned(1024))) int8_t ztile[2][kAssign2StageBytes];
+  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ntiles_all = (C + kAssignTile - 1) / kAssignTile;
+  const int per_slice = (ntiles_all + int(gridDim.y) - 1) / int(gridDim.y);
+  const int t_lo = int(blockIdx.y) * per_slice;
+  const int ntiles = min(ntiles_all, t_lo + per_slice);
+  const int cl = lane & 31, hi = lane >> 5;
+  auto issue = [&](int u) {
+    const int c0 = u * kAssignTile;
+    const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
+    int8_t* buf = ztile[u & 1];
+    for (int r = wv; r < nc; r += 4)   // one 1 KB row per load
+      __builtin_amdgcn_global_load_lds(Z + int64_t(c0 + r) * d + 16 * lane, buf + r * kAssignPitch, 16, 0, 0);
+    if (wv == 0 && lane < nc)
+      __builtin_amdgcn_global_load_lds(w + c0 + lane, buf + kAssign2StageW, 4, 0, 0);
+  };
+  for (int64_t r0 = int64_t(blockIdx.x) * kAssignRows; r0 < nrows;
+       r0 += int64_t(gridDim.x) * kAssignRows) {
+    const int64_t arow = r0 + 32 * wv + cl;
+    const bool arow_ok = arow < nrows;
+    const int8_t* abase = q + (row_off + (arow_ok ? arow : 0) * row_stride) * d + 16 * hi;
+    v4i a[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
       a[ks] = arow_ok ? *reinterpret_cast<const v4i*>(abase + 32 * ks) : v4i{0, 0, 0, 0};
     float best[16];
     int32_t bidx[16];
@@ -598,66 +757,58 @@ __global__ __launch_bounds__(512) void k_assign_ksplit(const int8_t* __restrict_
       best[j] = -INFINITY;
       bidx[j] = 0x7FFFFFFF;
     }
-    v16i prev;          // h = 0: own half of the previous tile
-    float w_prev = 0.f;
-    // h = 0: add the other half of tile u (part[u & 1]) to prev and fold it into best/bidx
-    auto finish = [&](int u) {
-      const int c0 = u * kAssignTile;
-      const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
+    // fold tile t's scores into best/bidx with selects; on a partial tile
+    // (FULL false) a column past its cells scores -inf, which never wins
+    auto fold = [&](int t, const v16i& acc, auto full) {
+      constexpr bool FULL = decltype(full)::value;
+      const int c0 = t * kAssignTile;
       const int cell = c0 + cl;
-      const bool ok = cl < nc;
-      const int32_t* pp = part[u & 1][rg];
+      const bool ok = FULL || cl < C - c0;
+      const float* wt = reinterpret_cast<const float*>(ztile[t & 1] + kAssign2StageW);
+      const float wc = ok ? wt[cl] : 0.f;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        const float sc = float(prev[j] + pp[j * 64 + lane]) * w_prev;
-        const bool self = EXCL && (r0 + 32 * rg + 8 * (j >> 2) + 4 * hi + (j & 3)) == cell;
-        if (ok && !self && sc > best[j]) {   // cells arrive in increasing order: ties keep the smaller
-          best[j] = sc;
-          bidx[j] = cell;
+        float sc = float(acc[j]) * wc;
+        if constexpr (EXCL) {
+          const bool self = (r0 + 32 * wv + 8 * (j >> 2) + 4 * hi + (j & 3)) == cell;
+          sc = (ok && !self) ? sc : -INFINITY;
+        } else if constexpr (!FULL) {
+          sc = ok ? sc : -INFINITY;
         }
+        const bool take = sc > best[j];   // cells arrive in increasing order: ties keep the smaller
+        best[j] = take ? sc : best[j];
+        bidx[j] = take ? cell : bidx[j];
       }
     };
-    __syncthreads();   // both stages and both partial buffers free (previous row tile done)
+    __syncthreads();   // both stages free (previous row tile done)
     if (t_lo < ntiles) issue(t_lo);
     for (int t = t_lo; t < ntiles; ++t) {
       __builtin_amdgcn_s_waitcnt(KNNG_VMCNT(0));   // this wave's loads of tile t have landed
-      __syncthreads();   // tile t and part[(t-1) & 1] visible; stage (t+1) & 1 no longer read
+      __syncthreads();   // tile t visible; stage (t+1) & 1 no longer read
       if (t + 1 < ntiles) issue(t + 1);
-      if (h == 0 && t > t_lo) finish(t - 1);
-      const int8_t* rb = ztile[t & 1] + cl * d;
-      v16i acc0, acc1;
+      const int8_t* rb = ztile[t & 1] + cl * kAssignPitch + 16 * hi;
+      v4i b[KS];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        acc0[j] = 0;
-        acc1[j] = 0;
+      for (int ks = 0; ks < KS; ++ks) b[ks] = *reinterpret_cast<const v4i*>(rb + 32 * ks);
+      v16i acc;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) acc[j] = 0;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], b[ks], acc, 0, 0, 0);
+      // keep kAhead fragment reads in flight ahead of the MFMAs
+      constexpr int kAhead = 4;
+      __builtin_amdgcn_sched_group_barrier(0x100, kAhead, 0);   // DS reads
+#pragma unroll
+      for (int i = 0; i < KS - kAhead; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);      // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);      // DS read
       }
-#pragma unroll
-      for (int ks = 0; ks < KH; ks += 2) {
-        const int kg = ks + KH * h;   // global k-step
-        int p0 = 2 * kg + hi - rrot;
-        if (p0 < 0) p0 += cpr;
-        if (p0 >= cpr) p0 -= cpr;
-        int p1 = p0 + 2;
-        if (p1 >= cpr) p1 -= cpr;
-        acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], *reinterpret_cast<const v4i*>(rb + 16 * p0), acc0,
-                                                     0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks + 1], *reinterpret_cast<const v4i*>(rb + 16 * p1),
-                                                     acc1, 0, 0, 0);
-      }
-      if (h == 1) {
-        int32_t* pp = part[t & 1][rg];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) pp[j * 64 + lane] = acc0[j] + acc1[j];
-      } else {
-        const float* wt = reinterpret_cast<const float*>(ztile[t & 1] + kAssignStageW);
-        w_prev = cl < ((C - t * kAssignTile) < kAssignTile ? (C - t * kAssignTile) : kAssignTile) ? wt[cl] : 0.f;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) prev[j] = acc0[j] + acc1[j];
-      }
+      __builtin_amdgcn_sched_group_barrier(0x008, kAhead, 0);
+      if (C - t * kAssignTile >= kAssignTile)
+        fold(t, acc, std::true_type{});
+      else
+        fold(t, acc, std::false_type{});
     }
-    __syncthreads();   // the last tile's partial sums visible
-    if (h == 1) continue;
-    if (ntiles > t_lo) finish(ntiles - 1);
     // reduce over the 32 lanes that share the accumulator rows
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -670,7 +821,7 @@ __global__ __launch_bounds__(512) void k_assign_ksplit(const int8_t* __restrict_
           bidx[j] = oi;
         }
       }
-      const int64_t row = r0 + 32 * rg + 8 * (j >> 2) + 4 * hi + (j & 3);
+      const int64_t row = r0 + 32 * wv + 8 * (j >> 2) + 4 * hi + (j & 3);
       if (cl == 0 && row < nrows) {
         if (gridDim.y > 1) {
           pbest[int64_t(blockIdx.y) * nrows + row] = best[j];
@@ -1020,97 +1171,80 @@ __global__ __launch_bounds__(256) void k_leaf_scores(const int8_t* __restrict__ 
   }
 }
 
-// k_leaf_scores at d = 1024 with the k dimension split over two waves per
-// row group (8 waves, two per SIMD), as k_assign_ksplit: wave (rg, h) holds
-// the A fragments of its 32 rows for k-steps 16h..16h+15; the h = 1 wave
-// leaves its partial sums in LDS and the h = 0 wave adds them one tile later
-// and writes the exact int32 scores.
-__global__ __launch_bounds__(512) void k_leaf_scores_ksplit(const int8_t* __restrict__ q, int64_t n,
-                                                            const int32_t* __restrict__ perm,
-                                                            const int4* __restrict__ items, int64_t item0,
-                                                            int64_t nitems, int32_t* __restrict__ scratch,
-                                                            int mpad) {
-  constexpr int d = kMaxDim, cpr = kMaxDim / 16, KH = kMaxDim / 64;
+// k_leaf_scores at d = 1024 with two blocks per CU, as k_assign_wide: 4
+// waves, each with the A fragments of 32 member rows over all of k, member
+// tiles double-buffered in LDS in the padded layout (one 1 KB direct load per
+// member row), fragment reads kept ahead of the MFMAs; each wave writes its
+// 32 x 32 exact int32 scores per tile.
+__global__ __launch_bounds__(256, 2) void k_leaf_scores_wide(const int8_t* __restrict__ q, int64_t n,
+                                                             const int32_t* __restrict__ perm,
+                                                             const int4* __restrict__ items, int64_t item0,
+                                                             int64_t nitems, int32_t* __restrict__ scratch,
+                                                             int mpad) {
+  constexpr int d = kMaxDim, KS = kMaxDim / 32;
   __shared__ int32_t mid[kMaxLeaf];
-  __shared__ __attribute__((aligned(1024))) int8_t tile[2][32 * kMaxDim];
-  __shared__ __attribute__((aligned(16))) int32_t part[2][4][16 * 64];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int rg = wv & 3, h = wv >> 2;
+  __shared__ __attribute__((aligned(1024))) int8_t tile[2][kAssign2StageW];
+  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int cl = lane & 31, hi = lane >> 5;
-  const int rrot = cl % cpr;
+  auto issue = [&](int c0, int nc, int8_t* buf) {
+    for (int r = wv; r < nc; r += 4)   // member row c0 + r: one 1 KB load
+      __builtin_amdgcn_global_load_lds(q + int64_t(mid[c0 + r]) * d + 16 * lane, buf + r * kAssignPitch, 16, 0,
+                                       0);
+  };
   for (int64_t bi = blockIdx.x; bi < nitems; bi += gridDim.x) {
     const int4 item = items[item0 + bi];   // {tree, start, size, first member}
     const int t = item.x, start = item.y, size = item.z, r0 = item.w;
     int32_t* scr = scratch + bi * kBigRows * mpad;
-    __syncthreads();   // the previous item is done with mid[], tile[] and part[]
-    for (int j = tid; j < size; j += 512) mid[j] = perm[int64_t(t) * n + start + j];
+    __syncthreads();   // the previous item is done with mid[] and tile[]
+    for (int j = tid; j < size; j += 256) mid[j] = perm[int64_t(t) * n + start + j];
     __syncthreads();
-    const bool wave_on = r0 + 32 * rg < size;
-    v4i a[KH];
+    const bool wave_on = r0 + 32 * wv < size;
+    v4i a[KS];
     {
-      const int arow = r0 + 32 * rg + cl;
+      const int arow = r0 + 32 * wv + cl;
       const bool ok = arow < size;
-      const int8_t* abase = q + int64_t(ok ? mid[arow] : 0) * d + 16 * hi + 32 * KH * h;
+      const int8_t* abase = q + int64_t(ok ? mid[arow] : 0) * d + 16 * hi;
 #pragma unroll
-      for (int ks = 0; ks < KH; ++ks)
+      for (int ks = 0; ks < KS; ++ks)
         a[ks] = ok ? *reinterpret_cast<const v4i*>(abase + 32 * ks) : v4i{0, 0, 0, 0};
     }
-    v16i prev;
-    // h = 0: scores of tile u = own half (prev) + the other half (part[u & 1])
-    auto finish = [&](int u) {
-      const int c0 = u * 32;
-      const int nc = (size - c0) < 32 ? (size - c0) : 32;
-      if (cl >= nc) return;
-      const int32_t* pp = part[u & 1][rg];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int rl = 32 * rg + 8 * (j >> 2) + 4 * hi + (j & 3);
-        scr[int64_t(rl) * mpad + c0 + cl] = prev[j] + pp[j * 64 + lane];
-      }
-    };
     const int ntiles = (size + 31) >> 5;
-    big_issue_tile(q, d, cpr, mid, 0, size < 32 ? size : 32, tile[0], wv, lane, kAssign2Waves);
+    issue(0, size < 32 ? size : 32, tile[0]);
     for (int tt = 0; tt < ntiles; ++tt) {
-      __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of tile tt have landed
-      __syncthreads();                 // tile tt and part[(tt-1) & 1] visible; tile tt-1's buffer free
+      __builtin_amdgcn_s_waitcnt(KNNG_VMCNT(0));   // this wave's loads of tile tt have landed
+      __syncthreads();                              // tile tt visible; tile tt-1's buffer free
       if (tt + 1 < ntiles) {
         const int c1 = (tt + 1) * 32;
-        big_issue_tile(q, d, cpr, mid, c1, (size - c1) < 32 ? (size - c1) : 32, tile[(tt + 1) & 1], wv, lane,
-                       kAssign2Waves);
+        issue(c1, (size - c1) < 32 ? (size - c1) : 32, tile[(tt + 1) & 1]);
       }
       if (!wave_on) continue;
-      if (h == 0 && tt > 0) finish(tt - 1);
-      const int8_t* rb = tile[tt & 1] + cl * d;
-      v16i acc0, acc1;
+      const int8_t* rb = tile[tt & 1] + cl * kAssignPitch + 16 * hi;
+      v4i b[KS];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        acc0[j] = 0;
-        acc1[j] = 0;
+      for (int ks = 0; ks < KS; ++ks) b[ks] = *reinterpret_cast<const v4i*>(rb + 32 * ks);
+      v16i acc;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) acc[j] = 0;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], b[ks], acc, 0, 0, 0);
+      constexpr int kAhead = 4;   // fragment reads in flight ahead of the MFMAs
+      __builtin_amdgcn_sched_group_barrier(0x100, kAhead, 0);
+#pragma unroll
+      for (int i = 0; i < KS - kAhead; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
       }
+      __builtin_amdgcn_sched_group_barrier(0x008, kAhead, 0);
+      const int c0 = tt * 32;
+      if (cl < size - c0) {
 #pragma unroll
-      for (int ks = 0; ks < KH; ks += 2) {
-        const int kg = ks + KH * h;
-        int p0 = 2 * kg + hi - rrot;
-        if (p0 < 0) p0 += cpr;
-        if (p0 >= cpr) p0 -= cpr;
-        int p1 = p0 + 2;
-        if (p1 >= cpr) p1 -= cpr;
-        acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], *reinterpret_cast<const v4i*>(rb + 16 * p0), acc0,
-                                                     0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks + 1], *reinterpret_cast<const v4i*>(rb + 16 * p1),
-                                                     acc1, 0, 0, 0);
-      }
-      if (h == 1) {
-        int32_t* pp = part[tt & 1][rg];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) pp[j * 64 + lane] = acc0[j] + acc1[j];
-      } else {
-#pragma unroll
-        for (int j = 0; j < 16; ++j) prev[j] = acc0[j] + acc1[j];
+        for (int j = 0; j < 16; ++j) {
+          // acc element j of lane l: row 8*(j/4) + 4*(l/32) + (j%4) of the wave's 32, column l%32
+          const int rl = 32 * wv + 8 * (j >> 2) + 4 * hi + (j & 3);
+          scr[int64_t(rl) * mpad + c0 + cl] = acc[j];
+        }
       }
     }
-    __syncthreads();   // the last tile's partial sums visible
-    if (wave_on && h == 0 && ntiles > 0) finish(ntiles - 1);
   }
 }
 
@@ -1968,20 +2102,21 @@ int launch_assign(hipStream_t stream, const int8_t* q, int64_t nrows, int d, con
   const int64_t cap = int64_t(num_cu) * 8;
   if (g > cap) g = cap;
   const int S = (pbest && pidx) ? assign_splits(nrows, C, num_cu, part_cap) : 1;
-  if (exclude_self)
-    if (d == kMaxDim)
-      hipLaunchKernelGGL((k_assign_ksplit<true>), dim3(g, S), dim3(512), 0, stream, q, nrows, row_off,
-                         row_stride, Z, w, C, out, score, pbest, pidx);
-    else
-      hipLaunchKernelGGL((k_assign<true, 0>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
-                         row_stride, d, Z, w, C, out, score, pbest, pidx);
+  // d = 1024: k_assign_wide (two blocks per CU); with self exclusion (the
+  // cell merging rounds) its extra compares would spill, so those use
+  // k_assign_ksplit
+  if (d == kMaxDim && !exclude_self)
+    hipLaunchKernelGGL((k_assign_wide<false>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
+                       row_stride, Z, w, C, out, score, pbest, pidx);
+  else if (d == kMaxDim)
+    hipLaunchKernelGGL((k_assign_ksplit<true>), dim3(g, S), dim3(512), 0, stream, q, nrows, row_off,
+                       row_stride, Z, w, C, out, score, pbest, pidx);
+  else if (exclude_self)
+    hipLaunchKernelGGL((k_assign<true, 0>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
+                       row_stride, d, Z, w, C, out, score, pbest, pidx);
   else
-    if (d == kMaxDim)
-      hipLaunchKernelGGL((k_assign_ksplit<false>), dim3(g, S), dim3(512), 0, stream, q, nrows, row_off,
-                         row_stride, Z, w, C, out, score, pbest, pidx);
-    else
-      hipLaunchKernelGGL((k_assign<false, 0>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
-                         row_stride, d, Z, w, C, out, score, pbest, pidx);
+    hipLaunchKernelGGL((k_assign<false, 0>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
+                       row_stride, d, Z, w, C, out, score, pbest, pidx);
   if (S > 1)
     hipLaunchKernelGGL(k_assign_reduce, dim3(int(std::min<int64_t>((nrows + 255) / 256, 4 * num_cu))),
                        dim3(256), 0, stream, pbest, pidx, S, nrows, out, score);
@@ -2999,8 +3134,9 @@ int knng_build_init(knng_ctx* c) {
       for (int64_t i0 = 0; i0 < int64_t(items.size()) && e == hipSuccess; i0 += chunk) {
         const int64_t ni = std::min<int64_t>(chunk, int64_t(items.size()) - i0);
         const int sgrid = int(std::min<int64_t>(ni, int64_t(c->num_cu) * 2));
+        const int wgrid = int(std::min<int64_t>(ni, int64_t(c->num_cu) * 4));   // two resident per CU
         if (c->d == kMaxDim)
-          hipLaunchKernelGGL(k_leaf_scores_ksplit, dim3(sgrid), dim3(512), 0, c->stream, c->q, n, c->perm, ditems,
+          hipLaunchKernelGGL(k_leaf_scores_wide, dim3(wgrid), dim3(256), 0, c->stream, c->q, n, c->perm, ditems,
                              i0, ni, scratch, mpad);
         else
           hipLaunchKernelGGL((k_leaf_scores<0>), dim3(sgrid), dim3(256), 0, c->stream, c->q, c->d, n, c->perm,

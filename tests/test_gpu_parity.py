@@ -110,6 +110,10 @@ def test_score_pairs_bitexact(torch_cuda):
     (1500, 64, 2, 64, 700, 1),      # cells of 0-3 rows, empty cells
     (6000, 256, 2, 512, -1, 2),     # big leaves (k_leaf_big) under RP splits of the cells
     (5000, 128, 3, 2048, 3, 2),     # whole cells as big leaves
+    (6000, 1024, 3, 64, 150, 2),    # d = 1024 assignment over 19 centroid tiles (last one partial)
+    (4000, 1024, 2, 512, 300, 1),   # 38 centroid tiles, one partition
+    (5000, 1024, 2, 2048, 3, 2),    # d = 1024 big leaves: whole cells of ~1700 rows
+    (6000, 1024, 2, 512, -1, 2),    # d = 1024 big leaves under RP splits
 ])
 def test_forest_and_init_bitexact(torch_cuda, n, d, T, leaf, cells, parts):
     say("forest+init", n, d, T, leaf, cells, parts)
