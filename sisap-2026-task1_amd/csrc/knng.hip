@@ -1248,16 +1248,77 @@ __global__ __launch_bounds__(256, 2) void k_leaf_scores_wide(const int8_t* __res
   }
 }
 
+// The R entries of largest (score, then smaller id) among v[] (N per lane,
+// INT32_MIN = none) whose scores lie in [lo, hi], lo low enough that at least
+// R entries reach it; written to dst[0..R) in the order of index lane + 64 e.
+template <int N>
+__device__ __forceinline__ void leaf_top_r(const int32_t (&v)[N], const int32_t (&ids)[N], int lo, int hi,
+                                           int R, int64_t n, uint64_t* dst, int lane) {
+  // largest theta with |{v >= theta}| >= R
+  while (lo < hi) {
+    const int md = lo + (hi - lo + 1) / 2;
+    int cnt = 0;
+#pragma unroll
+    for (int e = 0; e < N; ++e) cnt += ballot_count(v[e] >= md);
+    if (cnt >= R)
+      lo = md;
+    else
+      hi = md - 1;
+  }
+  const int theta = lo;
+  int cgt = 0, ceq = 0;
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    cgt += ballot_count(v[e] > theta);
+    ceq += ballot_count(v[e] == theta);
+  }
+  const int need = R - cgt;
+  int idcut = INT32_MAX;
+  if (ceq > need) {
+    // smallest idcut with |{v == theta, id <= idcut}| >= need
+    int ilo = 0, ihi = int(n - 1);
+    while (ilo < ihi) {
+      const int md = ilo + (ihi - ilo) / 2;
+      int cnt = 0;
+#pragma unroll
+      for (int e = 0; e < N; ++e) cnt += ballot_count(v[e] == theta && ids[e] <= md);
+      if (cnt >= need)
+        ihi = md;
+      else
+        ilo = md + 1;
+    }
+    idcut = ilo;
+  }
+  int base = 0;
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    const bool sel = v[e] > theta || (v[e] == theta && ids[e] <= idcut);
+    const uint64_t bal = __ballot(sel);
+    const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+    if (sel) dst[pos] = pack_key(v[e], ids[e]);
+    base += __popcll(bal);
+  }
+  if (lane >= R) dst[lane] = 0ull;
+}
+
+constexpr int kSelCap = 512;   // compacted candidates per wave in k_leaf_select
+
 // Top-R mates (R = min(size - 1, 64)) of one member row of a big leaf by
 // (score desc, id asc), self excluded, from its scores row; one wave per row,
 // rows (bi, rr) of the chunk's items taken in turn.  The member's 64 slots of
-// this tree get the keys (0 beyond R).
+// this tree get the keys in column order (0 beyond R).  The entries at or
+// above the smallest lane maximum (at least 64 of them when every lane holds
+// one) are compacted, in column order, into LDS (typically a few hundred of
+// the 2048), and the selection runs on those, 8 per lane instead of 32; a row
+// with more than kSelCap such entries selects from all of them.
 __global__ __launch_bounds__(256) void k_leaf_select(int64_t n, int T, const int32_t* __restrict__ perm,
                                                      const int4* __restrict__ items, int64_t item0,
                                                      int64_t nitems, const int32_t* __restrict__ scratch,
                                                      int mpad, uint64_t* __restrict__ cand) {
-  constexpr int kPer = kMaxLeaf / 64;
+  constexpr int kPer = kMaxLeaf / 64, kC = kSelCap / 64;
+  __shared__ int2 cbuf[4][kSelCap];   // (score, column) of this wave's compacted entries
   const int lane = threadIdx.x & 63;
+  int2* cb = cbuf[threadIdx.x >> 6];
   const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
   for (int64_t wr = wave; wr < nitems * kBigRows; wr += nwaves) {
@@ -1278,62 +1339,45 @@ __global__ __launch_bounds__(256) void k_leaf_select(int64_t n, int T, const int
       v[e] = (col < size && col != m) ? row[col] : INT32_MIN;   // scores are > -2^25
       lmax = max(lmax, v[e]);
     }
-    // largest theta with |{v >= theta}| >= R.  When every lane holds a valid
-    // entry, the 64 lane maxima are distinct entries, so their minimum has at
-    // least 64 >= R entries at or above it; otherwise search the full range.
-    int lo = wave_min_i(lmax), hi2 = wave_max_i(lmax);
+    // When every lane holds a valid entry, the 64 lane maxima are distinct
+    // entries, so their minimum has at least 64 >= R entries at or above it;
+    // otherwise every entry is a candidate.
+    int lo = wave_min_i(lmax);
+    const int hi = wave_max_i(lmax);
     if (ballot_count(lmax != INT32_MIN) < 64) lo = -(1 << 25);
-    while (lo < hi2) {
-      const int md = lo + (hi2 - lo + 1) / 2;
-      int cnt = 0;
+    int c0 = 0;
 #pragma unroll
-      for (int e = 0; e < kPer; ++e) cnt += ballot_count(v[e] >= md);
-      if (cnt >= R)
-        lo = md;
-      else
-        hi2 = md - 1;
-    }
-    const int theta = lo;
-    int cgt = 0, ceq = 0;
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      cgt += ballot_count(v[e] > theta);
-      ceq += ballot_count(v[e] == theta);
-    }
-    const int need = R - cgt;
-    int32_t ids[kPer];
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      const int col = lane + 64 * e;
-      ids[e] = col < size ? mid[col] : 0;
-    }
-    int idcut = INT32_MAX;
-    if (ceq > need) {
-      // smallest idcut with |{v == theta, id <= idcut}| >= need
-      int ilo = 0, ihi = int(n - 1);
-      while (ilo < ihi) {
-        const int md = ilo + (ihi - ilo) / 2;
-        int cnt = 0;
-#pragma unroll
-        for (int e = 0; e < kPer; ++e) cnt += ballot_count(v[e] == theta && ids[e] <= md);
-        if (cnt >= need)
-          ihi = md;
-        else
-          ilo = md + 1;
-      }
-      idcut = ilo;
-    }
+    for (int e = 0; e < kPer; ++e) c0 += ballot_count(v[e] >= lo);
     uint64_t* dst = cand + int64_t(mid[m]) * (T * kLeafSlots) + t * kLeafSlots;
-    int base = 0;
+    if (c0 <= kSelCap) {
+      int base = 0;
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      const bool sel = v[e] > theta || (v[e] == theta && ids[e] <= idcut);
-      const uint64_t bal = __ballot(sel);
-      const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
-      if (sel) dst[pos] = pack_key(v[e], ids[e]);
-      base += __popcll(bal);
+      for (int e = 0; e < kPer; ++e) {
+        const bool sel = v[e] >= lo;
+        const uint64_t bal = __ballot(sel);
+        if (sel) cb[base + __popcll(bal & ((1ull << lane) - 1ull))] = make_int2(v[e], lane + 64 * e);
+        base += __popcll(bal);
+      }
+      __builtin_amdgcn_wave_barrier();
+      int32_t cv[kC], ids[kC];
+#pragma unroll
+      for (int k = 0; k < kC; ++k) {
+        const int i = lane + 64 * k;
+        const int2 p = i < c0 ? cb[i] : make_int2(INT32_MIN, 0);
+        cv[k] = p.x;
+        ids[k] = i < c0 ? mid[p.y] : 0;
+      }
+      __builtin_amdgcn_wave_barrier();   // cbuf is free for the next row once read
+      leaf_top_r<kC>(cv, ids, lo, hi, R, n, dst, lane);
+    } else {
+      int32_t ids[kPer];
+#pragma unroll
+      for (int e = 0; e < kPer; ++e) {
+        const int col = lane + 64 * e;
+        ids[e] = col < size ? mid[col] : 0;
+      }
+      leaf_top_r<kPer>(v, ids, lo, hi, R, n, dst, lane);
     }
-    if (lane >= R) dst[lane] = 0ull;
   }
 }
 
