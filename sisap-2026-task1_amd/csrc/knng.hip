@@ -2024,14 +2024,23 @@ __global__ void k_export_plain(const int32_t* __restrict__ gid, const int32_t* _
   }
 }
 
+// Rows are visited in `order` (tree 0's leaf order; null: index order), each
+// XCD (blockIdx.x & 7; the grid is a multiple of 8 blocks) taking a contiguous
+// eighth of it, so the rows in flight on an XCD are cluster-mates whose
+// candidate rows largely coincide and stay in its L2 and the MALL.
 __global__ __launch_bounds__(256) void k_export_rerank(const float* __restrict__ x, int d,
                                                        int64_t n, int K, int R, int k,
                                                        const int32_t* __restrict__ gid,
                                                        int32_t* __restrict__ knns,
-                                                       float* __restrict__ dists, int64_t row_base) {
+                                                       float* __restrict__ dists, int64_t row_base,
+                                                       const int32_t* __restrict__ order) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nch = d >> 2;   // float4 chunks
-  for (int64_t i = int64_t(blockIdx.x) * 4 + w; i < n; i += int64_t(gridDim.x) * 4) {
+  const int xcd = blockIdx.x & 7;
+  const int64_t nbx = gridDim.x >> 3, bx = blockIdx.x >> 3;
+  const int64_t jhi = n * (xcd + 1) / 8;
+  for (int64_t j = n * xcd / 8 + bx * 4 + w; j < jhi; j += nbx * 4) {
+    const int64_t i = order ? order[j] : j;
     float4 own[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -3315,8 +3324,9 @@ int knng_export_device(knng_ctx* c, int32_t* dk, float* dd, int32_t k, int32_t r
     if (!x) return fail(c, KNNG_ESTATE, "knng_export: rerank needs float32 rows (keep_fp32 or attach)");
     int R = c->p.rerank_width ? c->p.rerank_width : K;
     if (R < k) R = k;
-    hipLaunchKernelGGL(k_export_rerank, dim3(grid_for(c, n, 4, 16)), dim3(256), 0, c->stream, x, c->d,
-                       n, K, R, k, c->gid[c->cur], dk, dd, int64_t(0));
+    const int32_t* order = c->have_forest ? c->perm : nullptr;   // tree 0's leaf order
+    hipLaunchKernelGGL(k_export_rerank, dim3(rerank_grid(c, n)), dim3(256), 0, c->stream, x, c->d,
+                       n, K, R, k, c->gid[c->cur], dk, dd, int64_t(0), order);
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -3666,8 +3676,8 @@ int export_segment(knng_ctx* c, int s, int cur, int k, int rerank, int32_t* dk, 
     if (!x) return fail(c, KNNG_ESTATE, "export_segment: rerank needs float32 rows");
     int R = c->p.rerank_width ? c->p.rerank_width : K;
     if (R < k) R = k;
-    hipLaunchKernelGGL(k_export_rerank, dim3(grid_for(c, ns, 4, 16)), dim3(256), 0, c->stream, x, c->d,
-                       ns, K, R, k, c->gid[cur], dk, dd, base);
+    hipLaunchKernelGGL(k_export_rerank, dim3(rerank_grid(c, ns)), dim3(256), 0, c->stream, x, c->d,
+                       ns, K, R, k, c->gid[cur], dk, dd, base, static_cast<const int32_t*>(nullptr));
   }
   HIPCHK(c, hipGetLastError());
   return 0;

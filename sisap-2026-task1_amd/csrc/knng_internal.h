@@ -153,6 +153,10 @@ inline int grid_for(const knng_ctx* c, int64_t work, int per_block, int blocks_p
   return int(g);
 }
 
+// k_export_rerank's grid: four rows per block, a multiple of 8 blocks (the
+// kernel gives each XCD its own eighth of the rows)
+inline int rerank_grid(const knng_ctx* c, int64_t n) { return (grid_for(c, n, 4, 16) + 7) & ~7; }
+
 // Cells of the k-means partition for n rows (0: plain RP trees).
 inline int32_t cells_for(const knng_params& p, int64_t n) {
   if (p.cells >= 0) return p.cells;
