@@ -2239,6 +2239,12 @@ struct KmeansBufs {
   float* pbest = nullptr;    // split k_assign partials [pcap]
   int32_t* pidx = nullptr;
   int64_t pcap = 0;
+  int32_t* mlist = nullptr;  // a merging round's [pairs | merged | killed] cells [2 cap]
+  // pinned host staging of the merging rounds
+  int32_t* hbest = nullptr;  // [cap]
+  float* hval = nullptr;     // [cap]
+  float* hw = nullptr;       // [cap]
+  int32_t* hlist = nullptr;  // [2 cap]
 };
 
 // exact sums of the listed rows per cell into B.sums (device); counts on the host
@@ -2333,46 +2339,47 @@ int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* zfin,
                        B.a, int(dead.size()));
     HIPCHK(c, hipGetLastError());
   }
-  std::vector<int32_t> best(static_cast<size_t>(Cs)), pairs, merged_into, killed;
-  std::vector<float> bval(static_cast<size_t>(Cs)), w(static_cast<size_t>(Cs));
+  // One host round trip per round: the download of the round's nearest
+  // cells, values and weights (its sync also retires the previous round's
+  // upload, so the pinned list buffer is free again); the round's pairs,
+  // merged and killed cells go up in one copy, and the sum merge, the
+  // requantisation and the kills follow in stream order.
   std::vector<char> gone(static_cast<size_t>(Cs), 0);
   for (int round = 0; round < 64 && live.size() >= 2; ++round) {
     if ((rc = assign_launch(c, B, B.z, Cs, 0, 1, B.z, B.w, Cs, B.a, true, B.sc))) return rc;
-    HIPCHK(c, hipMemcpyAsync(best.data(), B.a, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(bval.data(), B.sc, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(w.data(), B.w, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(B.hbest, B.a, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(B.hval, B.sc, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(B.hw, B.w, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    pairs.clear();
-    merged_into.clear();
-    killed.clear();
+    int m = 0;
     for (const int32_t j : live) {
-      const int32_t k = best[size_t(j)];
-      if (k <= j || k >= Cs || best[size_t(k)] != j) continue;
-      if (!(bval[size_t(j)] * w[size_t(j)] > c->p.cell_merge)) continue;
-      pairs.push_back(j);
-      pairs.push_back(k);
-      merged_into.push_back(j);
-      killed.push_back(k);
+      const int32_t k = B.hbest[j];
+      if (k <= j || k >= Cs || B.hbest[k] != j) continue;
+      if (!(B.hval[j] * B.hw[j] > c->p.cell_merge)) continue;
+      B.hlist[2 * m] = j;   // pairs (j, k): mutual nearest cells are disjoint, so m <= Cs / 2
+      B.hlist[2 * m + 1] = k;
+      ++m;
       cnt[size_t(j)] += cnt[size_t(k)];
       gone[size_t(k)] = 1;
     }
-    if (pairs.empty()) break;
+    if (m == 0) break;
+    for (int i = 0; i < m; ++i) {
+      B.hlist[2 * m + i] = B.hlist[2 * i];       // merged into
+      B.hlist[3 * m + i] = B.hlist[2 * i + 1];   // killed
+    }
     std::vector<int32_t> next;
     for (const int32_t j : live)
       if (!gone[size_t(j)]) next.push_back(j);
     live.swap(next);
-    const int m = int(killed.size());
-    HIPCHK(c, hipMemcpyAsync(B.a, pairs.data(), pairs.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(B.mlist, B.hlist, size_t(4 * m) * 4, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_merge_sums, dim3(std::min(m, c->num_cu * 16)), dim3(256), 0, c->stream, B.sums,
-                       d, B.a, m);
+                       d, B.mlist, m);
+    hipLaunchKernelGGL(k_quant_centroids<int64_t>, dim3(std::min(m, c->num_cu * 16)), dim3(256), 0,
+                       c->stream, B.sums, d, B.mlist + 2 * m, m, B.z, B.w);
+    hipLaunchKernelGGL(k_kill_cells, dim3((m + 255) / 256), dim3(256), 0, c->stream, B.w, B.mlist + 3 * m, m);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if ((rc = requantize(c, B, merged_into, B.z, B.w))) return rc;
-    HIPCHK(c, hipMemcpyAsync(B.a, killed.data(), killed.size() * 4, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_kill_cells, dim3((m + 255) / 256), dim3(256), 0, c->stream, B.w, B.a, m);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
   }
+  std::vector<float> w(static_cast<size_t>(Cs));
   trace_mark(c, "kmeans merging");
   // 4. pruning
   {
@@ -2437,6 +2444,14 @@ void free_kmeans(KmeansBufs& B, int8_t** merge_rows) {
   dfree(&B.ids);
   dfree(&B.pbest);
   dfree(&B.pidx);
+  dfree(&B.mlist);
+  for (void* h : {static_cast<void*>(B.hbest), static_cast<void*>(B.hval), static_cast<void*>(B.hw),
+                  static_cast<void*>(B.hlist)})
+    if (h) (void)hipHostFree(h);
+  B.hbest = nullptr;
+  B.hval = nullptr;
+  B.hw = nullptr;
+  B.hlist = nullptr;
   dfree(merge_rows);
 }
 
@@ -2466,9 +2481,18 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
       (rc = dalloc(c, &B.ids, size_t(cap), "seed ids")) ||
       (rc = dalloc(c, &B.pbest, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
       (rc = dalloc(c, &B.pidx, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
-      (rc = dalloc(c, &merge_rows, size_t(cap) * d, "merge rows"))) {
+      (rc = dalloc(c, &merge_rows, size_t(cap) * d, "merge rows")) ||
+      (rc = dalloc(c, &B.mlist, size_t(2 * cap), "merge lists"))) {
     free_kmeans(B, &merge_rows);
     return rc;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&B.hbest), size_t(cap) * 4, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&B.hval), size_t(cap) * 4, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&B.hw), size_t(cap) * 4, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&B.hlist), size_t(2 * cap) * 4, hipHostMallocDefault) !=
+          hipSuccess) {
+    free_kmeans(B, &merge_rows);
+    return fail(c, KNNG_ENOMEM, "hipHostMalloc failed (k-means merge staging)");
   }
   std::vector<std::vector<int8_t>> zs(G);
   std::vector<std::vector<float>> ws(G);
