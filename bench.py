@@ -283,6 +283,7 @@ def main():
                 if traffic is not None and jms > 0 else None,
                 "kernel": "k_join", "join_shape": st["join_variant"], "launches": it,
                 "algorithmic_bytes_per_launch": round(jb / max(it, 1)),
+                "algorithmic_bytes_per_iter": [int(v) for v in st["join_bytes"][:it]],
                 "avg_launch_ms": round(jms / max(it, 1), 3)}
 
     cpu = None

@@ -1589,7 +1589,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   // the per-chunk bounds of the sketch and row gathers into straight-line code
   if (DIM > 0) d = DIM;
   const int words = DIM > 0 ? DIM / 64 : d >> 6, nch = DIM > 0 ? DIM / 16 : d >> 4;
-  unsigned long long my_upd = 0, my_cand = 0, my_sel = 0, my_ins = 0;
+  unsigned long long my_upd = 0, my_cand = 0, my_sel = 0, my_ins = 0, my_skc = 0;
 
   // Schedule: blocks are dealt to the 8 XCDs round-robin, so XCD x = block % 8
   // takes the contiguous slice [x*n/8, (x+1)*n/8) of the processing order and
@@ -1662,8 +1662,10 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     __syncthreads();
     const int U = S.cnt[0];
     // ---- 3: sketch similarity, 8 lanes per candidate, 8 candidates per group
-    //         in flight (16 B per lane each)
-    {
+    //         in flight (16 B per lane each).  With at most M candidates the
+    //         threshold is 0 and every candidate is re-scored whatever its
+    //         sketch, so the sketches are not gathered.
+    if (U > M) {
       constexpr int kSkPer = SKPER;
       const int sub = tid & 7, grp = tid >> 3;
       uint64_t o0 = 0, o1 = 0;
@@ -1737,7 +1739,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
       const int thr = S.cnt[3];
       for (int u0 = 0; u0 < U; u0 += 256) {
         const int u = u0 + tid;
-        const bool take = u < U && ch[u] >= thr;
+        const bool take = u < U && (U <= M || ch[u] >= thr);   // U <= M: every candidate
         const int s = wave_append(take, &S.cnt[1]);
         if (take) sel_idx[s] = uint16_t(u);
       }
@@ -1873,6 +1875,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
       if (lane == 0) {
         my_upd += __popcll(m);
         my_cand += U;
+        if (U > M) my_skc += U;
         my_sel += NS;
         my_ins += S.cnt[4];
         if (segc && tid == 0) {
@@ -1898,30 +1901,32 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     atomicAdd(&segc[2 * seg_cur + 1], seg_ins);
   }
   if (tid == 0) {
-    unsigned long long* c = counters + 4 * (blockIdx.x % kNumCounters);
+    unsigned long long* c = counters + kCounterFields * (blockIdx.x % kNumCounters);
     atomicAdd(c + 0, my_upd);
     atomicAdd(c + 1, my_cand);
     atomicAdd(c + 2, my_sel);
     atomicAdd(c + 3, my_ins);
+    atomicAdd(c + 4, my_skc);
   }
 }
 
-// Sums the per-block counters of a k_join launch into out[0..3] and leaves
-// the counters and the row-queue counters behind them at zero for the next
-// launch (no memset node, so the sequence can be captured in a graph).
+// Sums the per-block counters of a k_join launch into out[0..kCounterFields)
+// and leaves the counters and the row-queue counters behind them at zero for
+// the next launch (no memset node, so the sequence can be captured in a graph).
 __global__ void k_sum_counters(unsigned long long* __restrict__ c,
                                unsigned long long* __restrict__ out) {
-  __shared__ unsigned long long part[4][256];
-  unsigned long long a[4] = {0, 0, 0, 0};
+  constexpr int F = kCounterFields;
+  __shared__ unsigned long long part[F][256];
+  unsigned long long a[F] = {};
   for (int j = threadIdx.x; j < kNumCounters; j += 256)
-    for (int f = 0; f < 4; ++f) {
-      a[f] += c[4 * j + f];
-      c[4 * j + f] = 0;
+    for (int f = 0; f < F; ++f) {
+      a[f] += c[F * j + f];
+      c[F * j + f] = 0;
     }
-  if (threadIdx.x < kRowQueueWords) c[4 * kNumCounters + threadIdx.x] = 0;
-  for (int f = 0; f < 4; ++f) part[f][threadIdx.x] = a[f];
+  if (threadIdx.x < kRowQueueWords) c[F * kNumCounters + threadIdx.x] = 0;
+  for (int f = 0; f < F; ++f) part[f][threadIdx.x] = a[f];
   __syncthreads();
-  if (threadIdx.x < 4) {
+  if (threadIdx.x < F) {
     unsigned long long s = 0;
     for (int j = 0; j < 256; ++j) s += part[threadIdx.x][j];
     out[threadIdx.x] = s;
@@ -2674,7 +2679,7 @@ int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const in
   // grid: a multiple of the 8 XCDs (the schedule needs it), four blocks per
   // CU; blocks beyond what is resident just find their row queue drained
   const int grid = std::max(8, (c->num_cu * 4) / 8 * 8);
-  unsigned long long* rowq = c->counters + 4 * kNumCounters;
+  unsigned long long* rowq = c->counters + kCounterFields * kNumCounters;
   const uint64_t* fin = c->p.incremental ? c->fresh[a] : nullptr;
 #define KNNG_JOIN_ARGS                                                                          \
   c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order, \
@@ -2834,12 +2839,12 @@ int knng_create(knng_ctx** out, int32_t device, const knng_params* p, int64_t n)
   if ((rc = dalloc(c, &c->q, size_t(n) * c->d, "int8 rows"))) return bail(rc);
   if ((rc = dalloc(c, &c->sk, size_t(n) * c->words, "sketches"))) return bail(rc);
   if (p->keep_fp32 && (rc = dalloc(c, &c->x, size_t(n) * c->d, "float32 rows"))) return bail(rc);
-  if ((rc = dalloc(c, &c->counters, 4 * kNumCounters + kRowQueueWords, "counters"))) return bail(rc);
+  if ((rc = dalloc(c, &c->counters, kCounterFields * kNumCounters + kRowQueueWords, "counters"))) return bail(rc);
   // zero once; k_sum_counters re-zeroes after every join launch
-  if (hipMemset(c->counters, 0, (4 * kNumCounters + kRowQueueWords) * sizeof(unsigned long long)) !=
+  if (hipMemset(c->counters, 0, (kCounterFields * kNumCounters + kRowQueueWords) * sizeof(unsigned long long)) !=
       hipSuccess)
     return bail(fail(c, KNNG_EHIP, "hipMemset of the join counters failed"));
-  if ((rc = dalloc(c, &c->csum, 4, "counter sums"))) return bail(rc);
+  if ((rc = dalloc(c, &c->csum, kCounterFields, "counter sums"))) return bail(rc);
   c->st.n = n;
   c->st.dim = c->d;
   *out = c;
@@ -3274,7 +3279,7 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   if (int rc = launch_join(c, c->join_variant, a, b, order, nullptr, nullptr, 0, nullptr)) return rc;
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(256), 0, c->stream, c->counters, c->csum);
-  unsigned long long sums[4];
+  unsigned long long sums[kCounterFields];
   HIPCHK(c, hipMemcpyAsync(sums, c->csum, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0;
@@ -3289,10 +3294,11 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
     c->st.join_rescored[it] = int64_t(sums[2]);
     c->st.join_inserted[it] = int64_t(sums[3]);
     c->st.insert_rates[it] = float(sums[3]) / float(size_t(n) * size_t(K));
-    // algorithmic bytes: own list + own row/sketch, K neighbour id rows,
-    // one sketch per unique candidate, one row per re-scored candidate, writes
+    // algorithmic bytes: own list + own row/sketch, K neighbour id rows, one
+    // sketch per candidate of a row that gathers them, one row per re-scored
+    // candidate, writes
     c->st.join_bytes[it] = double(n) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) +
-                           double(sums[1]) * 8.0 * c->words + double(sums[2]) * c->d;
+                           double(sums[4]) * 8.0 * c->words + double(sums[2]) * c->d;
   }
   c->st.iterations = it + 1;
   c->st.ms_join += ms;
@@ -3565,7 +3571,7 @@ int enqueue_packed_iteration(knng_ctx* c, PackedLoop& L, int a, int b) {
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(L.snap_h, L.snap, size_t(2 * S) * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(L.csum_h, c->csum, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+  HIPCHK(c, hipMemcpyAsync(L.csum_h, c->csum, kCounterFields * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            c->stream));
   return 0;
 }
@@ -3596,7 +3602,7 @@ int build_packed(knng_ctx* c, std::vector<int32_t>* iters, std::vector<int32_t>*
     return bail(rc);
   if (hipHostMalloc(&L.act_h, size_t(S) * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&L.snap_h, size_t(2 * S) * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&L.csum_h, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+      hipHostMalloc(&L.csum_h, kCounterFields * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
     return bail(fail(c, KNNG_ENOMEM, "hipHostMalloc failed for the packed join loop"));
   HIPCHK(c, hipMemsetAsync(L.segc, 0, size_t(2 * S) * sizeof(unsigned long long), c->stream));
   for (int s = 0; s < S; ++s) L.act_h[s] = 1;
@@ -3658,7 +3664,7 @@ int build_packed(knng_ctx* c, std::vector<int32_t>* iters, std::vector<int32_t>*
       c->st.insert_rates[it] = float(L.csum_h[3]) / float(size_t(c->n) * size_t(K));
       // algorithmic bytes as knng_join_iteration counts them, over the rows processed
       c->st.join_bytes[it] = double(rows) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) +
-                             double(L.csum_h[1]) * 8.0 * c->words + double(L.csum_h[2]) * c->d;
+                             double(L.csum_h[4]) * 8.0 * c->words + double(L.csum_h[2]) * c->d;
     }
     c->st.iterations = it + 1;
     c->st.ms_join += ms;
