@@ -872,64 +872,104 @@ __global__ __launch_bounds__(256) void k_scatter_rows(const int8_t* __restrict__
 }
 
 // Centroid of cell c from its exact sum S_c (int8 rows or int64 sums):
-// z_k = trunc(S_k * 127 / max_k |S_k|), w = float(1 / sqrt(sum z_k^2)); an
-// all-zero sum gives z = 0, w = 0.  One block per listed cell; in place is
-// allowed (every S value is read before any z value is written).
+// z = trunc(S * 127 / max |S|), *w = float(1 / sqrt(sum z^2)) for one cell's
+// sum row S (d <= 1024) by the whole block; an all-zero sum gives z = 0,
+// w = 0.  In place is allowed (every S value is read before any z value is
+// written).
+template <typename TS>
+__device__ __forceinline__ void quant_cell(const TS* S, int d, int8_t* z, float* w, long long* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int64_t v[4];
+  long long mx = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int k = tid + 256 * u;
+    v[u] = k < d ? int64_t(S[k]) : 0;
+    const long long av = v[u] < 0 ? -v[u] : v[u];
+    mx = av > mx ? av : mx;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const long long ot = __shfl_xor(mx, o);
+    mx = ot > mx ? ot : mx;
+  }
+  if (lane == 0) red[wv] = mx;
+  __syncthreads();
+  long long M = red[0];
+  for (int u = 1; u < 4; ++u) M = red[u] > M ? red[u] : M;
+  __syncthreads();
+  long long ss = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int k = tid + 256 * u;
+    if (k < d) {
+      const int8_t zk = M ? int8_t((v[u] * 127) / M) : int8_t(0);
+      z[k] = zk;
+      ss += int(zk) * int(zk);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
+  if (lane == 0) red[wv] = ss;
+  __syncthreads();
+  if (tid == 0) {
+    const long long t = red[0] + red[1] + red[2] + red[3];
+    *w = t > 0 ? float(1.0 / sqrt(double(t))) : 0.0f;
+  }
+  __syncthreads();
+}
+
+// z_k, w_k from S_k (quant_cell) for the listed cells (all m if cells is
+// null); one block per cell, 256 threads
 template <typename TS>
 __global__ __launch_bounds__(256) void k_quant_centroids(const TS* S, int d, const int32_t* __restrict__ cells,
                                                          int m, int8_t* z, float* __restrict__ w) {
   __shared__ long long red[4];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int i = blockIdx.x; i < m; i += gridDim.x) {
     const int64_t cc = cells ? cells[i] : i;
-    int64_t v[4];
-    long long mx = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = tid + 256 * u;
-      v[u] = k < d ? int64_t(S[cc * d + k]) : 0;
-      const long long av = v[u] < 0 ? -v[u] : v[u];
-      mx = av > mx ? av : mx;
-    }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-      const long long ot = __shfl_xor(mx, o);
-      mx = ot > mx ? ot : mx;
-    }
-    if (lane == 0) red[wv] = mx;
-    __syncthreads();
-    long long M = red[0];
-    for (int u = 1; u < 4; ++u) M = red[u] > M ? red[u] : M;
-    __syncthreads();
-    long long ss = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = tid + 256 * u;
-      if (k < d) {
-        const int8_t zk = M ? int8_t((v[u] * 127) / M) : int8_t(0);
-        z[cc * d + k] = zk;
-        ss += int(zk) * int(zk);
-      }
-    }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
-    if (lane == 0) red[wv] = ss;
-    __syncthreads();
-    if (tid == 0) {
-      const long long t = red[0] + red[1] + red[2] + red[3];
-      w[cc] = t > 0 ? float(1.0 / sqrt(double(t))) : 0.0f;
-    }
-    __syncthreads();
+    quant_cell(S + cc * d, d, z + cc * d, w + cc, red);
   }
 }
 
-// S[pairs[2i]] += S[pairs[2i+1]] (int64 rows of d)
-__global__ __launch_bounds__(256) void k_merge_sums(int64_t* __restrict__ S, int d,
-                                                    const int32_t* __restrict__ pairs, int m) {
-  for (int i = blockIdx.x; i < m; i += gridDim.x) {
-    int64_t* dst = S + int64_t(pairs[2 * i]) * d;
-    const int64_t* src = S + int64_t(pairs[2 * i + 1]) * d;
-    for (int k = threadIdx.x; k < d; k += 256) dst[k] += src[k];
+// A k-means merging round's decisions from the round's nearest cells: live
+// cell j (weight not NaN) absorbs k = best[j] when k > j, best[k] = j and the
+// cosine bval[j] * w[j] exceeds `merge`; partner[j] = k, else -1.  Mutually
+// nearest pairs are disjoint.  *npairs counts the pairs.
+__global__ __launch_bounds__(256) void k_merge_decide(const int32_t* __restrict__ best,
+                                                      const float* __restrict__ bval,
+                                                      const float* __restrict__ w, int C, float merge,
+                                                      int32_t* __restrict__ partner,
+                                                      unsigned long long* __restrict__ npairs) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= C) return;
+  int32_t p = -1;
+  const float wj = w[j];
+  if (wj == wj) {
+    const int32_t k = best[j];
+    if (k > j && k < C && best[k] == j && bval[j] * wj > merge) p = k;
+  }
+  partner[j] = p;
+  if (p >= 0) atomicAdd(npairs, 1ull);
+}
+
+// The round's merges: S[j] += S[k], cnt[j] += cnt[k], z_j / w_j requantised,
+// w[k] = NaN (dead: never wins an argmax again) for every pair (j, k).
+__global__ __launch_bounds__(256) void k_merge_apply(int64_t* __restrict__ S, int64_t* __restrict__ cnt,
+                                                     const int32_t* __restrict__ partner, int C, int d,
+                                                     int8_t* z, float* __restrict__ w) {
+  __shared__ long long red[4];
+  for (int j = blockIdx.x; j < C; j += gridDim.x) {
+    const int32_t k = partner[j];
+    if (k < 0) continue;
+    int64_t* sj = S + int64_t(j) * d;
+    const int64_t* sk = S + int64_t(k) * d;
+    for (int t = threadIdx.x; t < d; t += 256) sj[t] += sk[t];
+    if (threadIdx.x == 0) {
+      cnt[j] += cnt[k];
+      w[k] = __builtin_nanf("");
+    }
+    __syncthreads();
+    quant_cell(sj, d, z + int64_t(j) * d, w + j, red);
   }
 }
 
@@ -2244,12 +2284,10 @@ struct KmeansBufs {
   float* pbest = nullptr;    // split k_assign partials [pcap]
   int32_t* pidx = nullptr;
   int64_t pcap = 0;
-  int32_t* mlist = nullptr;  // a merging round's [pairs | merged | killed] cells [2 cap]
-  // pinned host staging of the merging rounds
-  int32_t* hbest = nullptr;  // [cap]
-  float* hval = nullptr;     // [cap]
-  float* hw = nullptr;       // [cap]
-  int32_t* hlist = nullptr;  // [2 cap]
+  int32_t* partner = nullptr;           // a merging round's partner of each cell [cap]
+  int64_t* dcnt = nullptr;              // sample rows per cell during merging [cap]
+  unsigned long long* dpairs = nullptr;  // pairs merged since the last check
+  unsigned long long* hpairs = nullptr;  // its pinned host copy
 };
 
 // exact sums of the listed rows per cell into B.sums (device); counts on the host
@@ -2344,45 +2382,39 @@ int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* zfin,
                        B.a, int(dead.size()));
     HIPCHK(c, hipGetLastError());
   }
-  // One host round trip per round: the download of the round's nearest
-  // cells, values and weights (its sync also retires the previous round's
-  // upload, so the pinned list buffer is free again); the round's pairs,
-  // merged and killed cells go up in one copy, and the sum merge, the
-  // requantisation and the kills follow in stream order.
-  std::vector<char> gone(static_cast<size_t>(Cs), 0);
-  for (int round = 0; round < 64 && live.size() >= 2; ++round) {
-    if ((rc = assign_launch(c, B, B.z, Cs, 0, 1, B.z, B.w, Cs, B.a, true, B.sc))) return rc;
-    HIPCHK(c, hipMemcpyAsync(B.hbest, B.a, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(B.hval, B.sc, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(B.hw, B.w, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
+  // Rounds run on the device: the cells' nearest live cells (k_assign with
+  // self exclusion), the round's mutually nearest pairs (k_merge_decide) and
+  // their merges (k_merge_apply), counts and liveness (w not NaN) kept on the
+  // device.  A round that merges nothing leaves the state as it was, so every
+  // later round merges nothing too: the host checks the pair count every 8
+  // rounds and stops at the first check that saw none, with the state of the
+  // loop that stops at the first empty round.
+  if (live.size() >= 2) {
+    HIPCHK(c, hipMemcpyAsync(B.dcnt, cnt.data(), size_t(Cs) * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(B.dpairs, 0, sizeof(unsigned long long), c->stream));
+    for (int round = 0; round < 64; ++round) {
+      if ((rc = assign_launch(c, B, B.z, Cs, 0, 1, B.z, B.w, Cs, B.a, true, B.sc))) return rc;
+      hipLaunchKernelGGL(k_merge_decide, dim3((Cs + 255) / 256), dim3(256), 0, c->stream, B.a, B.sc, B.w, Cs,
+                         c->p.cell_merge, B.partner, B.dpairs);
+      hipLaunchKernelGGL(k_merge_apply, dim3(std::min(Cs, c->num_cu * 16)), dim3(256), 0, c->stream, B.sums,
+                         B.dcnt, B.partner, Cs, d, B.z, B.w);
+      HIPCHK(c, hipGetLastError());
+      if ((round & 7) == 7) {
+        HIPCHK(c, hipMemcpyAsync(B.hpairs, B.dpairs, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (*B.hpairs == 0) break;
+        HIPCHK(c, hipMemsetAsync(B.dpairs, 0, sizeof(unsigned long long), c->stream));
+      }
+    }
+    // counts and liveness back on the host (live stays in increasing order)
+    std::vector<float> wl(static_cast<size_t>(Cs));
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), B.dcnt, size_t(Cs) * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(wl.data(), B.w, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    int m = 0;
-    for (const int32_t j : live) {
-      const int32_t k = B.hbest[j];
-      if (k <= j || k >= Cs || B.hbest[k] != j) continue;
-      if (!(B.hval[j] * B.hw[j] > c->p.cell_merge)) continue;
-      B.hlist[2 * m] = j;   // pairs (j, k): mutual nearest cells are disjoint, so m <= Cs / 2
-      B.hlist[2 * m + 1] = k;
-      ++m;
-      cnt[size_t(j)] += cnt[size_t(k)];
-      gone[size_t(k)] = 1;
-    }
-    if (m == 0) break;
-    for (int i = 0; i < m; ++i) {
-      B.hlist[2 * m + i] = B.hlist[2 * i];       // merged into
-      B.hlist[3 * m + i] = B.hlist[2 * i + 1];   // killed
-    }
-    std::vector<int32_t> next;
-    for (const int32_t j : live)
-      if (!gone[size_t(j)]) next.push_back(j);
-    live.swap(next);
-    HIPCHK(c, hipMemcpyAsync(B.mlist, B.hlist, size_t(4 * m) * 4, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_merge_sums, dim3(std::min(m, c->num_cu * 16)), dim3(256), 0, c->stream, B.sums,
-                       d, B.mlist, m);
-    hipLaunchKernelGGL(k_quant_centroids<int64_t>, dim3(std::min(m, c->num_cu * 16)), dim3(256), 0,
-                       c->stream, B.sums, d, B.mlist + 2 * m, m, B.z, B.w);
-    hipLaunchKernelGGL(k_kill_cells, dim3((m + 255) / 256), dim3(256), 0, c->stream, B.w, B.mlist + 3 * m, m);
-    HIPCHK(c, hipGetLastError());
+    live.clear();
+    for (int k = 0; k < Cs; ++k)
+      if (wl[size_t(k)] == wl[size_t(k)]) live.push_back(k);
   }
   std::vector<float> w(static_cast<size_t>(Cs));
   trace_mark(c, "kmeans merging");
@@ -2449,14 +2481,11 @@ void free_kmeans(KmeansBufs& B, int8_t** merge_rows) {
   dfree(&B.ids);
   dfree(&B.pbest);
   dfree(&B.pidx);
-  dfree(&B.mlist);
-  for (void* h : {static_cast<void*>(B.hbest), static_cast<void*>(B.hval), static_cast<void*>(B.hw),
-                  static_cast<void*>(B.hlist)})
-    if (h) (void)hipHostFree(h);
-  B.hbest = nullptr;
-  B.hval = nullptr;
-  B.hw = nullptr;
-  B.hlist = nullptr;
+  dfree(&B.partner);
+  dfree(&B.dcnt);
+  dfree(&B.dpairs);
+  if (B.hpairs) (void)hipHostFree(B.hpairs);
+  B.hpairs = nullptr;
   dfree(merge_rows);
 }
 
@@ -2487,17 +2516,16 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
       (rc = dalloc(c, &B.pbest, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
       (rc = dalloc(c, &B.pidx, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
       (rc = dalloc(c, &merge_rows, size_t(cap) * d, "merge rows")) ||
-      (rc = dalloc(c, &B.mlist, size_t(2 * cap), "merge lists"))) {
+      (rc = dalloc(c, &B.partner, size_t(cap), "merge partners")) ||
+      (rc = dalloc(c, &B.dcnt, size_t(cap), "merge counts")) ||
+      (rc = dalloc(c, &B.dpairs, 1, "merge pair count"))) {
     free_kmeans(B, &merge_rows);
     return rc;
   }
-  if (hipHostMalloc(reinterpret_cast<void**>(&B.hbest), size_t(cap) * 4, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&B.hval), size_t(cap) * 4, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&B.hw), size_t(cap) * 4, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&B.hlist), size_t(2 * cap) * 4, hipHostMallocDefault) !=
-          hipSuccess) {
+  if (hipHostMalloc(reinterpret_cast<void**>(&B.hpairs), sizeof(unsigned long long), hipHostMallocDefault) !=
+      hipSuccess) {
     free_kmeans(B, &merge_rows);
-    return fail(c, KNNG_ENOMEM, "hipHostMalloc failed (k-means merge staging)");
+    return fail(c, KNNG_ENOMEM, "hipHostMalloc failed (k-means merge pair count)");
   }
   std::vector<std::vector<int8_t>> zs(G);
   std::vector<std::vector<float>> ws(G);
