@@ -2249,7 +2249,8 @@ int quantize_rows(knng_ctx* c, const float* dx, int64_t row0, int64_t nrows) {
 //  3. merging rounds: every live cell finds its nearest other live cell
 //     (argmax_k float(<z_j, z_k>) * w_k); mutual pairs whose cosine
 //     (that value * w_j) exceeds cell_merge merge into the lower index,
-//     summing their sample sums and counts; repeat until no pair merges;
+//     summing their sample sums and counts; repeat until no pair merges
+//     (at most 64 rounds);
 //  4. pruning: cells with fewer sample rows than cell_prune * the median
 //     cell's (upper median) are dropped;
 //  5. cell_full_iters Lloyd rounds over all rows, then the final assignment.
