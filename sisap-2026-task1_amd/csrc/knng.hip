@@ -973,6 +973,55 @@ __global__ __launch_bounds__(256) void k_merge_apply(int64_t* __restrict__ S, in
   }
 }
 
+// Cell member lists of an assignment on the device (k_cell_count, then
+// these two): offs[c] = sum of cnt[c'] for c' < c (one block), then row
+// off + j * stride of list position j at members[offs[a[j]] + k].  The order
+// within a cell is arbitrary: the cell sums over it are exact integer sums.
+__global__ __launch_bounds__(1024) void k_count_offsets(const int32_t* __restrict__ cnt, int C,
+                                                        int64_t* __restrict__ offs) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x, per = (C + 1023) / 1024;
+  const int b = t * per, e = min(C, b + per);
+  int64_t sum = 0;
+  for (int k = b; k < e; ++k) sum += cnt[k];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int64_t v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int64_t run = part[t] - sum;
+  if (t == 0) offs[0] = 0;
+  for (int k = b; k < e; ++k) {
+    run += cnt[k];
+    offs[k + 1] = run;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cell_scatter(const int32_t* __restrict__ a, int64_t nrows, int C,
+                                                      int64_t off, int64_t stride,
+                                                      const int64_t* __restrict__ offs,
+                                                      int32_t* __restrict__ cur, int32_t* __restrict__ members) {
+  for (int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x; j < nrows; j += int64_t(gridDim.x) * 256) {
+    const int32_t c = a[j];
+    if (c < 0 || c >= C) continue;
+    members[offs[c] + atomicAdd(&cur[c], 1)] = int32_t(off + j * stride);
+  }
+}
+
+// z_c, w_c from S_c (quant_cell) for every cell with cnt[c] > 0; an empty
+// cell keeps its centroid
+__global__ __launch_bounds__(256) void k_quant_nonempty(const int64_t* S, int d, const int32_t* __restrict__ cnt,
+                                                        int C, int8_t* z, float* __restrict__ w) {
+  __shared__ long long red[4];
+  for (int c = blockIdx.x; c < C; c += gridDim.x) {
+    if (cnt[c] == 0) continue;
+    quant_cell(S + int64_t(c) * d, d, z + int64_t(c) * d, w + c, red);
+  }
+}
+
 // w[cells[i]] = NaN: a dead cell never wins an argmax
 __global__ void k_kill_cells(float* __restrict__ w, const int32_t* __restrict__ cells, int m) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2033,10 +2082,12 @@ __global__ __launch_bounds__(256) void k_iota(int32_t* __restrict__ v, int64_t n
     v[i] = int32_t(i);
 }
 
-__global__ __launch_bounds__(256) void k_cell_count(const int32_t* __restrict__ cell, int64_t n,
+__global__ __launch_bounds__(256) void k_cell_count(const int32_t* __restrict__ cell, int64_t n, int C,
                                                     int32_t* __restrict__ cnt) {
-  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
-    atomicAdd(&cnt[cell[i]], 1);
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+    const int32_t c = cell[i];
+    if (c >= 0 && c < C) atomicAdd(&cnt[c], 1);
+  }
 }
 
 // nodepos[p] = cellnode[cell of the row at position p] (rows sorted by cell)
@@ -2285,6 +2336,8 @@ struct KmeansBufs {
   float* pbest = nullptr;    // split k_assign partials [pcap]
   int32_t* pidx = nullptr;
   int64_t pcap = 0;
+  int32_t* ccount = nullptr;            // sample rows per cell (device cell lists) [cap]
+  int32_t* ccur = nullptr;              // fill cursors of the device cell lists [cap]
   int32_t* partner = nullptr;           // a merging round's partner of each cell [cap]
   int64_t* dcnt = nullptr;              // sample rows per cell during merging [cap]
   unsigned long long* dpairs = nullptr;  // pairs merged since the last check
@@ -2306,6 +2359,23 @@ int cell_sums(knng_ctx* c, KmeansBufs& B, const std::vector<int32_t>& a, int C, 
   cnt->resize(size_t(C));
   for (int k = 0; k < C; ++k) (*cnt)[size_t(k)] = offs[size_t(k) + 1] - offs[size_t(k)];
   HIPCHK(c, hipStreamSynchronize(c->stream));   // members / offs are host temporaries
+  return 0;
+}
+
+// cell_sums on the device for the assignment a (device, nrows entries): the
+// counts stay in B.ccount, the sums in B.sums; no host round trip
+int cell_sums_dev(knng_ctx* c, KmeansBufs& B, const int32_t* a, int64_t nrows, int C, int64_t off,
+                  int64_t stride) {
+  HIPCHK(c, hipMemsetAsync(B.ccount, 0, size_t(C) * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(B.ccur, 0, size_t(C) * 4, c->stream));
+  hipLaunchKernelGGL(k_cell_count, dim3(grid_for(c, nrows, 256, 8)), dim3(256), 0, c->stream, a, nrows, C,
+                     B.ccount);
+  hipLaunchKernelGGL(k_count_offsets, dim3(1), dim3(1024), 0, c->stream, B.ccount, C, B.offs);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(c, nrows, 256, 8)), dim3(256), 0, c->stream, a, nrows, C,
+                     off, stride, B.offs, B.ccur, B.members);
+  hipLaunchKernelGGL(k_cell_sum, dim3(std::max(1, std::min(C, c->num_cu * 8))), dim3(256), 0, c->stream, c->q,
+                     c->d, B.members, B.offs, C, B.sums);
+  HIPCHK(c, hipGetLastError());
   return 0;
 }
 
@@ -2354,21 +2424,24 @@ int run_kmeans(knng_ctx* c, int g, int Ct, KmeansBufs& B, int8_t* zfin,
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   trace_mark(c, "kmeans seeds");
-  // 2. Lloyd on the sample
-  std::vector<int32_t> a(ns), upd;
-  std::vector<int64_t> cnt;
+  // 2. Lloyd on the sample, on the device; the counts come back once
+  std::vector<int32_t> upd;
+  std::vector<int64_t> cnt(static_cast<size_t>(Cs));
   for (int it = 0;; ++it) {
     if ((rc = assign_launch(c, B, c->q, ns, off, stride, B.z, B.w, Cs, B.a, false, nullptr))) return rc;
-    HIPCHK(c, hipMemcpyAsync(a.data(), B.a, size_t(ns) * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     trace_mark(c, "kmeans sample assign");
-    if ((rc = cell_sums(c, B, a, Cs, off, stride, &cnt))) return rc;
+    if ((rc = cell_sums_dev(c, B, B.a, ns, Cs, off, stride))) return rc;
     if (it == c->p.kmeans_iters) break;
-    upd.clear();
-    for (int k = 0; k < Cs; ++k)
-      if (cnt[size_t(k)]) upd.push_back(k);   // an empty cell keeps its centroid
-    if ((rc = requantize(c, B, upd, B.z, B.w))) return rc;
+    hipLaunchKernelGGL(k_quant_nonempty, dim3(std::min(Cs, c->num_cu * 16)), dim3(256), 0, c->stream, B.sums,
+                       d, B.ccount, Cs, B.z, B.w);
+    HIPCHK(c, hipGetLastError());
     trace_mark(c, "kmeans sample update");
+  }
+  {
+    std::vector<int32_t> cnt32(static_cast<size_t>(Cs));
+    HIPCHK(c, hipMemcpyAsync(cnt32.data(), B.ccount, size_t(Cs) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < Cs; ++k) cnt[size_t(k)] = cnt32[size_t(k)];
   }
   // 3. merging.  Live cells get centroids from their final sample sums; a dead
   // cell's weight is NaN on the device, so it never wins an argmax (float
@@ -2482,6 +2555,8 @@ void free_kmeans(KmeansBufs& B, int8_t** merge_rows) {
   dfree(&B.ids);
   dfree(&B.pbest);
   dfree(&B.pidx);
+  dfree(&B.ccount);
+  dfree(&B.ccur);
   dfree(&B.partner);
   dfree(&B.dcnt);
   dfree(&B.dpairs);
@@ -2517,6 +2592,8 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
       (rc = dalloc(c, &B.pbest, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
       (rc = dalloc(c, &B.pidx, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
       (rc = dalloc(c, &merge_rows, size_t(cap) * d, "merge rows")) ||
+      (rc = dalloc(c, &B.ccount, size_t(cap), "cell counts")) ||
+      (rc = dalloc(c, &B.ccur, size_t(cap), "cell cursors")) ||
       (rc = dalloc(c, &B.partner, size_t(cap), "merge partners")) ||
       (rc = dalloc(c, &B.dcnt, size_t(cap), "merge counts")) ||
       (rc = dalloc(c, &B.dpairs, 1, "merge pair count"))) {
@@ -2593,7 +2670,7 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
     HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, c->cassign + size_t(g) * n, skeys + size_t(g) * n,
                                                  ivals, svals + size_t(g) * n, int(n), 0, bits, c->stream));
     hipLaunchKernelGGL(k_cell_count, dim3(grid_for(c, n, 256, 8)), dim3(256), 0, c->stream,
-                       c->cassign + size_t(g) * n, n, dcnt + size_t(g) * Cm);
+                       c->cassign + size_t(g) * n, n, Cm, dcnt + size_t(g) * Cm);
   }
   HIPCHK(c, hipGetLastError());
   std::vector<int32_t> hcnt(size_t(G) * Cm);
