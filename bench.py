@@ -188,6 +188,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4, help="concurrent bulk launches")
     ap.add_argument("--express-slots", type=int, default=2, help="express-lane slots (batches <= 64)")
     ap.add_argument("--express-cus", type=int, default=0, help="CUs reserved for the express lane")
+    ap.add_argument("--bulk-yield", type=int, default=0,
+                    help="bulk slots kept busy while express requests wait or run (0: no limit)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for the serve workload (0: leave the environment's)")
     args = ap.parse_args()
@@ -444,7 +446,7 @@ def serve_main(args):
     ins_rows, q_rows = xh[:n_ins], xh[n_ins:]
     sp = dict(beam=64, k=15, rerank=1, entry_trees=4, max_batch=args.serve_batch,
               max_slots=args.serve_slots, express_rows=64, express_slots=args.express_slots,
-              express_cus=args.express_cus)
+              express_cus=args.express_cus, bulk_yield=args.bulk_yield)
     srv = knng.Server(b, insert_capacity=n_ins + 16, **sp)
     srv.search(q_rows[:2048])   # warm-up (code objects, slots)
     qoff = ioff = 0

@@ -222,6 +222,9 @@ typedef struct knng_server_params {
   int32_t strict_order;    /* 0 (default): inserts run beside searches on their own thread and
                               stream; 1: an insert waits for the queries submitted before it and
                               a query for the inserts submitted before it */
+  int32_t bulk_yield;      /* while express requests wait or run, start no bulk launch once this
+                              many bulk slots are busy (0, the default: no limit); launches in
+                              flight are never held back, so bulk work always progresses */
 } knng_server_params;
 
 int knng_server_default_params(knng_server_params* sp);

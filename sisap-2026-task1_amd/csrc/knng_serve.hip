@@ -848,6 +848,18 @@ void scheduler(knng_server* s, bool express_lane) {
         busy = true;
       }
     } else {
+      // bulk_yield: while express requests wait or run, at most that many
+      // bulk slots stay busy, so small launches find more free CUs
+      int bulk_busy = 0;
+      bool express_active = !s->express.empty();
+      for (const Slot* x : s->slots) {
+        if (!x->busy) continue;
+        if (x->express)
+          express_active = true;
+        else
+          ++bulk_busy;
+      }
+      const bool hold = s->sp.bulk_yield > 0 && express_active;
       while (!s->queue.empty()) {
         Request* r = s->queue.front();
         if (r->kind == 1) {   // strict_order: the insert runs here, alone
@@ -873,10 +885,12 @@ void scheduler(knng_server* s, bool express_lane) {
         }
         bool fatal = false;
         while (r->next_row < r->nrows) {
+          if (hold && bulk_busy >= s->sp.bulk_yield) break;
           Slot* sl = acquire_slot(s, false, &fatal);
           if (!sl) break;
           take_rows(s, r, sl, s->sp.max_batch);
           launch_parts(s, sl, lk);
+          ++bulk_busy;
           progress = true;
           busy = true;
         }
@@ -923,6 +937,7 @@ int knng_server_default_params(knng_server_params* sp) {
   sp->express_slots = 2;
   sp->express_cus = 0;
   sp->strict_order = 0;
+  sp->bulk_yield = 0;
   return 0;
 }
 
@@ -934,7 +949,7 @@ int knng_server_create(knng_server** out, knng_ctx* c, const knng_server_params*
   if (!c->have_graph || !c->have_forest) return KNNG_ESTATE;
   if (sp->beam < 1 || sp->beam > kBeamMax || sp->k < 1 || sp->k > sp->beam || sp->entry_trees < 1 ||
       sp->max_batch < 1 || sp->max_slots < 1 || sp->max_steps < 0 || sp->insert_capacity < 0 ||
-      sp->express_rows < 0 || sp->express_slots < 0 || sp->express_cus < 0)
+      sp->express_rows < 0 || sp->express_slots < 0 || sp->express_cus < 0 || sp->bulk_yield < 0)
     return KNNG_EINVAL;
   const float* xsrc = c->x_ext ? c->x_ext : c->x;
   if (!xsrc) return KNNG_ESTATE;   // the index keeps float32 rows (rerank, inserts)

@@ -382,7 +382,7 @@ class ServerParams(C.Structure):
         ("max_steps", C.c_int32), ("max_batch", C.c_int32), ("max_slots", C.c_int32),
         ("pad0", C.c_int32), ("insert_capacity", C.c_int64), ("reserve_bytes", C.c_int64),
         ("express_rows", C.c_int32), ("express_slots", C.c_int32), ("express_cus", C.c_int32),
-        ("strict_order", C.c_int32),
+        ("strict_order", C.c_int32), ("bulk_yield", C.c_int32),
     ]
 
 

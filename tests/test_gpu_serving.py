@@ -170,3 +170,24 @@ def test_concurrent_inserts_beside_queries(torch_cuda):
         assert s.size() == 9600
         got, _ = s.search(new)
         assert (got[:, 0] == np.arange(9000, 9600)).mean() > 0.95
+
+
+def test_bulk_yield_answers_unchanged(torch_cuda):
+    """bulk_yield = 1 holds new bulk launches back while express requests wait
+    or run; every request still gets the answer it gets served alone."""
+    say("bulk yield")
+    x = synth.gaussian_mixture(51, 8000, 256, 16)
+    qs = synth.gaussian_mixture(52, 1600, 256, 16)
+    b = _built(x)
+    with knng.Server(b, beam=32, k=10, rerank=1, max_batch=128, max_slots=3, express_rows=64,
+                     express_slots=2, bulk_yield=1) as s:
+        reqs = []
+        for j in range(0, 1600, 200):   # a 160-row bulk request, then five 8-row express ones
+            reqs.append((j, 160, s.submit(qs[j:j + 160])))
+            for u in range(5):
+                a = j + 160 + 8 * u
+                reqs.append((a, 8, s.submit(qs[a:a + 8])))
+        got = [(a, m, s.wait(t, m)[:2]) for a, m, t in reqs]
+        for a, m, (ids, sc) in got:
+            ids1, sc1 = s.search(qs[a:a + m])
+            assert np.array_equal(ids, ids1) and np.array_equal(sc, sc1), (a, m)
