@@ -544,171 +544,12 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
   }
 }
 
-// k_assign at d = 1024 with the k dimension split over two waves per row
-// group: 8 waves, wave (rg, h) holds the A fragments of rows 32rg..32rg+31
-// for k-steps 16h..16h+15 only (64 VGPRs instead of 128), which lets two
-// waves share each SIMD, so one wave's LDS fragment reads overlap the other's
-// MFMAs.  Per centroid tile, each wave of the pair hands the other, through
-// LDS (double-buffered by tile parity), its k-half sums of the 8 accumulator
-// elements the other finishes, and one tile later adds the other's half to
-// its own 8 and runs k_assign's epilogue on them: the pair splits the
-// epilogue evenly.  int32 sums are exact in any order, so the result is
-// k_assign's.
-constexpr int kAssign2Waves = 8;
-// centroid rows in LDS with a 1040-byte pitch: one direct 1 KB load per row,
+// Centroid rows in LDS with a 1040-byte pitch: one direct 1 KB load per row,
 // and the 16 B column reads of 8 consecutive rows cover all 32 banks, so
-// every lane's fragment address is one base register plus an immediate
+// every lane's fragment address is one base register plus an immediate.
 constexpr int kAssignPitch = kMaxDim + 16;
 constexpr int kAssign2StageW = kAssignTile * kAssignPitch;        // weights' offset in a stage
 constexpr int kAssign2StageBytes = kAssign2StageW + 256;
-
-template <bool EXCL>
-__global__ __launch_bounds__(512) void k_assign_ksplit(const int8_t* __restrict__ q, int64_t nrows,
-                                                       int64_t row_off, int64_t row_stride,
-                                                       const int8_t* __restrict__ Z,
-                                                       const float* __restrict__ w, int C,
-                                                       int32_t* __restrict__ out,
-                                                       float* __restrict__ score,
-                                                       float* __restrict__ pbest,
-                                                       int32_t* __restrict__ pidx) {
-  constexpr int d = kMaxDim, KH = kMaxDim / 64;   // k-steps per half
-  __shared__ __attribute__((aligned(1024))) int8_t ztile[2][kAssign2StageBytes];
-  // part[parity][rg][hd]: the other wave's k-half sums of the 8 accumulator
-  // elements that wave (rg, hd) finishes
-  __shared__ __attribute__((aligned(16))) int32_t part[2][4][2][8 * 64];
-  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int rg = wv & 3, h = wv >> 2;
-  const int ntiles_all = (C + kAssignTile - 1) / kAssignTile;
-  const int per_slice = (ntiles_all + int(gridDim.y) - 1) / int(gridDim.y);
-  const int t_lo = int(blockIdx.y) * per_slice;
-  const int ntiles = min(ntiles_all, t_lo + per_slice);
-  const int cl = lane & 31, hi = lane >> 5;
-  // wave h = 1 takes its 32 rows in the order cl ^ 16, so accumulator element
-  // j of either wave is row 16h + 8(j/4) + 4(lane/32) + j%4 of the group for
-  // j < 8, and elements 8..15 of one wave are rows that elements 0..7 of the
-  // other hold: each wave finishes its elements 0..7 and hands over 8..15
-  const int rsw = 16 * h;
-  auto issue = [&](int u) {
-    const int c0 = u * kAssignTile;
-    const int nc = (C - c0) < kAssignTile ? (C - c0) : kAssignTile;
-    int8_t* buf = ztile[u & 1];
-    for (int r = wv; r < nc; r += kAssign2Waves)   // one 1 KB row per load
-      __builtin_amdgcn_global_load_lds(Z + int64_t(c0 + r) * d + 16 * lane, buf + r * kAssignPitch, 16, 0, 0);
-    if (wv == 0 && lane < nc)
-      __builtin_amdgcn_global_load_lds(w + c0 + lane, buf + kAssign2StageW, 4, 0, 0);
-  };
-  for (int64_t r0 = int64_t(blockIdx.x) * kAssignRows; r0 < nrows;
-       r0 += int64_t(gridDim.x) * kAssignRows) {
-    const int64_t arow = r0 + 32 * rg + (cl ^ rsw);
-    const bool arow_ok = arow < nrows;
-    const int8_t* abase = q + (row_off + (arow_ok ? arow : 0) * row_stride) * d + 16 * hi + 32 * KH * h;
-    v4i a[KH];
-#pragma unroll
-    for (int ks = 0; ks < KH; ++ks)
-      a[ks] = arow_ok ? *reinterpret_cast<const v4i*>(abase + 32 * ks) : v4i{0, 0, 0, 0};
-    float best[8];
-    int32_t bidx[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      best[j] = -INFINITY;
-      bidx[j] = 0x7FFFFFFF;
-    }
-    int32_t prev[8];    // own k-half sums of elements 0..7 of the previous tile
-    float w_prev = 0.f;
-    // add the other wave's k half of tile u (part[u & 1]) to prev and fold the
-    // 8 elements into best/bidx with selects; on a partial tile (FULL false)
-    // a column past its cells scores -inf, which never wins
-    auto fold = [&](int u, auto full) {
-      constexpr bool FULL = decltype(full)::value;
-      const int c0 = u * kAssignTile;
-      const int cell = c0 + cl;
-      const bool ok = FULL || cl < C - c0;
-      const int32_t* pp = part[u & 1][rg][h];
-#pragma unroll
-      for (int jj = 0; jj < 8; ++jj) {
-        float sc = float(prev[jj] + pp[jj * 64 + lane]) * w_prev;
-        if constexpr (EXCL) {
-          const bool self = (r0 + 32 * rg + rsw + 8 * (jj >> 2) + 4 * hi + (jj & 3)) == cell;
-          sc = (ok && !self) ? sc : -INFINITY;
-        } else if constexpr (!FULL) {
-          sc = ok ? sc : -INFINITY;
-        }
-        const bool take = sc > best[jj];   // cells arrive in increasing order: ties keep the smaller
-        best[jj] = take ? sc : best[jj];
-        bidx[jj] = take ? cell : bidx[jj];
-      }
-    };
-    auto finish = [&](int u) {
-      if (C - u * kAssignTile >= kAssignTile)
-        fold(u, std::true_type{});
-      else
-        fold(u, std::false_type{});
-    };
-    __syncthreads();   // both stages and both partial buffers free (previous row tile done)
-    if (t_lo < ntiles) issue(t_lo);
-    for (int t = t_lo; t < ntiles; ++t) {
-      __builtin_amdgcn_s_waitcnt(KNNG_VMCNT(0));   // this wave's loads of tile t have landed
-      __syncthreads();   // tile t and part[(t-1) & 1] visible; stage (t+1) & 1 no longer read
-      if (t + 1 < ntiles) issue(t + 1);
-      if (t > t_lo) finish(t - 1);
-      // B fragments of this wave's k half: column cl, bytes 32 kg + 16 hi
-      const int8_t* rb = ztile[t & 1] + cl * kAssignPitch + 16 * hi + 32 * KH * h;
-      v4i b[KH];
-#pragma unroll
-      for (int ks = 0; ks < KH; ++ks) b[ks] = *reinterpret_cast<const v4i*>(rb + 32 * ks);
-      v16i acc;
-#pragma unroll
-      for (int j = 0; j < 16; ++j) acc[j] = 0;
-#pragma unroll
-      for (int ks = 0; ks < KH; ++ks) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks], b[ks], acc, 0, 0, 0);
-      // keep kAhead fragment reads in flight ahead of the MFMAs (left alone,
-      // the scheduler puts each read right before its MFMA and the wave waits
-      // out the LDS latency every 32 cycles)
-      constexpr int kAhead = 6;
-      __builtin_amdgcn_sched_group_barrier(0x100, kAhead, 0);   // DS reads
-#pragma unroll
-      for (int i = 0; i < KH - kAhead; ++i) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);      // MFMA
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);      // DS read
-      }
-      __builtin_amdgcn_sched_group_barrier(0x008, kAhead, 0);
-      // hand the other wave elements 8..15, keep 0..7
-      int32_t* pp = part[t & 1][rg][1 - h];
-#pragma unroll
-      for (int jj = 0; jj < 8; ++jj) {
-        pp[jj * 64 + lane] = acc[8 + jj];
-        prev[jj] = acc[jj];
-      }
-      const float* wt = reinterpret_cast<const float*>(ztile[t & 1] + kAssign2StageW);
-      w_prev = cl < ((C - t * kAssignTile) < kAssignTile ? (C - t * kAssignTile) : kAssignTile) ? wt[cl] : 0.f;
-    }
-    __syncthreads();   // the last tile's partial sums visible
-    if (ntiles > t_lo) finish(ntiles - 1);
-    // reduce over the 32 lanes that share the accumulator rows
-#pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-#pragma unroll
-      for (int m = 16; m >= 1; m >>= 1) {
-        const float ov = __shfl_xor(best[jj], m);
-        const int32_t oi = __shfl_xor(bidx[jj], m);
-        if (ov > best[jj] || (ov == best[jj] && oi < bidx[jj])) {
-          best[jj] = ov;
-          bidx[jj] = oi;
-        }
-      }
-      const int64_t row = r0 + 32 * rg + rsw + 8 * (jj >> 2) + 4 * hi + (jj & 3);
-      if (cl == 0 && row < nrows) {
-        if (gridDim.y > 1) {
-          pbest[int64_t(blockIdx.y) * nrows + row] = best[jj];
-          pidx[int64_t(blockIdx.y) * nrows + row] = bidx[jj];
-        } else {
-          out[row] = bidx[jj];
-          if (score) score[row] = best[jj];
-        }
-      }
-    }
-  }
-}
 
 // k_assign at d = 1024 with two blocks per CU: 4 waves per block, each with
 // the A fragments of 32 rows over all of k (128 VGPRs), centroid tiles
@@ -766,11 +607,18 @@ __global__ __launch_bounds__(256, 2) void k_assign_wide(const int8_t* __restrict
       const bool ok = FULL || cl < C - c0;
       const float* wt = reinterpret_cast<const float*>(ztile[t & 1] + kAssign2StageW);
       const float wc = ok ? wt[cl] : 0.f;
+      // self exclusion: this lane's column is the row of element j exactly
+      // when its offset from the lane's first row equals 8 (j / 4) + j % 4
+      int self_at = -1;
+      if constexpr (EXCL) {
+        const int64_t off = int64_t(cell) - (r0 + 32 * wv + 4 * hi);
+        self_at = (off >= 0 && off < 32) ? int(off) : -1;
+      }
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         float sc = float(acc[j]) * wc;
         if constexpr (EXCL) {
-          const bool self = (r0 + 32 * wv + 8 * (j >> 2) + 4 * hi + (j & 3)) == cell;
+          const bool self = self_at == 8 * (j >> 2) + (j & 3);
           sc = (ok && !self) ? sc : -INFINITY;
         } else if constexpr (!FULL) {
           sc = ok ? sc : -INFINITY;
@@ -2251,14 +2099,12 @@ int launch_assign(hipStream_t stream, const int8_t* q, int64_t nrows, int d, con
   const int64_t cap = int64_t(num_cu) * 8;
   if (g > cap) g = cap;
   const int S = (pbest && pidx) ? assign_splits(nrows, C, num_cu, part_cap) : 1;
-  // d = 1024: k_assign_wide (two blocks per CU); with self exclusion (the
-  // cell merging rounds) its extra compares would spill, so those use
-  // k_assign_ksplit
+  // d = 1024: k_assign_wide (two blocks per CU)
   if (d == kMaxDim && !exclude_self)
     hipLaunchKernelGGL((k_assign_wide<false>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
                        row_stride, Z, w, C, out, score, pbest, pidx);
   else if (d == kMaxDim)
-    hipLaunchKernelGGL((k_assign_ksplit<true>), dim3(g, S), dim3(512), 0, stream, q, nrows, row_off,
+    hipLaunchKernelGGL((k_assign_wide<true>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
                        row_stride, Z, w, C, out, score, pbest, pidx);
   else if (exclude_self)
     hipLaunchKernelGGL((k_assign<true, 0>), dim3(g, S), dim3(256), 0, stream, q, nrows, row_off,
