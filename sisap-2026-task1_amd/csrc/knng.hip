@@ -1326,6 +1326,166 @@ __global__ __launch_bounds__(256) void k_leaf_select(int64_t n, int T, const int
 // (src/main.cpp:271-312), done by one thread.
 // ---------------------------------------------------------------------------
 
+// K4 with one wave per row (S2 <= 1024 slots): the row's S2 keys sit in
+// registers, E = S2 / 64 per lane at positions lane * E + e, and are bitonic
+// sorted descending there (within-lane steps as register swaps, the others
+// with lane shuffles), so no block barrier is needed.  The sorted order of a
+// multiset is unique, so the list equals k_init_merge's; so does the padding
+// path (fewer than K distinct candidates), run by lane 0 as there by thread 0.
+template <int S2>
+__global__ __launch_bounds__(256) void k_init_merge_wave(const int8_t* __restrict__ q, int d, int64_t n, int T,
+                                                         int K, const uint64_t* __restrict__ cand,
+                                                         int32_t* __restrict__ gid, int32_t* __restrict__ gsc,
+                                                         const int64_t* __restrict__ seg_base, int nseg) {
+  constexpr int E = S2 / 64;
+  static_assert(E >= 1 && E <= 16, "S2 must be in [64, 1024]");
+  __shared__ uint64_t pool_w[4][2 * kMaxK];
+  __shared__ int32_t pad_id_w[4][kMaxK];
+  __shared__ int32_t pad_sc_w[4][kMaxK];
+  __shared__ int np_w[4];
+  __shared__ uint32_t state_w[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t* pool = pool_w[w];
+  int32_t* pad_id = pad_id_w[w];
+  int32_t* pad_sc = pad_sc_w[w];
+  const int slots = T * kLeafSlots;
+  for (int64_t i = int64_t(blockIdx.x) * 4 + w; i < n; i += int64_t(gridDim.x) * 4) {
+    const uint64_t* src = cand + i * slots;
+    uint64_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int x = lane * E + e;
+      v[e] = x < slots ? src[x] : 0ull;
+    }
+#pragma unroll
+    for (int k = 2; k <= S2; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        if (j < E) {   // partner in this lane's registers
+#pragma unroll
+          for (int e = 0; e < E; ++e)
+            if ((e & j) == 0) {
+              const bool desc = ((lane * E + e) & k) == 0;
+              const uint64_t a = v[e], b = v[e + j];
+              const bool sw = (a < b) == desc;
+              v[e] = sw ? b : a;
+              v[e + j] = sw ? a : b;
+            }
+        } else {       // partner lane lane ^ (j / E), same register
+          const int lj = j / E;
+          const bool lower = (lane & lj) == 0;
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const uint64_t o = __shfl_xor(v[e], lj);
+            const int lo = (lower ? lane : (lane ^ lj)) * E + e;   // the pair's lower position
+            const bool desc = (lo & k) == 0;
+            const uint64_t mx = v[e] > o ? v[e] : o, mn = v[e] > o ? o : v[e];
+            v[e] = (lower == desc) ? mx : mn;
+          }
+        }
+      }
+    }
+    // distinct valid keys in order: the first K go to the list
+    const uint64_t prev_last = __shfl_up(v[E - 1], 1);
+    int flag[E];
+    int cnt = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const uint64_t pv = e > 0 ? v[e - 1] : prev_last;
+      flag[e] = (v[e] != 0ull && (lane * E + e == 0 || pv != v[e])) ? 1 : 0;
+      cnt += flag[e];
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o);
+      if (lane >= o) incl += t;
+    }
+    const int U0 = __shfl(incl, 63);
+    int pos = incl - cnt;
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (flag[e]) {
+        if (pos < K) {
+          gid[i * K + pos] = key_id(v[e]);
+          gsc[i * K + pos] = key_score(v[e]);
+          pool[pos] = v[e];
+        }
+        ++pos;
+      }
+    if (U0 < K) {   // uniform across the wave
+      __builtin_amdgcn_wave_barrier();
+      int64_t sb = 0, sn = n;
+      if (nseg > 0) {
+        const int sg = seg_of(seg_base, nseg, i);
+        sb = seg_base[sg];
+        sn = seg_base[sg + 1] - sb;
+      }
+      // padding ids: src/main.cpp:271-286, in the build's own row numbers
+      if (lane == 0) {
+        uint32_t st = uint32_t(1337 + (i - sb));
+        int np = 0;
+        while (U0 + np < K) {
+          st = xorshift32(st);
+          const int32_t r = int32_t(sb + int64_t(st % uint64_t(sn)));
+          if (r == i) continue;
+          pad_id[np++] = r;
+        }
+        np_w[w] = np;
+        state_w[w] = st;
+      }
+      __builtin_amdgcn_wave_barrier();
+      const int np = np_w[w];
+      for (int p = 0; p < np; ++p) {
+        const int8_t* a = q + i * d;
+        const int8_t* b = q + int64_t(pad_id[p]) * d;
+        int sdot = 0;
+        for (int c = lane; c < (d >> 4); c += 64)
+          sdot = dot16_i8(*reinterpret_cast<const v4i*>(a + 16 * c), *reinterpret_cast<const v4i*>(b + 16 * c),
+                          sdot);
+        sdot = wave_sum_i(sdot);
+        if (lane == 0) pad_sc[p] = sdot;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) {
+        int m = U0;
+        for (int p = 0; p < np; ++p) pool[m++] = pack_key(pad_sc[p], pad_id[p]);
+        for (int a = 1; a < m; ++a) {   // insertion sort, descending
+          const uint64_t val = pool[a];
+          int b = a;
+          while (b > 0 && pool[b - 1] < val) {
+            pool[b] = pool[b - 1];
+            --b;
+          }
+          pool[b] = val;
+        }
+        int kept = 0;
+        int32_t prev = -1;
+        for (int a = 0; a < m && kept < K; ++a) {
+          const int32_t id = key_id(pool[a]);
+          if (id != prev) {
+            gid[i * K + kept] = id;
+            gsc[i * K + kept] = key_score(pool[a]);
+            prev = id;
+            ++kept;
+          }
+        }
+        uint32_t st = state_w[w];
+        while (kept < K) {   // src/main.cpp:303-312
+          st = xorshift32(st);
+          const int32_t r = int32_t(sb + int64_t(st % uint64_t(sn)));
+          if (r != i) {
+            gid[i * K + kept] = r;
+            gsc[i * K + kept] = INT32_MIN;
+            ++kept;
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
 template <int S2>
 __global__ __launch_bounds__(256) void k_init_merge(const int8_t* __restrict__ q, int d, int64_t n,
                                                     int T, int K, const uint64_t* __restrict__ cand,
@@ -3192,14 +3352,16 @@ int knng_build_init(knng_ctx* c) {
   const int mgrid = grid_for(c, n, 1, 32);
   c->cur = 0;
   const int64_t* sb = c->nseg ? c->seg_base : nullptr;
+  // one wave per row up to 1024 slots (16 trees), one block per row beyond
+  const int wgrid = grid_for(c, n, 4, 16);
   if (slots <= 256)
-    hipLaunchKernelGGL(k_init_merge<256>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
+    hipLaunchKernelGGL(k_init_merge_wave<256>, dim3(wgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
                        cand, c->gid[0], c->gsc[0], sb, c->nseg);
   else if (slots <= 512)
-    hipLaunchKernelGGL(k_init_merge<512>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
+    hipLaunchKernelGGL(k_init_merge_wave<512>, dim3(wgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
                        cand, c->gid[0], c->gsc[0], sb, c->nseg);
   else if (slots <= 1024)
-    hipLaunchKernelGGL(k_init_merge<1024>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
+    hipLaunchKernelGGL(k_init_merge_wave<1024>, dim3(wgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
                        cand, c->gid[0], c->gsc[0], sb, c->nseg);
   else if (slots <= 2048)
     hipLaunchKernelGGL(k_init_merge<2048>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
