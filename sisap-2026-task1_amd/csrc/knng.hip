@@ -58,10 +58,19 @@ __global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ x, i
     const float* xr = x + r * d;
     int8_t* qr = q + r * d;
     uint64_t mine = 0;
-    for (int g = 0; g * 256 < d; ++g) {
+    // the row's four 1 KB quarters (d <= 1024) loaded before any is used
+    float4 vq[kMaxDim / 256];
+#pragma unroll
+    for (int g = 0; g < kMaxDim / 256; ++g) {
+      const int col = g * 256 + lane * 4;
+      vq[g] = col < d ? *reinterpret_cast<const float4*>(xr + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int g = 0; g < kMaxDim / 256; ++g) {
+      if (g * 256 >= d) break;
       const int col = g * 256 + lane * 4;
       const bool ok = col < d;
-      float4 v = ok ? *reinterpret_cast<const float4*>(xr + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 v = vq[g];
       const float c0 = q_clamp(v.x), c1 = q_clamp(v.y), c2 = q_clamp(v.z), c3 = q_clamp(v.w);
       const uint32_t packed = uint32_t(uint8_t(int8_t(int(c0)))) |
                               (uint32_t(uint8_t(int8_t(int(c1)))) << 8) |
