@@ -66,7 +66,8 @@ def build_params(knng, args, d):
                                leaf_size=args.leaf, cells=args.cells,
                                cell_partitions=args.partitions, rerank_width=args.rerank_width,
                                incremental=args.incremental, cell_oversample=args.oversample,
-                               max_iters=args.max_iters)
+                               max_iters=args.max_iters, kmeans_iters=args.kmeans_iters,
+                               cell_sample=args.cell_sample)
 
 
 def exact_topk(torch, x, rows, k, chunk=262144):
@@ -164,6 +165,8 @@ def main():
     ap.add_argument("--k-build", type=int, default=50)
     ap.add_argument("--oversample", type=int, default=2, help="k-means seeds per target cell")
     ap.add_argument("--max-iters", type=int, default=2, help="join iteration cap")
+    ap.add_argument("--kmeans-iters", type=int, default=2, help="Lloyd rounds on the k-means sample")
+    ap.add_argument("--cell-sample", type=int, default=8, help="k-means sample: every n-th row")
     ap.add_argument("--incremental", type=int, default=1,
                     help="1: NN-Descent's new/old rule in the join (a 2-hop pair needs a fresh edge)")
     ap.add_argument("--rerank-width", type=int, default=32,
@@ -304,6 +307,7 @@ def main():
                    "trees": args.trees, "leaf": args.leaf, "cells": st["cells"],
                    "rerank_width": args.rerank_width or args.k_build, "incremental": args.incremental,
                    "cell_oversample": args.oversample, "max_iters": args.max_iters,
+                   "kmeans_iters": args.kmeans_iters, "cell_sample": args.cell_sample,
                    "cell_partitions": args.partitions, "parallelism": "single-gpu"},
         "recall_at_15": {"plain": round(rec_plain, 4), "rerank": round(rec_rerank, 4),
                          "rows": int(len(rows_np)), "rerank_rel_err_max": float(rel.max())},
