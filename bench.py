@@ -319,6 +319,7 @@ def main():
                       "join_iter": [round(v, 1) for v in st["ms_join_iter"]],
                       "export_rerank": round(st["ms_export"], 1)},
         "iterations": it, "update_rates": [round(r, 5) for r in st["rates"]],
+        "join_passed_on": st["join_passed_on"],
         "insert_rates": [round(r, 5) for r in st["insert_rates"]],
         "stop_rule": "new-entry rate < 0.01 (reference rule's rate reported as update_rates)",
         "roofline": roofline, "cpu_baseline": cpu,

@@ -95,6 +95,9 @@ typedef struct knng_stats {
   float insert_rates[32];      /* join_inserted / (n * k_build) */
   int32_t join_variant;        /* k_join shape the autotuner chose (0..2) */
   float ms_tune;               /* time spent timing k_join shapes (first build of a shape) */
+  int64_t join_passed_on[32];  /* rows the small-row join kernel passed to the full one (iterations
+                                  under the new/old rule after the first); -1: the full kernel
+                                  ran every row */
 } knng_stats;
 
 typedef struct knng_ctx knng_ctx;

@@ -1661,7 +1661,7 @@ struct JoinScratch {
 
 
 
-template <int HSBITS, int CAP, int OCC, int DOTPER, int SKPER, int DIM>
+template <int HSBITS, int CAP, int OCC, int DOTPER, int SKPER, int DIM, bool SMALL = false>
 __global__ __launch_bounds__(256, OCC) void k_join(
     const int8_t* __restrict__ q, const uint64_t* __restrict__ sk, int d, int64_t n, int K, int M,
     const int32_t* __restrict__ ids_in, const int32_t* __restrict__ sc_in,
@@ -1670,7 +1670,8 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     const int32_t* __restrict__ order,
     const uint64_t* __restrict__ fresh_in, uint64_t* __restrict__ fresh_out,
     const int64_t* __restrict__ nwork_dev, const int64_t* __restrict__ seg_base, int nseg,
-    unsigned long long* __restrict__ segc) {
+    unsigned long long* __restrict__ segc, int32_t* __restrict__ ovf,
+    unsigned long long* __restrict__ ovf_n) {
   // order[0 .. nwork) lists the rows to process (nwork = *nwork_dev if given,
   // else n).  With segments (independent builds packed together, nseg > 0),
   // each row's changed-slot and new-id counts are also added to its build's
@@ -1679,6 +1680,14 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   // fresh_in != nullptr: incremental join -- the pair (i -> n1 -> c) is a
   // candidate only if n1 is fresh in i's list or c is fresh in n1's list
   // (otherwise the previous iteration already considered it)
+  // SMALL: the variant for rows with few candidates (iterations under the
+  // new/old rule after the first): a small hash and candidate array, no
+  // sketches (every row it finishes has at most M candidates, so all are
+  // re-scored, as the full kernel would), more blocks per CU.  A row whose
+  // unique candidates pass lim = min(M, CAP - 256) stops inserting (each of
+  // the 4 waves checks before inserting at most 64 more, so at most CAP are
+  // ever held) and is appended to ovf[*ovf_n]; the full kernel then runs
+  // those rows.  It writes nothing for such a row.
   constexpr int HS = 1 << HSBITS;
   static_assert(HS * 4 >= CAP * 2 + CAP * 4, "selection arrays must fit in the hash area");
   __shared__ __attribute__((aligned(16))) int32_t hs[HS];
@@ -1718,7 +1727,8 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     seg_cur = -1;
     seg_upd = seg_ins = 0;
   }
-  unsigned long long* const qctr = rowq + 32 * xcd;
+  unsigned long long* const qctr = rowq + 32 * xcd + (SMALL ? 1 : 0);
+  const int lim = SMALL ? min(M, CAP - 256) : CAP;
   if (tid == 0) s_row = lo + int64_t(atomicAdd(qctr, 1ull));
   __syncthreads();
   for (int64_t j = s_row; j < hi; j = s_row) {
@@ -1727,7 +1737,8 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     const int64_t i = order ? int64_t(order[j]) : j;
     // ---- 0/1: reset, own list
     for (int j = tid; j < HS; j += 256) hs[j] = -1;
-    for (int j = tid; j <= d; j += 256) S.hist[j] = 0;
+    if constexpr (!SMALL)
+      for (int j = tid; j <= d; j += 256) S.hist[j] = 0;
     if (tid < 5) S.cnt[tid] = 0;
     for (int c = tid; c < nch; c += 256)
       *reinterpret_cast<v4i*>(own_row + 16 * c) = *reinterpret_cast<const v4i*>(q + i * d + 16 * c);
@@ -1760,6 +1771,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
       }
 #pragma unroll
       for (int r = 0; r < kRowsPerWave; ++r) {
+        if (SMALL && S.cnt[0] > lim) break;   // the row overflows (wave-uniform read)
         const bool fresh_id = c[r] >= 0 && hs_insert(hs, HSBITS, c[r]);
         const int slot = wave_append(fresh_id, &S.cnt[0]);
         if (fresh_id) cid[slot] = c[r];
@@ -1767,235 +1779,245 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     }
     __syncthreads();
     const int U = S.cnt[0];
-    // ---- 3: sketch similarity, 8 lanes per candidate, 8 candidates per group
-    //         in flight (16 B per lane each).  With at most M candidates the
-    //         threshold is 0 and every candidate is re-scored whatever its
-    //         sketch, so the sketches are not gathered.
-    if (U > M) {
-      constexpr int kSkPer = SKPER;
-      const int sub = tid & 7, grp = tid >> 3;
-      uint64_t o0 = 0, o1 = 0;
-      if (2 * sub < words) o0 = S.own_sk[2 * sub];
-      if (2 * sub + 1 < words) o1 = S.own_sk[2 * sub + 1];
-      for (int u0 = 0; u0 < U; u0 += 32 * kSkPer) {
-        uint64_t a[kSkPer], b[kSkPer];
-#pragma unroll
-        for (int j = 0; j < kSkPer; ++j) {
-          const int u = u0 + grp + 32 * j;
-          a[j] = b[j] = 0;
-          if (u < U) {
-            const uint64_t* cs = sk + int64_t(cid[u]) * words;
-            if (2 * sub < words) a[j] = cs[2 * sub] ^ o0;
-            if (2 * sub + 1 < words) b[j] = cs[2 * sub + 1] ^ o1;
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < kSkPer; ++j) {
-          const int u = u0 + grp + 32 * j;
-          int pc = __popcll(a[j]) + __popcll(b[j]);
-          pc += __shfl_xor(pc, 1);
-          pc += __shfl_xor(pc, 2);
-          pc += __shfl_xor(pc, 4);
-          if (u < U && sub == 0) {
-            const int h = d - pc;
-            ch[u] = uint16_t(h);
-            atomicAdd(&S.hist[h], 1);
-          }
-        }
-      }
-    }
-    __syncthreads();
-    // ---- 4: threshold (wave 0): largest t with count(h >= t) >= M
-    if (w == 0) {
-      int thr = 0;
-      if (U > M) {
-        const int per = (d + 1 + 63) / 64;
-        const int hi = d - lane * per;            // this lane's bins: hi, hi-1, ..., hi-per+1
-        int local = 0;
-        for (int b = 0; b < per; ++b)
-          if (hi - b >= 0) local += S.hist[hi - b];
-        int inc = local;
-        for (int o = 1; o < 64; o <<= 1) {
-          const int t = __shfl_up(inc, o);
-          if (lane >= o) inc += t;
-        }
-        const int excl = inc - local;
-        const unsigned long long hit = __ballot(inc >= M && excl < M);
-        const int src = __ffsll(hit) - 1;
-        int t_lane = 0;
-        if (lane == src) {
-          int acc = excl;
-          for (int b = 0; b < per; ++b) {
-            const int bin = hi - b;
-            if (bin < 0) break;
-            acc += S.hist[bin];
-            if (acc >= M) {
-              t_lane = bin;
-              break;
+    if (SMALL && U > lim) {   // uniform: the full kernel runs this row
+      if (tid == 0) ovf[atomicAdd(ovf_n, 1ull)] = int32_t(i);
+    } else {
+      if constexpr (SMALL) {
+        // at most M candidates: the selection is the identity
+        for (int u = tid; u < U; u += 256) sel_idx[u] = uint16_t(u);
+        if (tid == 0) S.cnt[1] = U;
+      } else {
+        // ---- 3: sketch similarity, 8 lanes per candidate, 8 candidates per group
+        //         in flight (16 B per lane each).  With at most M candidates the
+        //         threshold is 0 and every candidate is re-scored whatever its
+        //         sketch, so the sketches are not gathered.
+        if (U > M) {
+          constexpr int kSkPer = SKPER;
+          const int sub = tid & 7, grp = tid >> 3;
+          uint64_t o0 = 0, o1 = 0;
+          if (2 * sub < words) o0 = S.own_sk[2 * sub];
+          if (2 * sub + 1 < words) o1 = S.own_sk[2 * sub + 1];
+          for (int u0 = 0; u0 < U; u0 += 32 * kSkPer) {
+            uint64_t a[kSkPer], b[kSkPer];
+    #pragma unroll
+            for (int j = 0; j < kSkPer; ++j) {
+              const int u = u0 + grp + 32 * j;
+              a[j] = b[j] = 0;
+              if (u < U) {
+                const uint64_t* cs = sk + int64_t(cid[u]) * words;
+                if (2 * sub < words) a[j] = cs[2 * sub] ^ o0;
+                if (2 * sub + 1 < words) b[j] = cs[2 * sub + 1] ^ o1;
+              }
+            }
+    #pragma unroll
+            for (int j = 0; j < kSkPer; ++j) {
+              const int u = u0 + grp + 32 * j;
+              int pc = __popcll(a[j]) + __popcll(b[j]);
+              pc += __shfl_xor(pc, 1);
+              pc += __shfl_xor(pc, 2);
+              pc += __shfl_xor(pc, 4);
+              if (u < U && sub == 0) {
+                const int h = d - pc;
+                ch[u] = uint16_t(h);
+                atomicAdd(&S.hist[h], 1);
+              }
             }
           }
         }
-        thr = __shfl(t_lane, src);
-      }
-      if (lane == 0) S.cnt[3] = thr;
-    }
-    __syncthreads();
-    // ---- selection
-    {
-      const int thr = S.cnt[3];
-      for (int u0 = 0; u0 < U; u0 += 256) {
-        const int u = u0 + tid;
-        const bool take = u < U && (U <= M || ch[u] >= thr);   // U <= M: every candidate
-        const int s = wave_append(take, &S.cnt[1]);
-        if (take) sel_idx[s] = uint16_t(u);
-      }
-    }
-    __syncthreads();
-    const int NS = S.cnt[1];
-    // ---- 5: int8 dots, 16 lanes per candidate, 4 candidates per wave step
-    {
-      const int g = lane >> 4, s16 = lane & 15;
-      v4i own[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int chn = s16 + 16 * c;
-        own[c] = chn < nch ? *reinterpret_cast<const v4i*>(own_row + 16 * chn) : v4i{0, 0, 0, 0};
-      }
-      // DOTPER candidates per 16-lane group in flight (64 B per lane each);
-      // at 2 the kernel fits 128 VGPRs, i.e. four blocks per CU
-      constexpr int kDotPer = DOTPER;
-      for (int s0 = 0; s0 < NS; s0 += 16 * kDotPer) {
-        v4i v[kDotPer][4];
-#pragma unroll
-        for (int j = 0; j < kDotPer; ++j) {
-          const int sj = s0 + w * 4 + g + 16 * j;
-          const int8_t* rj = sj < NS ? q + int64_t(cid[sel_idx[sj]]) * d : nullptr;
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const int chn = s16 + 16 * c;
-            v[j][c] = (rj && chn < nch) ? *reinterpret_cast<const v4i*>(rj + 16 * chn) : v4i{0, 0, 0, 0};
+        __syncthreads();
+        // ---- 4: threshold (wave 0): largest t with count(h >= t) >= M
+        if (w == 0) {
+          int thr = 0;
+          if (U > M) {
+            const int per = (d + 1 + 63) / 64;
+            const int hi = d - lane * per;            // this lane's bins: hi, hi-1, ..., hi-per+1
+            int local = 0;
+            for (int b = 0; b < per; ++b)
+              if (hi - b >= 0) local += S.hist[hi - b];
+            int inc = local;
+            for (int o = 1; o < 64; o <<= 1) {
+              const int t = __shfl_up(inc, o);
+              if (lane >= o) inc += t;
+            }
+            const int excl = inc - local;
+            const unsigned long long hit = __ballot(inc >= M && excl < M);
+            const int src = __ffsll(hit) - 1;
+            int t_lane = 0;
+            if (lane == src) {
+              int acc = excl;
+              for (int b = 0; b < per; ++b) {
+                const int bin = hi - b;
+                if (bin < 0) break;
+                acc += S.hist[bin];
+                if (acc >= M) {
+                  t_lane = bin;
+                  break;
+                }
+              }
+            }
+            thr = __shfl(t_lane, src);
+          }
+          if (lane == 0) S.cnt[3] = thr;
+        }
+        __syncthreads();
+        // ---- selection
+        {
+          const int thr = S.cnt[3];
+          for (int u0 = 0; u0 < U; u0 += 256) {
+            const int u = u0 + tid;
+            const bool take = u < U && (U <= M || ch[u] >= thr);   // U <= M: every candidate
+            const int s = wave_append(take, &S.cnt[1]);
+            if (take) sel_idx[s] = uint16_t(u);
           }
         }
-#pragma unroll
-        for (int j = 0; j < kDotPer; ++j) {
-          int dj = 0;
-#pragma unroll
-          for (int c = 0; c < 4; ++c) dj = dot16_i8(v[j][c], own[c], dj);
-#pragma unroll
-          for (int m = 8; m >= 1; m >>= 1) dj += __shfl_xor(dj, m);
-          const int sj = s0 + w * 4 + g + 16 * j;
-          if (s16 == 0 && sj < NS) sel_sc[sj] = dj;
+      }
+      __syncthreads();
+      const int NS = S.cnt[1];
+      // ---- 5: int8 dots, 16 lanes per candidate, 4 candidates per wave step
+      {
+        const int g = lane >> 4, s16 = lane & 15;
+        v4i own[4];
+  #pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int chn = s16 + 16 * c;
+          own[c] = chn < nch ? *reinterpret_cast<const v4i*>(own_row + 16 * chn) : v4i{0, 0, 0, 0};
         }
-      }
-    }
-    __syncthreads();
-    // ---- 6a: survivors (fewer than K own entries precede them).  Keys order
-    // (score desc, id asc); a candidate's id is never an own id, and own ids
-    // are distinct, so key comparisons decide every pair the slot rule would.
-    for (int s0 = 0; s0 < NS; s0 += 256) {
-      const int s = s0 + tid;
-      bool keep = false;
-      if (s < NS) {
-        const uint64_t key = pack_key(sel_sc[s], cid[sel_idx[s]]);
-        int c = 0;
-        for (int k = 0; k < K; ++k) c += S.own_key[k] > key ? 1 : 0;
-        keep = c < K;
-      }
-      const int p = wave_append(keep, &S.cnt[2]);
-      if (keep) surv[p] = uint16_t(s);
-    }
-    __syncthreads();
-    const int P = S.cnt[2];
-    // survivors' keys packed over cid[] (no longer needed once read) when they fit
-    constexpr int kPackCap = CAP / 2, kPackPer = (kPackCap + 255) / 256;
-    uint64_t* pkey = reinterpret_cast<uint64_t*>(cid);
-    const bool packed = P <= kPackCap;
-    if (packed) {
-      uint64_t kk[kPackPer];
-#pragma unroll
-      for (int j = 0; j < kPackPer; ++j) {
-        const int p = tid + 256 * j;
-        kk[j] = 0;
-        if (p < P) {
-          const int sv = surv[p];
-          kk[j] = pack_key(sel_sc[sv], cid[sel_idx[sv]]);
+        // DOTPER candidates per 16-lane group in flight (64 B per lane each);
+        // at 2 the kernel fits 128 VGPRs, i.e. four blocks per CU
+        constexpr int kDotPer = DOTPER;
+        for (int s0 = 0; s0 < NS; s0 += 16 * kDotPer) {
+          v4i v[kDotPer][4];
+  #pragma unroll
+          for (int j = 0; j < kDotPer; ++j) {
+            const int sj = s0 + w * 4 + g + 16 * j;
+            const int8_t* rj = sj < NS ? q + int64_t(cid[sel_idx[sj]]) * d : nullptr;
+  #pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const int chn = s16 + 16 * c;
+              v[j][c] = (rj && chn < nch) ? *reinterpret_cast<const v4i*>(rj + 16 * chn) : v4i{0, 0, 0, 0};
+            }
+          }
+  #pragma unroll
+          for (int j = 0; j < kDotPer; ++j) {
+            int dj = 0;
+  #pragma unroll
+            for (int c = 0; c < 4; ++c) dj = dot16_i8(v[j][c], own[c], dj);
+  #pragma unroll
+            for (int m = 8; m >= 1; m >>= 1) dj += __shfl_xor(dj, m);
+            const int sj = s0 + w * 4 + g + 16 * j;
+            if (s16 == 0 && sj < NS) sel_sc[sj] = dj;
+          }
         }
       }
       __syncthreads();
-#pragma unroll
-      for (int j = 0; j < kPackPer; ++j) {
-        const int p = tid + 256 * j;
-        if (p < P) pkey[p] = kk[j];
+      // ---- 6a: survivors (fewer than K own entries precede them).  Keys order
+      // (score desc, id asc); a candidate's id is never an own id, and own ids
+      // are distinct, so key comparisons decide every pair the slot rule would.
+      for (int s0 = 0; s0 < NS; s0 += 256) {
+        const int s = s0 + tid;
+        bool keep = false;
+        if (s < NS) {
+          const uint64_t key = pack_key(sel_sc[s], cid[sel_idx[s]]);
+          int c = 0;
+          for (int k = 0; k < K; ++k) c += S.own_key[k] > key ? 1 : 0;
+          keep = c < K;
+        }
+        const int p = wave_append(keep, &S.cnt[2]);
+        if (keep) surv[p] = uint16_t(s);
       }
       __syncthreads();
-    }
-    // ---- 6b: ranks
-    for (int e = tid; e < K + P; e += 256) {
-      int32_t sc, id, sl;
-      if (e < K) {
-        sc = S.own_sc[e];
-        id = S.own_id[e];
-        sl = e;
-      } else if (packed) {
-        const uint64_t key = pkey[e - K];
-        sc = key_score(key);
-        id = key_id(key);
-        sl = K;
-      } else {
-        const int s = surv[e - K];
-        sc = sel_sc[s];
-        id = cid[sel_idx[s]];
-        sl = K;
-      }
-      int r = 0;
-      for (int k = 0; k < K; ++k) r += before(S.own_sc[k], S.own_id[k], k, sc, id, sl) ? 1 : 0;
+      const int P = S.cnt[2];
+      // survivors' keys packed over cid[] (no longer needed once read) when they fit
+      constexpr int kPackCap = CAP / 2, kPackPer = (kPackCap + 255) / 256;
+      uint64_t* pkey = reinterpret_cast<uint64_t*>(cid);
+      const bool packed = P <= kPackCap;
       if (packed) {
-        const uint64_t key = pack_key(sc, id);
-        for (int p = 0; p < P && r < K; ++p) r += pkey[p] > key ? 1 : 0;
-      } else {
-        for (int p = 0; p < P && r < K; ++p) {
-          const int s = surv[p];
-          r += before(sel_sc[s], cid[sel_idx[s]], K, sc, id, sl) ? 1 : 0;
+        uint64_t kk[kPackPer];
+  #pragma unroll
+        for (int j = 0; j < kPackPer; ++j) {
+          const int p = tid + 256 * j;
+          kk[j] = 0;
+          if (p < P) {
+            const int sv = surv[p];
+            kk[j] = pack_key(sel_sc[sv], cid[sel_idx[sv]]);
+          }
+        }
+        __syncthreads();
+  #pragma unroll
+        for (int j = 0; j < kPackPer; ++j) {
+          const int p = tid + 256 * j;
+          if (p < P) pkey[p] = kk[j];
+        }
+        __syncthreads();
+      }
+      // ---- 6b: ranks
+      for (int e = tid; e < K + P; e += 256) {
+        int32_t sc, id, sl;
+        if (e < K) {
+          sc = S.own_sc[e];
+          id = S.own_id[e];
+          sl = e;
+        } else if (packed) {
+          const uint64_t key = pkey[e - K];
+          sc = key_score(key);
+          id = key_id(key);
+          sl = K;
+        } else {
+          const int s = surv[e - K];
+          sc = sel_sc[s];
+          id = cid[sel_idx[s]];
+          sl = K;
+        }
+        int r = 0;
+        for (int k = 0; k < K; ++k) r += before(S.own_sc[k], S.own_id[k], k, sc, id, sl) ? 1 : 0;
+        if (packed) {
+          const uint64_t key = pack_key(sc, id);
+          for (int p = 0; p < P && r < K; ++p) r += pkey[p] > key ? 1 : 0;
+        } else {
+          for (int p = 0; p < P && r < K; ++p) {
+            const int s = surv[p];
+            r += before(sel_sc[s], cid[sel_idx[s]], K, sc, id, sl) ? 1 : 0;
+          }
+        }
+        if (r < K) {
+          S.new_id[r] = id;
+          S.new_sc[r] = sc;
+          S.new_fresh[r] = e >= K ? 1 : 0;
+          if (e >= K) atomicAdd(&S.cnt[4], 1);   // a new id entered the list
         }
       }
-      if (r < K) {
-        S.new_id[r] = id;
-        S.new_sc[r] = sc;
-        S.new_fresh[r] = e >= K ? 1 : 0;
-        if (e >= K) atomicAdd(&S.cnt[4], 1);   // a new id entered the list
-      }
-    }
-    __syncthreads();
-    // ---- 7: write back, count changes
-    if (w == 0) {
-      bool changed = false, fresh = false;
-      if (lane < K) {
-        ids_out[i * K + lane] = S.new_id[lane];
-        sc_out[i * K + lane] = S.new_sc[lane];
-        changed = S.new_id[lane] != S.own_id[lane];
-        fresh = S.new_fresh[lane] != 0;
-      }
-      const unsigned long long m = __ballot(changed);
-      const unsigned long long fm = __ballot(fresh);
-      if (fresh_out && lane == 0) fresh_out[i] = fm;
-      if (lane == 0) {
-        my_upd += __popcll(m);
-        my_cand += U;
-        if (U > M) my_skc += U;
-        my_sel += NS;
-        my_ins += S.cnt[4];
-        if (segc && tid == 0) {
-          const int sg = seg_of(seg_base, nseg, i);
-          if (sg != seg_cur) {
-            if (seg_cur >= 0 && (seg_upd | seg_ins)) {
-              atomicAdd(&segc[2 * seg_cur], seg_upd);
-              atomicAdd(&segc[2 * seg_cur + 1], seg_ins);
+      __syncthreads();
+      // ---- 7: write back, count changes
+      if (w == 0) {
+        bool changed = false, fresh = false;
+        if (lane < K) {
+          ids_out[i * K + lane] = S.new_id[lane];
+          sc_out[i * K + lane] = S.new_sc[lane];
+          changed = S.new_id[lane] != S.own_id[lane];
+          fresh = S.new_fresh[lane] != 0;
+        }
+        const unsigned long long m = __ballot(changed);
+        const unsigned long long fm = __ballot(fresh);
+        if (fresh_out && lane == 0) fresh_out[i] = fm;
+        if (lane == 0) {
+          my_upd += __popcll(m);
+          my_cand += U;
+          if (U > M) my_skc += U;
+          my_sel += NS;
+          my_ins += S.cnt[4];
+          if (segc && tid == 0) {
+            const int sg = seg_of(seg_base, nseg, i);
+            if (sg != seg_cur) {
+              if (seg_cur >= 0 && (seg_upd | seg_ins)) {
+                atomicAdd(&segc[2 * seg_cur], seg_upd);
+                atomicAdd(&segc[2 * seg_cur + 1], seg_ins);
+              }
+              seg_cur = sg;
+              seg_upd = seg_ins = 0;
             }
-            seg_cur = sg;
-            seg_upd = seg_ins = 0;
+            seg_upd += __popcll(m);
+            seg_ins += S.cnt[4];
           }
-          seg_upd += __popcll(m);
-          seg_ins += S.cnt[4];
         }
       }
     }
@@ -2804,7 +2826,8 @@ int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const in
   const uint64_t* fin = c->p.incremental ? c->fresh[a] : nullptr;
 #define KNNG_JOIN_ARGS                                                                          \
   c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order, \
-      fin, c->fresh[b], nwork_dev, seg_base, nseg, segc
+      fin, c->fresh[b], nwork_dev, seg_base, nseg, segc, static_cast<int32_t*>(nullptr),             \
+      static_cast<unsigned long long*>(nullptr)
   const bool d1024 = c->d == kMaxDim;   // the rows' length as a compile-time constant
   if (K > 50 && d1024)
     hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
@@ -2825,6 +2848,42 @@ int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const in
 #undef KNNG_JOIN_ARGS
   HIPCHK(c, hipGetLastError());
   return 0;
+}
+
+// A join iteration under the new/old rule after the first: the small-row
+// variant of k_join over every row, then the full kernel over the rows it
+// passed on (device count, no host round trip).  Eight blocks per CU (the
+// most a CU holds: 32 waves; 64 VGPRs with some spilling, 15 KB of LDS each)
+// measured fastest: the iteration is bound by rows in flight (120 ms at
+// 6.35M against 135 / 141 ms at six / five blocks and 193 ms for the full
+// kernel at four).
+constexpr int kSmallJoinOcc = 8;
+int launch_join_small(knng_ctx* c, int a, int b, const int32_t* order) {
+  const int64_t n = c->n;
+  const int K = c->p.k_build, M = K * c->p.rerank_mult;
+  if (!c->ovf) {
+    int rc;
+    if ((rc = dalloc(c, &c->ovf, size_t(n), "join overflow rows")) ||
+        (rc = dalloc(c, &c->ovf_n, 1, "join overflow count")))
+      return rc;
+  }
+  HIPCHK(c, hipMemsetAsync(c->ovf_n, 0, sizeof(unsigned long long), c->stream));
+  const int grid = std::max(8, (c->num_cu * kSmallJoinOcc) / 8 * 8);
+  unsigned long long* rowq = c->counters + kCounterFields * kNumCounters;
+#define KNNG_SMALL_ARGS                                                                                 \
+  c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order,     \
+      c->fresh[a], c->fresh[b], static_cast<const int64_t*>(nullptr), static_cast<const int64_t*>(nullptr), \
+      0, static_cast<unsigned long long*>(nullptr), c->ovf, c->ovf_n
+  if (c->d == kMaxDim)
+    hipLaunchKernelGGL((k_join<10, 640, kSmallJoinOcc, 1, 8, kMaxDim, true>), dim3(grid), dim3(256), 0, c->stream,
+                       KNNG_SMALL_ARGS);
+  else
+    hipLaunchKernelGGL((k_join<10, 640, kSmallJoinOcc, 1, 8, 0, true>), dim3(grid), dim3(256), 0, c->stream,
+                       KNNG_SMALL_ARGS);
+#undef KNNG_SMALL_ARGS
+  HIPCHK(c, hipGetLastError());
+  return launch_join(c, c->join_variant, a, b, c->ovf, reinterpret_cast<const int64_t*>(c->ovf_n), nullptr, 0,
+                     nullptr);
 }
 
 struct TuneCache {
@@ -3000,6 +3059,10 @@ int knng_destroy(knng_ctx* c) {
     dfree(&c->gsc[b]);
     dfree(&c->fresh[b]);
     dfree(&c->dstage[b]);
+    if (b == 0) {
+      dfree(&c->ovf);
+      dfree(&c->ovf_n);
+    }
     if (c->pin[b]) hipHostFree(c->pin[b]);
     if (c->pin_ev[b]) hipEventDestroy(c->pin_ev[b]);
   }
@@ -3295,6 +3358,7 @@ int knng_build_init(knng_ctx* c) {
         (rc = dalloc(c, &c->fresh[b], size_t(n), "fresh masks")))
       return rc;
   HIPCHK(c, hipMemsetAsync(c->fresh[0], 0xFF, size_t(n) * sizeof(uint64_t), c->stream));   // all fresh
+  c->fresh_all = true;
   uint64_t* cand = nullptr;
   if ((rc = dalloc(c, &cand, size_t(n) * T * kLeafSlots, "init candidates"))) return rc;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -3398,12 +3462,22 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   const int a = c->cur, b = c->cur ^ 1;
   const int32_t* order = c->have_forest ? c->perm : nullptr;   // tree 0's leaf order
   if (int rc = ensure_join_variant(c, a, b, order, n, nullptr, 0, nullptr)) return rc;
+  // after the first iteration under the new/old rule most rows have few
+  // candidates: the small-row kernel, the full one for the rest
+  const bool small = c->p.incremental && !c->fresh_all && K <= 50;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  if (int rc = launch_join(c, c->join_variant, a, b, order, nullptr, nullptr, 0, nullptr)) return rc;
+  if (small) {
+    if (int rc = launch_join_small(c, a, b, order)) return rc;
+  } else if (int rc = launch_join(c, c->join_variant, a, b, order, nullptr, nullptr, 0, nullptr)) {
+    return rc;
+  }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  c->fresh_all = false;
   hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(256), 0, c->stream, c->counters, c->csum);
   unsigned long long sums[kCounterFields];
+  unsigned long long passed = 0;
   HIPCHK(c, hipMemcpyAsync(sums, c->csum, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+  if (small) HIPCHK(c, hipMemcpyAsync(&passed, c->ovf_n, sizeof(passed), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
@@ -3417,6 +3491,7 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
     c->st.join_rescored[it] = int64_t(sums[2]);
     c->st.join_inserted[it] = int64_t(sums[3]);
     c->st.insert_rates[it] = float(sums[3]) / float(size_t(n) * size_t(K));
+    c->st.join_passed_on[it] = small ? int64_t(passed) : -1;
     // algorithmic bytes: own list + own row/sketch, K neighbour id rows, one
     // sketch per candidate of a row that gathers them, one row per re-scored
     // candidate, writes
@@ -3605,6 +3680,7 @@ int knng_load_graph(knng_ctx* c, const int32_t* ids, const int32_t* scores) {
   HIPCHK(c, hipMemcpyAsync(c->gid[0], ids, bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->gsc[0], scores, bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemsetAsync(c->fresh[0], 0xFF, size_t(n) * sizeof(uint64_t), c->stream));   // all fresh
+  c->fresh_all = true;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_graph = true;
   return 0;
@@ -3784,6 +3860,7 @@ int build_packed(knng_ctx* c, std::vector<int32_t>* iters, std::vector<int32_t>*
       c->st.join_candidates[it] = int64_t(L.csum_h[1]);
       c->st.join_rescored[it] = int64_t(L.csum_h[2]);
       c->st.join_inserted[it] = int64_t(L.csum_h[3]);
+      c->st.join_passed_on[it] = -1;
       c->st.insert_rates[it] = float(L.csum_h[3]) / float(size_t(c->n) * size_t(K));
       // algorithmic bytes as knng_join_iteration counts them, over the rows processed
       c->st.join_bytes[it] = double(rows) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) +

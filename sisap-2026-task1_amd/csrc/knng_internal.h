@@ -54,6 +54,9 @@ struct knng_ctx {
   // graph
   int32_t *gid[2] = {nullptr, nullptr}, *gsc[2] = {nullptr, nullptr};
   uint64_t* fresh[2] = {nullptr, nullptr};   // per row: bit k = entry k entered last iteration
+  bool fresh_all = false;                    // fresh[cur] is all ones (no join iteration yet)
+  int32_t* ovf = nullptr;                    // rows the small-row join passed on [n]
+  unsigned long long* ovf_n = nullptr;       // their count
   int cur = 0;
   bool have_graph = false;
   unsigned long long *counters = nullptr, *csum = nullptr;

@@ -189,6 +189,11 @@ def test_incremental_join_bitexact(torch_cuda):
         g_ids, g_sc = b.graph()
         assert upd == o_upd, it
         assert np.array_equal(g_ids, ids) and np.array_equal(g_sc, sc), it
+    # iterations after the first ran the small-row kernel, which passed the
+    # rows with many candidates to the full one: both kernels wrote rows
+    passed = b.stats()["join_passed_on"]
+    assert passed[0] == -1 and all(p >= 0 for p in passed[1:]), passed
+    assert any(0 < p < 5000 for p in passed[1:]), passed
 
 
 def test_incremental_full_build_bitexact(torch_cuda):
