@@ -287,6 +287,8 @@ def main():
                 "traffic_GBps": round(traffic / (jms / max(it, 1) / 1e3) / 1e9, 1)
                 if traffic is not None and jms > 0 else None,
                 "kernel": "k_join", "join_shape": st["join_variant"], "launches": it,
+                "launch_unit": "join iteration (its HIP events bracket all of its k_join launches: the full "
+                               "kernel, or the small-row kernel and the full one over the rows it passed on)",
                 "algorithmic_bytes_per_launch": round(jb / max(it, 1)),
                 "algorithmic_bytes_per_iter": [int(v) for v in st["join_bytes"][:it]],
                 "avg_launch_ms": round(jms / max(it, 1), 3)}
