@@ -1652,7 +1652,9 @@ struct JoinScratch {
   int32_t new_id[kMaxK];
   int32_t new_sc[kMaxK];
   int32_t new_fresh[kMaxK];   // 1: the entry came from this iteration's candidates
-  uint64_t own_new;           // bit k: own entry k entered the list last iteration
+  uint8_t own_bit[kMaxK];     // 1: own entry k entered the list last iteration (one byte per entry:
+                              // 64-bit masks per neighbour would be hoisted out of the row loop and
+                              // spilled)
   uint64_t own_sk[kMaxDim / 64];
   int32_t hist[kMaxDim + 1];
   int32_t cnt[5];   // 0 unique candidates, 1 selected, 2 survivors, 3 threshold, 4 new ids
@@ -1749,7 +1751,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
       S.own_sc[tid] = osc;
       S.own_key[tid] = pack_key(osc, oid);
     }
-    if (tid == 0) S.own_new = fresh_in ? fresh_in[i] : ~0ull;
+    if (tid < K) S.own_bit[tid] = fresh_in ? uint8_t((fresh_in[i] >> tid) & 1ull) : uint8_t(1);
     __syncthreads();
     if (tid < K) hs_insert(hs, HSBITS, S.own_id[tid]);
     if (tid == K) hs_insert(hs, HSBITS, int32_t(i));
@@ -1758,14 +1760,13 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     {
       constexpr int kRowsPerWave = kMaxK / 4;
       int32_t c[kRowsPerWave];
-      const uint64_t own_new = S.own_new;
 #pragma unroll
       for (int r = 0; r < kRowsPerWave; ++r) {
         const int k = w + 4 * r;
         c[r] = -1;
         if (k < K && lane < K) {
           const int32_t n1 = S.own_id[k];
-          const bool take = ((own_new >> k) & 1ull) || !fresh_in || ((fresh_in[n1] >> lane) & 1ull);
+          const bool take = S.own_bit[k] != 0 || !fresh_in || ((fresh_in[n1] >> lane) & 1ull);
           if (take) c[r] = ids_in[int64_t(n1) * K + lane];
         }
       }
