@@ -1706,7 +1706,10 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   // the per-chunk bounds of the sketch and row gathers into straight-line code
   if (DIM > 0) d = DIM;
   const int words = DIM > 0 ? DIM / 64 : d >> 6, nch = DIM > 0 ? DIM / 16 : d >> 4;
-  unsigned long long my_upd = 0, my_cand = 0, my_sel = 0, my_ins = 0, my_skc = 0;
+  // the block's counters (changed slots, candidates, selected, new ids,
+  // sketched candidates), thread 0 only; LDS keeps them out of the register
+  // budget, where they would stay live through every phase of the row loop
+  __shared__ unsigned long long my_ctr[kCounterFields];
 
   // Schedule: blocks are dealt to the 8 XCDs round-robin, so XCD x = block % 8
   // takes the contiguous slice [x*n/8, (x+1)*n/8) of the processing order and
@@ -1718,6 +1721,10 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   // fewer rows, so the slices drain together.  The result does not depend on
   // the order (Jacobi iteration).
   __shared__ int64_t s_row;
+  // the next row's queue slot, parked in LDS as soon as the atomic returns: a
+  // register held through the whole row would be spilled (a scratch store per
+  // thread and row) by the small-row kernel
+  __shared__ unsigned long long s_next;
   const int xcd = blockIdx.x & 7;
   const int64_t nwork = nwork_dev ? *nwork_dev : n;
   const int64_t lo = nwork * xcd / 8, hi = nwork * (xcd + 1) / 8;
@@ -1729,6 +1736,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     seg_cur = -1;
     seg_upd = seg_ins = 0;
   }
+  if (tid < kCounterFields) my_ctr[tid] = 0;
   unsigned long long* const qctr = rowq + 32 * xcd + (SMALL ? 1 : 0);
   const int lim = SMALL ? min(M, CAP - 256) : CAP;
   if (tid == 0) s_row = lo + int64_t(atomicAdd(qctr, 1ull));
@@ -1752,6 +1760,8 @@ __global__ __launch_bounds__(256, OCC) void k_join(
       S.own_key[tid] = pack_key(osc, oid);
     }
     if (tid < K) S.own_bit[tid] = fresh_in ? uint8_t((fresh_in[i] >> tid) & 1ull) : uint8_t(1);
+    // wave 0 waits for its own-row loads here anyway, so the atomic's latency is hidden
+    if (tid == 0) s_next = next_row;
     __syncthreads();
     if (tid < K) hs_insert(hs, HSBITS, S.own_id[tid]);
     if (tid == K) hs_insert(hs, HSBITS, int32_t(i));
@@ -1877,11 +1887,17 @@ __global__ __launch_bounds__(256, OCC) void k_join(
       // ---- 5: int8 dots, 16 lanes per candidate, 4 candidates per wave step
       {
         const int g = lane >> 4, s16 = lane & 15;
-        v4i own[4];
-  #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        // this lane's four 16-byte chunks of the own row: held in registers by
+        // the full kernel; the small-row kernel (64 VGPRs) re-reads them from
+        // LDS at each step instead
+        auto own_chunk = [&](int c) -> v4i {
           const int chn = s16 + 16 * c;
-          own[c] = chn < nch ? *reinterpret_cast<const v4i*>(own_row + 16 * chn) : v4i{0, 0, 0, 0};
+          return chn < nch ? *reinterpret_cast<const v4i*>(own_row + 16 * chn) : v4i{0, 0, 0, 0};
+        };
+        v4i own[4];
+        if constexpr (!SMALL) {
+  #pragma unroll
+          for (int c = 0; c < 4; ++c) own[c] = own_chunk(c);
         }
         // DOTPER candidates per 16-lane group in flight (64 B per lane each);
         // at 2 the kernel fits 128 VGPRs, i.e. four blocks per CU
@@ -1902,7 +1918,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
           for (int j = 0; j < kDotPer; ++j) {
             int dj = 0;
   #pragma unroll
-            for (int c = 0; c < 4; ++c) dj = dot16_i8(v[j][c], own[c], dj);
+            for (int c = 0; c < 4; ++c) dj = dot16_i8(v[j][c], SMALL ? own_chunk(c) : own[c], dj);
   #pragma unroll
             for (int m = 8; m >= 1; m >>= 1) dj += __shfl_xor(dj, m);
             const int sj = s0 + w * 4 + g + 16 * j;
@@ -2001,11 +2017,11 @@ __global__ __launch_bounds__(256, OCC) void k_join(
         const unsigned long long fm = __ballot(fresh);
         if (fresh_out && lane == 0) fresh_out[i] = fm;
         if (lane == 0) {
-          my_upd += __popcll(m);
-          my_cand += U;
-          if (U > M) my_skc += U;
-          my_sel += NS;
-          my_ins += S.cnt[4];
+          my_ctr[0] += __popcll(m);
+          my_ctr[1] += U;
+          my_ctr[2] += NS;
+          my_ctr[3] += S.cnt[4];
+          if (U > M) my_ctr[4] += U;
           if (segc && tid == 0) {
             const int sg = seg_of(seg_base, nseg, i);
             if (sg != seg_cur) {
@@ -2022,7 +2038,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
         }
       }
     }
-    if (tid == 0) s_row = lo + int64_t(next_row);
+    if (tid == 0) s_row = lo + int64_t(s_next);
     __syncthreads();
   }
   if (tid == 0 && segc && seg_cur >= 0 && (seg_upd | seg_ins)) {
@@ -2031,11 +2047,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   }
   if (tid == 0) {
     unsigned long long* c = counters + kCounterFields * (blockIdx.x % kNumCounters);
-    atomicAdd(c + 0, my_upd);
-    atomicAdd(c + 1, my_cand);
-    atomicAdd(c + 2, my_sel);
-    atomicAdd(c + 3, my_ins);
-    atomicAdd(c + 4, my_skc);
+    for (int f = 0; f < kCounterFields; ++f) atomicAdd(c + f, my_ctr[f]);
   }
 }
 
