@@ -2817,9 +2817,9 @@ int check_params(const knng_params* p, int64_t n, std::string* why) {
 // k_join shapes and the autotuner.  Every shape computes the same lists (the
 // iteration is a deterministic function of the previous graph); they differ
 // in blocks per CU and in how many gathers each lane keeps in flight:
-//   0: 4 blocks/CU, 2 rows + 8 sketches in flight per lane group (128 VGPRs)
-//   1: 3 blocks/CU, 4 rows + 8 sketches
-//   2: 4 blocks/CU, 2 rows + 4 sketches
+//   0: 4 blocks/CU, 2 rows + 8 sketches in flight per lane group (106 VGPRs at d = 1024)
+//   1: 3 blocks/CU, 4 rows + 8 sketches (138 VGPRs)
+//   2: 4 blocks/CU, 3 rows + 8 sketches (122 VGPRs)
 // k_build > 50 has one shape (its 64 KB of LDS allows 2 blocks per CU).
 // The first join iteration of a (dim, k_build, rule) shape in a process times
 // each shape on a prefix of the work list (same rows, same outputs, so the
@@ -2851,9 +2851,9 @@ int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const in
   else if (v == 1)
     hipLaunchKernelGGL((k_join<12, 2560, 3, 4, 8, 0>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
   else if (v == 2 && d1024)
-    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 4, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+    hipLaunchKernelGGL((k_join<12, 2560, 4, 3, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
   else if (v == 2)
-    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 4, 0>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
+    hipLaunchKernelGGL((k_join<12, 2560, 4, 3, 8, 0>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
   else if (d1024)
     hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
   else
