@@ -1782,7 +1782,9 @@ __global__ __launch_bounds__(256, OCC) void k_join(
       }
 #pragma unroll
       for (int r = 0; r < kRowsPerWave; ++r) {
-        if (SMALL && S.cnt[0] > lim) break;   // the row overflows (wave-uniform read)
+        // the row overflows (a wave-uniform read): insert nothing more (a guard,
+        // not a break, so the loop unrolls and c[] stays in fixed registers)
+        if (SMALL && S.cnt[0] > lim) c[r] = -1;
         const bool fresh_id = c[r] >= 0 && hs_insert(hs, HSBITS, c[r]);
         const int slot = wave_append(fresh_id, &S.cnt[0]);
         if (fresh_id) cid[slot] = c[r];
