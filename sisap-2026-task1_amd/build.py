@@ -30,7 +30,6 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # keep the compiler away from packed-FP32 VALU forms on this target
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-result", "-Wno-unused-value",
 ]
 
 

@@ -3028,8 +3028,8 @@ int knng_create(knng_ctx** out, int32_t device, const knng_params* p, int64_t n)
   // kernels that produced a device input) completes before ours starts
   if (hipStreamCreateWithFlags(&c->stream, hipStreamDefault) != hipSuccess)
     return bail(fail(c, KNNG_EHIP, "hipStreamCreate failed"));
-  hipEventCreate(&c->ev0);
-  hipEventCreate(&c->ev1);
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return bail(fail(c, KNNG_EHIP, "hipEventCreate failed"));
   int rc;
   if ((rc = dalloc(c, &c->q, size_t(n) * c->d, "int8 rows"))) return bail(rc);
   if ((rc = dalloc(c, &c->sk, size_t(n) * c->words, "sketches"))) return bail(rc);
@@ -3048,8 +3048,8 @@ int knng_create(knng_ctx** out, int32_t device, const knng_params* p, int64_t n)
 
 int knng_destroy(knng_ctx* c) {
   if (!c) return 0;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   dfree(&c->x);
   dfree(&c->q);
   dfree(&c->sk);
@@ -3078,8 +3078,8 @@ int knng_destroy(knng_ctx* c) {
       dfree(&c->ovf);
       dfree(&c->ovf_n);
     }
-    if (c->pin[b]) hipHostFree(c->pin[b]);
-    if (c->pin_ev[b]) hipEventDestroy(c->pin_ev[b]);
+    if (c->pin[b]) (void)hipHostFree(c->pin[b]);
+    if (c->pin_ev[b]) (void)hipEventDestroy(c->pin_ev[b]);
   }
   dfree(&c->counters);
   dfree(&c->csum);
@@ -3087,9 +3087,9 @@ int knng_destroy(knng_ctx* c) {
   dfree(&c->tune_n);
   dfree(&c->ex_knns);
   dfree(&c->ex_dists);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
@@ -3135,7 +3135,7 @@ int knng_ingest_device(knng_ctx* c, const float* dev_rows, int64_t row0, int64_t
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);   // ms stays 0 on failure (a timing stat only)
   c->st.ms_ingest += ms;
   c->ingested += nrows;
   return 0;
@@ -3183,7 +3183,7 @@ int knng_ingest_chunk(knng_ctx* c, const float* rows, int64_t row0, int64_t nrow
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);   // ms stays 0 on failure (a timing stat only)
   c->st.ms_ingest += ms;
   c->ingested += nrows;
   return 0;
@@ -3350,7 +3350,7 @@ int knng_build_forest(knng_ctx* c) {
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);   // ms stays 0 on failure (a timing stat only)
   c->st.ms_forest = ms;
   c->st.forest_levels = levels;
   c->st.num_leaves = nl;
@@ -3461,7 +3461,7 @@ int knng_build_init(knng_ctx* c) {
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);   // ms stays 0 on failure (a timing stat only)
   c->st.ms_init = ms;
   dfree(&cand);
   c->have_graph = true;
@@ -3495,7 +3495,7 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   if (small) HIPCHK(c, hipMemcpyAsync(&passed, c->ovf_n, sizeof(passed), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);   // ms stays 0 on failure (a timing stat only)
   c->cur = b;
   const int it = c->st.iterations;
   if (it < 32) {
@@ -3575,7 +3575,7 @@ int knng_export_device(knng_ctx* c, int32_t* dk, float* dd, int32_t k, int32_t r
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);   // ms stays 0 on failure (a timing stat only)
   c->st.ms_export = ms;
   return 0;
 }

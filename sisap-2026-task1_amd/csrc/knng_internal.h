@@ -115,7 +115,7 @@ inline int dalloc(knng_ctx* c, T** ptr, size_t count, const char* what) {
 
 template <typename T>
 void dfree(T** ptr) {
-  if (*ptr) hipFree(reinterpret_cast<void*>(const_cast<typename std::remove_const<T>::type*>(*ptr)));
+  if (*ptr) (void)hipFree(reinterpret_cast<void*>(const_cast<typename std::remove_const<T>::type*>(*ptr)));
   *ptr = nullptr;
 }
 
