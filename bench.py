@@ -72,75 +72,85 @@ def build_params(knng, args, d):
 
 def exact_topk(torch, x, rows, k, chunk=262144):
     """Exact FP32 dot-product top-k of x[rows] over all rows, self excluded."""
-    q = x[rows]
-    R = q.shape[0]
-    best_s = torch.full((R, k + 1), float("-inf"), device=x.device)
-    best_i = torch.zeros((R, k + 1), dtype=torch.int64, device=x.device)
-    ar = torch.arange(R, device=x.device)
-    for a in range(0, x.shape[0], chunk):
-        b = min(x.shape[0], a + chunk)
-        s = q @ x[a:b].T
-        inside = (rows >= a) & (rows < b)
-        s[ar[inside], rows[inside] - a] = float("-inf")
-        ts, ti = s.topk(k + 1, dim=1)
-        cs = torch.cat([best_s, ts], 1)
-        ci = torch.cat([best_i, ti + a], 1)
-        best_s, o = cs.topk(k + 1, dim=1)
-        best_i = ci.gather(1, o)
-    return best_i[:, :k]
+    import synth
+    return synth.exact_knn_torch(x, rows, k, chunk)
 
 
 def recall(knns_rows_1b, gt_0b, k):
-    import numpy as np
-    hits = 0
-    for p, g in zip(knns_rows_1b, gt_0b):
-        hits += len(np.intersect1d(p - 1, g[:k]))
-    return hits / (len(gt_0b) * k)
+    import synth
+    return synth.recall_rows(knns_rows_1b, gt_0b, k)
 
 
-def cpu_baseline(args, nproc):
-    """The compiled reference engine (oracle/_ref/apex_main, built from
-    src/main.cpp with its own flags) on a bounded sample of the workload, at
-    the reference's own 8 OpenMP threads; its buildtime attribute is scaled
-    linearly to the full row count.  Falls back to the single-thread oracle
-    port when the reference binary cannot run."""
+def _reference_run(args, ns, threads):
+    """The compiled reference engine on ns rows of the workload's generator
+    (same rows per cluster as the workload): (buildtime s, wall s, iterations,
+    recall@15 on 2000 sampled rows) or None."""
     import numpy as np
 
     import knng
     import oracle
     import synth
 
-    ns = args.cpu_sample
-    # same points per cluster as the workload, so the sample's kNN structure matches
     cl = max(1, round(args.clusters * ns / args.n))
     x = synth.gaussian_mixture(args.seed, ns, args.dim, cl)
+    with tempfile.TemporaryDirectory() as td:
+        h5p = os.path.join(td, "sample.h5")
+        knng.h5_write_train(h5p, x)
+        t0 = time.time()
+        try:
+            r = oracle.run_reference_binary(h5p, td, threads=threads, timeout=args.cpu_timeout)
+        except subprocess.TimeoutExpired:
+            log(f"reference binary timed out at {ns} rows")
+            return None
+        wall = time.time() - t0
+        if r["returncode"] != 0:
+            log("reference binary failed:", r["stdout"][-400:])
+            return None
+        bt = knng.h5_attr_f32(r["out"], "buildtime")
+        kn = knng.h5_read(r["out"], "knns", np.int32)
+    rows = np.sort(np.random.default_rng(7).choice(ns, min(2000, ns), replace=False))
+    rec = synth.recall_at_k(kn, synth.exact_knn(x, rows, 15), rows, 15)
+    return bt, wall, r["stdout"].count("-> Iteration"), rec, cl
+
+
+def cpu_baseline(args, nproc):
+    """The compiled reference engine (oracle/_ref/apex_main, built from
+    src/main.cpp with its own flags) at the reference's own 8 OpenMP threads,
+    measured on two bounded samples of the workload (the same generator and
+    rows per cluster); its buildtime is extrapolated to the full row count by
+    the power law through the two measurements (the reference's time grows
+    faster than linearly: 2.75 s at 50k rows, 92.6 s at 1M).  Falls back to
+    the single-thread oracle port when the reference binary cannot run."""
+    import math
+
+    import oracle
+    import synth
+
     threads = min(8, nproc)
+    n2 = args.cpu_sample
+    n1 = max(2000, n2 // 4)
     if os.path.exists(oracle.REF_BIN) and args.dim == 1024:
-        with tempfile.TemporaryDirectory() as td:
-            h5p = os.path.join(td, "sample.h5")
-            knng.h5_write_train(h5p, x)
-            t0 = time.time()
-            try:
-                r = oracle.run_reference_binary(h5p, td, threads=threads, timeout=args.cpu_timeout)
-                wall = time.time() - t0
-                if r["returncode"] == 0:
-                    bt = knng.h5_attr_f32(r["out"], "buildtime")
-                    kn = knng.h5_read(r["out"], "knns", np.int32)
-                    rows = np.sort(np.random.default_rng(7).choice(ns, min(2000, ns), replace=False))
-                    rec = synth.recall_at_k(kn, synth.exact_knn(x, rows, 15), rows, 15)
-                    iters = r["stdout"].count("-> Iteration")
-                    return {"value": round(bt * args.n / ns, 2), "unit": "s (extrapolated)",
-                            "cores": threads, "kind": "reference",
-                            "sample": f"{ns} rows of the same generator (seed {args.seed}, {cl} "
-                                      f"clusters: the workload's rows per cluster); reference "
-                                      f"buildtime {bt:.2f} s (wall {wall:.2f} s incl. HDF5 read, "
-                                      f"{iters} iterations) x {args.n}/{ns}",
-                            "measured_s": round(bt, 3), "sample_recall_at_15": round(rec, 4)}
-                log("reference binary failed:", r["stdout"][-400:])
-            except subprocess.TimeoutExpired:
-                log("reference binary timed out")
-    ns = min(ns, 10000)
-    x = x[:ns]
+        m1 = _reference_run(args, n1, threads)
+        m2 = _reference_run(args, n2, threads) if m1 else None
+        if m1 and m2:
+            (t1, w1, i1, r1, c1), (t2, w2, i2, r2, c2) = m1, m2
+            if n2 >= args.n:
+                value, how = t2, "measured on the whole workload"
+            else:
+                expo = math.log(t2 / t1) / math.log(n2 / n1)
+                value = t2 * (args.n / n2) ** expo
+                how = f"power law t = t2 (n/{n2})^{expo:.3f} through both samples"
+            return {"value": round(value, 2), "unit": "s" if n2 >= args.n else "s (extrapolated)",
+                    "cores": threads, "kind": "reference",
+                    "sample": f"reference buildtime (src/main.cpp:203-442) on {n1} and {n2} rows of the "
+                              f"same generator (seed {args.seed}, {c1} and {c2} clusters: the workload's rows "
+                              f"per cluster): {t1:.2f} s ({i1} iterations) and {t2:.2f} s ({i2} iterations); "
+                              f"{how}",
+                    "measured_s": [round(t1, 3), round(t2, 3)], "measured_rows": [n1, n2],
+                    "measured_wall_s": [round(w1, 3), round(w2, 3)],
+                    "sample_recall_at_15": [round(r1, 4), round(r2, 4)]}
+    ns = min(n2, 10000)
+    x = synth.gaussian_mixture(args.seed, ns, args.dim, max(1, round(args.clusters * ns / args.n)))
     q, sk = oracle.quantize(x)
     t0 = time.time()
     oracle.build_reference(q, sk)
@@ -170,11 +180,11 @@ def main():
     ap.add_argument("--incremental", type=int, default=1,
                     help="1: NN-Descent's new/old rule in the join (a 2-hop pair needs a fresh edge)")
     ap.add_argument("--rerank-width", type=int, default=32,
-                    help="list entries the FP32 rerank re-scores (0: all k_build); 32 keeps the rerank "
-                         "under 10%% of the build at 6.35M rows")
+                    help="list entries the FP32 rerank re-scores (0: all k_build)")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--recall-rows", type=int, default=10000)
-    ap.add_argument("--cpu-sample", type=int, default=50000)
+    ap.add_argument("--cpu-sample", type=int, default=100000,
+                    help="rows of the larger CPU-baseline sample (a quarter of it is the smaller)")
     ap.add_argument("--cpu-timeout", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["build", "serve", "batch"], default="build",
@@ -283,8 +293,12 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": PMC_PROFILE if traffic is not None else None,
-                # HBM bytes actually moved (rocprofv3 FETCH_SIZE + WRITE_SIZE) per second
+                # HBM bytes actually moved (rocprofv3 FETCH_SIZE + WRITE_SIZE) per second, and
+                # that rate as a share of the HBM peak: `frac` counts algorithmic bytes, cache
+                # hits included, so it can exceed what the HBM actually delivered
                 "traffic_GBps": round(traffic / (jms / max(it, 1) / 1e3) / 1e9, 1)
+                if traffic is not None and jms > 0 else None,
+                "hbm_frac": round(traffic / (jms / max(it, 1) / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
                 if traffic is not None and jms > 0 else None,
                 "kernel": "k_join", "join_shape": st["join_variant"], "launches": it,
                 "launch_unit": "join iteration (its HIP events bracket all of its k_join launches: the full "
@@ -295,7 +309,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log(f"CPU baseline on a {args.cpu_sample}-row sample")
+        log(f"CPU baseline on {max(2000, args.cpu_sample // 4)}- and {args.cpu_sample}-row samples")
         cpu = cpu_baseline(args, os.cpu_count() or 1)
 
     out = {
@@ -315,6 +329,9 @@ def main():
                          "rows": int(len(rows_np)), "rerank_rel_err_max": float(rel.max())},
         "plain_s": round(stage["ingest_s"] + stage["build_s"] + stage["export_plain_s"], 4),
         "rerank_extra_s": round(stage["export_rerank_s"], 4),
+        # the FP32 rerank's cost relative to the plain int8 path (ingest + build + plain export)
+        "rerank_extra_frac": round(stage["export_rerank_s"] / max(1e-9, stage["ingest_s"] + stage["build_s"]
+                                                                   + stage["export_plain_s"]), 4),
         "stages_ms": {"ingest": round(stage["ingest_s"] * 1e3, 1), "forest": round(st["ms_forest"], 1),
                       "kmeans": round(st["ms_kmeans"], 1),
                       "init": round(st["ms_init"], 1), "join": round(st["ms_join"], 1),
@@ -389,22 +406,8 @@ def batch_main(args):
 
 def exact_topk_prefix(torch, x, qv, limits, k, chunk=262144):
     """Exact FP32 top-k of query vectors qv over rows x[:limits[j]] (per query)."""
-    R = qv.shape[0]
-    best_s = torch.full((R, k), float("-inf"), device=x.device)
-    best_i = torch.zeros((R, k), dtype=torch.int64, device=x.device)
-    lim = limits.view(-1, 1)
-    top = int(limits.max().item())
-    for a in range(0, top, chunk):
-        b = min(top, a + chunk)
-        s = qv @ x[a:b].T
-        cols = torch.arange(a, b, device=x.device).view(1, -1)
-        s.masked_fill_(cols >= lim, float("-inf"))
-        ts, ti = s.topk(min(k, b - a), dim=1)
-        cs = torch.cat([best_s, ts], 1)
-        ci = torch.cat([best_i, ti + a], 1)
-        best_s, o = cs.topk(k, dim=1)
-        best_i = ci.gather(1, o)
-    return best_i
+    import synth
+    return synth.exact_knn_prefix_torch([x], qv, limits, k, chunk)
 
 
 def serve_main(args):

@@ -395,6 +395,17 @@ __device__ __forceinline__ void assign_issue_tile(const int8_t* __restrict__ Z, 
   }
 }
 
+// Ordinary zero stores over an LDS array that direct global->LDS loads
+// (__builtin_amdgcn_global_load_lds) then fill, by the whole block, at the
+// kernel's start.  Without an ordinary store the optimiser may treat the array
+// as never written, drop it and fold its reads to undefined values; the build
+// checks every such kernel's LDS size against its declared arrays
+// (kernel-resource-usage remarks) so a dropped array fails the build.
+__device__ __forceinline__ void lds_claim(void* p, int bytes) {
+  v4i* v = static_cast<v4i*>(p);
+  for (int j = int(threadIdx.x); j < (bytes >> 4); j += int(blockDim.x)) v[j] = v4i{0, 0, 0, 0};
+}
+
 // s_waitcnt vmcnt(n), other counters unconstrained (gfx9 encoding:
 // vmcnt[3:0] | expcnt 7 << 4 | lgkmcnt 15 << 8 | vmcnt[5:4] << 14)
 #define KNNG_VMCNT(n) (((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
@@ -442,6 +453,7 @@ __global__ __launch_bounds__(256) void k_assign(const int8_t* __restrict__ q, in
   // the optimizer sees LDS that only direct loads write as unwritten and
   // folds reads of a separate weight array away)
   __shared__ __attribute__((aligned(1024))) int8_t ztile[kAssignStages][kAssignStageBytes];
+  lds_claim(ztile, int(sizeof(ztile)));   // ordered before the first fill by the row loop's barrier
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // KS > 0: d = 32 * KS known at compile time (straight-line MFMA chain the
   // scheduler can feed from LDS ahead of use); KS = 0: any d
@@ -576,6 +588,7 @@ __global__ __launch_bounds__(256, 2) void k_assign_wide(const int8_t* __restrict
                                                         int32_t* __restrict__ pidx) {
   constexpr int d = kMaxDim, KS = kMaxDim / 32;
   __shared__ __attribute__((aligned(1024))) int8_t ztile[2][kAssign2StageBytes];
+  lds_claim(ztile, int(sizeof(ztile)));   // ordered before the first fill by the row loop's barrier
   const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ntiles_all = (C + kAssignTile - 1) / kAssignTile;
   const int per_slice = (ntiles_all + int(gridDim.y) - 1) / int(gridDim.y);
@@ -1050,6 +1063,7 @@ __global__ __launch_bounds__(256) void k_leaf_scores(const int8_t* __restrict__ 
                                                      int mpad) {
   __shared__ int32_t mid[kMaxLeaf];
   __shared__ __attribute__((aligned(1024))) int8_t tile[2][32 * kMaxDim];
+  lds_claim(tile, int(sizeof(tile)));   // ordered before the first fill by the item loop's barrier
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int ksteps = KS > 0 ? KS : d >> 5, cpr = KS > 0 ? 2 * KS : d >> 4;
   const int cl = lane & 31, hi = lane >> 5;
@@ -1130,6 +1144,7 @@ __global__ __launch_bounds__(256, 2) void k_leaf_scores_wide(const int8_t* __res
   constexpr int d = kMaxDim, KS = kMaxDim / 32;
   __shared__ int32_t mid[kMaxLeaf];
   __shared__ __attribute__((aligned(1024))) int8_t tile[2][kAssign2StageW];
+  lds_claim(tile, int(sizeof(tile)));   // ordered before the first fill by the item loop's barrier
   const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int cl = lane & 31, hi = lane >> 5;
   auto issue = [&](int c0, int nc, int8_t* buf) {

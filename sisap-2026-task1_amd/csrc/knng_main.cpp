@@ -13,8 +13,24 @@
 // Extra flags (all optional; the reference has none of them):
 //   --rerank            FP32 rerank of each list before export (dists become
 //                       FP32 dot products; attribute dists_kind = fp32_dot)
+//   --plain             the plain int8 export (the default; undoes --rerank)
+//   --form chosen       the measured build form for large clustered inputs: 4 trees
+//                       seeded by 4 k-means cell partitions (2 seeds per target
+//                       cell), leaves of up to 2048 rows, a full join iteration then
+//                       one under NN-Descent's new/old rule, FP32 rerank of each
+//                       list's 32 best entries (implies --rerank)
+//   --form reference    the reference's own shape: 16 plain RP trees (no cells),
+//                       leaf 64, the reference stop rule
 //   --trees N  --k-build N  --leaf N  --iters N  --rerank-mult N  --device N
-//   --chunk-rows N      rows per HDF5 hyperslab / pinned buffer (default 275,000)
+//   --cells N           k-means cells seeding the trees (-1: n/1536, 0: plain RP trees)
+//   --partitions N  --oversample N  --kmeans-iters N  --cell-sample N
+//   --incremental 0|1   NN-Descent's new/old rule in join iterations after the first
+//   --rerank-width N    list entries the FP32 rerank re-scores (0: all k_build)
+//   --stop-rule reference|inserted|none
+//   --chunk-rows N      rows per HDF5 hyperslab / pinned buffer (default 275,000;
+//                       the reference reads 2.2M, src/main.cpp:26)
+// Flags apply in command-line order, so a --form preset can be adjusted by
+// the flags after it.
 
 #include <sys/stat.h>
 
@@ -79,6 +95,8 @@ int main(int argc, char** argv) {
       task = argv[++i];
     } else if (a == "--rerank") {
       rerank = 1;
+    } else if (a == "--plain") {
+      rerank = 0;
     } else if (a == "--trees" && num(&tmp)) {
       p.num_trees = int32_t(tmp);
     } else if (a == "--k-build" && num(&tmp)) {
@@ -89,6 +107,46 @@ int main(int argc, char** argv) {
       p.max_iters = int32_t(tmp);
     } else if (a == "--rerank-mult" && num(&tmp)) {
       p.rerank_mult = int32_t(tmp);
+    } else if (a == "--form" && i + 1 < argc) {
+      const std::string f = argv[++i];
+      if (f == "chosen") {
+        p.num_trees = 4;
+        p.leaf_size = 2048;
+        p.cells = -1;
+        p.cell_partitions = 4;
+        p.cell_oversample = 2;
+        p.incremental = 1;
+        p.max_iters = 2;
+        p.rerank_width = 32;
+        p.stop_rule = KNNG_STOP_INSERTED;
+        rerank = 1;
+      } else if (f == "reference") {
+        knng_default_params(&p);
+        p.cells = 0;
+        p.stop_rule = KNNG_STOP_REFERENCE;
+      } else {
+        std::cerr << "[WARN] unknown --form " << f << " (ignored)\n";
+      }
+    } else if (a == "--cells" && num(&tmp)) {
+      p.cells = int32_t(tmp);
+    } else if (a == "--partitions" && num(&tmp)) {
+      p.cell_partitions = int32_t(tmp);
+    } else if (a == "--oversample" && num(&tmp)) {
+      p.cell_oversample = int32_t(tmp);
+    } else if (a == "--kmeans-iters" && num(&tmp)) {
+      p.kmeans_iters = int32_t(tmp);
+    } else if (a == "--cell-sample" && num(&tmp)) {
+      p.cell_sample = int32_t(tmp);
+    } else if (a == "--incremental" && num(&tmp)) {
+      p.incremental = tmp ? 1 : 0;
+    } else if (a == "--rerank-width" && num(&tmp)) {
+      p.rerank_width = int32_t(tmp);
+    } else if (a == "--stop-rule" && i + 1 < argc) {
+      const std::string r = argv[++i];
+      if (r == "reference") p.stop_rule = KNNG_STOP_REFERENCE;
+      else if (r == "inserted") p.stop_rule = KNNG_STOP_INSERTED;
+      else if (r == "none") p.stop_rule = KNNG_STOP_NONE;
+      else std::cerr << "[WARN] unknown --stop-rule " << r << " (ignored)\n";
     } else if (a == "--device" && num(&tmp)) {
       device = int(tmp);
     } else if (a == "--chunk-rows" && num(&tmp)) {
@@ -201,11 +259,21 @@ int main(int argc, char** argv) {
   const std::string dir = "results/" + task;
   mkdir(dir.c_str(), 0777);
   const std::string out = dir + "/apex_simd.h5";
-  char params[256];
+  // the parameters in force, with the stop rule the build actually applied
+  char rule[96];
+  if (p.stop_rule == KNNG_STOP_REFERENCE)
+    snprintf(rule, sizeof(rule), "STOP=%.2f, DIFF=%.2f", double(p.early_stop_rate), double(p.rate_diff_tol));
+  else if (p.stop_rule == KNNG_STOP_INSERTED)
+    snprintf(rule, sizeof(rule), "STOP_NEW_IDS=%.3f", double(p.insert_stop_rate));
+  else
+    snprintf(rule, sizeof(rule), "STOP=none");
+  char params[512];
   snprintf(params, sizeof(params),
-           "K_BUILD=%d, RERANK=%d, TREES=%d, LEAF=%d, STOP=%.2f, DIFF=%.2f, DIM=%d, FP32_RERANK=%d",
-           p.k_build, p.rerank_mult, p.num_trees, p.leaf_size, double(p.early_stop_rate),
-           double(p.rate_diff_tol), p.dim, rerank);
+           "K_BUILD=%d, RERANK=%d, TREES=%d, LEAF=%d, %s, MAX_ITER=%d, DIM=%d, CELLS=%d, PARTITIONS=%d, "
+           "OVERSAMPLE=%d, INCREMENTAL=%d, FP32_RERANK=%d, RERANK_WIDTH=%d, ITERATIONS=%d",
+           p.k_build, p.rerank_mult, p.num_trees, p.leaf_size, rule, p.max_iters, p.dim, st.cells,
+           p.cell_partitions, p.cell_oversample, p.incremental, rerank,
+           rerank ? (p.rerank_width ? p.rerank_width : p.k_build) : 0, st.iterations);
   rc = knng_h5_write_result(out.c_str(), knns.data(), dists.data(), n, p.k_eval,
                             "Apex Hardware SIMD (MI355X HIP)", task.c_str(), float(buildtime), 0.0f,
                             params, "dists_kind", rerank ? "fp32_dot" : "int8_dot");

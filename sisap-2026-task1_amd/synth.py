@@ -116,3 +116,63 @@ def recall_at_k(knns_1based: np.ndarray, gt_0based: np.ndarray, rows: np.ndarray
     for r, g in zip(rows, gt_0based):
         hits += len(np.intersect1d(knns_1based[r] - 1, g[:k]))
     return hits / (len(rows) * k)
+
+
+def exact_knn_torch(x, rows, k: int, chunk: int = 262144):
+    """Exact FP32 dot-product top-k (0-based ids, best first) of rows x[rows]
+    over all rows of the device tensor x, self excluded.  The product is taken
+    in row chunks: one unchunked 500 x 6.35M float32 product in torch returned
+    a different top-15 for a quarter of the entries."""
+    import torch
+    q = x[rows]
+    R = q.shape[0]
+    best_s = torch.full((R, k + 1), float("-inf"), device=x.device)
+    best_i = torch.zeros((R, k + 1), dtype=torch.int64, device=x.device)
+    ar = torch.arange(R, device=x.device)
+    for a in range(0, x.shape[0], chunk):
+        b = min(x.shape[0], a + chunk)
+        s = q @ x[a:b].T
+        inside = (rows >= a) & (rows < b)
+        s[ar[inside], rows[inside] - a] = float("-inf")
+        ts, ti = s.topk(min(k + 1, b - a), dim=1)
+        cs = torch.cat([best_s, ts], 1)
+        ci = torch.cat([best_i, ti + a], 1)
+        best_s, o = cs.topk(k + 1, dim=1)
+        best_i = ci.gather(1, o)
+    return best_i[:, :k]
+
+
+def recall_rows(pred_1based: np.ndarray, gt_0based: np.ndarray, k: int) -> float:
+    """Evaluator rule on aligned rows: pred_1based[j] (1-based ids) against gt_0based[j]."""
+    hits = 0
+    for p, g in zip(pred_1based, gt_0based):
+        hits += len(np.intersect1d(np.asarray(p) - 1, g[:k]))
+    return hits / (len(gt_0based) * k)
+
+
+def exact_knn_prefix_torch(parts, qv, limits, k: int, chunk: int = 262144):
+    """Exact FP32 top-k (0-based ids, best first) of query vectors qv over the
+    rows of the device tensors `parts`, taken as one concatenated row space,
+    each query j seeing only rows [0, limits[j]) (an index that grew by inserts)."""
+    import torch
+    R = qv.shape[0]
+    best_s = torch.full((R, k), float("-inf"), device=qv.device)
+    best_i = torch.zeros((R, k), dtype=torch.int64, device=qv.device)
+    lim = limits.view(-1, 1)
+    top = int(limits.max().item())
+    base = 0
+    for part in parts:
+        for a in range(0, part.shape[0], chunk):
+            if base + a >= top:
+                break
+            b = min(part.shape[0], a + chunk)
+            s = qv @ part[a:b].T
+            cols = torch.arange(base + a, base + b, device=qv.device).view(1, -1)
+            s.masked_fill_(cols >= lim, float("-inf"))
+            ts, ti = s.topk(min(k, b - a), dim=1)
+            cs = torch.cat([best_s, ts], 1)
+            ci = torch.cat([best_i, ti + base + a], 1)
+            best_s, o = cs.topk(k, dim=1)
+            best_i = ci.gather(1, o)
+        base += part.shape[0]
+    return best_i

@@ -122,3 +122,25 @@ def test_cli_without_device_fails_cleanly():
         knng.h5_write_train(p, np.ones((100, 64), np.float32))
         r = _cli(["--dataset", p], td)
         assert r.returncode == 255 and "Device set-up failed" in r.stderr
+
+
+def test_direct_load_lds_arrays_kept():
+    """Every kernel that fills LDS with direct global->LDS loads keeps all of
+    its declared __shared__ bytes in the built code object (a dropped array
+    would fold its reads to undefined values).  The build writes the per-kernel
+    resources from clang's kernel-resource-usage remarks and refuses to link a
+    library that fails this; the test re-reads that report."""
+    import build
+    path = build.RESOURCES
+    assert os.path.exists(path), "build the library first (sisap-2026-task1_amd/build.py)"
+    recs = []
+    for line in open(path):
+        if line.startswith("#"):
+            continue
+        kernel, vgpr, agpr, scratch, occ, lds, mangled = line.rstrip("\n").split("\t")
+        recs.append({"kernel": kernel, "mangled": mangled, "LDS Size [bytes/block]": lds})
+    assert build.check_direct_lds(recs) == []
+    # and the check itself notices a dropped array
+    shrunk = [dict(r, **{"LDS Size [bytes/block]": str(int(r["LDS Size [bytes/block]"]) - 3072)})
+              if r["kernel"] == "k_assign_wide" else r for r in recs]
+    assert any("k_assign_wide" in b for b in build.check_direct_lds(shrunk))
