@@ -287,6 +287,56 @@ def test_c2_bench_form(torch_cuda):
     b.close()
 
 
+# ------------------------------------------------------------ c4: 512 jobs
+
+@pytest.mark.timeout(900)
+def test_c4_batched_uneven_builds(torch_cuda):
+    """c4: 512 independent builds, n log-uniform in [2000, 200000], d from
+    {384, 768, 1024}, 16-512 clusters, k from {10, 15, 32}, rows in HBM,
+    packed per dimension.  Per-job recall@k on 200 sampled rows; the three
+    smallest jobs export plain int8 lists that must equal the oracle's
+    device-mode build bit for bit."""
+    torch = torch_cuda
+    rng = np.random.default_rng(4)
+    jobs, data = [], []
+    for j in range(512):
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(200000))))
+        d = int(rng.choice([384, 768, 1024]))
+        cl = int(np.exp(rng.uniform(np.log(16), np.log(512))))
+        k = int(rng.choice([10, 15, 32]))
+        x = synth.gaussian_mixture_torch(4 * (1 << 20) + j, n, d, cl)
+        data.append(x)
+        p = knng.default_params(dim=d, k_eval=k, k_build=max(50, k), num_trees=8, keep_fp32=0, cells=0)
+        jobs.append({"rows": x.data_ptr(), "n": n, "params": p, "rerank": 1, "k": k, "d": d})
+    oracle_jobs = sorted(range(512), key=lambda j: jobs[j]["n"])[:3]
+    for j in oracle_jobs:
+        jobs[j]["rerank"] = 0
+    torch.cuda.synchronize()
+    say("c4", sum(j["n"] for j in jobs), "rows")
+    assert knng.build_batch(jobs, 0, 8) == 0
+    recs = []
+    for j, job in enumerate(jobs):
+        assert job["status"] == 0, j
+        x = data[j]
+        rows = np.sort(np.random.default_rng(j).choice(job["n"], min(200, job["n"]), replace=False))
+        gt = synth.exact_knn_torch(x, torch.from_numpy(rows).cuda(), job["k"]).cpu().numpy()
+        recs.append(synth.recall_rows(job["knns"][rows], gt, job["k"]))
+    recs = np.array(recs)
+    say(f"c4 recall@k mean {recs.mean():.4f} min {recs.min():.4f}")
+    assert recs.mean() >= 0.9 and recs.min() >= 0.6
+    for j in oracle_jobs:
+        job = jobs[j]
+        xh = data[j].cpu().numpy()
+        q, sk = oracle.quantize(xh)
+        o_ids, o_sc, _ = oracle.build_device_semantics(q, sk, trees=8, leaf=64, K=max(50, job["k"]),
+                                                       cells=0)
+        e_knns, e_dists = oracle.export(o_ids, o_sc, job["k"])
+        assert np.array_equal(job["knns"], e_knns), j
+        assert np.array_equal(job["dists"], e_dists), j
+    del data
+    torch.cuda.empty_cache()
+
+
 # ------------------------------------------------- c3 (6.35M) and c5 (serving)
 
 C3_N = 6_350_000
@@ -448,51 +498,3 @@ def test_c5_serving_on_c3_graph(torch_cuda, c3_build):
     rec = np.mean([len(np.intersect1d(s[2], gt[j])) / 15 for j, s in enumerate(samp)])
     say(f"c5 phase 2: {len(plan2)} requests, {n_ins} inserted rows, query recall@15 {rec:.4f}")
     assert rec >= 0.8
-
-
-# ------------------------------------------------------------ c4: 512 jobs
-
-@pytest.mark.timeout(900)
-def test_c4_batched_uneven_builds(torch_cuda):
-    """c4: 512 independent builds, n log-uniform in [2000, 200000], d from
-    {384, 768, 1024}, 16-512 clusters, k from {10, 15, 32}, rows in HBM,
-    packed per dimension.  Per-job recall@k on 200 sampled rows; the three
-    smallest jobs export plain int8 lists that must equal the oracle's
-    device-mode build bit for bit."""
-    torch = torch_cuda
-    rng = np.random.default_rng(4)
-    jobs, data = [], []
-    for j in range(512):
-        n = int(np.exp(rng.uniform(np.log(2000), np.log(200000))))
-        d = int(rng.choice([384, 768, 1024]))
-        cl = int(np.exp(rng.uniform(np.log(16), np.log(512))))
-        k = int(rng.choice([10, 15, 32]))
-        x = synth.gaussian_mixture_torch(4 * (1 << 20) + j, n, d, cl)
-        data.append(x)
-        p = knng.default_params(dim=d, k_eval=k, k_build=max(50, k), num_trees=8, keep_fp32=0, cells=0)
-        jobs.append({"rows": x.data_ptr(), "n": n, "params": p, "rerank": 1, "k": k, "d": d})
-    oracle_jobs = sorted(range(512), key=lambda j: jobs[j]["n"])[:3]
-    for j in oracle_jobs:
-        jobs[j]["rerank"] = 0
-    torch.cuda.synchronize()
-    say("c4", sum(j["n"] for j in jobs), "rows")
-    assert knng.build_batch(jobs, 0, 8) == 0
-    recs = []
-    for j, job in enumerate(jobs):
-        assert job["status"] == 0, j
-        x = data[j]
-        rows = np.sort(np.random.default_rng(j).choice(job["n"], min(200, job["n"]), replace=False))
-        gt = synth.exact_knn_torch(x, torch.from_numpy(rows).cuda(), job["k"]).cpu().numpy()
-        recs.append(synth.recall_rows(job["knns"][rows], gt, job["k"]))
-    recs = np.array(recs)
-    say(f"c4 recall@k mean {recs.mean():.4f} min {recs.min():.4f}")
-    assert recs.mean() >= 0.9 and recs.min() >= 0.6
-    for j in oracle_jobs:
-        job = jobs[j]
-        xh = data[j].cpu().numpy()
-        q, sk = oracle.quantize(xh)
-        o_ids, o_sc, _ = oracle.build_device_semantics(q, sk, trees=8, leaf=64, K=max(50, job["k"]),
-                                                       cells=0)
-        e_knns, e_dists = oracle.export(o_ids, o_sc, job["k"])
-        assert np.array_equal(job["knns"], e_knns), j
-        assert np.array_equal(job["dists"], e_dists), j
