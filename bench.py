@@ -404,12 +404,6 @@ def batch_main(args):
     print(json.dumps(out), flush=True)
 
 
-def exact_topk_prefix(torch, x, qv, limits, k, chunk=262144):
-    """Exact FP32 top-k of query vectors qv over rows x[:limits[j]] (per query)."""
-    import synth
-    return synth.exact_knn_prefix_torch([x], qv, limits, k, chunk)
-
-
 def serve_main(args):
     """Serving loop over a device-resident graph: a Poisson request stream
     (query batches log-uniform in [1, 4096], 10% inserts of 1-1024 rows) is
@@ -442,8 +436,11 @@ def serve_main(args):
     n_ins = sum(r[2] for r in reqs if r[1] == 1)
     n_q = sum(r[2] for r in reqs if r[1] == 0)
     log(f"serve: {len(reqs)} requests over {args.serve_seconds}s, {n_q} queries, {n_ins} inserted rows")
-    # rows: [base n][inserted rows, submission order][query rows]; one mixture
-    x = synth.gaussian_mixture_torch(5, n + n_ins + n_q, d, args.clusters, device="cuda")
+    # the build workload's rows (same seed), and held-out rows of the same mixture for the
+    # inserts (in submission order) and the queries
+    x = synth.gaussian_mixture_torch(args.seed, n, d, args.clusters, device="cuda")
+    extra = synth.gaussian_mixture_torch(args.seed, n_ins + n_q, d, args.clusters, device="cuda",
+                                         draw_seed=5)
     torch.cuda.synchronize()
     b = knng.Builder(n, build_params(knng, args, d))
     b.attach_fp32(x.data_ptr())
@@ -452,7 +449,7 @@ def serve_main(args):
     b.build()
     b.synchronize()
     log(f"graph built in {time.time() - t0:.2f}s")
-    xh = x[n:].cpu().numpy()
+    xh = extra.cpu().numpy()
     ins_rows, q_rows = xh[:n_ins], xh[n_ins:]
     sp = dict(beam=64, k=15, rerank=1, entry_trees=4, max_batch=args.serve_batch,
               max_slots=args.serve_slots, express_rows=64, express_slots=args.express_slots,
@@ -508,7 +505,7 @@ def serve_main(args):
     pick = np.random.default_rng(1).choice(len(samp), min(2000, len(samp)), replace=False)
     qv = torch.from_numpy(np.stack([q_rows[samp[p][0]] for p in pick])).cuda()
     lim = torch.tensor([samp[p][1] for p in pick], device="cuda")
-    gt = exact_topk_prefix(torch, x[: n + n_ins], qv, lim, 15).cpu().numpy()
+    gt = synth.exact_knn_prefix_torch([x, extra[:n_ins]], qv, lim, 15).cpu().numpy()
     rec = np.mean([len(np.intersect1d(samp[p][2], gt[j])) / 15 for j, p in enumerate(pick)])
     out = {
         "metric": "query p50/p99 ms (serving a 6.35M x 1024 graph)", "value": round(float(np.percentile(small, 99)), 3) if len(small) else None,

@@ -44,13 +44,18 @@ def gaussian_mixture(seed: int, n: int, d: int, clusters: int, chunk: int = 6553
 
 
 def gaussian_mixture_torch(seed: int, n: int, d: int, clusters: int, device="cuda",
-                           chunk: int = 262144, out=None):
-    """Same distribution generated directly in device memory (float32)."""
+                           chunk: int = 262144, out=None, draw_seed: int | None = None):
+    """Same distribution generated directly in device memory (float32).
+    draw_seed: held-out rows of the mixture `seed` describes -- the same cluster
+    centres, with assignments and noise drawn from a stream seeded by draw_seed."""
     import torch
     g = torch.Generator(device=device)
     g.manual_seed(seed)
     cent = torch.randn((clusters, d), generator=g, device=device, dtype=torch.float32)
     cent /= cent.norm(dim=1, keepdim=True)
+    if draw_seed is not None:
+        g = torch.Generator(device=device)
+        g.manual_seed(draw_seed)
     assign = torch.randint(0, clusters, (n,), generator=g, device=device)
     scale = torch.tensor(noise_scale(d), device=device, dtype=torch.float32)
     if out is None:

@@ -433,7 +433,8 @@ def test_c5_serving_on_c3_graph(torch_cuda, c3_build):
     ph2 = stream(0.25, 0.1)
     nq = sum(r[2] for r in ph1 + ph2 if r[1] == 0)
     n_ins = sum(r[2] for r in ph2 if r[1] == 1)
-    extra = synth.gaussian_mixture_torch(5, nq + n_ins, d, 4096)   # held-out rows, same mixture
+    # held-out rows of the same mixture: the c3 centres, another draw stream
+    extra = synth.gaussian_mixture_torch(3, nq + n_ins, d, 4096, draw_seed=5)
     eh = extra.cpu().numpy()
     qrows, irows = eh[:nq], eh[nq:]
     sp = dict(beam=64, k=15, rerank=1, entry_trees=4, max_batch=2048, max_slots=4, express_rows=64,
