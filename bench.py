@@ -119,7 +119,9 @@ def cpu_baseline(args, nproc):
     measured on two bounded samples of the workload (the same generator and
     rows per cluster); its buildtime is extrapolated to the full row count by
     the power law through the two measurements (the reference's time grows
-    faster than linearly: 2.75 s at 50k rows, 92.6 s at 1M).  Falls back to
+    faster than linearly: 2.75 s at 50k rows, 92.6 s at 1M; below 100k rows
+    the iteration count still changes with the size, so the samples are 100k
+    and 200k rows, about 30 s of CPU work).  Falls back to
     the single-thread oracle port when the reference binary cannot run."""
     import math
 
@@ -128,7 +130,7 @@ def cpu_baseline(args, nproc):
 
     threads = min(8, nproc)
     n2 = args.cpu_sample
-    n1 = max(2000, n2 // 4)
+    n1 = max(2000, n2 // 2)
     if os.path.exists(oracle.REF_BIN) and args.dim == 1024:
         m1 = _reference_run(args, n1, threads)
         m2 = _reference_run(args, n2, threads) if m1 else None
@@ -183,8 +185,8 @@ def main():
                     help="list entries the FP32 rerank re-scores (0: all k_build)")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--recall-rows", type=int, default=10000)
-    ap.add_argument("--cpu-sample", type=int, default=100000,
-                    help="rows of the larger CPU-baseline sample (a quarter of it is the smaller)")
+    ap.add_argument("--cpu-sample", type=int, default=200000,
+                    help="rows of the larger CPU-baseline sample (half of it is the smaller)")
     ap.add_argument("--cpu-timeout", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["build", "serve", "batch"], default="build",
@@ -309,7 +311,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log(f"CPU baseline on {max(2000, args.cpu_sample // 4)}- and {args.cpu_sample}-row samples")
+        log(f"CPU baseline on {max(2000, args.cpu_sample // 2)}- and {args.cpu_sample}-row samples")
         cpu = cpu_baseline(args, os.cpu_count() or 1)
 
     out = {

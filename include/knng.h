@@ -173,6 +173,33 @@ int knng_score_pairs_device(const int8_t* dev_q, const uint64_t* dev_sketch, int
                             const int32_t* dev_a, const int32_t* dev_b, int64_t npairs,
                             int32_t* dev_dot, int32_t* dev_sim);
 
+/* One join iteration (the full join: own list, 2-hop candidates, sketch
+ * prefilter keeping k_build * rerank_mult, int8 re-scoring, rank merge) over
+ * caller-owned DEVICE buffers, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * under HIP graph capture, tight serving or refinement loops, harnesses that
+ * forbid hipMalloc.  It computes exactly the lists knng_join_iteration
+ * computes from the same graph without the new/old rule.
+ *   q [n][d] int8, sketch [n][d/64]: the quantised rows (knng_quantize_device);
+ *   ids_in / scores_in [n][k_build]: the current lists, (score desc, id asc);
+ *   ids_out / scores_out [n][k_build]: the next lists (must not alias the inputs);
+ *   order: optional processing order of the rows (n int32, a permutation; NULL
+ *          = index order; the result does not depend on it);
+ *   counts [5] uint64 (device): changed slots (src/main.cpp:421-427 rule), unique
+ *          candidates, re-scored candidates, ids new to their list, candidates
+ *          whose sketches were gathered;
+ *   stream: a hipStream_t (NULL: the null stream).
+ * Stream-ordered, no host synchronisation.  Its only scratch is a static
+ * device array of 43,008 bytes per GPU (join counters and row queues), held
+ * until the process ends and shared by every call on that GPU: calls that
+ * overlap on one GPU race.  Limits: d a multiple of 64 up to 1024, k_build
+ * 1..64, rerank_mult >= 1, 2 <= n < 2^31 - 1 (KNNG_ERANGE otherwise). */
+int knng_join_iteration_noalloc(const int8_t* dev_q, const uint64_t* dev_sketch, int32_t d, int64_t n,
+                                int32_t k_build, int32_t rerank_mult, const int32_t* dev_ids_in,
+                                const int32_t* dev_scores_in, int32_t* dev_ids_out,
+                                int32_t* dev_scores_out, const int32_t* dev_order,
+                                uint64_t* dev_counts, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched builds of many independent graphs of uneven size (no reference
  * counterpart: the reference builds one graph of DIM = 1024).  Each job is a

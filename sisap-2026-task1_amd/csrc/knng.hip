@@ -2950,13 +2950,17 @@ int ensure_join_variant(knng_ctx* c, int a, int b, const int32_t* order, int64_t
   int rc;
   if ((rc = dalloc(c, &c->tune_n, 1, "tuning size"))) return rc;
   HIPCHK(c, hipMemcpyAsync(c->tune_n, &rows, sizeof(rows), hipMemcpyHostToDevice, c->stream));
-  hipEvent_t e0, e1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   HIPCHK(c, hipEventCreate(&e0));
-  HIPCHK(c, hipEventCreate(&e1));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(c, KNNG_EHIP, "hipEventCreate failed (join autotuner)");
+  }
   float best_ms = 0, total = 0;
   int best = 0;
-  for (int round = 0; round < 2; ++round)
-    for (int v = 0; v < kJoinVariants; ++v) {
+  // a failed launch ends the tuning at once: no later timing may pick a shape
+  for (int round = 0; round < 2 && !rc; ++round)
+    for (int v = 0; v < kJoinVariants && !rc; ++v) {
       (void)hipEventRecord(e0, c->stream);
       if ((rc = launch_join(c, v, a, b, order, c->tune_n, seg_base, nseg, segc))) break;
       (void)hipEventRecord(e1, c->stream);
@@ -2979,6 +2983,55 @@ int ensure_join_variant(knng_ctx* c, int a, int b, const int32_t* order, int64_t
   std::lock_guard<std::mutex> lk(tune_cache().mu);
   tune_cache().best[key] = best;
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free join iteration (knng_join_iteration_noalloc).  Its only
+// device memory beyond the caller's inputs and outputs is `scratch`: the
+// join's per-block counters and per-XCD row-queue counters, 43,008 bytes per
+// GPU, held until the process ends.  Host code never passes the array's name
+// to a kernel: each device's address comes from hipGetSymbolAddress, taken
+// once per device with that device current.  Every call on a GPU shares that
+// GPU's array, so calls that overlap on one GPU (on different streams) race.
+// ---------------------------------------------------------------------------
+
+constexpr int kScratchWords = kCounterFields * kNumCounters + kRowQueueWords;
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+struct ScratchTable {
+  std::mutex mu;
+  unsigned long long* ptr[kMaxDevices] = {};
+  int num_cu[kMaxDevices] = {};
+};
+
+ScratchTable& scratch_table() {
+  static ScratchTable t;
+  return t;
+}
+
+// this device's copy of `scratch` and its CU count (hipGetSymbolAddress on first use)
+int scratch_for_device(int dev, unsigned long long** ptr, int* num_cu) {
+  if (dev < 0 || dev >= kMaxDevices) return KNNG_ERANGE;
+  ScratchTable& t = scratch_table();
+  std::lock_guard<std::mutex> lk(t.mu);
+  if (!t.ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return KNNG_EHIP;
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
+      return KNNG_EHIP;
+    t.ptr[dev] = static_cast<unsigned long long*>(p);
+    t.num_cu[dev] = cu;
+  }
+  *ptr = t.ptr[dev];
+  *num_cu = t.num_cu[dev];
+  return 0;
+}
+
+// v[0..m) = 0 (a kernel, not hipMemsetAsync, so a captured graph replays it correctly)
+__global__ void k_zero_u64(unsigned long long* __restrict__ v, int m) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x) v[j] = 0ull;
 }
 
 }  // namespace
@@ -3724,6 +3777,56 @@ int knng_quantize_device(const float* dx, int64_t n, int32_t d, int8_t* dq, uint
   hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, 0, dx, n, d, dq, dsk);
   if (hipGetLastError() != hipSuccess) return KNNG_EHIP;
   return hipDeviceSynchronize() == hipSuccess ? 0 : KNNG_EHIP;
+}
+
+int knng_join_iteration_noalloc(const int8_t* dq, const uint64_t* dsk, int32_t d, int64_t n,
+                                int32_t k_build, int32_t rerank_mult, const int32_t* ids_in,
+                                const int32_t* sc_in, int32_t* ids_out, int32_t* sc_out,
+                                const int32_t* order, uint64_t* counts, void* stream) {
+  if (!dq || !dsk || !ids_in || !sc_in || !ids_out || !sc_out || !counts) return KNNG_EINVAL;
+  if (ids_in == ids_out || sc_in == sc_out) {
+    fprintf(stderr, "[knng] knng_join_iteration_noalloc: input and output lists must differ\n");
+    return KNNG_EINVAL;
+  }
+  if (d <= 0 || d % 64 || d > kMaxDim || k_build < 1 || k_build > kMaxK || rerank_mult < 1 || n < 2 ||
+      n >= (int64_t(1) << 31) - 1) {
+    fprintf(stderr,
+            "[knng] knng_join_iteration_noalloc: sizes beyond its limits (d a multiple of 64 up to %d, "
+            "1 <= k_build <= %d, rerank_mult >= 1, 2 <= n < 2^31 - 1)\n",
+            kMaxDim, kMaxK);
+    return KNNG_ERANGE;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return KNNG_EHIP;
+  unsigned long long* ctr = nullptr;
+  int num_cu = 0;
+  if (int rc = scratch_for_device(dev, &ctr, &num_cu)) {
+    fprintf(stderr, "[knng] knng_join_iteration_noalloc: no scratch address on device %d\n", dev);
+    return rc;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int K = k_build, M = k_build * rerank_mult;
+  const int grid = std::max(8, (num_cu * 4) / 8 * 8);   // a multiple of the 8 XCDs, as launch_join
+  unsigned long long* rowq = ctr + kCounterFields * kNumCounters;
+  unsigned long long* out = reinterpret_cast<unsigned long long*>(counts);
+  hipLaunchKernelGGL(k_zero_u64, dim3((kScratchWords + 255) / 256), dim3(256), 0, st, ctr, kScratchWords);
+#define KNNG_NOALLOC_ARGS                                                                                  \
+  dq, dsk, d, n, K, M, ids_in, sc_in, ids_out, sc_out, ctr, rowq, order, static_cast<const uint64_t*>(nullptr), \
+      static_cast<uint64_t*>(nullptr), static_cast<const int64_t*>(nullptr),                                   \
+      static_cast<const int64_t*>(nullptr), 0, static_cast<unsigned long long*>(nullptr),                      \
+      static_cast<int32_t*>(nullptr), static_cast<unsigned long long*>(nullptr)
+  if (K > 50 && d == kMaxDim)
+    hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, st, KNNG_NOALLOC_ARGS);
+  else if (K > 50)
+    hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, 0>), dim3(grid), dim3(256), 0, st, KNNG_NOALLOC_ARGS);
+  else if (d == kMaxDim)
+    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, st, KNNG_NOALLOC_ARGS);
+  else
+    hipLaunchKernelGGL((k_join<12, 2560, 4, 2, 8, 0>), dim3(grid), dim3(256), 0, st, KNNG_NOALLOC_ARGS);
+#undef KNNG_NOALLOC_ARGS
+  // sums into the caller's counts and leaves the counters at zero
+  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(256), 0, st, ctr, out);
+  return hipGetLastError() == hipSuccess ? 0 : KNNG_EHIP;
 }
 
 int knng_score_pairs_device(const int8_t* dq, const uint64_t* dsk, int32_t d, const int32_t* da,

@@ -25,7 +25,7 @@ KNNG_SYMBOLS = [
     "knng_export", "knng_export_device", "knng_get_stats", "knng_synchronize",
     "knng_copy_quantized", "knng_copy_forest", "knng_copy_graph", "knng_copy_cells",
     "knng_load_graph",
-    "knng_quantize_device", "knng_score_pairs_device",
+    "knng_quantize_device", "knng_score_pairs_device", "knng_join_iteration_noalloc",
     "knng_server_default_params", "knng_server_create", "knng_server_destroy",
     "knng_server_last_error", "knng_server_submit", "knng_server_poll", "knng_server_wait",
     "knng_server_size", "knng_server_search", "knng_server_wait_info", "knng_server_info",
@@ -139,6 +139,7 @@ def lib() -> C.CDLL:
             "knng_load_graph": ([V, V, V], i32),
             "knng_quantize_device": ([V, i64, i32, V, V], i32),
             "knng_score_pairs_device": ([V, V, i32, V, V, i64, V, V], i32),
+            "knng_join_iteration_noalloc": ([V, V, i32, i64, i32, i32, V, V, V, V, V, V, V], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -329,6 +330,18 @@ def score_pairs_device(q_ptr, sk_ptr, d, a_ptr, b_ptr, npairs, dot_ptr, sim_ptr)
                                        C.c_void_p(sim_ptr))
     if rc:
         raise KnngError(f"knng_score_pairs_device failed ({rc})")
+
+
+def join_iteration_noalloc(q_ptr, sk_ptr, d, n, k_build, rerank_mult, ids_in_ptr, sc_in_ptr,
+                           ids_out_ptr, sc_out_ptr, counts_ptr, order_ptr=None, stream=None) -> None:
+    """knng_join_iteration_noalloc on device buffers (stream-ordered, no sync)."""
+    rc = lib().knng_join_iteration_noalloc(
+        C.c_void_p(q_ptr), C.c_void_p(sk_ptr), d, n, k_build, rerank_mult, C.c_void_p(ids_in_ptr),
+        C.c_void_p(sc_in_ptr), C.c_void_p(ids_out_ptr), C.c_void_p(sc_out_ptr),
+        C.c_void_p(order_ptr) if order_ptr else None, C.c_void_p(counts_ptr),
+        C.c_void_p(stream) if stream else None)
+    if rc:
+        raise KnngError(f"knng_join_iteration_noalloc failed ({rc})")
 
 
 # -- HDF5 helpers -----------------------------------------------------------
