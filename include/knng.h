@@ -98,6 +98,8 @@ typedef struct knng_stats {
   int64_t join_passed_on[32];  /* rows the small-row join kernel passed to the full one (iterations
                                   under the new/old rule after the first); -1: the full kernel
                                   ran every row */
+  int64_t join_leafmates[32];  /* unique 2-hop candidates dropped because they share a leaf with
+                                  the row (scored exactly at init, so they cannot enter its list) */
 } knng_stats;
 
 typedef struct knng_ctx knng_ctx;
@@ -185,12 +187,12 @@ int knng_score_pairs_device(const int8_t* dev_q, const uint64_t* dev_sketch, int
  *   ids_out / scores_out [n][k_build]: the next lists (must not alias the inputs);
  *   order: optional processing order of the rows (n int32, a permutation; NULL
  *          = index order; the result does not depend on it);
- *   counts [5] uint64 (device): changed slots (src/main.cpp:421-427 rule), unique
+ *   counts [6] uint64 (device): changed slots (src/main.cpp:421-427 rule), unique
  *          candidates, re-scored candidates, ids new to their list, candidates
- *          whose sketches were gathered;
+ *          whose sketches were gathered, 0 (no leaf-mate rule without a forest);
  *   stream: a hipStream_t (NULL: the null stream).
  * Stream-ordered, no host synchronisation.  Its only scratch is a static
- * device array of 43,008 bytes per GPU (join counters and row queues), held
+ * device array of 51,200 bytes per GPU (join counters and row queues), held
  * until the process ends and shared by every call on that GPU: calls that
  * overlap on one GPU race.  Limits: d a multiple of 64 up to 1024, k_build
  * 1..64, rerank_mult >= 1, 2 <= n < 2^31 - 1 (KNNG_ERANGE otherwise). */

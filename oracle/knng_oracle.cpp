@@ -36,7 +36,8 @@
 //                        separated clusters are recovered exactly; each
 //                        centroid equals the requantised sum of its cell
 //   orc_init_device      brute-force restatement in numpy (tests)
-//   orc_join_device      brute-force restatement in numpy (tests)
+//   orc_join_device      brute-force restatement in numpy (tests), with and
+//                        without the leaf-mate rule
 //   orc_rerank_device    FP64 numpy dot products
 //   orc_export           reference golden outputs (through reference mode)
 
@@ -821,10 +822,15 @@ int32_t orc_sketch_threshold(const int32_t* h, int64_t cnt, int32_t M, int32_t b
 // the previous iteration): incremental join -- the pair (i -> n1 -> c) is a
 // candidate only if n1 is fresh in i's list or c is fresh in n1's list.
 // fresh_out (optional) receives the same masks for the new lists.
+// leafkey (optional, [n][T]: row r's leaf in tree t, as its first position in
+// the tree's permutation) when the lists descend from that forest's leaf init:
+// a candidate sharing a leaf with row i in any tree is dropped before the
+// sketch prefilter (the init scored it exactly, so it cannot enter i's list).
 int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t d, int32_t K,
                         int32_t M, const int32_t* ids_in, const int32_t* sc_in,
                         int32_t* ids_out, int32_t* sc_out, int64_t* inserted,
-                        const uint64_t* fresh_in, uint64_t* fresh_out) {
+                        const uint64_t* fresh_in, uint64_t* fresh_out, const int32_t* leafkey,
+                        int32_t T) {
   int64_t ins = 0;
   const int words = d / 64;
   int64_t updates = 0;
@@ -846,6 +852,15 @@ int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t 
         if (!own_fresh && !((fresh_in[own[k]] >> k2) & 1ull)) continue;
         if (seen.insert(nb[k2]).second) cand.push_back(nb[k2]);
       }
+    }
+    if (leafkey) {
+      size_t keep = 0;
+      for (int32_t c : cand) {
+        bool mate = false;
+        for (int t = 0; t < T && !mate; ++t) mate = leafkey[int64_t(c) * T + t] == leafkey[i * T + t];
+        if (!mate) cand[keep++] = c;
+      }
+      cand.resize(keep);
     }
     const uint64_t* si = sk + i * words;
     for (int32_t c : cand) hval.push_back(sketch_sim(si, sk + int64_t(c) * words, words));

@@ -53,7 +53,7 @@ def lib() -> C.CDLL:
             "orc_forest_device": ([V, i64, i32, i32, i32, u64, V, V, i64], i64),
             "orc_init_device": ([V, i64, i32, i32, i32, V, V, i64, V, V], i32),
             "orc_sketch_threshold": ([V, i64, i32, i32], i32),
-            "orc_join_device": ([V, V, i64, i32, i32, i32, V, V, V, V, C.POINTER(i64), V, V], i64),
+            "orc_join_device": ([V, V, i64, i32, i32, i32, V, V, V, V, C.POINTER(i64), V, V, V, i32], i64),
             "orc_rerank_device": ([V, i64, i32, i32, i32, i32, V, V, V], i32),
             "orc_search_device": ([V, i64, i32, V, i32, i32, i32, u64, i32, i32, i32, i32, V, i64,
                                    V, V, i32, V, V, V], i32),
@@ -223,9 +223,20 @@ def sketch_threshold(h: np.ndarray, M: int, bits: int) -> int:
     return int(lib().orc_sketch_threshold(_p(h), h.shape[0], M, bits))
 
 
-def join_device(q, sk, ids, sc, M: int, with_inserted: bool = False, fresh=None):
+def leaf_keys(perm: np.ndarray, leaves: np.ndarray) -> np.ndarray:
+    """[n][T] int32: row r's leaf in tree t, as the leaf's first position in perm[t]."""
+    T, n = perm.shape
+    key = np.full((n, T), -1, np.int32)
+    for t, s, z in np.asarray(leaves, np.int64):
+        key[perm[t, s:s + z], t] = s
+    assert (key >= 0).all(), "every row lies in one leaf of every tree"
+    return key
+
+
+def join_device(q, sk, ids, sc, M: int, with_inserted: bool = False, fresh=None, leafkey=None):
     """One Jacobi join iteration -> (ids, scores, updates[, inserted][, fresh_out]).
-    fresh (u64 per row) selects the incremental join and adds fresh_out."""
+    fresh (u64 per row) selects the incremental join and adds fresh_out;
+    leafkey ([n][T], from leaf_keys) applies the leaf-mate rule."""
     n, d = q.shape
     K = ids.shape[1]
     oi = np.empty_like(ids)
@@ -233,9 +244,11 @@ def join_device(q, sk, ids, sc, M: int, with_inserted: bool = False, fresh=None)
     ins = C.c_int64()
     fo = np.empty(n, np.uint64)
     fi = None if fresh is None else _p(np.ascontiguousarray(fresh, np.uint64))
+    lk = None if leafkey is None else np.ascontiguousarray(leafkey, np.int32)
     upd = lib().orc_join_device(_p(q), _p(sk), n, d, K, M, _p(np.ascontiguousarray(ids)),
                                 _p(np.ascontiguousarray(sc)), _p(oi), _p(os_), C.byref(ins),
-                                fi, _p(fo))
+                                fi, _p(fo), None if lk is None else _p(lk),
+                                0 if lk is None else lk.shape[1])
     assert upd >= 0
     out = [oi, os_, int(upd)]
     if with_inserted:
@@ -273,6 +286,7 @@ def build_device_semantics(q, sk, trees=16, leaf=64, K=50, mult=8, max_iters=30,
                         full_iters, prune)
     perm, leaves = forest_cells_device(q, trees, leaf, plan, seed)
     ids, sc = init_device(q, perm, leaves, K)
+    lk = leaf_keys(perm, leaves)   # the lists descend from this forest: leaf-mate rule
     n = q.shape[0]
     prev = np.float32(1.0)
     rates = []
@@ -280,9 +294,9 @@ def build_device_semantics(q, sk, trees=16, leaf=64, K=50, mult=8, max_iters=30,
     for it in range(max_iters):
         if incremental:
             ids, sc, upd, ins, fresh = join_device(q, sk, ids, sc, K * mult, with_inserted=True,
-                                                   fresh=fresh)
+                                                   fresh=fresh, leafkey=lk)
         else:
-            ids, sc, upd, ins = join_device(q, sk, ids, sc, K * mult, with_inserted=True)
+            ids, sc, upd, ins = join_device(q, sk, ids, sc, K * mult, with_inserted=True, leafkey=lk)
         rate = np.float32(upd) / np.float32(n * K)
         rates.append(float(rate))
         if stop_rule == STOP_INSERTED:

@@ -343,6 +343,21 @@ __global__ __launch_bounds__(256) void k_forest_partition(
   }
 }
 
+// leafkey[r * TP + t] = first perm position of row r's leaf in tree t, one
+// wave per leaf (every row lies in exactly one leaf of every tree)
+__global__ __launch_bounds__(256) void k_leaf_keys(const int4* __restrict__ leaves, int64_t nleaves,
+                                                   const int32_t* __restrict__ perm, int64_t n, int TP,
+                                                   int32_t* __restrict__ leafkey) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  for (int64_t l = wave; l < nleaves; l += nwaves) {
+    const int4 lf = leaves[l];
+    const int32_t* mem = perm + int64_t(lf.x) * n + lf.y;
+    for (int j = lane; j < lf.z; j += 64) leafkey[int64_t(mem[j]) * TP + lf.x] = lf.y;
+  }
+}
+
 // Append one level's nodes to the persistent hierarchy, turning next-level
 // child indices into global ones (the next level is appended right after).
 __global__ void k_hier_append(const Node* __restrict__ nodes, int nnodes, Node* __restrict__ hier,
@@ -1671,8 +1686,10 @@ struct JoinScratch {
                               // 64-bit masks per neighbour would be hoisted out of the row loop and
                               // spilled)
   uint64_t own_sk[kMaxDim / 64];
+  int32_t own_leaf[kMaxTrees];   // the row's leaf (first perm position) in every tree
   int32_t hist[kMaxDim + 1];
-  int32_t cnt[5];   // 0 unique candidates, 1 selected, 2 survivors, 3 threshold, 4 new ids
+  int32_t cnt[6];   // 0 unique candidates, 1 selected, 2 survivors, 3 threshold, 4 new ids,
+                    // 5 candidates left after dropping leaf-mates
   int lds[8];
 };
 
@@ -1688,7 +1705,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     const uint64_t* __restrict__ fresh_in, uint64_t* __restrict__ fresh_out,
     const int64_t* __restrict__ nwork_dev, const int64_t* __restrict__ seg_base, int nseg,
     unsigned long long* __restrict__ segc, int32_t* __restrict__ ovf,
-    unsigned long long* __restrict__ ovf_n) {
+    unsigned long long* __restrict__ ovf_n, const int32_t* __restrict__ leafkey, int T, int TP) {
   // order[0 .. nwork) lists the rows to process (nwork = *nwork_dev if given,
   // else n).  With segments (independent builds packed together, nseg > 0),
   // each row's changed-slot and new-id counts are also added to its build's
@@ -1705,6 +1722,13 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   // the 4 waves checks before inserting at most 64 more, so at most CAP are
   // ever held) and is appended to ovf[*ovf_n]; the full kernel then runs
   // those rows.  It writes nothing for such a row.
+  // leafkey != nullptr (the graph came from this forest's leaf init): a
+  // candidate that shares a leaf with the row in any of the T trees is
+  // dropped.  The init scored every leaf-mate exactly and kept it in the
+  // row's 64 slots of that tree unless 64 mates beat it, so it is either in
+  // the list already or below the init list's K-th entry (K <= 64), and lists
+  // only improve: it could never enter.  leafkey[r * TP + t] = the first perm
+  // position of row r's leaf in tree t.
   constexpr int HS = 1 << HSBITS;
   static_assert(HS * 4 >= CAP * 2 + CAP * 4, "selection arrays must fit in the hash area");
   __shared__ __attribute__((aligned(16))) int32_t hs[HS];
@@ -1764,7 +1788,8 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     for (int j = tid; j < HS; j += 256) hs[j] = -1;
     if constexpr (!SMALL)
       for (int j = tid; j <= d; j += 256) S.hist[j] = 0;
-    if (tid < 5) S.cnt[tid] = 0;
+    if (tid < 6) S.cnt[tid] = 0;
+    if (leafkey && tid < T) S.own_leaf[tid] = leafkey[i * TP + tid];
     for (int c = tid; c < nch; c += 256)
       *reinterpret_cast<v4i*>(own_row + 16 * c) = *reinterpret_cast<const v4i*>(q + i * d + 16 * c);
     if (tid < words) S.own_sk[tid] = sk[i * words + tid];
@@ -1806,10 +1831,46 @@ __global__ __launch_bounds__(256, OCC) void k_join(
       }
     }
     __syncthreads();
-    const int U = S.cnt[0];
+    int U = S.cnt[0];
     if (SMALL && U > lim) {   // uniform: the full kernel runs this row
       if (tid == 0) ovf[atomicAdd(ovf_n, 1ull)] = int32_t(i);
     } else {
+      if (leafkey) {
+        // ---- 2b: drop leaf-mates (one 4 * TP byte gather per unique candidate,
+        //          all of a thread's in flight together), compact the rest
+        constexpr int kPer = (CAP + 255) / 256;
+        int32_t v[kPer];
+        bool keep[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+          const int u = tid + 256 * j;
+          v[j] = u < U ? cid[u] : -1;
+          keep[j] = v[j] >= 0;
+        }
+        for (int t0 = 0; t0 < T; t0 += 4) {
+          int4 lk[kPer];
+#pragma unroll
+          for (int j = 0; j < kPer; ++j)
+            lk[j] = keep[j] ? *reinterpret_cast<const int4*>(leafkey + int64_t(v[j]) * TP + t0)
+                            : make_int4(-1, -1, -1, -1);
+#pragma unroll
+          for (int j = 0; j < kPer; ++j) {
+            const bool mate = lk[j].x == S.own_leaf[t0] || (t0 + 1 < T && lk[j].y == S.own_leaf[t0 + 1]) ||
+                              (t0 + 2 < T && lk[j].z == S.own_leaf[t0 + 2]) ||
+                              (t0 + 3 < T && lk[j].w == S.own_leaf[t0 + 3]);
+            keep[j] = keep[j] && !mate;
+          }
+        }
+        __syncthreads();   // every cid entry read
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+          const int s = wave_append(keep[j], &S.cnt[5]);
+          if (keep[j]) cid[s] = v[j];
+        }
+        __syncthreads();
+        if (tid == 0) my_ctr[5] += U - S.cnt[5];
+        U = S.cnt[5];
+      }
       if constexpr (SMALL) {
         // at most M candidates: the selection is the identity
         for (int u = tid; u < U; u += 256) sel_idx[u] = uint16_t(u);
@@ -2854,10 +2915,11 @@ int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const in
   const int grid = std::max(8, (c->num_cu * 4) / 8 * 8);
   unsigned long long* rowq = c->counters + kCounterFields * kNumCounters;
   const uint64_t* fin = c->p.incremental ? c->fresh[a] : nullptr;
+  const int32_t* leafkey = c->leaf_valid ? c->leafkey : nullptr;
 #define KNNG_JOIN_ARGS                                                                          \
   c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order, \
       fin, c->fresh[b], nwork_dev, seg_base, nseg, segc, static_cast<int32_t*>(nullptr),             \
-      static_cast<unsigned long long*>(nullptr)
+      static_cast<unsigned long long*>(nullptr), leafkey, int(c->p.num_trees), c->leaf_tp
   const bool d1024 = c->d == kMaxDim;   // the rows' length as a compile-time constant
   if (K > 50 && d1024)
     hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
@@ -2899,11 +2961,12 @@ int launch_join_small(knng_ctx* c, int a, int b, const int32_t* order) {
   }
   HIPCHK(c, hipMemsetAsync(c->ovf_n, 0, sizeof(unsigned long long), c->stream));
   const int grid = std::max(8, (c->num_cu * kSmallJoinOcc) / 8 * 8);
+  const int32_t* leafkey = c->leaf_valid ? c->leafkey : nullptr;
   unsigned long long* rowq = c->counters + kCounterFields * kNumCounters;
 #define KNNG_SMALL_ARGS                                                                                 \
   c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order,     \
       c->fresh[a], c->fresh[b], static_cast<const int64_t*>(nullptr), static_cast<const int64_t*>(nullptr), \
-      0, static_cast<unsigned long long*>(nullptr), c->ovf, c->ovf_n
+      0, static_cast<unsigned long long*>(nullptr), c->ovf, c->ovf_n, leafkey, int(c->p.num_trees), c->leaf_tp
   if (c->d == kMaxDim)
     hipLaunchKernelGGL((k_join<10, 640, kSmallJoinOcc, 1, 8, kMaxDim, true>), dim3(grid), dim3(256), 0, c->stream,
                        KNNG_SMALL_ARGS);
@@ -2988,7 +3051,7 @@ int ensure_join_variant(knng_ctx* c, int a, int b, const int32_t* order, int64_t
 // ---------------------------------------------------------------------------
 // Allocation-free join iteration (knng_join_iteration_noalloc).  Its only
 // device memory beyond the caller's inputs and outputs is `scratch`: the
-// join's per-block counters and per-XCD row-queue counters, 43,008 bytes per
+// join's per-block counters and per-XCD row-queue counters, 51,200 bytes per
 // GPU, held until the process ends.  Host code never passes the array's name
 // to a kernel: each device's address comes from hipGetSymbolAddress, taken
 // once per device with that device current.  Every call on a GPU shares that
@@ -3151,6 +3214,7 @@ int knng_destroy(knng_ctx* c) {
   }
   dfree(&c->counters);
   dfree(&c->csum);
+  dfree(&c->leafkey);
   dfree(&c->seg_base);
   dfree(&c->tune_n);
   dfree(&c->ex_knns);
@@ -3526,6 +3590,16 @@ int knng_build_init(knng_ctx* c) {
     hipLaunchKernelGGL(k_init_merge<4096>, dim3(mgrid), dim3(256), 0, c->stream, c->q, c->d, n, T, K,
                        cand, c->gid[0], c->gsc[0], sb, c->nseg);
   HIPCHK(c, hipGetLastError());
+  // every row's leaf in every tree, for the join's leaf-mate rule
+  c->leaf_tp = (T + 3) & ~3;
+  c->leaf_valid = false;
+  if (!c->leafkey && (rc = dalloc(c, &c->leafkey, size_t(n) * c->leaf_tp, "leaf keys"))) {
+    dfree(&cand);
+    return rc;
+  }
+  hipLaunchKernelGGL(k_leaf_keys, dim3(grid_for(c, c->nleaves, 4, 16)), dim3(256), 0, c->stream, c->leaves,
+                     c->nleaves, c->perm, n, c->leaf_tp, c->leafkey);
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
@@ -3533,6 +3607,7 @@ int knng_build_init(knng_ctx* c) {
   c->st.ms_init = ms;
   dfree(&cand);
   c->have_graph = true;
+  c->leaf_valid = true;
   return 0;
 }
 
@@ -3575,10 +3650,13 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
     c->st.join_inserted[it] = int64_t(sums[3]);
     c->st.insert_rates[it] = float(sums[3]) / float(size_t(n) * size_t(K));
     c->st.join_passed_on[it] = small ? int64_t(passed) : -1;
+    c->st.join_leafmates[it] = int64_t(sums[5]);
     // algorithmic bytes: own list + own row/sketch, K neighbour id rows, one
-    // sketch per candidate of a row that gathers them, one row per re-scored
-    // candidate, writes
-    c->st.join_bytes[it] = double(n) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) +
+    // leaf-key row per unique candidate (leaf-mate rule), one sketch per
+    // candidate of a row that gathers them, one row per re-scored candidate,
+    // writes
+    const double lk_bytes = c->leaf_valid ? double(sums[1] + sums[5]) * 4.0 * c->leaf_tp : 0.0;
+    c->st.join_bytes[it] = double(n) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) + lk_bytes +
                            double(sums[4]) * 8.0 * c->words + double(sums[2]) * c->d;
   }
   c->st.iterations = it + 1;
@@ -3766,6 +3844,7 @@ int knng_load_graph(knng_ctx* c, const int32_t* ids, const int32_t* scores) {
   c->fresh_all = true;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_graph = true;
+  c->leaf_valid = false;   // a graph from elsewhere: its lists do not bound leaf-mates
   return 0;
 }
 
@@ -3814,7 +3893,8 @@ int knng_join_iteration_noalloc(const int8_t* dq, const uint64_t* dsk, int32_t d
   dq, dsk, d, n, K, M, ids_in, sc_in, ids_out, sc_out, ctr, rowq, order, static_cast<const uint64_t*>(nullptr), \
       static_cast<uint64_t*>(nullptr), static_cast<const int64_t*>(nullptr),                                   \
       static_cast<const int64_t*>(nullptr), 0, static_cast<unsigned long long*>(nullptr),                      \
-      static_cast<int32_t*>(nullptr), static_cast<unsigned long long*>(nullptr)
+      static_cast<int32_t*>(nullptr), static_cast<unsigned long long*>(nullptr),                               \
+      static_cast<const int32_t*>(nullptr), 0, 4
   if (K > 50 && d == kMaxDim)
     hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, st, KNNG_NOALLOC_ARGS);
   else if (K > 50)
@@ -3994,10 +4074,12 @@ int build_packed(knng_ctx* c, std::vector<int32_t>* iters, std::vector<int32_t>*
       c->st.join_rescored[it] = int64_t(L.csum_h[2]);
       c->st.join_inserted[it] = int64_t(L.csum_h[3]);
       c->st.join_passed_on[it] = -1;
+      c->st.join_leafmates[it] = int64_t(L.csum_h[5]);
       c->st.insert_rates[it] = float(L.csum_h[3]) / float(size_t(c->n) * size_t(K));
       // algorithmic bytes as knng_join_iteration counts them, over the rows processed
+      const double lk_bytes = c->leaf_valid ? double(L.csum_h[1] + L.csum_h[5]) * 4.0 * c->leaf_tp : 0.0;
       c->st.join_bytes[it] = double(rows) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) +
-                             double(L.csum_h[4]) * 8.0 * c->words + double(L.csum_h[2]) * c->d;
+                             lk_bytes + double(L.csum_h[4]) * 8.0 * c->words + double(L.csum_h[2]) * c->d;
     }
     c->st.iterations = it + 1;
     c->st.ms_join += ms;

@@ -19,8 +19,8 @@ constexpr int kLeafSlots = 64;          // candidate slots per (point, tree)
 constexpr int kMaxLeaf = 2048;          // largest leaf (leaves above kLeafSlots keep 64 mates)
 constexpr int kNumCounters = 1024;      // spread of the per-block atomic counters
 // fields per counter: changed slots, unique candidates, re-scored candidates,
-// inserted ids, candidates whose sketches were gathered
-constexpr int kCounterFields = 5;
+// inserted ids, candidates whose sketches were gathered, leaf-mates dropped
+constexpr int kCounterFields = 6;
 constexpr int kRowQueueWords = 8 * 32;  // k_join row-queue counters, one 256 B line per XCD
 constexpr float kFltMin = 1.17549435e-38f;
 

@@ -59,6 +59,11 @@ struct knng_ctx {
   unsigned long long* ovf_n = nullptr;       // their count
   int cur = 0;
   bool have_graph = false;
+  // leaf of every row in every tree ([n][leaf_tp], first perm position of the
+  // leaf), valid while the graph descends from this forest's leaf init
+  int32_t* leafkey = nullptr;
+  int leaf_tp = 4;
+  bool leaf_valid = false;
   unsigned long long *counters = nullptr, *csum = nullptr;
 
   // ingest staging

@@ -74,6 +74,7 @@ class Stats(C.Structure):
         ("join_candidates", C.c_int64 * 32), ("join_rescored", C.c_int64 * 32),
         ("join_inserted", C.c_int64 * 32), ("insert_rates", C.c_float * 32),
         ("join_variant", C.c_int32), ("ms_tune", C.c_float), ("join_passed_on", C.c_int64 * 32),
+        ("join_leafmates", C.c_int64 * 32),
     ]
 
     def as_dict(self) -> dict:
@@ -93,6 +94,7 @@ class Stats(C.Structure):
             "insert_rates": [float(r) for r in self.insert_rates[:it]],
             "join_variant": self.join_variant, "ms_tune": self.ms_tune,
             "join_passed_on": list(self.join_passed_on[:it]),
+            "join_leafmates": list(self.join_leafmates[:it]),
         }
 
 

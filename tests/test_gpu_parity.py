@@ -374,3 +374,27 @@ def test_large_build_properties(torch_cuda):
     nb = xt[torch.from_numpy(kr[rows].reshape(-1) - 1).cuda()].double().view(500, 15, 1024)
     exact = (X[:, None, :] * nb).sum(-1).cpu().numpy()
     assert np.abs(dr[rows] - exact).max() / np.abs(exact).min() <= 1e-5
+
+
+@pytest.mark.parametrize("d,K,mult,T,leaf", [(1024, 64, 8, 4, 64), (768, 10, 3, 3, 200), (512, 50, 8, 16, 64)])
+def test_leaf_rule_join_bitexact(torch_cuda, d, K, mult, T, leaf):
+    """Join iterations after a leaf init drop candidates that share a leaf with
+    the row (leaf-mate rule), in every join shape: the K > 50 kernel, a small
+    K with leaves above 64 rows, 16 trees; two iterations bit-exact against
+    the oracle's join with the same leaf table."""
+    say("leaf rule join", d, K, mult, T, leaf)
+    x = synth.gaussian_mixture(90 + K, 4000, d, 40)
+    q, sk = oracle.quantize(x)
+    b = _builder(x, num_trees=T, leaf_size=leaf, k_build=K, k_eval=min(15, K), rerank_mult=mult, cells=0)
+    b.build_forest()
+    b.build_init()
+    perm, leaves = b.forest()
+    ids, sc = b.graph()
+    lk = oracle.leaf_keys(perm, leaves)
+    for it in range(2):
+        upd = b.join_iteration()
+        ids, sc, o_upd = oracle.join_device(q, sk, ids, sc, K * mult, leafkey=lk)
+        g_ids, g_sc = b.graph()
+        assert upd == o_upd, it
+        assert np.array_equal(g_ids, ids) and np.array_equal(g_sc, sc), it
+    assert b.stats()["join_leafmates"][0] > 0

@@ -460,12 +460,24 @@ def test_sketch_threshold_bruteforce():
             assert (h >= t).sum() >= M and (h >= t + 1).sum() < M
 
 
-def test_join_device_bruteforce():
+def _leaf_sets(perm, leaves, n):
+    """per row, the set of (tree, leaf start) it belongs to (numpy, independent of the oracle)"""
+    mine = [set() for _ in range(n)]
+    for t, s0, z in leaves:
+        for r in perm[t, s0:s0 + z]:
+            mine[int(r)].add((int(t), int(s0)))
+    return mine
+
+
+@pytest.mark.parametrize("leaf_rule", [False, True])
+def test_join_device_bruteforce(leaf_rule):
     x, q, sk = _small(n=250, d=128)
     K, M = 10, 30
     perm, leaves = oracle.forest_device(q, 2, 16)
     ids, sc = oracle.init_device(q, perm, leaves, K)
-    oi, osc, upd = oracle.join_device(q, sk, ids, sc, M)
+    lk = oracle.leaf_keys(perm, leaves) if leaf_rule else None
+    oi, osc, upd = oracle.join_device(q, sk, ids, sc, M, leafkey=lk)
+    mine = _leaf_sets(perm, leaves, 250)
     Q = q.astype(np.int64)
     bits = np.unpackbits(sk.view(np.uint8), axis=1, bitorder="little")
     changes = 0
@@ -475,6 +487,8 @@ def test_join_device_bruteforce():
         for n1 in ids[i]:
             for c in ids[n1]:
                 c = int(c)
+                if leaf_rule and mine[i] & mine[c]:
+                    continue
                 if c not in excl and c not in cands:
                     cands.append(c)
         h = np.array([128 - int((bits[i] ^ bits[c]).sum()) for c in cands], np.int32)
@@ -490,6 +504,27 @@ def test_join_device_bruteforce():
         assert osc[i].tolist() == [e[1] for e in pool[:K]]
         changes += sum(int(a != b) for a, b in zip(oi[i], ids[i]))
     assert upd == changes
+
+
+@pytest.mark.parametrize("leaf", [24, 150])   # 150: leaves above 64 rows keep 64 mates per member
+def test_leaf_mates_never_enter_the_list(leaf):
+    """What makes the leaf-mate rule safe: after a leaf init, a join without the
+    rule never inserts a candidate sharing a leaf with the row (the init scored
+    it exactly and kept it unless 64 mates beat it), over three iterations."""
+    x, q, sk = _small(n=400, d=128)
+    K, M = 10, 30
+    perm, leaves = oracle.forest_device(q, 3, leaf)
+    ids, sc = oracle.init_device(q, perm, leaves, K)
+    mine = _leaf_sets(perm, leaves, 400)
+    entered = 0
+    for _ in range(3):
+        oi, osc, _ = oracle.join_device(q, sk, ids, sc, M)
+        for i in range(400):
+            for c in set(oi[i].tolist()) - set(ids[i].tolist()):
+                entered += 1
+                assert not (mine[i] & mine[c]), (i, c)
+        ids, sc = oi, osc
+    assert entered > 50
 
 
 def test_incremental_join_bruteforce():

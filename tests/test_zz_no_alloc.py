@@ -48,7 +48,7 @@ def _noalloc(torch, x, ids, sc, K, mult, order=None):
     ds = torch.from_numpy(sc).cuda()
     oi = torch.full_like(di, -7)
     os_ = torch.full_like(ds, -7)
-    counts = torch.zeros(5, dtype=torch.int64, device="cuda")
+    counts = torch.zeros(6, dtype=torch.int64, device="cuda")
     dord = torch.from_numpy(order.astype(np.int32)).cuda() if order is not None else None
     torch.cuda.synchronize()
     knng.join_iteration_noalloc(dq.data_ptr(), dsk.data_ptr(), d, n, K, mult, di.data_ptr(), ds.data_ptr(),
@@ -70,7 +70,7 @@ def test_join_noalloc_equals_join_iteration(torch_cuda):
         assert np.array_equal(g_ids, e_ids), j
         assert np.array_equal(g_sc, e_sc), j
         assert int(counts[0]) == e_upd, j
-        assert counts[1] > 0 and 0 < counts[2] <= counts[1], j
+        assert counts[1] > 0 and 0 < counts[2] <= counts[1] and counts[5] == 0, j
         # and the oracle agrees
         o_ids, o_sc, o_upd = oracle.join_device(*oracle.quantize(x), ids, sc, K * mult)
         assert np.array_equal(g_ids, o_ids) and np.array_equal(g_sc, o_sc) and o_upd == e_upd
