@@ -95,11 +95,13 @@ typedef struct knng_stats {
   float insert_rates[32];      /* join_inserted / (n * k_build) */
   int32_t join_variant;        /* k_join shape the autotuner chose (0..2) */
   float ms_tune;               /* time spent timing k_join shapes (first build of a shape) */
-  int64_t join_passed_on[32];  /* rows the small-row join kernel passed to the full one (iterations
-                                  under the new/old rule after the first); -1: the full kernel
-                                  ran every row */
-  int64_t join_leafmates[32];  /* unique 2-hop candidates dropped because they share a leaf with
-                                  the row (scored exactly at init, so they cannot enter its list) */
+  int64_t join_passed_on[32];  /* rows the join passed to its block-per-row kernel because they
+                                  hold too many candidates for the wave-per-row phases (or, with
+                                  KNNG_JOIN_FUSED=1, that the small-row kernel passed on); -1:
+                                  the block-per-row kernel ran every row */
+  int64_t join_leafmates[32];  /* leaf-key rows gathered by the leaf-mate rule (a candidate that
+                                  shares a leaf with the row was scored exactly at init, so it
+                                  cannot enter the row's list and is dropped) */
 } knng_stats;
 
 typedef struct knng_ctx knng_ctx;

@@ -1868,7 +1868,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
           if (keep[j]) cid[s] = v[j];
         }
         __syncthreads();
-        if (tid == 0) my_ctr[5] += U - S.cnt[5];
+        if (tid == 0) my_ctr[5] += U;   // leaf-key rows gathered
         U = S.cnt[5];
       }
       if constexpr (SMALL) {
@@ -2126,6 +2126,330 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   if (tid == 0) {
     unsigned long long* c = counters + kCounterFields * (blockIdx.x % kNumCounters);
     for (int f = 0; f < kCounterFields; ++f) atomicAdd(c + f, my_ctr[f]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Phased join (single builds): the iteration of k_join split into two
+// wave-per-row kernels that hold little LDS, so each CU keeps 24-32 rows in
+// flight instead of four, and the rows no wave can hold go to k_join.
+//   k_jcand       own list -> 2-hop ids (new/old rule) -> leaf-mate rule ->
+//                 dedup in a per-wave LDS hash -> sketch prefilter when more
+//                 than M remain -> the row's candidates in cbuf[j][0..S),
+//                 ccount[j] = S; a row with more than kJCap candidates gets
+//                 ccount[j] = -1 and goes to ovf[] for k_join.
+//   k_jdot_merge  int8 dots of the candidates (16 lanes per 1 KB row), rank
+//                 merge with the own list by (score desc, id asc, slot),
+//                 write-back, change / new-id counts.
+// The lists are exactly k_join's: the candidate set, the prefilter threshold
+// and the merge order are the same functions of the previous graph.  Here the
+// leaf-mate rule is applied to every 2-hop id before deduplication (so the
+// hash never holds leaf-mates); counter 5 then counts those 2-hop ids.
+// ---------------------------------------------------------------------------
+
+constexpr int kJCap = 512;        // candidates a phased-join row may hold
+constexpr int kJHashBits = 10;    // per-wave LDS hash: 1024 slots (< 64 own + self + kJCap + 63 inserted)
+
+// blocks of 4 waves, dealt round-robin to the 8 XCDs: XCD x takes the
+// contiguous slice [x*nwork/8, (x+1)*nwork/8) of the work list and its waves
+// stride through it together (cluster-mates in flight share that XCD's L2)
+struct WaveSlice {
+  int64_t j0, hi, step;
+};
+__device__ __forceinline__ WaveSlice wave_slice(int64_t nwork) {
+  const int xcd = blockIdx.x & 7, w = threadIdx.x >> 6;
+  WaveSlice ws;
+  ws.j0 = nwork * xcd / 8 + int64_t(blockIdx.x >> 3) * 4 + w;
+  ws.hi = nwork * (xcd + 1) / 8;
+  ws.step = int64_t(gridDim.x >> 3) * 4;
+  return ws;
+}
+
+__global__ __launch_bounds__(256) void k_jcand(
+    const uint64_t* __restrict__ sk, int d, int K, int M, const int32_t* __restrict__ ids_in,
+    const uint64_t* __restrict__ fresh_in, const int32_t* __restrict__ leafkey, int T, int TP,
+    const int32_t* __restrict__ order, int64_t nwork, int32_t* __restrict__ cbuf,
+    int32_t* __restrict__ ccount, int32_t* __restrict__ ovf, unsigned long long* __restrict__ ovf_n,
+    unsigned long long* __restrict__ counters) {
+  __shared__ int32_t hs_w[4][1 << kJHashBits];   // dedup hash, then (low half) sketch similarities
+  __shared__ int32_t cid_w[4][kJCap];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int32_t* hs = hs_w[w];
+  int32_t* cid = cid_w[w];
+  const int words = d >> 6;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  unsigned long long n_u = 0, n_sk = 0, n_lk = 0;   // candidates, sketched candidates, leaf-key checks
+  const WaveSlice ws = wave_slice(nwork);
+  for (int64_t j = ws.j0; j < ws.hi; j += ws.step) {
+    const int64_t i = order ? int64_t(order[j]) : j;
+    for (int s = lane; s < (1 << kJHashBits); s += 64) hs[s] = -1;
+    const int32_t oid = lane < K ? ids_in[i * K + lane] : -1;
+    const bool obit = !fresh_in || (lane < K && ((fresh_in[i] >> lane) & 1ull));
+    const int32_t myleaf = (leafkey && lane < T) ? leafkey[i * TP + lane] : -1;
+    __builtin_amdgcn_wave_barrier();
+    if (lane < K) hs_insert(hs, kJHashBits, oid);
+    if (lane == K) hs_insert(hs, kJHashBits, int32_t(i));
+    __builtin_amdgcn_wave_barrier();
+    int U = 0;
+    // 2-hop ids, eight neighbour lists at a time
+    for (int k0 = 0; k0 < K && U <= kJCap; k0 += 8) {
+      int32_t c[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + u;
+        c[u] = -1;
+        if (k < K) {
+          const int32_t n1 = __shfl(oid, k);
+          const bool own_new = __shfl(int(obit), k) != 0;
+          const bool take = own_new || ((fresh_in[n1] >> lane) & 1ull);
+          if (lane < K && take) c[u] = ids_in[int64_t(n1) * K + lane];
+        }
+      }
+      if (leafkey) {
+        for (int t0 = 0; t0 < T; t0 += 4) {
+          const int32_t l0 = __shfl(myleaf, t0), l1 = __shfl(myleaf, t0 + 1 < T ? t0 + 1 : t0),
+                        l2 = __shfl(myleaf, t0 + 2 < T ? t0 + 2 : t0), l3 = __shfl(myleaf, t0 + 3 < T ? t0 + 3 : t0);
+          int4 lk[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            lk[u] = c[u] >= 0 ? *reinterpret_cast<const int4*>(leafkey + int64_t(c[u]) * TP + t0)
+                              : make_int4(-1, -1, -1, -1);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            if (t0 == 0) n_lk += __popcll(__ballot(c[u] >= 0));
+            const bool mate = lk[u].x == l0 || (t0 + 1 < T && lk[u].y == l1) || (t0 + 2 < T && lk[u].z == l2) ||
+                              (t0 + 3 < T && lk[u].w == l3);
+            if (mate) c[u] = -1;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (U > kJCap) c[u] = -1;   // the row overflows: insert nothing more (wave-uniform)
+        const bool nw = c[u] >= 0 && hs_insert(hs, kJHashBits, c[u]);
+        const unsigned long long bal = __ballot(nw);
+        const int pos = U + __popcll(bal & lt);
+        if (nw && pos < kJCap) cid[pos] = c[u];
+        U += __popcll(bal);
+      }
+    }
+    if (U > kJCap) {   // wave-uniform: k_join runs this row
+      if (lane == 0) {
+        ccount[j] = -1;
+        ovf[atomicAdd(ovf_n, 1ull)] = int32_t(i);
+      }
+      continue;
+    }
+    n_u += U;
+    __builtin_amdgcn_wave_barrier();
+    int S = U;
+    constexpr int kPer = kJCap / 64;
+    int32_t cv[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) cv[e] = lane + 64 * e < U ? cid[lane + 64 * e] : -1;
+    bool keep[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) keep[e] = cv[e] >= 0;
+    if (U > M) {
+      // sketch similarity h = d - popcount, 8 lanes per candidate (128 B)
+      n_sk += U;
+      const int sub = lane & 7, grp = lane >> 3;
+      const uint64_t o0 = 2 * sub < words ? sk[i * words + 2 * sub] : 0ull;
+      const uint64_t o1 = 2 * sub + 1 < words ? sk[i * words + 2 * sub + 1] : 0ull;
+      for (int u0 = 0; u0 < U; u0 += 32) {
+        uint64_t a[4], b[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int u = u0 + grp + 8 * r;
+          a[r] = b[r] = 0ull;
+          if (u < U) {
+            const uint64_t* cs = sk + int64_t(cid[u]) * words;
+            if (2 * sub < words) a[r] = cs[2 * sub] ^ o0;
+            if (2 * sub + 1 < words) b[r] = cs[2 * sub + 1] ^ o1;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int u = u0 + grp + 8 * r;
+          int pc = __popcll(a[r]) + __popcll(b[r]);
+          pc += __shfl_xor(pc, 1);
+          pc += __shfl_xor(pc, 2);
+          pc += __shfl_xor(pc, 4);
+          if (u < U && sub == 0) hs[u] = d - pc;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      int hv[kPer];
+#pragma unroll
+      for (int e = 0; e < kPer; ++e) hv[e] = keep[e] ? hs[lane + 64 * e] : -1;
+      // largest t with |{h >= t}| >= M
+      int lo = 0, hi = d;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        int cnt = 0;
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) cnt += __popcll(__ballot(hv[e] >= mid));
+        if (cnt >= M)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+#pragma unroll
+      for (int e = 0; e < kPer; ++e) keep[e] = keep[e] && hv[e] >= lo;
+    }
+    // the row's candidates to cbuf, in list order
+    int base = 0;
+    int32_t* dst = cbuf + j * kJCap;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const unsigned long long bal = __ballot(keep[e]);
+      if (keep[e]) dst[base + __popcll(bal & lt)] = cv[e];
+      base += __popcll(bal);
+    }
+    S = base;
+    if (lane == 0) ccount[j] = S;
+  }
+  if (lane == 0) {
+    unsigned long long* cc = counters + kCounterFields * (blockIdx.x % kNumCounters);
+    atomicAdd(cc + 1, n_u);
+    atomicAdd(cc + 4, n_sk);
+    atomicAdd(cc + 5, n_lk);
+  }
+}
+
+template <int DIM>   // DIM > 0: rows of DIM bytes at compile time; 0: any d
+__global__ __launch_bounds__(256) void k_jdot_merge(
+    const int8_t* __restrict__ q, int d, int K, const int32_t* __restrict__ ids_in,
+    const int32_t* __restrict__ sc_in, int32_t* __restrict__ ids_out, int32_t* __restrict__ sc_out,
+    uint64_t* __restrict__ fresh_out, const int32_t* __restrict__ order, int64_t nwork,
+    const int32_t* __restrict__ cbuf, const int32_t* __restrict__ ccount,
+    unsigned long long* __restrict__ counters) {
+  __shared__ uint64_t key_w[4][kJCap];   // candidate keys, then the survivors' keys
+  __shared__ uint64_t okey_w[4][kMaxK];
+  __shared__ int32_t nid_w[4][kMaxK], nsc_w[4][kMaxK], nfr_w[4][kMaxK];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t* key = key_w[w];
+  uint64_t* okey = okey_w[w];
+  int32_t* nid = nid_w[w];
+  int32_t* nsc = nsc_w[w];
+  int32_t* nfr = nfr_w[w];
+  if (DIM > 0) d = DIM;
+  const int nch = DIM > 0 ? DIM / 16 : d >> 4;
+  const int g = lane >> 4, s16 = lane & 15;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  unsigned long long n_chg = 0, n_ns = 0, n_ins = 0;
+  const WaveSlice ws = wave_slice(nwork);
+  for (int64_t j = ws.j0; j < ws.hi; j += ws.step) {
+    const int S = ccount[j];
+    if (S < 0) continue;   // k_join runs this row
+    const int64_t i = order ? int64_t(order[j]) : j;
+    const int32_t oid = lane < K ? ids_in[i * K + lane] : -1;
+    const int32_t osc = lane < K ? sc_in[i * K + lane] : 0;
+    const uint64_t ok = lane < K ? pack_key(osc, oid) : ~0ull;
+    if (lane < K) okey[lane] = ok;
+    // the K-th best own key: a candidate below it cannot enter the list
+    uint64_t mn = ok;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      const uint64_t o = __shfl_xor(mn, m);
+      mn = o < mn ? o : mn;
+    }
+    // ---- int8 dots: 16 lanes per candidate, two candidates per group in flight
+    v4i own[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int ch = s16 + 16 * c;
+      own[c] = ch < nch ? *reinterpret_cast<const v4i*>(q + i * d + 16 * ch) : v4i{0, 0, 0, 0};
+    }
+    const int32_t* cl = cbuf + j * kJCap;
+    for (int s0 = 0; s0 < S; s0 += 8) {
+      int32_t id[2];
+      v4i v[2][4];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int s = s0 + g + 4 * r;
+        id[r] = s < S ? cl[s] : -1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int ch = s16 + 16 * c;
+          v[r][c] = (id[r] >= 0 && ch < nch) ? *reinterpret_cast<const v4i*>(q + int64_t(id[r]) * d + 16 * ch)
+                                             : v4i{0, 0, 0, 0};
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        int dt = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) dt = dot16_i8(v[r][c], own[c], dt);
+#pragma unroll
+        for (int m = 8; m >= 1; m >>= 1) dt += __shfl_xor(dt, m);
+        const int s = s0 + g + 4 * r;
+        if (s16 == 0 && s < S) key[s] = pack_key(dt, id[r]);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // ---- survivors (key above the K-th own key), compacted in place
+    constexpr int kPer = kJCap / 64;
+    uint64_t kv[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) kv[e] = lane + 64 * e < S ? key[lane + 64 * e] : 0ull;
+    __builtin_amdgcn_wave_barrier();
+    int P = 0;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const bool sv = kv[e] > mn;
+      const unsigned long long bal = __ballot(sv);
+      if (sv) key[P + __popcll(bal & lt)] = kv[e];
+      P += __popcll(bal);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // ---- ranks in the union (own entries tie-break by slot; candidate ids are
+    //      neither own ids nor repeated, so their keys are distinct)
+    if (lane < K) {
+      int r = 0;
+      for (int k = 0; k < K; ++k) {
+        const uint64_t o = okey[k];
+        r += (o > ok || (o == ok && k < lane)) ? 1 : 0;
+      }
+      for (int p = 0; p < P && r < K; ++p) r += key[p] > ok ? 1 : 0;
+      if (r < K) {
+        nid[r] = oid;
+        nsc[r] = osc;
+        nfr[r] = 0;
+      }
+    }
+    for (int p = lane; p < P; p += 64) {
+      const uint64_t kp = key[p];
+      int r = 0;
+      for (int k = 0; k < K; ++k) r += okey[k] > kp ? 1 : 0;
+      for (int p2 = 0; p2 < P && r < K; ++p2) r += key[p2] > kp ? 1 : 0;
+      if (r < K) {
+        nid[r] = key_id(kp);
+        nsc[r] = key_score(kp);
+        nfr[r] = 1;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // ---- write back
+    bool changed = false, fr = false;
+    if (lane < K) {
+      ids_out[i * K + lane] = nid[lane];
+      sc_out[i * K + lane] = nsc[lane];
+      changed = nid[lane] != oid;
+      fr = nfr[lane] != 0;
+    }
+    const unsigned long long fm = __ballot(fr);
+    if (fresh_out && lane == 0) fresh_out[i] = fm;
+    n_chg += __popcll(__ballot(changed));
+    n_ins += __popcll(fm);
+    n_ns += S;
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane == 0) {
+    unsigned long long* cc = counters + kCounterFields * (blockIdx.x % kNumCounters);
+    atomicAdd(cc + 0, n_chg);
+    atomicAdd(cc + 2, n_ns);
+    atomicAdd(cc + 3, n_ins);
   }
 }
 
@@ -2979,6 +3303,66 @@ int launch_join_small(knng_ctx* c, int a, int b, const int32_t* order) {
                      nullptr);
 }
 
+// The phased join (k_jcand, k_jdot_merge) over every row, then k_join over
+// the rows with more than kJCap candidates (device count, no host round trip).
+// blocks of 256 threads one CU holds of kernel f (its registers and LDS), at least 1
+template <typename F>
+int resident_blocks(F f) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(f), 256, 0) != hipSuccess ||
+      nb < 1)
+    nb = 1;
+  return nb;
+}
+
+// a grid of every block the GPU holds at once, a multiple of the 8 XCDs (the
+// phased kernels give each wave a fixed share of its XCD's rows: a block
+// that waits for a slot would finish its share last)
+template <typename F>
+int resident_grid(const knng_ctx* c, F f) {
+  static std::mutex mu;
+  static std::map<const void*, int> per_cu;   // by kernel (instantiations share F)
+  int b;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = per_cu.find(reinterpret_cast<const void*>(f));
+    if (it == per_cu.end()) it = per_cu.emplace(reinterpret_cast<const void*>(f), resident_blocks(f)).first;
+    b = it->second;
+  }
+  return std::max(8, (c->num_cu * b) / 8 * 8);
+}
+
+int launch_join_phased(knng_ctx* c, int a, int b, const int32_t* order) {
+  const int64_t n = c->n;
+  const int K = c->p.k_build, M = K * c->p.rerank_mult;
+  int rc;
+  if ((rc = dalloc(c, &c->jcbuf, size_t(n) * kJCap, "join candidates")) ||
+      (rc = dalloc(c, &c->jccount, size_t(n), "join candidate counts")) ||
+      (rc = dalloc(c, &c->ovf, size_t(n), "join overflow rows")) ||
+      (rc = dalloc(c, &c->ovf_n, 1, "join overflow count")))
+    return rc;
+  HIPCHK(c, hipMemsetAsync(c->ovf_n, 0, sizeof(unsigned long long), c->stream));
+  const int32_t* leafkey = c->leaf_valid ? c->leafkey : nullptr;
+  const uint64_t* fin = c->p.incremental ? c->fresh[a] : nullptr;
+  const int ga = resident_grid(c, k_jcand);
+  hipLaunchKernelGGL(k_jcand, dim3(ga), dim3(256), 0, c->stream, c->sk, c->d, K, M, c->gid[a], fin, leafkey,
+                     int(c->p.num_trees), c->leaf_tp, order, n, c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters);
+  if (c->d == kMaxDim)
+    hipLaunchKernelGGL(k_jdot_merge<kMaxDim>, dim3(resident_grid(c, k_jdot_merge<kMaxDim>)), dim3(256), 0, c->stream,
+                       c->q, c->d, K, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->fresh[b], order, n, c->jcbuf,
+                       c->jccount, c->counters);
+  else
+    hipLaunchKernelGGL(k_jdot_merge<0>, dim3(resident_grid(c, k_jdot_merge<0>)), dim3(256), 0, c->stream, c->q, c->d, K, c->gid[a], c->gsc[a],
+                       c->gid[b], c->gsc[b], c->fresh[b], order, n, c->jcbuf, c->jccount, c->counters);
+  HIPCHK(c, hipGetLastError());
+  return launch_join(c, 0, a, b, c->ovf, reinterpret_cast<const int64_t*>(c->ovf_n), nullptr, 0, nullptr);
+}
+
+bool join_fused_forced() {
+  const char* e = std::getenv("KNNG_JOIN_FUSED");
+  return e && e[0] == '1';
+}
+
 struct TuneCache {
   std::mutex mu;
   std::map<std::tuple<int, int, int>, int> best;
@@ -3215,6 +3599,8 @@ int knng_destroy(knng_ctx* c) {
   dfree(&c->counters);
   dfree(&c->csum);
   dfree(&c->leafkey);
+  dfree(&c->jcbuf);
+  dfree(&c->jccount);
   dfree(&c->seg_base);
   dfree(&c->tune_n);
   dfree(&c->ex_knns);
@@ -3619,12 +4005,20 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   const int K = c->p.k_build, M = K * c->p.rerank_mult;
   const int a = c->cur, b = c->cur ^ 1;
   const int32_t* order = c->have_forest ? c->perm : nullptr;   // tree 0's leaf order
-  if (int rc = ensure_join_variant(c, a, b, order, n, nullptr, 0, nullptr)) return rc;
-  // after the first iteration under the new/old rule most rows have few
-  // candidates: the small-row kernel, the full one for the rest
-  const bool small = c->p.incremental && !c->fresh_all && K <= 50;
+  // default: the phased join, k_join for the rows it cannot hold; with
+  // KNNG_JOIN_FUSED=1 the block-per-row k_join (and, under the new/old rule
+  // after the first iteration, its small-row variant) over every row
+  const bool phased = !join_fused_forced();
+  if (phased) {
+    if (c->join_variant < 0) c->join_variant = 0;
+  } else if (int rc = ensure_join_variant(c, a, b, order, n, nullptr, 0, nullptr)) {
+    return rc;
+  }
+  const bool small = !phased && c->p.incremental && !c->fresh_all && K <= 50;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  if (small) {
+  if (phased) {
+    if (int rc = launch_join_phased(c, a, b, order)) return rc;
+  } else if (small) {
     if (int rc = launch_join_small(c, a, b, order)) return rc;
   } else if (int rc = launch_join(c, c->join_variant, a, b, order, nullptr, nullptr, 0, nullptr)) {
     return rc;
@@ -3635,7 +4029,7 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   unsigned long long sums[kCounterFields];
   unsigned long long passed = 0;
   HIPCHK(c, hipMemcpyAsync(sums, c->csum, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
-  if (small) HIPCHK(c, hipMemcpyAsync(&passed, c->ovf_n, sizeof(passed), hipMemcpyDeviceToHost, c->stream));
+  if (small || phased) HIPCHK(c, hipMemcpyAsync(&passed, c->ovf_n, sizeof(passed), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0;
   (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);   // ms stays 0 on failure (a timing stat only)
@@ -3649,13 +4043,13 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
     c->st.join_rescored[it] = int64_t(sums[2]);
     c->st.join_inserted[it] = int64_t(sums[3]);
     c->st.insert_rates[it] = float(sums[3]) / float(size_t(n) * size_t(K));
-    c->st.join_passed_on[it] = small ? int64_t(passed) : -1;
+    c->st.join_passed_on[it] = (small || phased) ? int64_t(passed) : -1;
     c->st.join_leafmates[it] = int64_t(sums[5]);
     // algorithmic bytes: own list + own row/sketch, K neighbour id rows, one
     // leaf-key row per unique candidate (leaf-mate rule), one sketch per
     // candidate of a row that gathers them, one row per re-scored candidate,
     // writes
-    const double lk_bytes = c->leaf_valid ? double(sums[1] + sums[5]) * 4.0 * c->leaf_tp : 0.0;
+    const double lk_bytes = double(sums[5]) * 4.0 * c->leaf_tp;
     c->st.join_bytes[it] = double(n) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) + lk_bytes +
                            double(sums[4]) * 8.0 * c->words + double(sums[2]) * c->d;
   }
@@ -4077,7 +4471,7 @@ int build_packed(knng_ctx* c, std::vector<int32_t>* iters, std::vector<int32_t>*
       c->st.join_leafmates[it] = int64_t(L.csum_h[5]);
       c->st.insert_rates[it] = float(L.csum_h[3]) / float(size_t(c->n) * size_t(K));
       // algorithmic bytes as knng_join_iteration counts them, over the rows processed
-      const double lk_bytes = c->leaf_valid ? double(L.csum_h[1] + L.csum_h[5]) * 4.0 * c->leaf_tp : 0.0;
+      const double lk_bytes = double(L.csum_h[5]) * 4.0 * c->leaf_tp;
       c->st.join_bytes[it] = double(rows) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) +
                              lk_bytes + double(L.csum_h[4]) * 8.0 * c->words + double(L.csum_h[2]) * c->d;
     }

@@ -55,7 +55,9 @@ struct knng_ctx {
   int32_t *gid[2] = {nullptr, nullptr}, *gsc[2] = {nullptr, nullptr};
   uint64_t* fresh[2] = {nullptr, nullptr};   // per row: bit k = entry k entered last iteration
   bool fresh_all = false;                    // fresh[cur] is all ones (no join iteration yet)
-  int32_t* ovf = nullptr;                    // rows the small-row join passed on [n]
+  int32_t* ovf = nullptr;                    // rows passed on to k_join [n]
+  int32_t* jcbuf = nullptr;                  // phased join: candidates per work position [n][kJCap]
+  int32_t* jccount = nullptr;                // their counts (-1: the row went to k_join) [n]
   unsigned long long* ovf_n = nullptr;       // their count
   int cur = 0;
   bool have_graph = false;

@@ -98,7 +98,7 @@ def test_bench_form_bitexact(torch_cuda, bench_form_case):
     ids, sc = b.graph()
     st = b.stats()
     assert st["cells"] >= 2 and st["iterations"] == 2 == len(o_rates)
-    assert st["join_passed_on"][0] == -1 and 0 <= st["join_passed_on"][1] < n
+    assert all(0 <= p < n for p in st["join_passed_on"])
     assert np.array_equal(ids, o_ids) and np.array_equal(sc, o_sc)
     knns, dists = b.export(15, rerank=False)
     e_knns, e_dists = oracle.export(o_ids, o_sc)
@@ -180,11 +180,15 @@ def test_cli_reference_form_stop_rule(torch_cuda):
         assert r.stdout.count("-> Iteration") == len(o_rates)
 
 
-def test_small_row_join_other_dims_bitexact(torch_cuda):
-    """The small-row join kernel at d != 1024 with a small K (M = K * mult = 30
-    candidates before rows pass on), over four iterations under the new/old
-    rule: both kernels write rows and every list is bit-exact."""
-    say("small-row join d=768 K=10")
+@pytest.mark.parametrize("fused", [False, True])
+def test_small_row_join_other_dims_bitexact(torch_cuda, monkeypatch, fused):
+    """Join iterations at d != 1024 with a small K (M = K * mult = 30), over four
+    iterations under the new/old rule: the wave-per-row phases with rows passed
+    to the block-per-row kernel, and (fused) the small-row kernel with rows
+    passed to the full one; both kernels write rows, every list bit-exact."""
+    say("small-row join d=768 K=10", "fused" if fused else "phased")
+    if fused:
+        monkeypatch.setenv("KNNG_JOIN_FUSED", "1")
     n = 4000
     x = synth.gaussian_mixture(27, n, 768, 40)
     q, sk = oracle.quantize(x)
@@ -202,8 +206,11 @@ def test_small_row_join_other_dims_bitexact(torch_cuda):
         assert upd == o_upd, it
         assert np.array_equal(g_ids, ids) and np.array_equal(g_sc, sc), it
     passed = b.stats()["join_passed_on"]
-    assert passed[0] == -1 and all(0 <= p <= n for p in passed[1:]), passed
-    assert any(0 < p < n for p in passed[1:]), passed
+    if fused:
+        assert passed[0] == -1 and all(0 <= p <= n for p in passed[1:]), passed
+        assert any(0 < p < n for p in passed[1:]), passed
+    else:   # at most K * K = 100 candidates: every row stays in the phases
+        assert all(p == 0 for p in passed), passed
 
 
 # ------------------------------------------------------------ c1: 100k rows

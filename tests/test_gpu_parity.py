@@ -173,9 +173,15 @@ def test_join_iterations_bitexact(torch_cuda):
         assert np.array_equal(g_sc, sc), it
 
 
-def test_incremental_join_bitexact(torch_cuda):
-    """New/old rule: only pairs with a fresh edge are candidates."""
-    say("incremental join")
+@pytest.mark.parametrize("fused", [False, True])
+def test_incremental_join_bitexact(torch_cuda, monkeypatch, fused):
+    """New/old rule: only pairs with a fresh edge are candidates.  Default
+    path: the wave-per-row phases with the rows that hold too many candidates
+    passed to the block-per-row kernel; fused: the block-per-row kernel, and
+    after the first iteration its small-row variant passing rows on."""
+    say("incremental join", "fused" if fused else "phased")
+    if fused:
+        monkeypatch.setenv("KNNG_JOIN_FUSED", "1")
     x = synth.gaussian_mixture(17, 5000, 1024, 32)
     q, sk = oracle.quantize(x)
     perm, leaves = oracle.forest_device(q, 4, 64, 1337)
@@ -189,11 +195,14 @@ def test_incremental_join_bitexact(torch_cuda):
         g_ids, g_sc = b.graph()
         assert upd == o_upd, it
         assert np.array_equal(g_ids, ids) and np.array_equal(g_sc, sc), it
-    # iterations after the first ran the small-row kernel, which passed the
-    # rows with many candidates to the full one: both kernels wrote rows
+    # rows were passed on to the block-per-row kernel in some iteration while
+    # others stayed: both kernels wrote rows
     passed = b.stats()["join_passed_on"]
-    assert passed[0] == -1 and all(p >= 0 for p in passed[1:]), passed
-    assert any(0 < p < 5000 for p in passed[1:]), passed
+    if fused:
+        assert passed[0] == -1 and all(p >= 0 for p in passed[1:]), passed
+    else:
+        assert all(0 <= p <= 5000 for p in passed), passed
+    assert any(0 < p < 5000 for p in passed), passed
 
 
 def test_incremental_full_build_bitexact(torch_cuda):
