@@ -2214,12 +2214,16 @@ __global__ __launch_bounds__(256) void k_jcand(
           for (int u = 0; u < 8; ++u)
             lk[u] = c[u] >= 0 ? *reinterpret_cast<const int4*>(leafkey + int64_t(c[u]) * TP + t0)
                               : make_int4(-1, -1, -1, -1);
+          // branch-free test (bitwise, then a select): as a chain of || with
+          // conditional assignment the compiler lost the update for matches
+          // in the .z/.w lanes
+          const int v1 = t0 + 1 < T, v2 = t0 + 2 < T, v3 = t0 + 3 < T;
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             if (t0 == 0) n_lk += __popcll(__ballot(c[u] >= 0));
-            const bool mate = lk[u].x == l0 || (t0 + 1 < T && lk[u].y == l1) || (t0 + 2 < T && lk[u].z == l2) ||
-                              (t0 + 3 < T && lk[u].w == l3);
-            if (mate) c[u] = -1;
+            const int mate = int(lk[u].x == l0) | (v1 & int(lk[u].y == l1)) | (v2 & int(lk[u].z == l2)) |
+                             (v3 & int(lk[u].w == l3));
+            c[u] = mate ? -1 : c[u];
           }
         }
       }
@@ -2409,7 +2413,7 @@ __global__ __launch_bounds__(256) void k_jdot_merge(
       int r = 0;
       for (int k = 0; k < K; ++k) {
         const uint64_t o = okey[k];
-        r += (o > ok || (o == ok && k < lane)) ? 1 : 0;
+        r += int(o > ok) | (int(o == ok) & int(k < lane));
       }
       for (int p = 0; p < P && r < K; ++p) r += key[p] > ok ? 1 : 0;
       if (r < K) {
