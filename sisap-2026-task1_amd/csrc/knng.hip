@@ -2188,7 +2188,7 @@ __global__ __launch_bounds__(256) void k_jcand(
     const int32_t myleaf = (leafkey && lane < T) ? leafkey[i * TP + lane] : -1;
     __builtin_amdgcn_wave_barrier();
     if (lane < K) hs_insert(hs, kJHashBits, oid);
-    if (lane == K) hs_insert(hs, kJHashBits, int32_t(i));
+    if (lane == 0) hs_insert(hs, kJHashBits, int32_t(i));   // K may be 64: no spare lane
     __builtin_amdgcn_wave_barrier();
     int U = 0;
     // 2-hop ids, eight neighbour lists at a time
