@@ -2191,20 +2191,28 @@ __global__ __launch_bounds__(256) void k_jcand(
     if (lane == 0) hs_insert(hs, kJHashBits, int32_t(i));   // K may be 64: no spare lane
     __builtin_amdgcn_wave_barrier();
     int U = 0;
-    // 2-hop ids, eight neighbour lists at a time
-    for (int k0 = 0; k0 < K && U <= kJCap; k0 += 8) {
-      int32_t c[8];
+    // the neighbours' fresh masks, lane k holding neighbour k's (one load per row)
+    const uint64_t fnb = (fresh_in && lane < K) ? fresh_in[oid] : ~0ull;
+    // 2-hop ids of eight neighbour lists at a time: batch b + 1's ids are
+    // loaded while batch b's leaf keys are in flight
+    int32_t cn[8];
+    auto load_ids = [&](int k0, int32_t (&dst)[8]) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int k = k0 + u;
-        c[u] = -1;
+        dst[u] = -1;
         if (k < K) {
           const int32_t n1 = __shfl(oid, k);
-          const bool own_new = __shfl(int(obit), k) != 0;
-          const bool take = own_new || ((fresh_in[n1] >> lane) & 1ull);
-          if (lane < K && take) c[u] = ids_in[int64_t(n1) * K + lane];
+          const bool take = __shfl(int(obit), k) != 0 || ((__shfl(fnb, k) >> lane) & 1ull);
+          if (lane < K && take) dst[u] = ids_in[int64_t(n1) * K + lane];
         }
       }
+    };
+    load_ids(0, cn);
+    for (int k0 = 0; k0 < K && U <= kJCap; k0 += 8) {
+      int32_t c[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) c[u] = cn[u];
       if (leafkey) {
         for (int t0 = 0; t0 < T; t0 += 4) {
           const int32_t l0 = __shfl(myleaf, t0), l1 = __shfl(myleaf, t0 + 1 < T ? t0 + 1 : t0),
@@ -2214,6 +2222,7 @@ __global__ __launch_bounds__(256) void k_jcand(
           for (int u = 0; u < 8; ++u)
             lk[u] = c[u] >= 0 ? *reinterpret_cast<const int4*>(leafkey + int64_t(c[u]) * TP + t0)
                               : make_int4(-1, -1, -1, -1);
+          if (t0 == 0 && k0 + 8 < K) load_ids(k0 + 8, cn);
           // branch-free test (bitwise, then a select): as a chain of || with
           // conditional assignment the compiler lost the update for matches
           // in the .z/.w lanes
@@ -2226,6 +2235,8 @@ __global__ __launch_bounds__(256) void k_jcand(
             c[u] = mate ? -1 : c[u];
           }
         }
+      } else if (k0 + 8 < K) {
+        load_ids(k0 + 8, cn);
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -3347,7 +3358,8 @@ int launch_join_phased(knng_ctx* c, int a, int b, const int32_t* order) {
     return rc;
   HIPCHK(c, hipMemsetAsync(c->ovf_n, 0, sizeof(unsigned long long), c->stream));
   const int32_t* leafkey = c->leaf_valid ? c->leafkey : nullptr;
-  const uint64_t* fin = c->p.incremental ? c->fresh[a] : nullptr;
+  // every entry still new (no join iteration yet): the new/old rule takes every pair
+  const uint64_t* fin = (c->p.incremental && !c->fresh_all) ? c->fresh[a] : nullptr;
   const int ga = resident_grid(c, k_jcand);
   hipLaunchKernelGGL(k_jcand, dim3(ga), dim3(256), 0, c->stream, c->sk, c->d, K, M, c->gid[a], fin, leafkey,
                      int(c->p.num_trees), c->leaf_tp, order, n, c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters);
