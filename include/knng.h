@@ -99,9 +99,9 @@ typedef struct knng_stats {
                                   hold too many candidates for the wave-per-row phases (or, with
                                   KNNG_JOIN_FUSED=1, that the small-row kernel passed on); -1:
                                   the block-per-row kernel ran every row */
-  int64_t join_leafmates[32];  /* leaf-key rows gathered by the leaf-mate rule (a candidate that
-                                  shares a leaf with the row was scored exactly at init, so it
-                                  cannot enter the row's list and is dropped) */
+  int64_t join_leafmates[32];  /* 2-hop paths i -> n1 -> c the leaf-mate rule dropped (both edges
+                                  share a leaf of one tree, so c shares it with i: scored exactly at
+                                  init, it cannot enter i's list) */
 } knng_stats;
 
 typedef struct knng_ctx knng_ctx;

@@ -824,8 +824,11 @@ int32_t orc_sketch_threshold(const int32_t* h, int64_t cnt, int32_t M, int32_t b
 // fresh_out (optional) receives the same masks for the new lists.
 // leafkey (optional, [n][T]: row r's leaf in tree t, as its first position in
 // the tree's permutation) when the lists descend from that forest's leaf init:
-// a candidate sharing a leaf with row i in any tree is dropped before the
-// sketch prefilter (the init scored it exactly, so it cannot enter i's list).
+// the leaf-mate rule.  The mask of an edge r -> e has bit t (trees 0..7) set
+// when r and e share tree t's leaf; the path i -> n1 -> c is dropped when the
+// masks of i -> n1 and n1 -> c share a bit (then c shares that leaf with i:
+// the init scored it exactly, so it cannot enter i's list).  c is a candidate
+// when some path to it survives.
 int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t d, int32_t K,
                         int32_t M, const int32_t* ids_in, const int32_t* sc_in,
                         int32_t* ids_out, int32_t* sc_out, int64_t* inserted,
@@ -845,22 +848,21 @@ int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t 
     hval.clear();
     seen.insert(int32_t(i));
     for (int k = 0; k < K; ++k) seen.insert(own[k]);
+    const int tm = T < 8 ? T : 8;
+    auto edge_mask = [&](int64_t r, int64_t e) {
+      uint32_t m = 0;
+      for (int t = 0; t < tm; ++t) m |= uint32_t(leafkey[r * T + t] == leafkey[e * T + t]) << t;
+      return m;
+    };
     for (int k = 0; k < K; ++k) {
       const int32_t* nb = ids_in + int64_t(own[k]) * K;
       const bool own_fresh = !fresh_in || ((fresh_in[i] >> k) & 1ull);
+      const uint32_t m1 = leafkey ? edge_mask(i, own[k]) : 0u;
       for (int k2 = 0; k2 < K; ++k2) {
         if (!own_fresh && !((fresh_in[own[k]] >> k2) & 1ull)) continue;
+        if (leafkey && (m1 & edge_mask(own[k], nb[k2]))) continue;   // leaf-mate path
         if (seen.insert(nb[k2]).second) cand.push_back(nb[k2]);
       }
-    }
-    if (leafkey) {
-      size_t keep = 0;
-      for (int32_t c : cand) {
-        bool mate = false;
-        for (int t = 0; t < T && !mate; ++t) mate = leafkey[int64_t(c) * T + t] == leafkey[i * T + t];
-        if (!mate) cand[keep++] = c;
-      }
-      cand.resize(keep);
     }
     const uint64_t* si = sk + i * words;
     for (int32_t c : cand) hval.push_back(sketch_sim(si, sk + int64_t(c) * words, words));

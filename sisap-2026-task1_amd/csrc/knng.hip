@@ -358,6 +358,17 @@ __global__ __launch_bounds__(256) void k_leaf_keys(const int4* __restrict__ leav
   }
 }
 
+// the mask byte of every list entry of the graph gid (see edge_mask)
+__global__ __launch_bounds__(256) void k_edge_masks(const int32_t* __restrict__ gid, int64_t n, int K,
+                                                    const int32_t* __restrict__ leafkey, int T, int TP,
+                                                    uint8_t* __restrict__ gmask) {
+  const int64_t total = n * K;
+  for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+    const int64_t r = e / K;
+    gmask[e] = uint8_t(edge_mask(leafkey + r * TP, leafkey + int64_t(gid[e]) * TP, T));
+  }
+}
+
 // Append one level's nodes to the persistent hierarchy, turning next-level
 // child indices into global ones (the next level is appended right after).
 __global__ void k_hier_append(const Node* __restrict__ nodes, int nnodes, Node* __restrict__ hier,
@@ -1685,6 +1696,8 @@ struct JoinScratch {
   uint8_t own_bit[kMaxK];     // 1: own entry k entered the list last iteration (one byte per entry:
                               // 64-bit masks per neighbour would be hoisted out of the row loop and
                               // spilled)
+  uint8_t own_mask[kMaxK];    // own entry k's leaf mask byte
+  uint8_t new_mask[kMaxK];
   uint64_t own_sk[kMaxDim / 64];
   int32_t own_leaf[kMaxTrees];   // the row's leaf (first perm position) in every tree
   int32_t hist[kMaxDim + 1];
@@ -1705,7 +1718,8 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     const uint64_t* __restrict__ fresh_in, uint64_t* __restrict__ fresh_out,
     const int64_t* __restrict__ nwork_dev, const int64_t* __restrict__ seg_base, int nseg,
     unsigned long long* __restrict__ segc, int32_t* __restrict__ ovf,
-    unsigned long long* __restrict__ ovf_n, const int32_t* __restrict__ leafkey, int T, int TP) {
+    unsigned long long* __restrict__ ovf_n, const uint8_t* __restrict__ masks_in,
+    uint8_t* __restrict__ masks_out, const int32_t* __restrict__ leafkey, int T, int TP) {
   // order[0 .. nwork) lists the rows to process (nwork = *nwork_dev if given,
   // else n).  With segments (independent builds packed together, nseg > 0),
   // each row's changed-slot and new-id counts are also added to its build's
@@ -1722,13 +1736,15 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   // the 4 waves checks before inserting at most 64 more, so at most CAP are
   // ever held) and is appended to ovf[*ovf_n]; the full kernel then runs
   // those rows.  It writes nothing for such a row.
-  // leafkey != nullptr (the graph came from this forest's leaf init): a
-  // candidate that shares a leaf with the row in any of the T trees is
-  // dropped.  The init scored every leaf-mate exactly and kept it in the
-  // row's 64 slots of that tree unless 64 mates beat it, so it is either in
-  // the list already or below the init list's K-th entry (K <= 64), and lists
-  // only improve: it could never enter.  leafkey[r * TP + t] = the first perm
-  // position of row r's leaf in tree t.
+  // masks_in != nullptr (the graph came from this forest's leaf init): list
+  // entry e of row r carries a mask byte, bit t = r and e share tree t's leaf
+  // (trees 0..7).  The path i -> n1 -> c is dropped when both of its edges
+  // share a leaf of the same tree: then c shares that leaf with i, the init
+  // scored it exactly and kept it in i's 64 slots of that tree unless 64 mates
+  // beat it, so it is either in i's list already or below the init list's
+  // K-th entry (K <= 64), and lists only improve: it could never enter.  New
+  // entries get their mask byte from the leaf keys (leafkey[r * TP + t] = the
+  // first perm position of row r's leaf in tree t).
   constexpr int HS = 1 << HSBITS;
   static_assert(HS * 4 >= CAP * 2 + CAP * 4, "selection arrays must fit in the hash area");
   __shared__ __attribute__((aligned(16))) int32_t hs[HS];
@@ -1789,7 +1805,8 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     if constexpr (!SMALL)
       for (int j = tid; j <= d; j += 256) S.hist[j] = 0;
     if (tid < 6) S.cnt[tid] = 0;
-    if (leafkey && tid < T) S.own_leaf[tid] = leafkey[i * TP + tid];
+    if (masks_out && tid < T && tid < kMaskTrees) S.own_leaf[tid] = leafkey[i * TP + tid];
+    if (tid < K) S.own_mask[tid] = masks_in ? masks_in[i * K + tid] : uint8_t(0);
     for (int c = tid; c < nch; c += 256)
       *reinterpret_cast<v4i*>(own_row + 16 * c) = *reinterpret_cast<const v4i*>(q + i * d + 16 * c);
     if (tid < words) S.own_sk[tid] = sk[i * words + tid];
@@ -1817,8 +1834,17 @@ __global__ __launch_bounds__(256, OCC) void k_join(
         if (k < K && lane < K) {
           const int32_t n1 = S.own_id[k];
           const bool take = S.own_bit[k] != 0 || !fresh_in || ((fresh_in[n1] >> lane) & 1ull);
-          if (take) c[r] = ids_in[int64_t(n1) * K + lane];
+          if (take) {
+            c[r] = ids_in[int64_t(n1) * K + lane];
+            if (masks_in && (masks_in[int64_t(n1) * K + lane] & S.own_mask[k])) c[r] = -2;   // leaf-mate path
+          }
         }
+      }
+      if (masks_in) {
+        int dropped = 0;
+#pragma unroll
+        for (int r = 0; r < kRowsPerWave; ++r) dropped += __popcll(__ballot(c[r] == -2));
+        if (lane == 0 && dropped) atomicAdd(&S.cnt[5], dropped);
       }
 #pragma unroll
       for (int r = 0; r < kRowsPerWave; ++r) {
@@ -1835,42 +1861,6 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     if (SMALL && U > lim) {   // uniform: the full kernel runs this row
       if (tid == 0) ovf[atomicAdd(ovf_n, 1ull)] = int32_t(i);
     } else {
-      if (leafkey) {
-        // ---- 2b: drop leaf-mates (one 4 * TP byte gather per unique candidate,
-        //          all of a thread's in flight together), compact the rest
-        constexpr int kPer = (CAP + 255) / 256;
-        int32_t v[kPer];
-        bool keep[kPer];
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-          const int u = tid + 256 * j;
-          v[j] = u < U ? cid[u] : -1;
-          keep[j] = v[j] >= 0;
-        }
-        for (int t0 = 0; t0 < T; t0 += 4) {
-          int4 lk[kPer];
-#pragma unroll
-          for (int j = 0; j < kPer; ++j)
-            lk[j] = keep[j] ? *reinterpret_cast<const int4*>(leafkey + int64_t(v[j]) * TP + t0)
-                            : make_int4(-1, -1, -1, -1);
-#pragma unroll
-          for (int j = 0; j < kPer; ++j) {
-            const bool mate = lk[j].x == S.own_leaf[t0] || (t0 + 1 < T && lk[j].y == S.own_leaf[t0 + 1]) ||
-                              (t0 + 2 < T && lk[j].z == S.own_leaf[t0 + 2]) ||
-                              (t0 + 3 < T && lk[j].w == S.own_leaf[t0 + 3]);
-            keep[j] = keep[j] && !mate;
-          }
-        }
-        __syncthreads();   // every cid entry read
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-          const int s = wave_append(keep[j], &S.cnt[5]);
-          if (keep[j]) cid[s] = v[j];
-        }
-        __syncthreads();
-        if (tid == 0) my_ctr[5] += U;   // leaf-key rows gathered
-        U = S.cnt[5];
-      }
       if constexpr (SMALL) {
         // at most M candidates: the selection is the identity
         for (int u = tid; u < U; u += 256) sel_idx[u] = uint16_t(u);
@@ -2078,6 +2068,8 @@ __global__ __launch_bounds__(256, OCC) void k_join(
           S.new_id[r] = id;
           S.new_sc[r] = sc;
           S.new_fresh[r] = e >= K ? 1 : 0;
+          if (masks_out)
+            S.new_mask[r] = uint8_t(e < K ? S.own_mask[e] : edge_mask(S.own_leaf, leafkey + int64_t(id) * TP, T));
           if (e >= K) atomicAdd(&S.cnt[4], 1);   // a new id entered the list
         }
       }
@@ -2088,6 +2080,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
         if (lane < K) {
           ids_out[i * K + lane] = S.new_id[lane];
           sc_out[i * K + lane] = S.new_sc[lane];
+          if (masks_out) masks_out[i * K + lane] = S.new_mask[lane];
           changed = S.new_id[lane] != S.own_id[lane];
           fresh = S.new_fresh[lane] != 0;
         }
@@ -2099,6 +2092,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
           my_ctr[1] += U;
           my_ctr[2] += NS;
           my_ctr[3] += S.cnt[4];
+          my_ctr[5] += S.cnt[5];
           if (U > M) my_ctr[4] += U;
           if (segc && tid == 0) {
             const int sg = seg_of(seg_base, nseg, i);
@@ -2167,7 +2161,7 @@ __device__ __forceinline__ WaveSlice wave_slice(int64_t nwork) {
 
 __global__ __launch_bounds__(256) void k_jcand(
     const uint64_t* __restrict__ sk, int d, int K, int M, const int32_t* __restrict__ ids_in,
-    const uint64_t* __restrict__ fresh_in, const int32_t* __restrict__ leafkey, int T, int TP,
+    const uint64_t* __restrict__ fresh_in, const uint8_t* __restrict__ masks_in,
     const int32_t* __restrict__ order, int64_t nwork, int32_t* __restrict__ cbuf,
     int32_t* __restrict__ ccount, int32_t* __restrict__ ovf, unsigned long long* __restrict__ ovf_n,
     unsigned long long* __restrict__ counters) {
@@ -2178,14 +2172,14 @@ __global__ __launch_bounds__(256) void k_jcand(
   int32_t* cid = cid_w[w];
   const int words = d >> 6;
   const unsigned long long lt = (1ull << lane) - 1ull;
-  unsigned long long n_u = 0, n_sk = 0, n_lk = 0;   // candidates, sketched candidates, leaf-key checks
+  unsigned long long n_u = 0, n_sk = 0, n_lk = 0;   // candidates, sketched candidates, paths dropped
   const WaveSlice ws = wave_slice(nwork);
   for (int64_t j = ws.j0; j < ws.hi; j += ws.step) {
     const int64_t i = order ? int64_t(order[j]) : j;
     for (int s = lane; s < (1 << kJHashBits); s += 64) hs[s] = -1;
     const int32_t oid = lane < K ? ids_in[i * K + lane] : -1;
     const bool obit = !fresh_in || (lane < K && ((fresh_in[i] >> lane) & 1ull));
-    const int32_t myleaf = (leafkey && lane < T) ? leafkey[i * TP + lane] : -1;
+    const int32_t omask = (masks_in && lane < K) ? int32_t(masks_in[i * K + lane]) : 0;
     __builtin_amdgcn_wave_barrier();
     if (lane < K) hs_insert(hs, kJHashBits, oid);
     if (lane == 0) hs_insert(hs, kJHashBits, int32_t(i));   // K may be 64: no spare lane
@@ -2193,8 +2187,9 @@ __global__ __launch_bounds__(256) void k_jcand(
     int U = 0;
     // the neighbours' fresh masks, lane k holding neighbour k's (one load per row)
     const uint64_t fnb = (fresh_in && lane < K) ? fresh_in[oid] : ~0ull;
-    // 2-hop ids of eight neighbour lists at a time: batch b + 1's ids are
-    // loaded while batch b's leaf keys are in flight
+    // 2-hop ids of eight neighbour lists at a time (batch b + 1 loads while
+    // batch b is hashed); the leaf-mate rule drops the path i -> n1 -> c when
+    // both of its edges share a leaf in the same tree (their mask bytes meet)
     int32_t cn[8];
     auto load_ids = [&](int k0, int32_t (&dst)[8]) {
 #pragma unroll
@@ -2204,7 +2199,12 @@ __global__ __launch_bounds__(256) void k_jcand(
         if (k < K) {
           const int32_t n1 = __shfl(oid, k);
           const bool take = __shfl(int(obit), k) != 0 || ((__shfl(fnb, k) >> lane) & 1ull);
-          if (lane < K && take) dst[u] = ids_in[int64_t(n1) * K + lane];
+          if (lane < K && take) {
+            const int32_t id = ids_in[int64_t(n1) * K + lane];
+            const int32_t m = masks_in ? int32_t(masks_in[int64_t(n1) * K + lane]) : 0;
+            // -2: a path the leaf-mate rule drops (counted, never hashed)
+            dst[u] = (m & __shfl(omask, k)) ? -2 : id;
+          }
         }
       }
     };
@@ -2213,33 +2213,10 @@ __global__ __launch_bounds__(256) void k_jcand(
       int32_t c[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) c[u] = cn[u];
-      if (leafkey) {
-        for (int t0 = 0; t0 < T; t0 += 4) {
-          const int32_t l0 = __shfl(myleaf, t0), l1 = __shfl(myleaf, t0 + 1 < T ? t0 + 1 : t0),
-                        l2 = __shfl(myleaf, t0 + 2 < T ? t0 + 2 : t0), l3 = __shfl(myleaf, t0 + 3 < T ? t0 + 3 : t0);
-          int4 lk[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u)
-            lk[u] = c[u] >= 0 ? *reinterpret_cast<const int4*>(leafkey + int64_t(c[u]) * TP + t0)
-                              : make_int4(-1, -1, -1, -1);
-          if (t0 == 0 && k0 + 8 < K) load_ids(k0 + 8, cn);
-          // branch-free test (bitwise, then a select): as a chain of || with
-          // conditional assignment the compiler lost the update for matches
-          // in the .z/.w lanes
-          const int v1 = t0 + 1 < T, v2 = t0 + 2 < T, v3 = t0 + 3 < T;
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            if (t0 == 0) n_lk += __popcll(__ballot(c[u] >= 0));
-            const int mate = int(lk[u].x == l0) | (v1 & int(lk[u].y == l1)) | (v2 & int(lk[u].z == l2)) |
-                             (v3 & int(lk[u].w == l3));
-            c[u] = mate ? -1 : c[u];
-          }
-        }
-      } else if (k0 + 8 < K) {
-        load_ids(k0 + 8, cn);
-      }
+      if (k0 + 8 < K) load_ids(k0 + 8, cn);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
+        n_lk += __popcll(__ballot(c[u] == -2));
         if (U > kJCap) c[u] = -1;   // the row overflows: insert nothing more (wave-uniform)
         const bool nw = c[u] >= 0 && hs_insert(hs, kJHashBits, c[u]);
         const unsigned long long bal = __ballot(nw);
@@ -2336,18 +2313,21 @@ template <int DIM>   // DIM > 0: rows of DIM bytes at compile time; 0: any d
 __global__ __launch_bounds__(256) void k_jdot_merge(
     const int8_t* __restrict__ q, int d, int K, const int32_t* __restrict__ ids_in,
     const int32_t* __restrict__ sc_in, int32_t* __restrict__ ids_out, int32_t* __restrict__ sc_out,
-    uint64_t* __restrict__ fresh_out, const int32_t* __restrict__ order, int64_t nwork,
+    uint64_t* __restrict__ fresh_out, const uint8_t* __restrict__ masks_in, uint8_t* __restrict__ masks_out,
+    const int32_t* __restrict__ leafkey, int T, int TP, const int32_t* __restrict__ order, int64_t nwork,
     const int32_t* __restrict__ cbuf, const int32_t* __restrict__ ccount,
     unsigned long long* __restrict__ counters) {
   __shared__ uint64_t key_w[4][kJCap];   // candidate keys, then the survivors' keys
   __shared__ uint64_t okey_w[4][kMaxK];
   __shared__ int32_t nid_w[4][kMaxK], nsc_w[4][kMaxK], nfr_w[4][kMaxK];
+  __shared__ int32_t oleaf_w[4][kMaskTrees];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint64_t* key = key_w[w];
   uint64_t* okey = okey_w[w];
   int32_t* nid = nid_w[w];
   int32_t* nsc = nsc_w[w];
-  int32_t* nfr = nfr_w[w];
+  int32_t* nfr = nfr_w[w];   // 1: from the candidates; else the own entry's mask byte << 1
+  int32_t* oleaf = oleaf_w[w];
   if (DIM > 0) d = DIM;
   const int nch = DIM > 0 ? DIM / 16 : d >> 4;
   const int g = lane >> 4, s16 = lane & 15;
@@ -2360,8 +2340,10 @@ __global__ __launch_bounds__(256) void k_jdot_merge(
     const int64_t i = order ? int64_t(order[j]) : j;
     const int32_t oid = lane < K ? ids_in[i * K + lane] : -1;
     const int32_t osc = lane < K ? sc_in[i * K + lane] : 0;
+    const int32_t om = (masks_in && lane < K) ? int32_t(masks_in[i * K + lane]) : 0;
     const uint64_t ok = lane < K ? pack_key(osc, oid) : ~0ull;
     if (lane < K) okey[lane] = ok;
+    if (masks_out && lane < T && lane < kMaskTrees) oleaf[lane] = leafkey[i * TP + lane];
     // the K-th best own key: a candidate below it cannot enter the list
     uint64_t mn = ok;
 #pragma unroll
@@ -2430,7 +2412,7 @@ __global__ __launch_bounds__(256) void k_jdot_merge(
       if (r < K) {
         nid[r] = oid;
         nsc[r] = osc;
-        nfr[r] = 0;
+        nfr[r] = om << 1;
       }
     }
     for (int p = lane; p < P; p += 64) {
@@ -2445,13 +2427,16 @@ __global__ __launch_bounds__(256) void k_jdot_merge(
       }
     }
     __builtin_amdgcn_wave_barrier();
-    // ---- write back
+    // ---- write back (a new entry's mask byte from its leaf keys)
     bool changed = false, fr = false;
     if (lane < K) {
-      ids_out[i * K + lane] = nid[lane];
+      const int32_t id = nid[lane];
+      ids_out[i * K + lane] = id;
       sc_out[i * K + lane] = nsc[lane];
-      changed = nid[lane] != oid;
-      fr = nfr[lane] != 0;
+      changed = id != oid;
+      fr = (nfr[lane] & 1) != 0;
+      if (masks_out)
+        masks_out[i * K + lane] = uint8_t(fr ? edge_mask(oleaf, leafkey + int64_t(id) * TP, T) : nfr[lane] >> 1);
     }
     const unsigned long long fm = __ballot(fr);
     if (fresh_out && lane == 0) fresh_out[i] = fm;
@@ -3255,10 +3240,12 @@ int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const in
   unsigned long long* rowq = c->counters + kCounterFields * kNumCounters;
   const uint64_t* fin = c->p.incremental ? c->fresh[a] : nullptr;
   const int32_t* leafkey = c->leaf_valid ? c->leafkey : nullptr;
+  const uint8_t* mk_in = c->leaf_valid ? c->gmask[a] : nullptr;
+  uint8_t* mk_out = c->leaf_valid ? c->gmask[b] : nullptr;
 #define KNNG_JOIN_ARGS                                                                          \
   c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order, \
       fin, c->fresh[b], nwork_dev, seg_base, nseg, segc, static_cast<int32_t*>(nullptr),             \
-      static_cast<unsigned long long*>(nullptr), leafkey, int(c->p.num_trees), c->leaf_tp
+      static_cast<unsigned long long*>(nullptr), mk_in, mk_out, leafkey, int(c->p.num_trees), c->leaf_tp
   const bool d1024 = c->d == kMaxDim;   // the rows' length as a compile-time constant
   if (K > 50 && d1024)
     hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
@@ -3301,11 +3288,14 @@ int launch_join_small(knng_ctx* c, int a, int b, const int32_t* order) {
   HIPCHK(c, hipMemsetAsync(c->ovf_n, 0, sizeof(unsigned long long), c->stream));
   const int grid = std::max(8, (c->num_cu * kSmallJoinOcc) / 8 * 8);
   const int32_t* leafkey = c->leaf_valid ? c->leafkey : nullptr;
+  const uint8_t* mk_in = c->leaf_valid ? c->gmask[a] : nullptr;
+  uint8_t* mk_out = c->leaf_valid ? c->gmask[b] : nullptr;
   unsigned long long* rowq = c->counters + kCounterFields * kNumCounters;
 #define KNNG_SMALL_ARGS                                                                                 \
   c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order,     \
       c->fresh[a], c->fresh[b], static_cast<const int64_t*>(nullptr), static_cast<const int64_t*>(nullptr), \
-      0, static_cast<unsigned long long*>(nullptr), c->ovf, c->ovf_n, leafkey, int(c->p.num_trees), c->leaf_tp
+      0, static_cast<unsigned long long*>(nullptr), c->ovf, c->ovf_n, mk_in, mk_out, leafkey, int(c->p.num_trees), \
+      c->leaf_tp
   if (c->d == kMaxDim)
     hipLaunchKernelGGL((k_join<10, 640, kSmallJoinOcc, 1, 8, kMaxDim, true>), dim3(grid), dim3(256), 0, c->stream,
                        KNNG_SMALL_ARGS);
@@ -3358,18 +3348,22 @@ int launch_join_phased(knng_ctx* c, int a, int b, const int32_t* order) {
     return rc;
   HIPCHK(c, hipMemsetAsync(c->ovf_n, 0, sizeof(unsigned long long), c->stream));
   const int32_t* leafkey = c->leaf_valid ? c->leafkey : nullptr;
+  const uint8_t* mk_in = c->leaf_valid ? c->gmask[a] : nullptr;
+  uint8_t* mk_out = c->leaf_valid ? c->gmask[b] : nullptr;
+  const int T = int(c->p.num_trees);
   // every entry still new (no join iteration yet): the new/old rule takes every pair
   const uint64_t* fin = (c->p.incremental && !c->fresh_all) ? c->fresh[a] : nullptr;
   const int ga = resident_grid(c, k_jcand);
-  hipLaunchKernelGGL(k_jcand, dim3(ga), dim3(256), 0, c->stream, c->sk, c->d, K, M, c->gid[a], fin, leafkey,
-                     int(c->p.num_trees), c->leaf_tp, order, n, c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters);
+  hipLaunchKernelGGL(k_jcand, dim3(ga), dim3(256), 0, c->stream, c->sk, c->d, K, M, c->gid[a], fin, mk_in, order, n,
+                     c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters);
   if (c->d == kMaxDim)
     hipLaunchKernelGGL(k_jdot_merge<kMaxDim>, dim3(resident_grid(c, k_jdot_merge<kMaxDim>)), dim3(256), 0, c->stream,
-                       c->q, c->d, K, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->fresh[b], order, n, c->jcbuf,
-                       c->jccount, c->counters);
+                       c->q, c->d, K, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->fresh[b], mk_in, mk_out, leafkey,
+                       T, c->leaf_tp, order, n, c->jcbuf, c->jccount, c->counters);
   else
-    hipLaunchKernelGGL(k_jdot_merge<0>, dim3(resident_grid(c, k_jdot_merge<0>)), dim3(256), 0, c->stream, c->q, c->d, K, c->gid[a], c->gsc[a],
-                       c->gid[b], c->gsc[b], c->fresh[b], order, n, c->jcbuf, c->jccount, c->counters);
+    hipLaunchKernelGGL(k_jdot_merge<0>, dim3(resident_grid(c, k_jdot_merge<0>)), dim3(256), 0, c->stream, c->q, c->d,
+                       K, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->fresh[b], mk_in, mk_out, leafkey, T,
+                       c->leaf_tp, order, n, c->jcbuf, c->jccount, c->counters);
   HIPCHK(c, hipGetLastError());
   return launch_join(c, 0, a, b, c->ovf, reinterpret_cast<const int64_t*>(c->ovf_n), nullptr, 0, nullptr);
 }
@@ -3615,6 +3609,8 @@ int knng_destroy(knng_ctx* c) {
   dfree(&c->counters);
   dfree(&c->csum);
   dfree(&c->leafkey);
+  dfree(&c->gmask[0]);
+  dfree(&c->gmask[1]);
   dfree(&c->jcbuf);
   dfree(&c->jccount);
   dfree(&c->seg_base);
@@ -3904,7 +3900,8 @@ int knng_build_init(knng_ctx* c) {
   for (int b = 0; b < 2; ++b)
     if ((rc = dalloc(c, &c->gid[b], size_t(n) * K, "graph ids")) ||
         (rc = dalloc(c, &c->gsc[b], size_t(n) * K, "graph scores")) ||
-        (rc = dalloc(c, &c->fresh[b], size_t(n), "fresh masks")))
+        (rc = dalloc(c, &c->fresh[b], size_t(n), "fresh masks")) ||
+        (rc = dalloc(c, &c->gmask[b], size_t(n) * K, "leaf mask bytes")))
       return rc;
   HIPCHK(c, hipMemsetAsync(c->fresh[0], 0xFF, size_t(n) * sizeof(uint64_t), c->stream));   // all fresh
   c->fresh_all = true;
@@ -4001,6 +3998,8 @@ int knng_build_init(knng_ctx* c) {
   }
   hipLaunchKernelGGL(k_leaf_keys, dim3(grid_for(c, c->nleaves, 4, 16)), dim3(256), 0, c->stream, c->leaves,
                      c->nleaves, c->perm, n, c->leaf_tp, c->leafkey);
+  hipLaunchKernelGGL(k_edge_masks, dim3(grid_for(c, n * K, 256, 16)), dim3(256), 0, c->stream, c->gid[0], n, K,
+                     c->leafkey, T, c->leaf_tp, c->gmask[0]);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -4062,10 +4061,11 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
     c->st.join_passed_on[it] = (small || phased) ? int64_t(passed) : -1;
     c->st.join_leafmates[it] = int64_t(sums[5]);
     // algorithmic bytes: own list + own row/sketch, K neighbour id rows, one
-    // leaf-key row per unique candidate (leaf-mate rule), one sketch per
-    // candidate of a row that gathers them, one row per re-scored candidate,
-    // writes
-    const double lk_bytes = double(sums[5]) * 4.0 * c->leaf_tp;
+    // sketch per candidate of a row that gathers them, one row per re-scored
+    // candidate, writes; with the leaf-mate rule also the mask bytes (own,
+    // neighbours', written) and one leaf-key row per new entry
+    const double lk_bytes =
+        c->leaf_valid ? double(n) * (2.0 * K + double(K) * K) + double(sums[3]) * 4.0 * c->leaf_tp : 0.0;
     c->st.join_bytes[it] = double(n) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) + lk_bytes +
                            double(sums[4]) * 8.0 * c->words + double(sums[2]) * c->d;
   }
@@ -4244,7 +4244,8 @@ int knng_load_graph(knng_ctx* c, const int32_t* ids, const int32_t* scores) {
   for (int b = 0; b < 2; ++b)
     if ((rc = dalloc(c, &c->gid[b], size_t(n) * K, "graph ids")) ||
         (rc = dalloc(c, &c->gsc[b], size_t(n) * K, "graph scores")) ||
-        (rc = dalloc(c, &c->fresh[b], size_t(n), "fresh masks")))
+        (rc = dalloc(c, &c->fresh[b], size_t(n), "fresh masks")) ||
+        (rc = dalloc(c, &c->gmask[b], size_t(n) * K, "leaf mask bytes")))
       return rc;
   c->cur = 0;
   const size_t bytes = size_t(n) * K * sizeof(int32_t);
@@ -4304,7 +4305,8 @@ int knng_join_iteration_noalloc(const int8_t* dq, const uint64_t* dsk, int32_t d
       static_cast<uint64_t*>(nullptr), static_cast<const int64_t*>(nullptr),                                   \
       static_cast<const int64_t*>(nullptr), 0, static_cast<unsigned long long*>(nullptr),                      \
       static_cast<int32_t*>(nullptr), static_cast<unsigned long long*>(nullptr),                               \
-      static_cast<const int32_t*>(nullptr), 0, 4
+      static_cast<const uint8_t*>(nullptr), static_cast<uint8_t*>(nullptr), static_cast<const int32_t*>(nullptr), \
+      0, 4
   if (K > 50 && d == kMaxDim)
     hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, st, KNNG_NOALLOC_ARGS);
   else if (K > 50)
@@ -4487,7 +4489,8 @@ int build_packed(knng_ctx* c, std::vector<int32_t>* iters, std::vector<int32_t>*
       c->st.join_leafmates[it] = int64_t(L.csum_h[5]);
       c->st.insert_rates[it] = float(L.csum_h[3]) / float(size_t(c->n) * size_t(K));
       // algorithmic bytes as knng_join_iteration counts them, over the rows processed
-      const double lk_bytes = double(L.csum_h[5]) * 4.0 * c->leaf_tp;
+      const double lk_bytes =
+          c->leaf_valid ? double(rows) * (2.0 * K + double(K) * K) + double(L.csum_h[3]) * 4.0 * c->leaf_tp : 0.0;
       c->st.join_bytes[it] = double(rows) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) +
                              lk_bytes + double(L.csum_h[4]) * 8.0 * c->words + double(L.csum_h[2]) * c->d;
     }

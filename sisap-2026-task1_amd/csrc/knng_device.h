@@ -19,8 +19,10 @@ constexpr int kLeafSlots = 64;          // candidate slots per (point, tree)
 constexpr int kMaxLeaf = 2048;          // largest leaf (leaves above kLeafSlots keep 64 mates)
 constexpr int kNumCounters = 1024;      // spread of the per-block atomic counters
 // fields per counter: changed slots, unique candidates, re-scored candidates,
-// inserted ids, candidates whose sketches were gathered, leaf-mates dropped
+// inserted ids, candidates whose sketches were gathered, 2-hop paths dropped
+// by the leaf-mate rule
 constexpr int kCounterFields = 6;
+constexpr int kMaskTrees = 8;           // trees whose leaves a list entry's mask byte records
 constexpr int kRowQueueWords = 8 * 32;  // k_join row-queue counters, one 256 B line per XCD
 constexpr float kFltMin = 1.17549435e-38f;
 
@@ -61,6 +63,15 @@ __device__ __forceinline__ int dot16_i8(const v4i& a, const v4i& b, int c) {
   c = dot4_i8(a.y, b.y, c);
   c = dot4_i8(a.z, b.z, c);
   return dot4_i8(a.w, b.w, c);
+}
+
+// Mask byte of a list entry: bit t (trees 0..7) = the row and the entry share
+// tree t's leaf.  own[t] = the row's leaf keys, lk = the entry's leaf-key row.
+__device__ __forceinline__ int32_t edge_mask(const int32_t* own, const int32_t* __restrict__ lk, int T) {
+  int32_t m = 0;
+  const int tm = T < kMaskTrees ? T : kMaskTrees;
+  for (int t = 0; t < tm; ++t) m |= int32_t(lk[t] == own[t]) << t;
+  return m;
 }
 
 // (score desc, id asc) as a descending u64 key; 0 is "no candidate".

@@ -66,6 +66,7 @@ struct knng_ctx {
   int32_t* leafkey = nullptr;
   int leaf_tp = 4;
   bool leaf_valid = false;
+  uint8_t* gmask[2] = {nullptr, nullptr};    // per list entry: bit t = row and entry share tree t's leaf
   unsigned long long *counters = nullptr, *csum = nullptr;
 
   // ingest staging

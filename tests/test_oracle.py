@@ -471,6 +471,10 @@ def _leaf_sets(perm, leaves, n):
 
 @pytest.mark.parametrize("leaf_rule", [False, True])
 def test_join_device_bruteforce(leaf_rule):
+    """One join iteration against a numpy restatement: 2-hop candidates (a path
+    i -> n1 -> c is dropped under the leaf-mate rule when i, n1 and c lie in one
+    leaf of one tree), sketch threshold, exact int8 scores, (score desc, id asc,
+    slot) merge, change count."""
     x, q, sk = _small(n=250, d=128)
     K, M = 10, 30
     perm, leaves = oracle.forest_device(q, 2, 16)
@@ -487,7 +491,8 @@ def test_join_device_bruteforce(leaf_rule):
         for n1 in ids[i]:
             for c in ids[n1]:
                 c = int(c)
-                if leaf_rule and mine[i] & mine[c]:
+                # leaf-mate rule: i, n1 and c in one leaf of one tree
+                if leaf_rule and mine[i] & mine[int(n1)] & mine[c]:
                     continue
                 if c not in excl and c not in cands:
                     cands.append(c)
