@@ -2310,7 +2310,7 @@ __global__ __launch_bounds__(256) void k_jcand(
 }
 
 template <int DIM>   // DIM > 0: rows of DIM bytes at compile time; 0: any d
-__global__ __launch_bounds__(256) void k_jdot_merge(
+__global__ __launch_bounds__(256, 4) void k_jdot_merge(
     const int8_t* __restrict__ q, int d, int K, const int32_t* __restrict__ ids_in,
     const int32_t* __restrict__ sc_in, int32_t* __restrict__ ids_out, int32_t* __restrict__ sc_out,
     uint64_t* __restrict__ fresh_out, const uint8_t* __restrict__ masks_in, uint8_t* __restrict__ masks_out,
@@ -2321,7 +2321,9 @@ __global__ __launch_bounds__(256) void k_jdot_merge(
   __shared__ uint64_t okey_w[4][kMaxK];
   __shared__ int32_t nid_w[4][kMaxK], nsc_w[4][kMaxK], nfr_w[4][kMaxK];
   __shared__ int32_t oleaf_w[4][kMaskTrees];
+  __shared__ __attribute__((aligned(16))) int8_t orow_w[4][kMaxDim];   // the row's int8 data
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int8_t* orow = orow_w[w];
   uint64_t* key = key_w[w];
   uint64_t* okey = okey_w[w];
   int32_t* nid = nid_w[w];
@@ -2351,19 +2353,23 @@ __global__ __launch_bounds__(256) void k_jdot_merge(
       const uint64_t o = __shfl_xor(mn, m);
       mn = o < mn ? o : mn;
     }
-    // ---- int8 dots: 16 lanes per candidate, two candidates per group in flight
-    v4i own[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    // ---- int8 dots: 16 lanes per candidate, four candidates per group in flight
+    //      (the kernel is bound by rows in flight per CU)
+    constexpr int ND = 2;
+    // the row's own data in LDS (read again at each step: registers go to rows in flight)
+    for (int ch = lane; ch < nch; ch += 64)
+      *reinterpret_cast<v4i*>(orow + 16 * ch) = *reinterpret_cast<const v4i*>(q + i * d + 16 * ch);
+    __builtin_amdgcn_wave_barrier();
+    auto own = [&](int c) -> v4i {
       const int ch = s16 + 16 * c;
-      own[c] = ch < nch ? *reinterpret_cast<const v4i*>(q + i * d + 16 * ch) : v4i{0, 0, 0, 0};
-    }
+      return ch < nch ? *reinterpret_cast<const v4i*>(orow + 16 * ch) : v4i{0, 0, 0, 0};
+    };
     const int32_t* cl = cbuf + j * kJCap;
-    for (int s0 = 0; s0 < S; s0 += 8) {
-      int32_t id[2];
-      v4i v[2][4];
+    for (int s0 = 0; s0 < S; s0 += 4 * ND) {
+      int32_t id[ND];
+      v4i v[ND][4];
 #pragma unroll
-      for (int r = 0; r < 2; ++r) {
+      for (int r = 0; r < ND; ++r) {
         const int s = s0 + g + 4 * r;
         id[r] = s < S ? cl[s] : -1;
 #pragma unroll
@@ -2374,10 +2380,10 @@ __global__ __launch_bounds__(256) void k_jdot_merge(
         }
       }
 #pragma unroll
-      for (int r = 0; r < 2; ++r) {
+      for (int r = 0; r < ND; ++r) {
         int dt = 0;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) dt = dot16_i8(v[r][c], own[c], dt);
+        for (int c = 0; c < 4; ++c) dt = dot16_i8(v[r][c], own(c), dt);
 #pragma unroll
         for (int m = 8; m >= 1; m >>= 1) dt += __shfl_xor(dt, m);
         const int s = s0 + g + 4 * r;
@@ -2385,21 +2391,19 @@ __global__ __launch_bounds__(256) void k_jdot_merge(
       }
     }
     __builtin_amdgcn_wave_barrier();
-    // ---- survivors (key above the K-th own key), compacted in place
-    constexpr int kPer = kJCap / 64;
-    uint64_t kv[kPer];
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) kv[e] = lane + 64 * e < S ? key[lane + 64 * e] : 0ull;
-    __builtin_amdgcn_wave_barrier();
+    // ---- survivors (key above the K-th own key), compacted in place one
+    //      64-entry chunk at a time (a chunk is read before any of it is
+    //      overwritten, and writes land below the next chunk)
     int P = 0;
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      const bool sv = kv[e] > mn;
+    for (int e0 = 0; e0 < S; e0 += 64) {
+      const uint64_t kv = e0 + lane < S ? key[e0 + lane] : 0ull;
+      __builtin_amdgcn_wave_barrier();
+      const bool sv = kv > mn;
       const unsigned long long bal = __ballot(sv);
-      if (sv) key[P + __popcll(bal & lt)] = kv[e];
+      if (sv) key[P + __popcll(bal & lt)] = kv;
       P += __popcll(bal);
+      __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_wave_barrier();
     // ---- ranks in the union (own entries tie-break by slot; candidate ids are
     //      neither own ids nor repeated, so their keys are distinct)
     if (lane < K) {
@@ -4124,7 +4128,10 @@ int knng_export_device(knng_ctx* c, int32_t* dk, float* dd, int32_t k, int32_t r
     int R = c->p.rerank_width ? c->p.rerank_width : K;
     if (R < k) R = k;
     const int32_t* order = c->have_forest ? c->perm : nullptr;   // tree 0's leaf order
-    hipLaunchKernelGGL(k_export_rerank, dim3(rerank_grid(c, n)), dim3(256), 0, c->stream, x, c->d,
+    // every block resident at once: each XCD's rows in flight are one window of
+    // consecutive cluster-mates, whose candidate rows then stay in its L2
+    const int grid = std::min(rerank_grid(c, n), resident_grid(c, k_export_rerank));
+    hipLaunchKernelGGL(k_export_rerank, dim3(grid), dim3(256), 0, c->stream, x, c->d,
                        n, K, R, k, c->gid[c->cur], dk, dd, int64_t(0), order);
   }
   HIPCHK(c, hipGetLastError());
