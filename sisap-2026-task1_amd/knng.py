@@ -374,6 +374,29 @@ def h5_write_train(path: str, x: np.ndarray, chunk_rows: int = 0) -> None:
         raise KnngError(f"h5 write of {path} failed ({rc})")
 
 
+class H5Writer:
+    """Chunked float32 /train writer (knng_h5_create_f32 / write_rows / finish)."""
+
+    def __init__(self, path: str, rows: int, cols: int, chunk_rows: int, dset: str = "train"):
+        self.w = C.c_void_p()
+        rc = h5().knng_h5_create_f32(path.encode(), dset.encode(), rows, cols, chunk_rows, C.byref(self.w))
+        if rc:
+            raise KnngError(f"h5 create of {path} failed ({rc})")
+
+    def write(self, row0: int, rows: np.ndarray) -> None:
+        rows = np.ascontiguousarray(rows, np.float32)
+        rc = h5().knng_h5_write_rows(self.w, row0, rows.shape[0], _ptr(rows))
+        if rc:
+            raise KnngError(f"h5 write of rows {row0}+{rows.shape[0]} failed ({rc})")
+
+    def close(self) -> None:
+        if self.w:
+            rc = h5().knng_h5_finish(self.w)
+            self.w = C.c_void_p()
+            if rc:
+                raise KnngError(f"h5 finish failed ({rc})")
+
+
 def h5_attr_str(path: str, name: str) -> str:
     buf = C.create_string_buffer(4096)
     rc = h5().knng_h5_read_attr_str(path.encode(), name.encode(), buf, 4096)
