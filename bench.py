@@ -205,7 +205,9 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0, help="Poisson request rate (req/s)")
     ap.add_argument("--insert-frac", type=float, default=0.1, help="share of requests that insert rows")
     ap.add_argument("--serve-batch", type=int, default=2048, help="queries per bulk launch")
-    ap.add_argument("--serve-slots", type=int, default=4, help="concurrent bulk launches")
+    ap.add_argument("--serve-slots", type=int, default=3,
+                    help="concurrent bulk launches (at 4 a bulk stream shares an express stream's "
+                         "hardware queue under HIP's default 4 queues: small-batch p99 9.3 ms vs 5.9)")
     ap.add_argument("--express-slots", type=int, default=2, help="express-lane slots (batches <= 64)")
     ap.add_argument("--express-cus", type=int, default=0, help="CUs reserved for the express lane")
     ap.add_argument("--bulk-yield", type=int, default=0,
