@@ -2646,6 +2646,94 @@ __global__ __launch_bounds__(256) void k_export_rerank(const float* __restrict__
   }
 }
 
+// FP32 rerank in dimension slices, one launch per slice.  A whole float32 row
+// is 4 KB, and the candidate rows of the ~900 rows in flight on one XCD (about
+// 6,000 distinct rows at 6.35M x 1024, k_build 50, width 22: a row's list
+// reaches well beyond its tree-0 leaf) overflow its 4 MB L2, so the one-pass
+// kernel above fetches 0.96 of its gathered bytes from HBM.  A 256-dim slice
+// is 1 KB; a pass's window then needs a quarter of the bytes, and the passes
+// fetch 0.89 of theirs (about 7 % less time at that size).
+// Pass s adds the FP32 dot products over dims [256 s, 256 s + 256) of the
+// row's first R list entries to part[i * R + r]; the last pass ranks the sums
+// (((p0 + p1) + p2) + p3) by (score desc, id asc, slot asc) and writes the
+// row's best k, reversed, as k_export_rerank does.  Rows in `order`, each XCD
+// taking a contiguous eighth of it (the grid is a multiple of 8 blocks).
+// One wave per row in four groups of 16 lanes: group g scores candidates
+// g, g + 4, ... (two per group in flight), lane t of a group holds float4
+// chunks t, t + 16, t + 32, t + 48 of the slice.  d % 64 == 0, so a chunk
+// column is either inside the row or past its end.
+constexpr int kRrSlice = 256;
+
+template <bool kFirst, bool kLast>
+__global__ __launch_bounds__(256) void k_rerank_slice(const float* __restrict__ x, int d, int64_t n, int K,
+                                                      int R, int k, int s, const int32_t* __restrict__ gid,
+                                                      float* __restrict__ part, int32_t* __restrict__ knns,
+                                                      float* __restrict__ dists,
+                                                      const int32_t* __restrict__ order) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int g = lane >> 4, t = lane & 15;
+  const int xcd = blockIdx.x & 7;
+  const int64_t nbx = gridDim.x >> 3, bx = blockIdx.x >> 3;
+  const int64_t jhi = n * (xcd + 1) / 8;
+  const int base = s * kRrSlice;
+  bool on[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) on[c] = base + 64 * c < d;
+  for (int64_t j = n * xcd / 8 + bx * 4 + w; j < jhi; j += nbx * 4) {
+    const int64_t i = order ? order[j] : j;
+    const float4* po = reinterpret_cast<const float4*>(x + i * d + base);
+    float4 own[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) own[c] = on[c] ? po[t + 16 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int32_t my_id = lane < R ? gid[i * K + lane] : 0;
+    float my_s = 0.f;
+    for (int r0 = 0; r0 < R; r0 += 8) {
+      float4 v[2][4];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int r = r0 + 4 * u + g;
+        const int32_t id = __shfl(my_id, r < R ? r : 0);
+        const float4* pc = reinterpret_cast<const float4*>(x + int64_t(id) * d + base);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          v[u][c] = (on[c] && r < R) ? pc[t + 16 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        float sa = 0.f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          sa = fmaf(own[c].x, v[u][c].x, sa);
+          sa = fmaf(own[c].y, v[u][c].y, sa);
+          sa = fmaf(own[c].z, v[u][c].z, sa);
+          sa = fmaf(own[c].w, v[u][c].w, sa);
+        }
+#pragma unroll
+        for (int m = 8; m >= 1; m >>= 1) sa += __shfl_xor(sa, m, 16);
+        // lane r0 + 4u + g' takes group g''s sum (candidate r0 + 4u + g')
+        const int lo = r0 + 4 * u;
+        const float got = __shfl(sa, ((lane - lo) & 3) * 16);
+        if (lane >= lo && lane < lo + 4) my_s = got;
+      }
+    }
+    if (!kFirst && lane < R) my_s = part[i * R + lane] + my_s;
+    if (!kLast) {
+      if (lane < R) part[i * R + lane] = my_s;
+    } else {
+      int rank = 0;
+      for (int r = 0; r < R; ++r) {
+        const float tv = __shfl(my_s, r);
+        const int32_t tid = __shfl(my_id, r);
+        rank += (tv > my_s || (tv == my_s && (tid < my_id || (tid == my_id && r < lane)))) ? 1 : 0;
+      }
+      if (lane < R && rank < k) {
+        knns[i * k + (k - 1 - rank)] = my_id + 1;
+        dists[i * k + (k - 1 - rank)] = my_s;
+      }
+    }
+  }
+}
+
 // stand-alone gathered pair scores (wave per pair)
 __global__ __launch_bounds__(256) void k_score_pairs(const int8_t* __restrict__ q,
                                                      const uint64_t* __restrict__ sk, int d,
@@ -3621,6 +3709,7 @@ int knng_destroy(knng_ctx* c) {
   dfree(&c->tune_n);
   dfree(&c->ex_knns);
   dfree(&c->ex_dists);
+  dfree(&c->rr_part);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -4128,11 +4217,27 @@ int knng_export_device(knng_ctx* c, int32_t* dk, float* dd, int32_t k, int32_t r
     int R = c->p.rerank_width ? c->p.rerank_width : K;
     if (R < k) R = k;
     const int32_t* order = c->have_forest ? c->perm : nullptr;   // tree 0's leaf order
-    // every block resident at once: each XCD's rows in flight are one window of
-    // consecutive cluster-mates, whose candidate rows then stay in its L2
-    const int grid = std::min(rerank_grid(c, n), resident_grid(c, k_export_rerank));
-    hipLaunchKernelGGL(k_export_rerank, dim3(grid), dim3(256), 0, c->stream, x, c->d,
-                       n, K, R, k, c->gid[c->cur], dk, dd, int64_t(0), order);
+    // one pass per 256-dim slice; every block resident at once, so each XCD's
+    // rows in flight are one window of consecutive cluster-mates whose
+    // candidate slices stay in its L2
+    const int S = (c->d + kRrSlice - 1) / kRrSlice;
+    int rc;
+    if (S > 1 && (rc = dalloc(c, &c->rr_part, size_t(n) * K, "rerank partial sums"))) return rc;
+    for (int s = 0; s < S; ++s) {
+      auto launch = [&](auto kern) {
+        const int grid = std::min(rerank_grid(c, n), resident_grid(c, kern));
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, x, c->d, n, K, R, k, s,
+                           c->gid[c->cur], c->rr_part, dk, dd, order);
+      };
+      if (S == 1)
+        launch(k_rerank_slice<true, true>);
+      else if (s == 0)
+        launch(k_rerank_slice<true, false>);
+      else if (s < S - 1)
+        launch(k_rerank_slice<false, false>);
+      else
+        launch(k_rerank_slice<false, true>);
+    }
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));

@@ -79,6 +79,7 @@ struct knng_ctx {
   int32_t* ex_knns = nullptr;
   float* ex_dists = nullptr;
   int64_t ex_cap = 0;
+  float* rr_part = nullptr;   // sliced FP32 rerank: running sums [n][R], R <= k_build
 
   // independent builds packed into one context (knng_build_batch): segment s
   // is rows [seg_host[s], seg_host[s+1]); trees, leaves and edges never

@@ -286,9 +286,12 @@ def test_full_build_bitexact(torch_cuda, n, rule):
     assert np.array_equal(knns, e_knns) and np.array_equal(dists, e_dists)
 
 
-def test_rerank_within_fp64_tolerance(torch_cuda):
-    say("rerank")
-    x = synth.gaussian_mixture(10, 4000, 1024, 16)
+# d = 1024 / 768: four / three full 256-dim slices; 320: a 64-dim last slice;
+# 64: a single pass
+@pytest.mark.parametrize("d", [1024, 768, 320, 64])
+def test_rerank_within_fp64_tolerance(torch_cuda, d):
+    say("rerank", d)
+    x = synth.gaussian_mixture(10, 4000, d, 16)
     b = _builder(x)
     b.build()
     ids, _ = b.graph()
