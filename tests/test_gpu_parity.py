@@ -136,6 +136,24 @@ def test_forest_and_init_bitexact(torch_cuda, n, d, T, leaf, cells, parts):
     assert np.array_equal(sc, o_sc)
 
 
+def test_large_assignment_forest_bitexact(torch_cuda):
+    """70,000 rows at d = 1024 (45 target cells, 2 partitions): the full-row
+    cell assignment spans 547 row tiles, the last one partial, over 69 / 68
+    cells (the last centroid tile partial); forest bit-exact."""
+    n, d, T, leaf = 70000, 1024, 2, 512
+    say("large assignment forest", n)
+    x = synth.gaussian_mixture(77, n, d, 64)
+    q, sk = oracle.quantize(x)
+    b = _builder(x, num_trees=T, leaf_size=leaf, cells=-1, cell_partitions=2)
+    b.build_forest()
+    perm, leaves = b.forest()
+    plan = oracle.cells_device(q, T, leaf, -1, partitions=2)
+    o_perm, o_leaves = oracle.forest_cells_device(q, T, leaf, plan, 1337)
+    assert np.array_equal(perm, o_perm)
+    assert np.array_equal(leaves, o_leaves)
+    b.close()
+
+
 def test_duplicate_rows_build_bitexact(torch_cuda):
     """Many identical rows: the depth cap cuts them into leaves; whole build
     still bit-exact."""
