@@ -43,12 +43,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-PMC_PROFILE = "profiles/k_join_c3_pmc_traffic.md"
+PMC_PROFILE = "profiles/join_phased_c3_pmc_traffic.md"
 
 
 def measured_join_traffic(args):
-    """HBM bytes per k_join launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary (calibrated against k_quantize), valid for the default workload only."""
+    """HBM bytes per join iteration (k_jcand + k_jdot_merge + k_join over the rows
+    passed on) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (calibrated against k_quantize), valid for the default workload only."""
     default = (args.n == 6_350_000 and args.dim == 1024 and args.clusters == 4096 and args.trees == 4
                and args.leaf == 2048 and args.cells == -1 and args.partitions == 4
                and args.k_build == 50 and args.seed == 3)
@@ -312,9 +313,9 @@ def main():
                 if traffic is not None and jms > 0 else None,
                 "hbm_frac": round(traffic / (jms / max(it, 1) / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
                 if traffic is not None and jms > 0 else None,
-                "kernel": "k_join", "join_shape": st["join_variant"], "launches": it,
-                "launch_unit": "join iteration (its HIP events bracket all of its k_join launches: the full "
-                               "kernel, or the small-row kernel and the full one over the rows it passed on)",
+                "kernel": "join iteration: k_jcand + k_jdot_merge + k_join", "launches": it,
+                "launch_unit": "join iteration (its HIP events bracket k_jcand, k_jdot_merge and k_join over "
+                               "the rows whose candidates overflow the wave-per-row phases)",
                 "algorithmic_bytes_per_launch": round(jb / max(it, 1)),
                 "algorithmic_bytes_per_iter": [int(v) for v in st["join_bytes"][:it]],
                 "avg_launch_ms": round(jms / max(it, 1), 3)}
@@ -352,6 +353,10 @@ def main():
         "iterations": it, "update_rates": [round(r, 5) for r in st["rates"]],
         "join_passed_on": st["join_passed_on"],
         "insert_rates": [round(r, 5) for r in st["insert_rates"]],
+        # int8 ops (2 per multiply-add) of the two MFMA stages, for their POP/s: the
+        # leaf tiles (init) and the cell k-means assignments (forest)
+        "int8_ops": {"init_leaf_tiles": st["init_tile_ops"], "kmeans_assign": st["assign_ops"],
+                     "kmeans_stage_pops": round(st["assign_ops"] / max(1e-9, st["ms_kmeans"] * 1e-3) / 1e15, 3)},
         "stop_rule": "new-entry rate < 0.01 (reference rule's rate reported as update_rates)",
         "roofline": roofline, "cpu_baseline": cpu,
     }

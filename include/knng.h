@@ -102,6 +102,11 @@ typedef struct knng_stats {
   int64_t join_leafmates[32];  /* 2-hop paths i -> n1 -> c the leaf-mate rule dropped (both edges
                                   share a leaf of one tree, so c shares it with i: scored exactly at
                                   init, it cannot enter i's list) */
+  double init_tile_ops;        /* int8 ops (2 per multiply-add) of the init's leaf tiles: every
+                                  member of each leaf above 64 rows against every member, m * m * 2 *
+                                  dim per leaf (padding not counted) */
+  double assign_ops;           /* int8 ops of the cell k-means assignments, rows * cells * 2 * dim
+                                  per assignment (sample Lloyd steps, merge rounds, full rows) */
 } knng_stats;
 
 typedef struct knng_ctx knng_ctx;

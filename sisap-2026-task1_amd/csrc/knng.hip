@@ -2937,6 +2937,7 @@ int requantize(knng_ctx* c, KmeansBufs& B, const std::vector<int32_t>& cells, in
 int assign_launch(knng_ctx* c, KmeansBufs& B, const int8_t* rows, int64_t nrows, int64_t off,
                   int64_t stride, const int8_t* z, const float* w, int C, int32_t* out, bool excl,
                   float* score) {
+  c->st.assign_ops += double(nrows) * double(C) * 2.0 * double(c->d);
   if (launch_assign(c->stream, rows, nrows, c->d, z, w, C, out, c->num_cu, off, stride, excl, score,
                     B.pbest, B.pidx, B.pcap))
     return fail(c, KNNG_EHIP, std::string("k_assign launch: ") + hipGetErrorString(hipGetLastError()));
@@ -3914,6 +3915,7 @@ int knng_build_forest(knng_ctx* c) {
   c->nparts = 0;
   c->cell_count.clear();
   c->st.ms_kmeans = 0;
+  c->st.assign_ops = 0;
   const int32_t ncells = c->nseg > 0 ? 0 : cells_for(c->p, n);
   if (nnodes > 0 && ncells >= 2) {
     // k-means cells replace the root split of every tree
@@ -4000,6 +4002,7 @@ int knng_build_init(knng_ctx* c) {
   c->fresh_all = true;
   uint64_t* cand = nullptr;
   if ((rc = dalloc(c, &cand, size_t(n) * T * kLeafSlots, "init candidates"))) return rc;
+  c->st.init_tile_ops = 0;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   hipLaunchKernelGGL(k_leaf_gram, dim3(grid_for(c, c->nleaves, 1, 64)), dim3(256), 0, c->stream, c->q,
                      c->d, n, T, c->perm, c->leaves, c->nleaves, cand);
@@ -4013,6 +4016,7 @@ int knng_build_init(knng_ctx* c) {
     int mmax = 0;
     for (const int4& l : lv)
       if (l.z > kLeafSlots) {
+        c->st.init_tile_ops += double(l.z) * double(l.z) * 2.0 * double(c->d);
         mmax = std::max(mmax, int(l.z));
         for (int r0 = 0; r0 < l.z; r0 += kBigRows) items.push_back(make_int4(l.x, l.y, l.z, r0));
       }
