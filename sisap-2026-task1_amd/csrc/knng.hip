@@ -2578,131 +2578,68 @@ __global__ void k_export_plain(const int32_t* __restrict__ gid, const int32_t* _
   }
 }
 
-// Rows are visited in `order` (tree 0's leaf order; null: index order), each
-// XCD (blockIdx.x & 7; the grid is a multiple of 8 blocks) taking a contiguous
-// eighth of it, so the rows in flight on an XCD are cluster-mates whose
-// candidate rows largely coincide and stay in its L2 and the MALL.
-__global__ __launch_bounds__(256) void k_export_rerank(const float* __restrict__ x, int d,
-                                                       int64_t n, int K, int R, int k,
-                                                       const int32_t* __restrict__ gid,
-                                                       int32_t* __restrict__ knns,
-                                                       float* __restrict__ dists, int64_t row_base,
-                                                       const int32_t* __restrict__ order) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nch = d >> 2;   // float4 chunks
-  const int xcd = blockIdx.x & 7;
-  const int64_t nbx = gridDim.x >> 3, bx = blockIdx.x >> 3;
-  const int64_t jhi = n * (xcd + 1) / 8;
-  for (int64_t j = n * xcd / 8 + bx * 4 + w; j < jhi; j += nbx * 4) {
-    const int64_t i = order ? order[j] : j;
-    float4 own[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int chn = lane + 64 * c;
-      own[c] = chn < nch ? reinterpret_cast<const float4*>(x + (row_base + i) * d)[chn]
-                         : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    // lane r ends up holding candidate r's id and FP32 score; four candidate
-    // rows (4 x 4 KB) in flight per wave
-    const int32_t my_id = lane < R ? gid[(row_base + i) * K + lane] : -1;
-    float my_s = 0.f;
-    for (int r = 0; r < R; r += 4) {
-      float4 v[4][4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int32_t id = __shfl(my_id, r + u < R ? r + u : r);
-        const float4* pr = reinterpret_cast<const float4*>(x + int64_t(id) * d);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int chn = lane + 64 * c;
-          v[u][c] = chn < nch ? pr[chn] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        float sa = 0.f;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          sa = fmaf(own[c].x, v[u][c].x, sa);
-          sa = fmaf(own[c].y, v[u][c].y, sa);
-          sa = fmaf(own[c].z, v[u][c].z, sa);
-          sa = fmaf(own[c].w, v[u][c].w, sa);
-        }
-        sa = wave_sum_f(sa);
-        if (lane == r + u) my_s = sa;
-      }
-    }
-    // rank among the R by (score desc, id asc, slot asc)
-    int rank = 0;
-    for (int r = 0; r < R; ++r) {
-      const float t = __shfl(my_s, r);
-      const int32_t tid = __shfl(my_id, r);
-      rank += (t > my_s || (t == my_s && (tid < my_id || (tid == my_id && r < lane)))) ? 1 : 0;
-    }
-    if (lane < R && rank < k) {
-      knns[i * k + (k - 1 - rank)] = int32_t(my_id - row_base) + 1;
-      dists[i * k + (k - 1 - rank)] = my_s;
-    }
-  }
-}
-
 // FP32 rerank in dimension slices, one launch per slice.  A whole float32 row
 // is 4 KB, and the candidate rows of the ~900 rows in flight on one XCD (about
 // 6,000 distinct rows at 6.35M x 1024, k_build 50, width 22: a row's list
-// reaches well beyond its tree-0 leaf) overflow its 4 MB L2, so the one-pass
-// kernel above fetches 0.96 of its gathered bytes from HBM.  A 256-dim slice
-// is 1 KB; a pass's window then needs a quarter of the bytes, and the passes
-// fetch 0.89 of theirs (about 7 % less time at that size).
-// Pass s adds the FP32 dot products over dims [256 s, 256 s + 256) of the
+// reaches well beyond its tree-0 leaf) overflow its 4 MB L2, so a one-pass
+// kernel fetched 0.96 of its gathered bytes from HBM; a slice of 64 CH dims
+// needs 1/(16/CH) of that per pass, so a pass's window of candidate slices
+// can stay in the L2 while its rows are in flight.
+// Pass s adds the FP32 dot products over dims [64 CH s, 64 CH (s + 1)) of the
 // row's first R list entries to part[i * R + r]; the last pass ranks the sums
-// (((p0 + p1) + p2) + p3) by (score desc, id asc, slot asc) and writes the
-// row's best k, reversed, as k_export_rerank does.  Rows in `order`, each XCD
-// taking a contiguous eighth of it (the grid is a multiple of 8 blocks).
+// (((p0 + p1) + p2) + ...) by (score desc, id asc, slot asc) and writes the
+// row's best k as 1-based ids, best last.  Rows are visited in `order` (tree
+// 0's leaf order; null: index order), each XCD (blockIdx.x & 7; the grid is a
+// multiple of 8 blocks) taking a contiguous eighth of it, so the rows in
+// flight on an XCD are cluster-mates whose candidate rows largely coincide.
+// row_base: a packed build's segment offset (rows, list ids and x are in the
+// packed numbering; part, knns and dists in the segment's own, ids written
+// relative to the segment); 0 for a single build.
 // One wave per row in four groups of 16 lanes: group g scores candidates
-// g, g + 4, ... (two per group in flight), lane t of a group holds float4
-// chunks t, t + 16, t + 32, t + 48 of the slice.  d % 64 == 0, so a chunk
-// column is either inside the row or past its end.
-constexpr int kRrSlice = 256;
-
-template <bool kFirst, bool kLast>
+// g, g + 4, ... (U = 8 / CH per group in flight), lane t of a group holds
+// float4 chunks t, t + 16, ... (CH of them) of the slice.  d % 64 == 0, so a
+// chunk column is either inside the row or past its end.
+template <int CH, bool kFirst, bool kLast>
 __global__ __launch_bounds__(256) void k_rerank_slice(const float* __restrict__ x, int d, int64_t n, int K,
                                                       int R, int k, int s, const int32_t* __restrict__ gid,
                                                       float* __restrict__ part, int32_t* __restrict__ knns,
                                                       float* __restrict__ dists,
-                                                      const int32_t* __restrict__ order) {
+                                                      const int32_t* __restrict__ order, int64_t row_base) {
+  static_assert(CH == 1 || CH == 2 || CH == 4, "slice of 64, 128 or 256 dims");
+  constexpr int U = 8 / CH;   // candidates in flight per 16-lane group
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int g = lane >> 4, t = lane & 15;
   const int xcd = blockIdx.x & 7;
   const int64_t nbx = gridDim.x >> 3, bx = blockIdx.x >> 3;
   const int64_t jhi = n * (xcd + 1) / 8;
-  const int base = s * kRrSlice;
-  bool on[4];
+  const int base = s * 64 * CH;
+  bool on[CH];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) on[c] = base + 64 * c < d;
+  for (int c = 0; c < CH; ++c) on[c] = base + 64 * c < d;
   for (int64_t j = n * xcd / 8 + bx * 4 + w; j < jhi; j += nbx * 4) {
     const int64_t i = order ? order[j] : j;
-    const float4* po = reinterpret_cast<const float4*>(x + i * d + base);
-    float4 own[4];
+    const float4* po = reinterpret_cast<const float4*>(x + (row_base + i) * d + base);
+    float4 own[CH];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) own[c] = on[c] ? po[t + 16 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const int32_t my_id = lane < R ? gid[i * K + lane] : 0;
+    for (int c = 0; c < CH; ++c) own[c] = on[c] ? po[t + 16 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int32_t my_id = lane < R ? gid[(row_base + i) * K + lane] : int32_t(row_base);
     float my_s = 0.f;
-    for (int r0 = 0; r0 < R; r0 += 8) {
-      float4 v[2][4];
+    for (int r0 = 0; r0 < R; r0 += 4 * U) {
+      float4 v[U][CH];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < U; ++u) {
         const int r = r0 + 4 * u + g;
         const int32_t id = __shfl(my_id, r < R ? r : 0);
         const float4* pc = reinterpret_cast<const float4*>(x + int64_t(id) * d + base);
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < CH; ++c)
           v[u][c] = (on[c] && r < R) ? pc[t + 16 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < U; ++u) {
         float sa = 0.f;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < CH; ++c) {
           sa = fmaf(own[c].x, v[u][c].x, sa);
           sa = fmaf(own[c].y, v[u][c].y, sa);
           sa = fmaf(own[c].z, v[u][c].z, sa);
@@ -2727,7 +2664,7 @@ __global__ __launch_bounds__(256) void k_rerank_slice(const float* __restrict__ 
         rank += (tv > my_s || (tv == my_s && (tid < my_id || (tid == my_id && r < lane)))) ? 1 : 0;
       }
       if (lane < R && rank < k) {
-        knns[i * k + (k - 1 - rank)] = my_id + 1;
+        knns[i * k + (k - 1 - rank)] = int32_t(my_id - row_base) + 1;
         dists[i * k + (k - 1 - rank)] = my_s;
       }
     }
@@ -3535,6 +3472,64 @@ int ensure_join_variant(knng_ctx* c, int a, int b, const int32_t* order, int64_t
   return 0;
 }
 
+// float4 chunks per lane of one rerank slice (slice = 64 * chunks dims);
+// KNNG_RERANK_SLICE = 64, 128 or 256 overrides the default.
+int rerank_chunks(int d) {
+  int dims = 256;
+  if (const char* e = std::getenv("KNNG_RERANK_SLICE")) {
+    const int v = std::atoi(e);
+    if (v == 64 || v == 128 || v == 256) dims = v;
+  }
+  (void)d;
+  return dims / 64;
+}
+
+// FP32 rerank export of rows [base, base + nrows) of graph buffer `cur`: one
+// k_rerank_slice pass per slice, so single and packed builds sum the slices
+// in the same order.  Stream-ordered on c->stream.
+int launch_rerank(knng_ctx* c, int cur, int64_t nrows, int64_t base, int k, int32_t* dk, float* dd,
+                  const int32_t* order) {
+  const float* x = c->x_ext ? c->x_ext : c->x;
+  if (!x) return fail(c, KNNG_ESTATE, "knng_export: rerank needs float32 rows (keep_fp32 or attach)");
+  const int K = c->p.k_build;
+  int R = c->p.rerank_width ? c->p.rerank_width : K;
+  if (R < k) R = k;
+  // every block resident at once, so each XCD's rows in flight are one
+  // window of consecutive cluster-mates whose candidate slices stay in its L2
+  const int ch = rerank_chunks(c->d);
+  const int S = (c->d + 64 * ch - 1) / (64 * ch);
+  int rc;
+  if (S > 1 && (rc = dalloc(c, &c->rr_part, size_t(c->n) * K, "rerank partial sums"))) return rc;
+  auto run = [&](auto first, auto mid, auto last, auto only) {
+    for (int s = 0; s < S; ++s) {
+      auto launch = [&](auto kern) {
+        const int grid = std::min(rerank_grid(c, nrows), resident_grid(c, kern));
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, x, c->d, nrows, K, R, k, s,
+                           c->gid[cur], c->rr_part, dk, dd, order, base);
+      };
+      if (S == 1)
+        launch(only);
+      else if (s == 0)
+        launch(first);
+      else if (s < S - 1)
+        launch(mid);
+      else
+        launch(last);
+    }
+  };
+  if (ch == 1)
+    run(k_rerank_slice<1, true, false>, k_rerank_slice<1, false, false>, k_rerank_slice<1, false, true>,
+        k_rerank_slice<1, true, true>);
+  else if (ch == 2)
+    run(k_rerank_slice<2, true, false>, k_rerank_slice<2, false, false>, k_rerank_slice<2, false, true>,
+        k_rerank_slice<2, true, true>);
+  else
+    run(k_rerank_slice<4, true, false>, k_rerank_slice<4, false, false>, k_rerank_slice<4, false, true>,
+        k_rerank_slice<4, true, true>);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Allocation-free join iteration (knng_join_iteration_noalloc).  Its only
 // device memory beyond the caller's inputs and outputs is `scratch`: the
@@ -4216,32 +4211,8 @@ int knng_export_device(knng_ctx* c, int32_t* dk, float* dd, int32_t k, int32_t r
     hipLaunchKernelGGL(k_export_plain, dim3(grid_for(c, n * k, 256, 16)), dim3(256), 0, c->stream,
                        c->gid[c->cur], c->gsc[c->cur], n, K, k, dk, dd, int64_t(0));
   } else {
-    const float* x = c->x_ext ? c->x_ext : c->x;
-    if (!x) return fail(c, KNNG_ESTATE, "knng_export: rerank needs float32 rows (keep_fp32 or attach)");
-    int R = c->p.rerank_width ? c->p.rerank_width : K;
-    if (R < k) R = k;
     const int32_t* order = c->have_forest ? c->perm : nullptr;   // tree 0's leaf order
-    // one pass per 256-dim slice; every block resident at once, so each XCD's
-    // rows in flight are one window of consecutive cluster-mates whose
-    // candidate slices stay in its L2
-    const int S = (c->d + kRrSlice - 1) / kRrSlice;
-    int rc;
-    if (S > 1 && (rc = dalloc(c, &c->rr_part, size_t(n) * K, "rerank partial sums"))) return rc;
-    for (int s = 0; s < S; ++s) {
-      auto launch = [&](auto kern) {
-        const int grid = std::min(rerank_grid(c, n), resident_grid(c, kern));
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, x, c->d, n, K, R, k, s,
-                           c->gid[c->cur], c->rr_part, dk, dd, order);
-      };
-      if (S == 1)
-        launch(k_rerank_slice<true, true>);
-      else if (s == 0)
-        launch(k_rerank_slice<true, false>);
-      else if (s < S - 1)
-        launch(k_rerank_slice<false, false>);
-      else
-        launch(k_rerank_slice<false, true>);
-    }
+    if (int rc = launch_rerank(c, c->cur, n, 0, k, dk, dd, order)) return rc;
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -4646,12 +4617,7 @@ int export_segment(knng_ctx* c, int s, int cur, int k, int rerank, int32_t* dk, 
     hipLaunchKernelGGL(k_export_plain, dim3(grid_for(c, ns * k, 256, 16)), dim3(256), 0, c->stream,
                        c->gid[cur], c->gsc[cur], ns, K, k, dk, dd, base);
   } else {
-    const float* x = c->x_ext ? c->x_ext : c->x;
-    if (!x) return fail(c, KNNG_ESTATE, "export_segment: rerank needs float32 rows");
-    int R = c->p.rerank_width ? c->p.rerank_width : K;
-    if (R < k) R = k;
-    hipLaunchKernelGGL(k_export_rerank, dim3(rerank_grid(c, ns)), dim3(256), 0, c->stream, x, c->d,
-                       ns, K, R, k, c->gid[cur], dk, dd, base, static_cast<const int32_t*>(nullptr));
+    if (int rc = launch_rerank(c, cur, ns, base, k, dk, dd, nullptr)) return rc;
   }
   HIPCHK(c, hipGetLastError());
   return 0;

@@ -165,7 +165,7 @@ inline int grid_for(const knng_ctx* c, int64_t work, int per_block, int blocks_p
   return int(g);
 }
 
-// k_export_rerank's grid: four rows per block, a multiple of 8 blocks (the
+// k_rerank_slice's grid: four rows per block, a multiple of 8 blocks (the
 // kernel gives each XCD its own eighth of the rows)
 inline int rerank_grid(const knng_ctx* c, int64_t n) { return (grid_for(c, n, 4, 16) + 7) & ~7; }
 
