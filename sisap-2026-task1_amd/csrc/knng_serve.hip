@@ -67,6 +67,129 @@ __device__ __forceinline__ bool ahead(int32_t sa, int32_t ia, int32_t sb, int32_
   return sa != sb ? sa > sb : ia < ib;   // (score desc, id asc)
 }
 
+// Cell routing of a small query batch: each query's best cell in every
+// partition, argmax_c float(<q, z_c>) * w_c with ties to the smaller c -- the
+// same values and the same order as k_assign / k_assign_wide, so routing (and
+// every search result) does not depend on which kernel ran.  One wave per
+// block (64 threads, 16 KB of LDS, few VGPRs), so its blocks start on CUs that
+// resident bulk search blocks already occupy instead of waiting for them to
+// drain (the two-block-per-CU k_assign_wide cannot fit beside them), and all
+// partitions go in one launch.  Block (s, qg, g): queries [16 qg, 16 qg + 16)
+// (staged in LDS) against the 16-cell tiles [8 s, 8 s + 8) of partition g,
+// with v_mfma_i32_16x16x64_i8: A = 16 queries, B = 16 cells, lane l holding
+// row / cell column l % 16 and the same 16-dim quarter l / 16 of every 64-dim
+// step in both operands, so the exact int32 dot is summed over all dims; the
+// accumulator holds queries 4 (l / 16) + j of cell column l % 16.  The last
+// block of a (g, qg) to finish (an agent-scope counter, release / acquire
+// fences around it) folds the slices' partial bests into out[g * nq + q] and
+// re-arms the counter for the slot's next launch.
+constexpr int kRouteQ = 16;       // queries per block
+constexpr int kRouteTiles = 8;    // 16-cell tiles per block
+constexpr int kRouteMaxQ = 256;   // queries per k_route_small launch
+constexpr int kRouteGroups = kRouteMaxQ / kRouteQ;
+
+__global__ __launch_bounds__(64) void k_route_small(const int8_t* __restrict__ q, int nq, int d,
+                                                    const int8_t* __restrict__ Z, const float* __restrict__ W,
+                                                    int64_t ncells, const int32_t* __restrict__ cells_of,
+                                                    float* __restrict__ pbest, int32_t* __restrict__ pidx,
+                                                    int32_t* __restrict__ ctr, int32_t* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) int8_t qs[kRouteQ * kMaxDim];
+  const int lane = threadIdx.x;
+  const int s = blockIdx.x, qg = blockIdx.y, g = blockIdx.z, smax = gridDim.x;
+  const int C = cells_of[g];
+  const int tiles = (C + 15) >> 4;
+  const int S = max(1, (tiles + kRouteTiles - 1) / kRouteTiles);
+  if (s >= S) return;
+  const int q0 = qg * kRouteQ, ks = d >> 6, cpr = d >> 4;
+  for (int e = lane; e < kRouteQ * cpr; e += 64) {
+    const int r = e / cpr, c = e - r * cpr;
+    const v4i v = q0 + r < nq ? *reinterpret_cast<const v4i*>(q + int64_t(q0 + r) * d + 16 * c)
+                              : v4i{0, 0, 0, 0};
+    *reinterpret_cast<v4i*>(qs + r * d + 16 * c) = v;
+  }
+  __syncthreads();
+  const int col = lane & 15, kq = lane >> 4;
+  const int8_t* zg = Z + int64_t(g) * ncells * d;
+  const float* wg = W + int64_t(g) * ncells;
+  float best[4];
+  int32_t bidx[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    best[j] = -INFINITY;
+    bidx[j] = 0x7FFFFFFF;
+  }
+  const int t_hi = min(tiles, (s + 1) * kRouteTiles);
+  for (int t = s * kRouteTiles; t < t_hi; ++t) {
+    const int cell = 16 * t + col;
+    const bool ok = cell < C;
+    const int8_t* zr = zg + int64_t(ok ? cell : 0) * d + 16 * kq;
+    const float wc = ok ? wg[cell] : 0.f;
+    v4i acc = {0, 0, 0, 0};
+    // the cell rows in two halves of 8 steps (32 VGPRs of loads in flight),
+    // so the block stays small enough to sit beside 4 resident search waves
+#pragma unroll
+    for (int h = 0; h < kMaxDim / 512; ++h) {
+      v4i b[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        b[k] = 8 * h + k < ks ? *reinterpret_cast<const v4i*>(zr + 64 * (8 * h + k)) : v4i{0, 0, 0, 0};
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (8 * h + k < ks)
+          acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+              *reinterpret_cast<const v4i*>(qs + col * d + 64 * (8 * h + k) + 16 * kq), b[k], acc, 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float sc = ok ? float(acc[j]) * wc : -INFINITY;
+      const bool take = sc > best[j];   // cells arrive in increasing order: ties keep the smaller
+      best[j] = take ? sc : best[j];
+      bidx[j] = take ? cell : bidx[j];
+    }
+  }
+  const int64_t pbase = (int64_t(g) * smax + s) * nq;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) {   // over the 16 lanes of this quarter
+      const float ov = __shfl_xor(best[j], m);
+      const int32_t oi = __shfl_xor(bidx[j], m);
+      if (ov > best[j] || (ov == best[j] && oi < bidx[j])) {
+        best[j] = ov;
+        bidx[j] = oi;
+      }
+    }
+    const int qq = q0 + 4 * kq + j;
+    if (col == 0 && qq < nq) {
+      pbest[pbase + qq] = best[j];
+      pidx[pbase + qq] = bidx[j];
+    }
+  }
+  __threadfence();   // release this block's partials
+  int prev = 0;
+  if (lane == 0) prev = atomicAdd(&ctr[g * kRouteGroups + qg], 1);
+  prev = __shfl(prev, 0);
+  if (prev != S - 1) return;
+  __threadfence();   // acquire the other slices' partials
+  if (lane < kRouteQ && q0 + lane < nq) {
+    const int qq = q0 + lane;
+    const int64_t base = int64_t(g) * smax * nq + qq;
+    float bv = pbest[base];
+    int32_t bi = pidx[base];
+    for (int y = 1; y < S; ++y) {
+      const float v = pbest[base + int64_t(y) * nq];
+      const int32_t vi = pidx[base + int64_t(y) * nq];
+      if (v > bv || (v == bv && vi < bi)) {
+        bv = v;
+        bi = vi;
+      }
+    }
+    out[int64_t(g) * nq + qq] = bi;
+  }
+  if (lane == 0) ctr[g * kRouteGroups + qg] = 0;   // re-armed for the slot's next launch
+}
+
 __global__ __launch_bounds__(256) void k_search(
     const int8_t* __restrict__ q, const float* __restrict__ x, int d,
     const int32_t* __restrict__ gid, int K, const Node* __restrict__ hier, int64_t hier_count,
@@ -387,6 +510,9 @@ struct Slot {
   float* pbest = nullptr;     // split cell-routing partials
   int32_t* pidx = nullptr;
   int64_t pcap = 0;
+  float* rbest = nullptr;     // k_route_small partials [nparts][route_smax][kRouteMaxQ]
+  int32_t* ridx = nullptr;
+  int32_t* rctr = nullptr;    // its last-block counters [nparts][kRouteGroups], zero between launches
   float* hx = nullptr;        // pinned staging
   int32_t* hid = nullptr;
   float* hsc = nullptr;
@@ -441,6 +567,8 @@ struct knng_server {
   int64_t inserts_submitted = 0, inserts_done = 0;
   bool cu_masked = false;         // the express / bulk CU split is in force
   int bulk_blocks = 0;            // search blocks per bulk launch
+  int32_t* route_cells = nullptr; // cells of each partition (device copy of ctx->cell_count)
+  int route_smax = 0;             // k_route_small slices per partition (16-cell tiles / 8)
   std::string mask_err;           // why the runtime refused the CU masks, if it did
   // concurrent inserts: queue, thread and preallocated scratch for ins_cap rows
   std::deque<Request*> ins_queue;
@@ -481,7 +609,8 @@ void free_slot(Slot* sl) {
   for (void* p : {static_cast<void*>(sl->dx), static_cast<void*>(sl->dq),
                   static_cast<void*>(sl->dsk), static_cast<void*>(sl->did),
                   static_cast<void*>(sl->dsc), static_cast<void*>(sl->dcell),
-                  static_cast<void*>(sl->pbest), static_cast<void*>(sl->pidx)})
+                  static_cast<void*>(sl->pbest), static_cast<void*>(sl->pidx),
+                  static_cast<void*>(sl->rbest), static_cast<void*>(sl->ridx), static_cast<void*>(sl->rctr)})
     if (p) (void)hipFree(p);
   for (void* p : {static_cast<void*>(sl->hx), static_cast<void*>(sl->hid), static_cast<void*>(sl->hsc)})
     if (p) (void)hipHostFree(p);
@@ -517,6 +646,12 @@ hipError_t make_stream(knng_server* s, bool express, hipStream_t* st) {
   return hipStreamCreateWithFlags(st, hipStreamNonBlocking);
 }
 
+// k_route_small buffers of a slot: partials and counters for every partition
+size_t route_entries(const knng_server* s) {
+  return size_t(std::max(1, s->ctx->nparts)) * size_t(std::max(1, s->route_smax)) * kRouteMaxQ;
+}
+size_t route_counters(const knng_server* s) { return size_t(std::max(1, s->ctx->nparts)) * kRouteGroups; }
+
 Slot* new_slot(knng_server* s, bool express) {
   auto* sl = new Slot();
   sl->express = express;
@@ -533,6 +668,10 @@ Slot* new_slot(knng_server* s, bool express) {
                                       sizeof(float)) == hipSuccess &&
             hipMalloc(&sl->pidx, size_t(knng_host::assign_part_entries(int64_t(B), s->num_cu)) *
                                      sizeof(int32_t)) == hipSuccess &&
+            hipMalloc(&sl->rbest, route_entries(s) * sizeof(float)) == hipSuccess &&
+            hipMalloc(&sl->ridx, route_entries(s) * sizeof(int32_t)) == hipSuccess &&
+            hipMalloc(&sl->rctr, route_counters(s) * sizeof(int32_t)) == hipSuccess &&
+            hipMemset(sl->rctr, 0, route_counters(s) * sizeof(int32_t)) == hipSuccess &&
             hipHostMalloc(&sl->hx, B * s->d * sizeof(float), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc(&sl->hid, B * s->sp.k * sizeof(int32_t), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc(&sl->hsc, B * s->sp.k * sizeof(float), hipHostMallocDefault) == hipSuccess;
@@ -555,16 +694,27 @@ Slot* new_slot(knng_server* s, bool express) {
 // slots that bulk launches leave free.
 int launch_search(knng_server* s, hipStream_t stream, const float* dx, int8_t* dq, uint64_t* dsk,
                   int32_t* dcell, float* pbest, int32_t* pidx, int64_t pcap, int32_t nq, int beam,
-                  int k, int rerank, int32_t* did, float* dsc, int max_blocks, int route_cu) {
+                  int k, int rerank, int32_t* did, float* dsc, int max_blocks, int route_cu,
+                  const Slot* sl) {
   knng_ctx* c = s->ctx;
   if (knng_host::launch_quantize(stream, dx, nq, s->d, dq, dsk, s->num_cu))
     return sfail(s, KNNG_EHIP, "quantise launch failed");
-  for (int g = 0; g < c->nparts; ++g)   // route the queries to cells
-    if (knng_host::launch_assign(stream, dq, nq, s->d, c->cz + size_t(g) * c->ncells * s->d,
-                                 c->cw + size_t(g) * c->ncells, c->cell_count[size_t(g)],
-                                 dcell + size_t(g) * nq, route_cu, 0, 1, false, nullptr,
-                                 pbest, pidx, pcap))
-      return sfail(s, KNNG_EHIP, "cell routing launch failed");
+  // route the queries to cells: small batches (a slot's launch of at most
+  // kRouteMaxQ queries) in one k_route_small launch over all partitions, the
+  // rest through k_assign per partition
+  if (c->nparts > 0 && sl && nq <= kRouteMaxQ) {
+    hipLaunchKernelGGL(k_route_small, dim3(s->route_smax, (nq + kRouteQ - 1) / kRouteQ, c->nparts), dim3(64),
+                       0, stream, dq, nq, s->d, c->cz, c->cw, int64_t(c->ncells), s->route_cells, sl->rbest,
+                       sl->ridx, sl->rctr, dcell);
+    SCHK(s, hipGetLastError());
+  } else {
+    for (int g = 0; g < c->nparts; ++g)
+      if (knng_host::launch_assign(stream, dq, nq, s->d, c->cz + size_t(g) * c->ncells * s->d,
+                                   c->cw + size_t(g) * c->ncells, c->cell_count[size_t(g)],
+                                   dcell + size_t(g) * nq, route_cu, 0, 1, false, nullptr,
+                                   pbest, pidx, pcap))
+        return sfail(s, KNNG_EHIP, "cell routing launch failed");
+  }
   const int grid = std::max(1, std::min<int>(nq, max_blocks));
   // at most 4 trees: their leaves (<= 64 rows each) fill the candidate buffer
   const int et = std::min(std::min<int>(s->sp.entry_trees, c->p.num_trees), kCandCap / 64);
@@ -594,7 +744,8 @@ int do_insert_rows(knng_server* s, const float* rows, int32_t m, int64_t* first_
   int rc = launch_search(s, st, s->x + base * s->d, s->q + base * s->d, s->sk + base * s->words,
                          s->ins_dcell, s->ins_pbest, s->ins_pidx,
                          knng_host::assign_part_entries(s->ins_cap, s->num_cu), m,
-                         std::min(beam, kBeamMax), K, 0, s->ins_did, s->ins_dsc, s->num_cu * 8, s->num_cu);
+                         std::min(beam, kBeamMax), K, 0, s->ins_did, s->ins_dsc, s->num_cu * 8, s->num_cu,
+                         nullptr);
   if (rc) return rc;
   std::vector<int32_t> ids(size_t(m) * K);
   std::vector<float> sc(size_t(m) * K);
@@ -769,7 +920,7 @@ void launch_parts(knng_server* s, Slot* sl, std::unique_lock<std::mutex>& lk) {
   const int route_cu = sl->express ? kExpressRouteCUs : s->num_cu;
   ok = ok && !launch_search(s, sl->stream, sl->dx, sl->dq, sl->dsk, sl->dcell, sl->pbest, sl->pidx,
                             sl->pcap, m, s->sp.beam, s->sp.k, s->sp.rerank, sl->did, sl->dsc, blocks,
-                            route_cu);
+                            route_cu, sl);
   ok = ok && hipMemcpyAsync(sl->hid, sl->did, size_t(m) * s->sp.k * sizeof(int32_t),
                             hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
   ok = ok && hipMemcpyAsync(sl->hsc, sl->dsc, size_t(m) * s->sp.k * sizeof(float),
@@ -1004,6 +1155,14 @@ int knng_server_create(knng_server** out, knng_ctx* c, const knng_server_params*
         hipMalloc(&s->ins_pf, mk * 4) != hipSuccess)
       return bail(KNNG_ENOMEM);
   }
+  // per-partition cell counts for k_route_small, and its slices per partition
+  if (c->nparts > 0) {
+    if (hipMalloc(&s->route_cells, size_t(c->nparts) * sizeof(int32_t)) != hipSuccess ||
+        hipMemcpy(s->route_cells, c->cell_count.data(), size_t(c->nparts) * sizeof(int32_t),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return bail(KNNG_ENOMEM);
+    s->route_smax = std::max(1, ((c->ncells + 15) / 16 + kRouteTiles - 1) / kRouteTiles);
+  }
   // express slots up front; one bulk slot (more as requests and HBM allow)
   for (int j = 0; j < (s->sp.express_rows > 0 ? s->sp.express_slots : 0); ++j) {
     Slot* sl = new_slot(s, true);
@@ -1044,7 +1203,8 @@ int knng_server_destroy(knng_server* s) {
                   static_cast<void*>(s->ins_dcell), static_cast<void*>(s->ins_pbest),
                   static_cast<void*>(s->ins_pidx), static_cast<void*>(s->ins_t),
                   static_cast<void*>(s->ins_off), static_cast<void*>(s->ins_pi),
-                  static_cast<void*>(s->ins_ps), static_cast<void*>(s->ins_pf)})
+                  static_cast<void*>(s->ins_ps), static_cast<void*>(s->ins_pf),
+                  static_cast<void*>(s->route_cells)})
     if (p) (void)hipFree(p);
   if (s->ins_stream) (void)hipStreamDestroy(s->ins_stream);
   delete s;

@@ -304,11 +304,13 @@ def test_full_build_bitexact(torch_cuda, n, rule):
     assert np.array_equal(knns, e_knns) and np.array_equal(dists, e_dists)
 
 
-# d = 1024 / 768: four / three full 256-dim slices; 320: a 64-dim last slice;
-# 64: a single pass
-@pytest.mark.parametrize("d", [1024, 768, 320, 64])
-def test_rerank_within_fp64_tolerance(torch_cuda, d):
-    say("rerank", d)
+# slices of 64 dims (the default: 16 / 12 / 5 / 1 passes at d = 1024 / 768 /
+# 320 / 64), 128 and 256 dims (320: a partial last slice; 64: a single pass)
+@pytest.mark.parametrize("d,slice_dims", [(1024, 64), (768, 64), (320, 64), (64, 64), (1024, 128),
+                                          (320, 128), (1024, 256), (768, 256), (320, 256)])
+def test_rerank_within_fp64_tolerance(torch_cuda, monkeypatch, d, slice_dims):
+    say("rerank", d, slice_dims)
+    monkeypatch.setenv("KNNG_RERANK_SLICE", str(slice_dims))
     x = synth.gaussian_mixture(10, 4000, d, 16)
     b = _builder(x)
     b.build()

@@ -46,6 +46,36 @@ def test_search_int8_bitexact_vs_oracle(torch_cuda, T, leaf):
     assert np.array_equal(g_sc, o_sc.astype(np.float32))
 
 
+@pytest.mark.parametrize("d", [1024, 320])
+def test_search_many_cells_routing_bitexact(torch_cuda, d):
+    """Queries routed through hundreds of cells per partition (several
+    8-tile slices of the small-batch routing kernel, partitions of different
+    sizes): launches of at most 256 queries (the one-launch routing kernel)
+    and one launch of 600 queries (the per-partition assignment kernel) both
+    return the oracle's search results over the builder's own cell plan."""
+    say("search many cells", d)
+    n = 20000
+    x = synth.gaussian_mixture(61, n, d, 64)
+    b = _built(x, num_trees=2, leaf_size=256, cells=320, cell_partitions=2, k_build=20, k_eval=15)
+    ids, _ = b.graph()
+    q, _ = oracle.quantize(x)
+    queries = synth.gaussian_mixture(62, 600, d, 64)
+    qq, _ = oracle.quantize(queries)
+    parts = [b.cells(g) for g in range(2)]
+    counts = np.array([len(p[1]) for p in parts], np.int32)
+    assert counts.min() > 128, counts   # more than 8 tiles of 16 cells: several slices
+    plan = (counts, 2, [p[0] for p in parts], [p[1] for p in parts], np.stack([p[2] for p in parts]))
+    o_ids, o_sc = oracle.search_device(q, ids, qq, T=2, leaf=256, entry_trees=4, beam=32, k=10,
+                                       max_steps=128, plan=plan)
+    for max_batch in (600, 100, 7):
+        with knng.Server(b, beam=32, k=10, rerank=0, entry_trees=4, max_batch=max_batch, max_slots=2,
+                         express_rows=0) as s:
+            g_ids, g_sc = s.search(queries)
+        assert np.array_equal(g_ids, o_ids), max_batch
+        assert np.array_equal(g_sc, o_sc.astype(np.float32)), max_batch
+    b.close()
+
+
 def test_search_recall_rerank(torch_cuda):
     say("search recall")
     n = 20000
