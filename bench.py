@@ -68,7 +68,7 @@ def build_params(knng, args, d):
                                cell_partitions=args.partitions, rerank_width=args.rerank_width,
                                incremental=args.incremental, cell_oversample=args.oversample,
                                max_iters=args.max_iters, kmeans_iters=args.kmeans_iters,
-                               cell_sample=args.cell_sample)
+                               cell_sample=args.cell_sample, reverse_width=args.reverse_width)
 
 
 def exact_topk(torch, x, rows, k, chunk=262144):
@@ -184,6 +184,8 @@ def main():
                     help="1: NN-Descent's new/old rule in the join (a 2-hop pair needs a fresh edge)")
     ap.add_argument("--rerank-width", type=int, default=22,
                     help="list entries the FP32 rerank re-scores (0: all k_build)")
+    ap.add_argument("--reverse-width", type=int, default=0,
+                    help="reverse-list entries per row in the join (0: pull-only 2-hop join)")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--recall-rows", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=200000,
@@ -337,7 +339,8 @@ def main():
                    "rerank_width": args.rerank_width or args.k_build, "incremental": args.incremental,
                    "cell_oversample": args.oversample, "max_iters": args.max_iters,
                    "kmeans_iters": args.kmeans_iters, "cell_sample": args.cell_sample,
-                   "cell_partitions": args.partitions, "parallelism": "single-gpu"},
+                   "cell_partitions": args.partitions, "reverse_width": args.reverse_width,
+                   "parallelism": "single-gpu"},
         "recall_at_15": {"plain": round(rec_plain, 4), "rerank": round(rec_rerank, 4),
                          "rows": int(len(rows_np)), "rerank_rel_err_max": float(rel.max())},
         "plain_s": round(stage["ingest_s"] + stage["build_s"] + stage["export_plain_s"], 4),

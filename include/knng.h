@@ -61,7 +61,13 @@ typedef struct knng_params {
   int32_t cell_full_iters; /* Lloyd iterations over all rows after merging */
   float cell_prune;        /* after merging, drop cells holding fewer sample rows than
                               cell_prune * the median cell (their rows go to the others) */
-  int32_t pad0;
+  int32_t reverse_width;   /* R (0..8, k_build <= 50): NN-Descent's reverse candidates in the
+                              join.  Row v's reverse list holds the R best rows (score desc, id
+                              asc) whose lists hold v as an entry the iteration treats as new;
+                              row i also takes its reverse entries, its neighbours' reverse
+                              entries and the lists of its reverse entries.  Single builds only
+                              (knng_build_batch needs 0).  0 (default): pull-only 2-hop join as
+                              src/main.cpp:335-419 */
 } knng_params;
 
 #define KNNG_STOP_REFERENCE 0  /* slots whose id changed, shifts included (src/main.cpp:421-427) */

@@ -37,7 +37,7 @@
 //                        centroid equals the requantised sum of its cell
 //   orc_init_device      brute-force restatement in numpy (tests)
 //   orc_join_device      brute-force restatement in numpy (tests), with and
-//                        without the leaf-mate rule
+//                        without the leaf-mate rule and the reverse lists
 //   orc_rerank_device    FP64 numpy dot products
 //   orc_export           reference golden outputs (through reference mode)
 
@@ -829,11 +829,19 @@ int32_t orc_sketch_threshold(const int32_t* h, int64_t cnt, int32_t M, int32_t b
 // masks of i -> n1 and n1 -> c share a bit (then c shares that leaf with i:
 // the init scored it exactly, so it cannot enter i's list).  c is a candidate
 // when some path to it survives.
+// R > 0 (reverse_width): reverse-neighbour candidates.  RV(v) = the R best
+// rows u by (score desc, u asc) whose lists hold v as an entry this iteration
+// treats as new (every entry without fresh_in, else the fresh ones).  Row i
+// also takes u in RV(i) (path i <- u), w in RV(n1) for its neighbours n1
+// (path i -> n1 <- w) and the entries w of u's list for u in RV(i) (path
+// i <- u -> w); every reverse edge is new, and each path is dropped under the
+// leaf-mate rule when its two edges' masks share a bit (for i <- u: when the
+// edge's mask is not 0).
 int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t d, int32_t K,
                         int32_t M, const int32_t* ids_in, const int32_t* sc_in,
                         int32_t* ids_out, int32_t* sc_out, int64_t* inserted,
                         const uint64_t* fresh_in, uint64_t* fresh_out, const int32_t* leafkey,
-                        int32_t T) {
+                        int32_t T, int32_t R) {
   int64_t ins = 0;
   const int words = d / 64;
   int64_t updates = 0;
@@ -841,6 +849,28 @@ int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t 
   std::vector<int32_t> cand, hval;
   struct Slot { Edge e; int32_t slot; };
   std::vector<Slot> all;
+  const int tm = T < 8 ? T : 8;
+  auto edge_mask = [&](int64_t r, int64_t e) {
+    uint32_t m = 0;
+    for (int t = 0; t < tm; ++t) m |= uint32_t(leafkey[r * T + t] == leafkey[e * T + t]) << t;
+    return m;
+  };
+  // reverse lists: every new edge u -> v offered to v, the R best kept
+  std::vector<int32_t> rv(size_t(R > 0 ? n * R : 0), -1);
+  if (R > 0) {
+    std::vector<std::vector<Edge>> in(static_cast<size_t>(n));
+    for (int64_t u = 0; u < n; ++u)
+      for (int k = 0; k < K; ++k)
+        if (!fresh_in || ((fresh_in[u] >> k) & 1ull)) {
+          const int32_t v = ids_in[u * K + k];
+          if (v >= 0 && v < n) in[size_t(v)].push_back(Edge{int32_t(u), sc_in[u * K + k]});
+        }
+    for (int64_t v = 0; v < n; ++v) {
+      std::vector<Edge>& e = in[size_t(v)];
+      std::sort(e.begin(), e.end(), edge_before);
+      for (size_t j = 0; j < e.size() && j < size_t(R); ++j) rv[size_t(v * R) + j] = e[j].id;
+    }
+  }
   for (int64_t i = 0; i < n; ++i) {
     const int32_t* own = ids_in + i * K;
     seen.clear();
@@ -848,12 +878,6 @@ int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t 
     hval.clear();
     seen.insert(int32_t(i));
     for (int k = 0; k < K; ++k) seen.insert(own[k]);
-    const int tm = T < 8 ? T : 8;
-    auto edge_mask = [&](int64_t r, int64_t e) {
-      uint32_t m = 0;
-      for (int t = 0; t < tm; ++t) m |= uint32_t(leafkey[r * T + t] == leafkey[e * T + t]) << t;
-      return m;
-    };
     for (int k = 0; k < K; ++k) {
       const int32_t* nb = ids_in + int64_t(own[k]) * K;
       const bool own_fresh = !fresh_in || ((fresh_in[i] >> k) & 1ull);
@@ -862,6 +886,33 @@ int64_t orc_join_device(const int8_t* q, const uint64_t* sk, int64_t n, int32_t 
         if (!own_fresh && !((fresh_in[own[k]] >> k2) & 1ull)) continue;
         if (leafkey && (m1 & edge_mask(own[k], nb[k2]))) continue;   // leaf-mate path
         if (seen.insert(nb[k2]).second) cand.push_back(nb[k2]);
+      }
+    }
+    if (R > 0) {
+      const int32_t* ri = rv.data() + i * R;
+      for (int j = 0; j < R; ++j) {   // i <- u
+        const int32_t u = ri[j];
+        if (u < 0 || (leafkey && edge_mask(u, i))) continue;
+        if (seen.insert(u).second) cand.push_back(u);
+      }
+      for (int k = 0; k < K; ++k) {   // i -> n1 <- w
+        const int32_t n1 = own[k];
+        const uint32_t m1 = leafkey ? edge_mask(i, n1) : 0u;
+        for (int j = 0; j < R; ++j) {
+          const int32_t w = rv[size_t(int64_t(n1) * R + j)];
+          if (w < 0 || (leafkey && (m1 & edge_mask(w, n1)))) continue;
+          if (seen.insert(w).second) cand.push_back(w);
+        }
+      }
+      for (int j = 0; j < R; ++j) {   // i <- u -> w
+        const int32_t u = ri[j];
+        if (u < 0) continue;
+        const uint32_t mu = leafkey ? edge_mask(u, i) : 0u;
+        for (int k = 0; k < K; ++k) {
+          const int32_t w = ids_in[int64_t(u) * K + k];
+          if (leafkey && (mu & edge_mask(u, w))) continue;
+          if (seen.insert(w).second) cand.push_back(w);
+        }
       }
     }
     const uint64_t* si = sk + i * words;

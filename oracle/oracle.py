@@ -53,7 +53,7 @@ def lib() -> C.CDLL:
             "orc_forest_device": ([V, i64, i32, i32, i32, u64, V, V, i64], i64),
             "orc_init_device": ([V, i64, i32, i32, i32, V, V, i64, V, V], i32),
             "orc_sketch_threshold": ([V, i64, i32, i32], i32),
-            "orc_join_device": ([V, V, i64, i32, i32, i32, V, V, V, V, C.POINTER(i64), V, V, V, i32], i64),
+            "orc_join_device": ([V, V, i64, i32, i32, i32, V, V, V, V, C.POINTER(i64), V, V, V, i32, i32], i64),
             "orc_rerank_device": ([V, i64, i32, i32, i32, i32, V, V, V], i32),
             "orc_search_device": ([V, i64, i32, V, i32, i32, i32, u64, i32, i32, i32, i32, V, i64,
                                    V, V, i32, V, V, V], i32),
@@ -233,10 +233,12 @@ def leaf_keys(perm: np.ndarray, leaves: np.ndarray) -> np.ndarray:
     return key
 
 
-def join_device(q, sk, ids, sc, M: int, with_inserted: bool = False, fresh=None, leafkey=None):
+def join_device(q, sk, ids, sc, M: int, with_inserted: bool = False, fresh=None, leafkey=None,
+                reverse: int = 0):
     """One Jacobi join iteration -> (ids, scores, updates[, inserted][, fresh_out]).
     fresh (u64 per row) selects the incremental join and adds fresh_out;
-    leafkey ([n][T], from leaf_keys) applies the leaf-mate rule."""
+    leafkey ([n][T], from leaf_keys) applies the leaf-mate rule; reverse = R > 0
+    adds the reverse-neighbour candidates of R-entry reverse lists."""
     n, d = q.shape
     K = ids.shape[1]
     oi = np.empty_like(ids)
@@ -248,7 +250,7 @@ def join_device(q, sk, ids, sc, M: int, with_inserted: bool = False, fresh=None,
     upd = lib().orc_join_device(_p(q), _p(sk), n, d, K, M, _p(np.ascontiguousarray(ids)),
                                 _p(np.ascontiguousarray(sc)), _p(oi), _p(os_), C.byref(ins),
                                 fi, _p(fo), None if lk is None else _p(lk),
-                                0 if lk is None else lk.shape[1])
+                                0 if lk is None else lk.shape[1], int(reverse))
     assert upd >= 0
     out = [oi, os_, int(upd)]
     if with_inserted:
@@ -276,12 +278,13 @@ def build_device_semantics(q, sk, trees=16, leaf=64, K=50, mult=8, max_iters=30,
                            stop=0.04, tol=0.01, seed=1337, stop_rule=STOP_INSERTED,
                            insert_stop=0.01, incremental=False, cells=-1, kmeans_iters=2,
                            partitions=2, over=4, sample=8, merge=0.6, full_iters=0,
-                           prune=0.25):
+                           prune=0.25, reverse=0):
     """Full device-mode build.  Stop rules: STOP_REFERENCE is src/main.cpp:432-439
     on the rate of slots whose id changed; STOP_INSERTED stops once fewer than
     insert_stop of all slots received a new id; STOP_NONE runs max_iters.
     cells / kmeans_iters / partitions select the cell seeding of the forest
-    with the library's rule (cells=-1: n // 1536 when >= 2, 0: none)."""
+    with the library's rule (cells=-1: n // 1536 when >= 2, 0: none); reverse
+    is the reverse-list width of the join iterations (0: none)."""
     plan = cells_device(q, trees, leaf, cells, kmeans_iters, partitions, over, sample, merge,
                         full_iters, prune)
     perm, leaves = forest_cells_device(q, trees, leaf, plan, seed)
@@ -294,9 +297,10 @@ def build_device_semantics(q, sk, trees=16, leaf=64, K=50, mult=8, max_iters=30,
     for it in range(max_iters):
         if incremental:
             ids, sc, upd, ins, fresh = join_device(q, sk, ids, sc, K * mult, with_inserted=True,
-                                                   fresh=fresh, leafkey=lk)
+                                                   fresh=fresh, leafkey=lk, reverse=reverse)
         else:
-            ids, sc, upd, ins = join_device(q, sk, ids, sc, K * mult, with_inserted=True, leafkey=lk)
+            ids, sc, upd, ins = join_device(q, sk, ids, sc, K * mult, with_inserted=True, leafkey=lk,
+                                            reverse=reverse)
         rate = np.float32(upd) / np.float32(n * K)
         rates.append(float(rate))
         if stop_rule == STOP_INSERTED:

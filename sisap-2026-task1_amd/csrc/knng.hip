@@ -1686,6 +1686,8 @@ __global__ __launch_bounds__(256) void k_init_merge(const int8_t* __restrict__ q
 //  7. write the new list, count changed slots (src/main.cpp:421-427)
 // ---------------------------------------------------------------------------
 
+constexpr int kRevMax = 8;   // reverse_width limit
+
 struct JoinScratch {
   uint64_t own_key[kMaxK];    // pack_key(own score, own id)
   int32_t own_id[kMaxK];
@@ -1700,6 +1702,8 @@ struct JoinScratch {
   uint8_t new_mask[kMaxK];
   uint64_t own_sk[kMaxDim / 64];
   int32_t own_leaf[kMaxTrees];   // the row's leaf (first perm position) in every tree
+  int32_t rv_id[kRevMax];     // the row's reverse entries (-1: none) and their mask bytes
+  int32_t rv_mk[kRevMax];
   int32_t hist[kMaxDim + 1];
   int32_t cnt[6];   // 0 unique candidates, 1 selected, 2 survivors, 3 threshold, 4 new ids,
                     // 5 candidates left after dropping leaf-mates
@@ -1719,7 +1723,8 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     const int64_t* __restrict__ nwork_dev, const int64_t* __restrict__ seg_base, int nseg,
     unsigned long long* __restrict__ segc, int32_t* __restrict__ ovf,
     unsigned long long* __restrict__ ovf_n, const uint8_t* __restrict__ masks_in,
-    uint8_t* __restrict__ masks_out, const int32_t* __restrict__ leafkey, int T, int TP) {
+    uint8_t* __restrict__ masks_out, const int32_t* __restrict__ leafkey, int T, int TP,
+    const int32_t* __restrict__ rv_id, const uint8_t* __restrict__ rv_mask, int R) {
   // order[0 .. nwork) lists the rows to process (nwork = *nwork_dev if given,
   // else n).  With segments (independent builds packed together, nseg > 0),
   // each row's changed-slot and new-id counts are also added to its build's
@@ -1745,6 +1750,8 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   // K-th entry (K <= 64), and lists only improve: it could never enter.  New
   // entries get their mask byte from the leaf keys (leafkey[r * TP + t] = the
   // first perm position of row r's leaf in tree t).
+  // R > 0: reverse-neighbour candidates from the R-entry reverse lists rv_id
+  // (k_rev_final), as k_jcand takes them (CAP must hold K * K + R + 2 K R).
   constexpr int HS = 1 << HSBITS;
   static_assert(HS * 4 >= CAP * 2 + CAP * 4, "selection arrays must fit in the hash area");
   __shared__ __attribute__((aligned(16))) int32_t hs[HS];
@@ -1807,6 +1814,10 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     if (tid < 6) S.cnt[tid] = 0;
     if (masks_out && tid < T && tid < kMaskTrees) S.own_leaf[tid] = leafkey[i * TP + tid];
     if (tid < K) S.own_mask[tid] = masks_in ? masks_in[i * K + tid] : uint8_t(0);
+    if (tid < R) {
+      S.rv_id[tid] = rv_id[i * R + tid];
+      S.rv_mk[tid] = rv_mask ? int32_t(rv_mask[i * R + tid]) : 0;
+    }
     for (int c = tid; c < nch; c += 256)
       *reinterpret_cast<v4i*>(own_row + 16 * c) = *reinterpret_cast<const v4i*>(q + i * d + 16 * c);
     if (tid < words) S.own_sk[tid] = sk[i * words + tid];
@@ -1855,6 +1866,42 @@ __global__ __launch_bounds__(256, OCC) void k_join(
         const int slot = wave_append(fresh_id, &S.cnt[0]);
         if (fresh_id) cid[slot] = c[r];
       }
+    }
+    // ---- 2b: reverse-neighbour candidates: the row's reverse entries u (path
+    //          i <- u), its neighbours' reverse entries w (i -> n1 <- w) and the
+    //          list entries w of its reverse entries (i <- u -> w); every reverse
+    //          edge is new, and a path is dropped when its edges' mask bytes meet
+    //          (i <- u: when the edge's own byte is not 0)
+    if (R > 0) {
+      const int nrev = R + 2 * K * R;
+      int dropped = 0;
+      for (int e0 = 0; e0 < nrev; e0 += 256) {
+        const int e = e0 + tid;
+        int32_t id = -1;
+        if (e < R) {
+          id = S.rv_id[e];
+          if (id >= 0 && S.rv_mk[e]) id = -2;
+        } else if (e < R + K * R) {
+          const int e2 = e - R, k = e2 / R;
+          const int64_t at = int64_t(S.own_id[k]) * R + (e2 - k * R);
+          id = rv_id[at];
+          if (id >= 0 && rv_mask && (int32_t(rv_mask[at]) & int32_t(S.own_mask[k]))) id = -2;
+        } else if (e < nrev) {
+          const int e3 = e - R - K * R, jr = e3 / K;
+          const int32_t u = S.rv_id[jr];
+          if (u >= 0) {
+            const int64_t at = int64_t(u) * K + (e3 - jr * K);
+            id = ids_in[at];
+            if (masks_in && (int32_t(masks_in[at]) & S.rv_mk[jr])) id = -2;
+          }
+        }
+        dropped += __popcll(__ballot(id == -2));
+        if (SMALL && S.cnt[0] > lim) id = -1;
+        const bool fresh_id = id >= 0 && hs_insert(hs, HSBITS, id);
+        const int slot = wave_append(fresh_id, &S.cnt[0]);
+        if (fresh_id) cid[slot] = id;
+      }
+      if (masks_in && lane == 0 && dropped) atomicAdd(&S.cnt[5], dropped);
     }
     __syncthreads();
     int U = S.cnt[0];
@@ -2124,6 +2171,81 @@ __global__ __launch_bounds__(256, OCC) void k_join(
 }
 
 // ---------------------------------------------------------------------------
+// Reverse-neighbour lists (NN-Descent's reverse candidates).  Row v's reverse
+// list holds the R best rows u, by (score desc, u asc), whose lists hold v as
+// an entry the coming iteration treats as new (every entry in an iteration
+// without the new/old rule, else the entries that came in during the previous
+// one).  k_rev_insert offers every such edge u -> v to v's R slots of packed
+// 64-bit (score, id) keys with a lock-free top-R insertion: find the slot
+// holding the smallest key; stop if the offered key is not larger; else swap
+// it in with a 64-bit compare-and-swap, and start over if another insertion
+// changed that slot first.  A key leaves the slots only for a larger one, so
+// once every offer has finished the slots hold exactly the R largest keys
+// offered, whatever the order of the offers (keys of one v are distinct: a
+// list holds v at most once).  Slots are read with agent-scope atomic loads,
+// so an offer never works from a stale copy in another XCD's L2.
+// k_rev_final sorts each row's slots, writes the ids (-1 past the last) and
+// the mask byte of each edge u -> v (trees whose leaf u and v share, from the
+// leaf keys; 0 without them), and zeroes the slots for the next iteration.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_rev_insert(const int32_t* __restrict__ gid,
+                                                    const int32_t* __restrict__ gsc,
+                                                    const uint64_t* __restrict__ fresh, int64_t n, int K,
+                                                    int R, unsigned long long* __restrict__ slots) {
+  const int64_t total = n * K;
+  for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+    const int64_t u = e / K;
+    const int k = int(e - u * K);
+    if (fresh && !((fresh[u] >> k) & 1ull)) continue;
+    const int32_t v = gid[e];
+    if (v < 0 || v >= n) continue;
+    const unsigned long long key = pack_key(gsc[e], int32_t(u));
+    unsigned long long* sl = slots + int64_t(v) * R;
+    while (true) {
+      unsigned long long mn = ~0ull;
+      int at = 0;
+      for (int j = 0; j < R; ++j) {
+        const unsigned long long x = __hip_atomic_load(sl + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (x < mn) {
+          mn = x;
+          at = j;
+        }
+      }
+      if (key <= mn) break;                                  // R larger keys are already in
+      if (atomicCAS(sl + at, mn, key) == mn) break;          // swapped in
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rev_final(unsigned long long* __restrict__ slots, int64_t n, int R,
+                                                   const int32_t* __restrict__ leafkey, int T, int TP,
+                                                   int32_t* __restrict__ rv_id, uint8_t* __restrict__ rv_mask) {
+  for (int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x; v < n; v += int64_t(gridDim.x) * 256) {
+    unsigned long long key[kRevMax];
+#pragma unroll
+    for (int j = 0; j < kRevMax; ++j) key[j] = j < R ? slots[v * R + j] : 0ull;
+    // descending insertion sort of at most kRevMax keys
+#pragma unroll
+    for (int j = 1; j < kRevMax; ++j)
+#pragma unroll
+      for (int t = j; t > 0; --t)
+        if (key[t] > key[t - 1]) {
+          const unsigned long long x = key[t];
+          key[t] = key[t - 1];
+          key[t - 1] = x;
+        }
+    for (int j = 0; j < R; ++j) {
+      const int32_t u = key[j] ? int32_t(0xFFFFFFFFu - uint32_t(key[j] & 0xFFFFFFFFull)) : -1;
+      rv_id[v * R + j] = u;
+      rv_mask[v * R + j] = (u >= 0 && leafkey) ? uint8_t(edge_mask(leafkey + v * TP, leafkey + int64_t(u) * TP, T))
+                                               : uint8_t(0);
+      slots[v * R + j] = 0ull;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Phased join (single builds): the iteration of k_join split into two
 // wave-per-row kernels that hold little LDS, so each CU keeps 24-32 rows in
 // flight instead of four, and the rows no wave can hold go to k_join.
@@ -2164,7 +2286,8 @@ __global__ __launch_bounds__(256) void k_jcand(
     const uint64_t* __restrict__ fresh_in, const uint8_t* __restrict__ masks_in,
     const int32_t* __restrict__ order, int64_t nwork, int32_t* __restrict__ cbuf,
     int32_t* __restrict__ ccount, int32_t* __restrict__ ovf, unsigned long long* __restrict__ ovf_n,
-    unsigned long long* __restrict__ counters) {
+    unsigned long long* __restrict__ counters, const int32_t* __restrict__ ids_rv,
+    const uint8_t* __restrict__ rv_mask, int R) {
   __shared__ int32_t hs_w[4][1 << kJHashBits];   // dedup hash, then (low half) sketch similarities
   __shared__ int32_t cid_w[4][kJCap];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -2223,6 +2346,56 @@ __global__ __launch_bounds__(256) void k_jcand(
         const int pos = U + __popcll(bal & lt);
         if (nw && pos < kJCap) cid[pos] = c[u];
         U += __popcll(bal);
+      }
+    }
+    // reverse-neighbour candidates (R > 0): the row's reverse entries u (path
+    // i <- u), its neighbours' reverse entries w (i -> n1 <- w) and the list
+    // entries w of its reverse entries (i <- u -> w); every reverse edge is
+    // new, and a path is dropped when its edges' mask bytes meet (i <- u: when
+    // the edge's own byte is not 0).  Lane j < R holds reverse entry j; each
+    // batch of 256 (entry, lane) pairs is loaded before any of it is hashed.
+    if (R > 0) {
+      const int32_t rid = lane < R ? ids_rv[i * R + lane] : -1;
+      const int32_t rmk = (lane < R && rv_mask) ? int32_t(rv_mask[i * R + lane]) : 0;
+      const int nrev = R + 2 * K * R;
+      for (int e0 = 0; e0 < nrev && U <= kJCap; e0 += 256) {
+        int32_t c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + 64 * u + lane;
+          const bool r1 = e < R;
+          const int e2 = e - R;
+          const bool r2 = !r1 && e2 < K * R;
+          const int e3 = e2 - K * R;
+          const bool r3 = !r1 && !r2 && e3 < R * K;
+          const int k2 = r2 ? e2 / R : 0, j3 = r3 ? e3 / K : 0;
+          const int src = r1 ? e : (r2 ? k2 : j3);   // the lane holding this pair's source
+          const int32_t s_oid = __shfl(oid, src), s_om = __shfl(omask, src);
+          const int32_t s_rid = __shfl(rid, src), s_rmk = __shfl(rmk, src);
+          int32_t id = -1;
+          if (r1) {
+            id = (s_rid >= 0 && s_rmk) ? -2 : s_rid;
+          } else if (r2) {
+            const int64_t at = int64_t(s_oid) * R + (e2 - k2 * R);
+            const int32_t w2 = ids_rv[at];
+            id = (w2 >= 0 && rv_mask && (int32_t(rv_mask[at]) & s_om)) ? -2 : w2;
+          } else if (r3 && s_rid >= 0) {
+            const int64_t at = int64_t(s_rid) * K + (e3 - j3 * K);
+            const int32_t w3 = ids_in[at];
+            id = (masks_in && (int32_t(masks_in[at]) & s_rmk)) ? -2 : w3;
+          }
+          c[u] = id;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          n_lk += __popcll(__ballot(c[u] == -2));
+          if (U > kJCap) c[u] = -1;
+          const bool nw = c[u] >= 0 && hs_insert(hs, kJHashBits, c[u]);
+          const unsigned long long bal = __ballot(nw);
+          const int pos = U + __popcll(bal & lt);
+          if (nw && pos < kJCap) cid[pos] = c[u];
+          U += __popcll(bal);
+        }
       }
     }
     if (U > kJCap) {   // wave-uniform: k_join runs this row
@@ -3239,6 +3412,9 @@ int check_params(const knng_params* p, int64_t n, std::string* why) {
   if (p->cell_full_iters < 0 || p->cell_full_iters > 100) return *why = "cell_full_iters must be in [0, 100]", KNNG_EINVAL;
   if (!(p->cell_merge == p->cell_merge)) return *why = "cell_merge must be a number", KNNG_EINVAL;
   if (!(p->cell_prune >= 0.0f && p->cell_prune <= 1.0f)) return *why = "cell_prune must be in [0, 1]", KNNG_EINVAL;
+  if (p->reverse_width < 0 || p->reverse_width > kRevMax) return *why = "reverse_width must be in [0, 8]", KNNG_ERANGE;
+  if (p->reverse_width > 0 && p->k_build > 50)
+    return *why = "reverse_width needs k_build <= 50 (the block-per-row join's candidate array)", KNNG_EINVAL;
   if (int64_t(p->num_trees) * n >= (int64_t(1) << 31) - 1) return *why = "num_trees * n must fit in int32", KNNG_ERANGE;
   if (n >= (int64_t(1) << 31) - 1) return *why = "n must fit in int32", KNNG_ERANGE;
   return 0;
@@ -3260,6 +3436,36 @@ int check_params(const knng_params* p, int64_t n, std::string* why) {
 
 constexpr int kJoinVariants = 3;
 
+// Reverse lists of graph buffer a for the coming iteration (reverse_width R >
+// 0): every entry offered when the iteration has no new/old rule (or no join
+// iteration ran yet), else the entries that came in during the previous one.
+int build_reverse_lists(knng_ctx* c, int a) {
+  const int64_t n = c->n;
+  const int K = c->p.k_build, R = c->p.reverse_width;
+  int rc;
+  if (!c->rv_slots) {
+    if ((rc = dalloc(c, &c->rv_slots, size_t(n) * R, "reverse-list slots"))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->rv_slots, 0, size_t(n) * R * sizeof(unsigned long long), c->stream));
+  }
+  if ((rc = dalloc(c, &c->rv_id, size_t(n) * R, "reverse lists")) ||
+      (rc = dalloc(c, &c->rv_mask, size_t(n) * R, "reverse-list masks")))
+    return rc;
+  const uint64_t* fin = (c->p.incremental && !c->fresh_all) ? c->fresh[a] : nullptr;
+  hipLaunchKernelGGL(k_rev_insert, dim3(grid_for(c, n * K, 256, 16)), dim3(256), 0, c->stream, c->gid[a],
+                     c->gsc[a], fin, n, K, R, c->rv_slots);
+  hipLaunchKernelGGL(k_rev_final, dim3(grid_for(c, n, 256, 16)), dim3(256), 0, c->stream, c->rv_slots, n, R,
+                     c->leaf_valid ? c->leafkey : static_cast<const int32_t*>(nullptr), int(c->p.num_trees),
+                     c->leaf_tp, c->rv_id, c->rv_mask);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// reverse-list arguments of the join kernels: (ids, masks, R), or none
+#define KNNG_REV_ARGS(c)                                                                   \
+  ((c)->p.reverse_width > 0 ? (c)->rv_id : static_cast<const int32_t*>(nullptr)),           \
+      ((c)->p.reverse_width > 0 && (c)->leaf_valid ? (c)->rv_mask : static_cast<const uint8_t*>(nullptr)), \
+      int((c)->p.reverse_width)
+
 int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const int64_t* nwork_dev,
                 const int64_t* seg_base, int nseg, unsigned long long* segc) {
   const int64_t n = c->n;
@@ -3275,11 +3481,13 @@ int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const in
 #define KNNG_JOIN_ARGS                                                                          \
   c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order, \
       fin, c->fresh[b], nwork_dev, seg_base, nseg, segc, static_cast<int32_t*>(nullptr),             \
-      static_cast<unsigned long long*>(nullptr), mk_in, mk_out, leafkey, int(c->p.num_trees), c->leaf_tp
+      static_cast<unsigned long long*>(nullptr), mk_in, mk_out, leafkey, int(c->p.num_trees), c->leaf_tp, \
+      KNNG_REV_ARGS(c)
   const bool d1024 = c->d == kMaxDim;   // the rows' length as a compile-time constant
-  if (K > 50 && d1024)
+  // reverse candidates: up to K * K + R + 2 K R of them, the 4096-slot shape
+  if ((K > 50 || c->p.reverse_width > 0) && d1024)
     hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
-  else if (K > 50)
+  else if (K > 50 || c->p.reverse_width > 0)
     hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, 0>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
   else if (v == 1 && d1024)
     hipLaunchKernelGGL((k_join<12, 2560, 3, 4, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
@@ -3325,7 +3533,7 @@ int launch_join_small(knng_ctx* c, int a, int b, const int32_t* order) {
   c->q, c->sk, c->d, n, K, M, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->counters, rowq, order,     \
       c->fresh[a], c->fresh[b], static_cast<const int64_t*>(nullptr), static_cast<const int64_t*>(nullptr), \
       0, static_cast<unsigned long long*>(nullptr), c->ovf, c->ovf_n, mk_in, mk_out, leafkey, int(c->p.num_trees), \
-      c->leaf_tp
+      c->leaf_tp, KNNG_REV_ARGS(c)
   if (c->d == kMaxDim)
     hipLaunchKernelGGL((k_join<10, 640, kSmallJoinOcc, 1, 8, kMaxDim, true>), dim3(grid), dim3(256), 0, c->stream,
                        KNNG_SMALL_ARGS);
@@ -3385,7 +3593,7 @@ int launch_join_phased(knng_ctx* c, int a, int b, const int32_t* order) {
   const uint64_t* fin = (c->p.incremental && !c->fresh_all) ? c->fresh[a] : nullptr;
   const int ga = resident_grid(c, k_jcand);
   hipLaunchKernelGGL(k_jcand, dim3(ga), dim3(256), 0, c->stream, c->sk, c->d, K, M, c->gid[a], fin, mk_in, order, n,
-                     c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters);
+                     c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters, KNNG_REV_ARGS(c));
   if (c->d == kMaxDim)
     hipLaunchKernelGGL(k_jdot_merge<kMaxDim>, dim3(resident_grid(c, k_jdot_merge<kMaxDim>)), dim3(256), 0, c->stream,
                        c->q, c->d, K, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->fresh[b], mk_in, mk_out, leafkey,
@@ -3610,7 +3818,7 @@ int knng_default_params(knng_params* p) {
   p->cell_merge = 0.6f;
   p->cell_full_iters = 0;
   p->cell_prune = 0.25f;
-  p->pad0 = 0;
+  p->reverse_width = 0;
   return 0;
 }
 
@@ -3708,6 +3916,9 @@ int knng_destroy(knng_ctx* c) {
   dfree(&c->ex_knns);
   dfree(&c->ex_dists);
   dfree(&c->rr_part);
+  dfree(&c->rv_slots);
+  dfree(&c->rv_id);
+  dfree(&c->rv_mask);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -4125,6 +4336,8 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
   }
   const bool small = !phased && c->p.incremental && !c->fresh_all && K <= 50;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  if (c->p.reverse_width > 0)
+    if (int rc = build_reverse_lists(c, a)) return rc;
   if (phased) {
     if (int rc = launch_join_phased(c, a, b, order)) return rc;
   } else if (small) {
@@ -4162,6 +4375,15 @@ int knng_join_iteration(knng_ctx* c, int64_t* updates) {
         c->leaf_valid ? double(n) * (2.0 * K + double(K) * K) + double(sums[3]) * 4.0 * c->leaf_tp : 0.0;
     c->st.join_bytes[it] = double(n) * (8.0 * K + c->d + 8.0 * c->words + 4.0 * K * K + 8.0 * K) + lk_bytes +
                            double(sums[4]) * 8.0 * c->words + double(sums[2]) * c->d;
+    // reverse lists: offered edges (ids, scores, fresh masks; R slot loads
+    // each), the lists written, and the candidate stage's reads (own and
+    // neighbours' reverse entries, the lists of the own reverse entries; an
+    // upper bound: every reverse slot taken as filled)
+    if (const int R = c->p.reverse_width) {
+      const double offered = (c->p.incremental && it > 0) ? double(c->st.join_inserted[it - 1]) : double(n) * K;
+      c->st.join_bytes[it] += double(n) * (8.0 * K + 8.0) + offered * 8.0 * R + double(n) * R * (16.0 + 5.0) +
+                              double(n) * (5.0 * R + 5.0 * K * R + 5.0 * K * R);
+    }
   }
   c->st.iterations = it + 1;
   c->st.ms_join += ms;
@@ -4395,7 +4617,7 @@ int knng_join_iteration_noalloc(const int8_t* dq, const uint64_t* dsk, int32_t d
       static_cast<const int64_t*>(nullptr), 0, static_cast<unsigned long long*>(nullptr),                      \
       static_cast<int32_t*>(nullptr), static_cast<unsigned long long*>(nullptr),                               \
       static_cast<const uint8_t*>(nullptr), static_cast<uint8_t*>(nullptr), static_cast<const int32_t*>(nullptr), \
-      0, 4
+      0, 4, static_cast<const int32_t*>(nullptr), static_cast<const uint8_t*>(nullptr), 0
   if (K > 50 && d == kMaxDim)
     hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, st, KNNG_NOALLOC_ARGS);
   else if (K > 50)

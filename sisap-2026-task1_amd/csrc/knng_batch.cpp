@@ -144,6 +144,7 @@ extern "C" int knng_build_batch(knng_job* jobs, int32_t njobs, int32_t device, i
     jobs[j].iterations = 0;
     jobs[j].build_ms = 0;
     if (!jobs[j].rows || jobs[j].n < 2 || !jobs[j].knns || !jobs[j].dists) return KNNG_EINVAL;
+    if (jobs[j].params.reverse_width != 0) return KNNG_EINVAL;   // reverse lists: single builds only
   }
   // packed groups: plain-tree jobs with equal build parameters (a job of
   // <= leaf_size rows has a single leaf per tree, which packs as well)

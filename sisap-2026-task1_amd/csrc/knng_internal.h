@@ -67,6 +67,11 @@ struct knng_ctx {
   int leaf_tp = 4;
   bool leaf_valid = false;
   uint8_t* gmask[2] = {nullptr, nullptr};    // per list entry: bit t = row and entry share tree t's leaf
+  // reverse lists (reverse_width R > 0): packed (score, id) top-R slots [n][R]
+  // (all zero between iterations), and the lists built from them
+  unsigned long long* rv_slots = nullptr;
+  int32_t* rv_id = nullptr;      // [n][R], -1 past the last entry
+  uint8_t* rv_mask = nullptr;    // [n][R] mask bytes of the edges u -> v
   unsigned long long *counters = nullptr, *csum = nullptr;
 
   // ingest staging

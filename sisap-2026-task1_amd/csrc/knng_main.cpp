@@ -26,6 +26,7 @@
 //   --partitions N  --oversample N  --kmeans-iters N  --cell-sample N
 //   --incremental 0|1   NN-Descent's new/old rule in join iterations after the first
 //   --rerank-width N    list entries the FP32 rerank re-scores (0: all k_build)
+//   --reverse-width N   reverse-list entries per row in the join (0: none)
 //   --stop-rule reference|inserted|none
 //   --chunk-rows N      rows per HDF5 hyperslab / pinned buffer (default 275,000;
 //                       the reference reads 2.2M, src/main.cpp:26)
@@ -141,6 +142,8 @@ int main(int argc, char** argv) {
       p.incremental = tmp ? 1 : 0;
     } else if (a == "--rerank-width" && num(&tmp)) {
       p.rerank_width = int32_t(tmp);
+    } else if (a == "--reverse-width" && num(&tmp)) {
+      p.reverse_width = int32_t(tmp);
     } else if (a == "--stop-rule" && i + 1 < argc) {
       const std::string r = argv[++i];
       if (r == "reference") p.stop_rule = KNNG_STOP_REFERENCE;
@@ -270,9 +273,9 @@ int main(int argc, char** argv) {
   char params[512];
   snprintf(params, sizeof(params),
            "K_BUILD=%d, RERANK=%d, TREES=%d, LEAF=%d, %s, MAX_ITER=%d, DIM=%d, CELLS=%d, PARTITIONS=%d, "
-           "OVERSAMPLE=%d, INCREMENTAL=%d, FP32_RERANK=%d, RERANK_WIDTH=%d, ITERATIONS=%d",
+           "OVERSAMPLE=%d, INCREMENTAL=%d, REVERSE_WIDTH=%d, FP32_RERANK=%d, RERANK_WIDTH=%d, ITERATIONS=%d",
            p.k_build, p.rerank_mult, p.num_trees, p.leaf_size, rule, p.max_iters, p.dim, st.cells,
-           p.cell_partitions, p.cell_oversample, p.incremental, rerank,
+           p.cell_partitions, p.cell_oversample, p.incremental, p.reverse_width, rerank,
            rerank ? (p.rerank_width ? p.rerank_width : p.k_build) : 0, st.iterations);
   rc = knng_h5_write_result(out.c_str(), knns.data(), dists.data(), n, p.k_eval,
                             "Apex Hardware SIMD (MI355X HIP)", task.c_str(), float(buildtime), 0.0f,

@@ -58,7 +58,7 @@ class Params(C.Structure):
         ("incremental", C.c_int32), ("cells", C.c_int32), ("kmeans_iters", C.c_int32),
         ("cell_partitions", C.c_int32), ("cell_oversample", C.c_int32),
         ("cell_sample", C.c_int32), ("cell_merge", C.c_float), ("cell_full_iters", C.c_int32),
-        ("cell_prune", C.c_float), ("pad0", C.c_int32),
+        ("cell_prune", C.c_float), ("reverse_width", C.c_int32),
     ]
 
 

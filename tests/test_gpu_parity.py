@@ -236,6 +236,36 @@ def test_incremental_full_build_bitexact(torch_cuda):
     assert b.stats()["iterations"] == len(o_rates)
 
 
+# reverse lists (reverse_width R): the phased join (rows over 512 candidates
+# passed to the block-per-row kernel), the fused join (its small-row variant
+# under the new/old rule), with and without the new/old rule
+@pytest.mark.parametrize("n,d,T,leaf,incremental,R,fused", [
+    (4000, 1024, 16, 64, 1, 4, False),
+    (3000, 512, 4, 32, 0, 8, False),
+    (3000, 384, 8, 64, 1, 3, True),
+    (2500, 1024, 4, 2048, 1, 2, True)])
+def test_reverse_lists_full_build_bitexact(torch_cuda, monkeypatch, n, d, T, leaf, incremental, R, fused):
+    say("reverse lists", n, d, T, leaf, incremental, R, fused)
+    if fused:
+        monkeypatch.setenv("KNNG_JOIN_FUSED", "1")
+    x = synth.gaussian_mixture(70 + R, n, d, 16)
+    q, sk = oracle.quantize(x)
+    o_ids, o_sc, o_rates = oracle.build_device_semantics(q, sk, trees=T, leaf=leaf, incremental=bool(incremental),
+                                                         reverse=R)
+    b = _builder(x, num_trees=T, leaf_size=leaf, incremental=incremental, reverse_width=R)
+    b.build()
+    ids, sc = b.graph()
+    st = b.stats()
+    assert np.array_equal(ids, o_ids) and np.array_equal(sc, o_sc)
+    assert st["iterations"] == len(o_rates)
+    if not fused and T == 16:
+        assert max(st["join_passed_on"][:st["iterations"]]) > 0   # k_join ran some rows
+    # the reverse candidates change the build
+    p_ids, _, _ = oracle.build_device_semantics(q, sk, trees=T, leaf=leaf, incremental=bool(incremental))
+    assert not np.array_equal(p_ids, o_ids)
+    b.close()
+
+
 @pytest.mark.parametrize("cells,parts,T", [(24, 2, 8), (9, 3, 3)])
 def test_cells_full_build_bitexact(torch_cuda, cells, parts, T):
     """k-means cells seed the forest; the whole build stays bit-exact."""

@@ -567,6 +567,79 @@ def test_incremental_join_bruteforce():
         assert int(fo[i]) == sum(1 << r for r, e in enumerate(pool[:K]) if e[2] == K)
 
 
+@pytest.mark.parametrize("R,leaf_rule,incremental", [(4, True, False), (2, True, True), (3, True, True),
+                                                    (8, False, True)])
+def test_reverse_join_bruteforce(R, leaf_rule, incremental):
+    """Reverse-neighbour candidates against a numpy restatement.  RV(v) = the R
+    best rows u by (score desc, u asc) whose lists hold v as an entry the
+    iteration treats as new (all entries, or the fresh ones); row i adds u in
+    RV(i), w in RV(n1) for its neighbours n1 and the list of every u in RV(i).
+    Under the leaf-mate rule a path is dropped when all of its rows lie in one
+    leaf of one tree (i <- u: i and u share a leaf)."""
+    x, q, sk = _small(n=250, d=128)
+    K, M = 10, 30
+    perm, leaves = oracle.forest_device(q, 2, 16)
+    ids, sc = oracle.init_device(q, perm, leaves, K)
+    lk = oracle.leaf_keys(perm, leaves) if leaf_rule else None
+    fresh = None
+    if incremental:
+        rng = np.random.default_rng(R)
+        fresh = rng.integers(0, 2 ** 63, 250, dtype=np.uint64) & np.uint64((1 << K) - 1)
+    out = oracle.join_device(q, sk, ids, sc, M, fresh=fresh, leafkey=lk, reverse=R)
+    oi, osc = out[0], out[1]
+    mine = _leaf_sets(perm, leaves, 250)
+    Q = q.astype(np.int64)
+    bits = np.unpackbits(sk.view(np.uint8), axis=1, bitorder="little")
+    new = lambda u, k: fresh is None or (int(fresh[u]) >> k) & 1
+    inc = [[] for _ in range(250)]
+    for u in range(250):
+        for k in range(K):
+            if new(u, k):
+                inc[int(ids[u, k])].append((-int(sc[u, k]), u))
+    rv = [[u for _, u in sorted(e)[:R]] for e in inc]
+    shared = lambda *rows: leaf_rule and bool(set.intersection(*(mine[r] for r in rows)))
+    n_rev = 0
+    for i in range(250):
+        excl = set(ids[i].tolist()) | {i}
+        cands = []
+
+        def add(c):
+            if c not in excl and c not in cands:
+                cands.append(c)
+        for k, n1 in enumerate(ids[i]):
+            for k2, c in enumerate(ids[n1]):
+                if fresh is not None and not (new(i, k) or new(int(n1), k2)):
+                    continue
+                if not shared(i, int(n1), int(c)):
+                    add(int(c))
+        before = len(cands)
+        for u in rv[i]:
+            if not shared(i, u):
+                add(u)
+        for n1 in ids[i]:
+            for w in rv[int(n1)]:
+                if not shared(i, int(n1), w):
+                    add(w)
+        for u in rv[i]:
+            for w in ids[u]:
+                if not shared(i, u, int(w)):
+                    add(int(w))
+        n_rev += len(cands) - before
+        h = np.array([128 - int((bits[i] ^ bits[c]).sum()) for c in cands], np.int32)
+        sel = cands
+        if len(cands) > M:
+            thr = max(t for t in range(129) if (h >= t).sum() >= M)
+            sel = [c for c, hv in zip(cands, h) if hv >= thr]
+        pool = [(int(ids[i, k]), int(sc[i, k]), k) for k in range(K)]
+        pool += [(c, int(Q[i] @ Q[c]), K) for c in sel]
+        pool.sort(key=lambda e: (-e[1], e[0], e[2]))
+        assert oi[i].tolist() == [e[0] for e in pool[:K]], i
+        assert osc[i].tolist() == [e[1] for e in pool[:K]], i
+    # the reverse lists add candidates the 2-hop paths miss (without the new/old
+    # rule almost every reverse candidate is also a 2-hop one at this size)
+    assert n_rev > (250 if incremental else 0)
+
+
 def test_rerank_device_fp64():
     x, q, sk = _small(n=200, d=128)
     perm, leaves = oracle.forest_device(q, 4, 16)
