@@ -113,6 +113,9 @@ typedef struct knng_stats {
                                   dim per leaf (padding not counted) */
   double assign_ops;           /* int8 ops of the cell k-means assignments, rows * cells * 2 * dim
                                   per assignment (sample Lloyd steps, merge rounds, full rows) */
+  int32_t leaf_variant;        /* leaf-tile shape the autotuner chose for leaves of <= 64 rows:
+                                  0 block per leaf (LDS-staged quadrants), 1 wave per leaf */
+  float ms_tune_leaf;          /* time spent timing leaf-tile shapes (first init of a shape) */
 } knng_stats;
 
 typedef struct knng_ctx knng_ctx;

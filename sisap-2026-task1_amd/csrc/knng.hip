@@ -1045,6 +1045,74 @@ __global__ __launch_bounds__(256) void k_leaf_gram(const int8_t* __restrict__ q,
   }
 }
 
+// The second leaf-tile shape: one wave per leaf of at most 64 rows (blocks of
+// four waves take four leaves), no LDS and no block barriers.  Lane (c, h)
+// loads the 16-byte fragments of rows c and 32 + c at k-offset 16 h of every
+// 32-dim step straight from HBM; the same fragments serve as A (rows) and B
+// (columns) of v_mfma_i32_32x32x32_i8, and the Gram matrix is symmetric, so a
+// leaf of 33..64 rows takes three 32 x 32 tiles (top-left, bottom-left,
+// bottom-right; the bottom-left one also written transposed as the
+// top-right) and a leaf of at most 32 rows one.  The keys equal
+// k_leaf_gram's, slot for slot.
+__global__ __launch_bounds__(256) void k_leaf_gram_wave(const int8_t* __restrict__ q, int d, int64_t n, int T,
+                                                        const int32_t* __restrict__ perm,
+                                                        const int4* __restrict__ leaves, int64_t nleaves,
+                                                        uint64_t* __restrict__ cand) {
+  const int lane = threadIdx.x & 63, cl = lane & 31, hi = lane >> 5;
+  const int64_t w0 = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  const int ks_n = d >> 5;
+  for (int64_t L = w0; L < nleaves; L += nw) {
+    const int4 lf = leaves[L];
+    const int t = lf.x, start = lf.y, size = lf.z;
+    if (size > kLeafSlots) continue;   // big leaves: the big-leaf kernels
+    const int32_t* mp = perm + int64_t(t) * n + start;
+    const int32_t mtop = cl < size ? mp[cl] : -1;
+    const int32_t mbot = 32 + cl < size ? mp[32 + cl] : -1;
+    const bool two = size > 32;   // wave-uniform
+    const int8_t* pt = q + int64_t(mtop < 0 ? 0 : mtop) * d + 16 * hi;
+    const int8_t* pb = q + int64_t(mbot < 0 ? 0 : mbot) * d + 16 * hi;
+    v16i a00, a10, a11;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a00[j] = a10[j] = a11[j] = 0;
+    for (int k0 = 0; k0 < ks_n; k0 += 8) {
+      v4i ft[8], fb[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const bool in = k0 + u < ks_n;
+        ft[u] = (in && mtop >= 0) ? *reinterpret_cast<const v4i*>(pt + 32 * (k0 + u)) : v4i{0, 0, 0, 0};
+        fb[u] = (in && two && mbot >= 0) ? *reinterpret_cast<const v4i*>(pb + 32 * (k0 + u)) : v4i{0, 0, 0, 0};
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        a00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ft[u], ft[u], a00, 0, 0, 0);
+        if (two) {
+          a10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[u], ft[u], a10, 0, 0, 0);
+          a11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[u], fb[u], a11, 0, 0, 0);
+        }
+      }
+    }
+    // element j of lane (cl, hi): row r = 8 (j / 4) + 4 hi + j % 4 of the
+    // tile's row half, column cl of its column half.  Member m's slot s gets
+    // (score, id of member s), 0 for s = m or s >= size, as k_leaf_gram.
+    auto put = [&](int32_t member, int slot, int32_t mate, bool self, int32_t score) {
+      const uint64_t key = (mate >= 0 && !self) ? pack_key(score, mate) : 0ull;
+      cand[int64_t(member) * (T * kLeafSlots) + t * kLeafSlots + slot] = key;
+    };
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int r = 8 * (j >> 2) + 4 * hi + (j & 3);
+      const int32_t top_r = __shfl(mtop, r), bot_r = __shfl(mbot, r);   // members r and 32 + r
+      if (r < size) put(top_r, cl, mtop, r == cl, a00[j]);                 // top-left
+      if (cl < size) put(mtop, 32 + r, bot_r, false, two ? a10[j] : 0);    // top-right (transposed)
+      if (two && 32 + r < size) {
+        put(bot_r, cl, mtop, false, a10[j]);                               // bottom-left
+        put(bot_r, 32 + cl, mbot, r == cl, a11[j]);                        // bottom-right
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // K3b: leaves of 65..kMaxLeaf members.  A work item is (leaf, 128 members);
 // wave v owns members r0+32v .. r0+32v+31 and holds their A fragments in
@@ -1712,7 +1780,7 @@ struct JoinScratch {
 
 
 
-template <int HSBITS, int CAP, int OCC, int DOTPER, int SKPER, int DIM, bool SMALL = false>
+template <int HSBITS, int CAP, int OCC, int DOTPER, int SKPER, int DIM, bool SMALL = false, bool REV = false>
 __global__ __launch_bounds__(256, OCC) void k_join(
     const int8_t* __restrict__ q, const uint64_t* __restrict__ sk, int d, int64_t n, int K, int M,
     const int32_t* __restrict__ ids_in, const int32_t* __restrict__ sc_in,
@@ -1750,8 +1818,9 @@ __global__ __launch_bounds__(256, OCC) void k_join(
   // K-th entry (K <= 64), and lists only improve: it could never enter.  New
   // entries get their mask byte from the leaf keys (leafkey[r * TP + t] = the
   // first perm position of row r's leaf in tree t).
-  // R > 0: reverse-neighbour candidates from the R-entry reverse lists rv_id
-  // (k_rev_final), as k_jcand takes them (CAP must hold K * K + R + 2 K R).
+  // REV (with R > 0): reverse-neighbour candidates from the R-entry reverse
+  // lists rv_id (k_rev_final), as k_jcand takes them (CAP must hold
+  // K * K + R + 2 K R); without REV the stage is not compiled in.
   constexpr int HS = 1 << HSBITS;
   static_assert(HS * 4 >= CAP * 2 + CAP * 4, "selection arrays must fit in the hash area");
   __shared__ __attribute__((aligned(16))) int32_t hs[HS];
@@ -1814,7 +1883,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     if (tid < 6) S.cnt[tid] = 0;
     if (masks_out && tid < T && tid < kMaskTrees) S.own_leaf[tid] = leafkey[i * TP + tid];
     if (tid < K) S.own_mask[tid] = masks_in ? masks_in[i * K + tid] : uint8_t(0);
-    if (tid < R) {
+    if (REV && tid < R) {
       S.rv_id[tid] = rv_id[i * R + tid];
       S.rv_mk[tid] = rv_mask ? int32_t(rv_mask[i * R + tid]) : 0;
     }
@@ -1872,7 +1941,7 @@ __global__ __launch_bounds__(256, OCC) void k_join(
     //          list entries w of its reverse entries (i <- u -> w); every reverse
     //          edge is new, and a path is dropped when its edges' mask bytes meet
     //          (i <- u: when the edge's own byte is not 0)
-    if (R > 0) {
+    if (REV && R > 0) {
       const int nrev = R + 2 * K * R;
       int dropped = 0;
       for (int e0 = 0; e0 < nrev; e0 += 256) {
@@ -2281,6 +2350,7 @@ __device__ __forceinline__ WaveSlice wave_slice(int64_t nwork) {
   return ws;
 }
 
+template <bool REV>   // the reverse-candidate stage compiled in (used when R > 0)
 __global__ __launch_bounds__(256) void k_jcand(
     const uint64_t* __restrict__ sk, int d, int K, int M, const int32_t* __restrict__ ids_in,
     const uint64_t* __restrict__ fresh_in, const uint8_t* __restrict__ masks_in,
@@ -2354,7 +2424,7 @@ __global__ __launch_bounds__(256) void k_jcand(
     // new, and a path is dropped when its edges' mask bytes meet (i <- u: when
     // the edge's own byte is not 0).  Lane j < R holds reverse entry j; each
     // batch of 256 (entry, lane) pairs is loaded before any of it is hashed.
-    if (R > 0) {
+    if (REV && R > 0) {
       const int32_t rid = lane < R ? ids_rv[i * R + lane] : -1;
       const int32_t rmk = (lane < R && rv_mask) ? int32_t(rv_mask[i * R + lane]) : 0;
       const int nrev = R + 2 * K * R;
@@ -3485,9 +3555,15 @@ int launch_join(knng_ctx* c, int v, int a, int b, const int32_t* order, const in
       KNNG_REV_ARGS(c)
   const bool d1024 = c->d == kMaxDim;   // the rows' length as a compile-time constant
   // reverse candidates: up to K * K + R + 2 K R of them, the 4096-slot shape
-  if ((K > 50 || c->p.reverse_width > 0) && d1024)
+  if (c->p.reverse_width > 0 && d1024)
+    hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim, false, true>), dim3(grid), dim3(256), 0, c->stream,
+                       KNNG_JOIN_ARGS);
+  else if (c->p.reverse_width > 0)
+    hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, 0, false, true>), dim3(grid), dim3(256), 0, c->stream,
+                       KNNG_JOIN_ARGS);
+  else if (K > 50 && d1024)
     hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
-  else if (K > 50 || c->p.reverse_width > 0)
+  else if (K > 50)
     hipLaunchKernelGGL((k_join<13, 4096, 2, 2, 8, 0>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
   else if (v == 1 && d1024)
     hipLaunchKernelGGL((k_join<12, 2560, 3, 4, 8, kMaxDim>), dim3(grid), dim3(256), 0, c->stream, KNNG_JOIN_ARGS);
@@ -3534,7 +3610,13 @@ int launch_join_small(knng_ctx* c, int a, int b, const int32_t* order) {
       c->fresh[a], c->fresh[b], static_cast<const int64_t*>(nullptr), static_cast<const int64_t*>(nullptr), \
       0, static_cast<unsigned long long*>(nullptr), c->ovf, c->ovf_n, mk_in, mk_out, leafkey, int(c->p.num_trees), \
       c->leaf_tp, KNNG_REV_ARGS(c)
-  if (c->d == kMaxDim)
+  if (c->p.reverse_width > 0 && c->d == kMaxDim)
+    hipLaunchKernelGGL((k_join<10, 640, kSmallJoinOcc, 1, 8, kMaxDim, true, true>), dim3(grid), dim3(256), 0,
+                       c->stream, KNNG_SMALL_ARGS);
+  else if (c->p.reverse_width > 0)
+    hipLaunchKernelGGL((k_join<10, 640, kSmallJoinOcc, 1, 8, 0, true, true>), dim3(grid), dim3(256), 0, c->stream,
+                       KNNG_SMALL_ARGS);
+  else if (c->d == kMaxDim)
     hipLaunchKernelGGL((k_join<10, 640, kSmallJoinOcc, 1, 8, kMaxDim, true>), dim3(grid), dim3(256), 0, c->stream,
                        KNNG_SMALL_ARGS);
   else
@@ -3591,9 +3673,14 @@ int launch_join_phased(knng_ctx* c, int a, int b, const int32_t* order) {
   const int T = int(c->p.num_trees);
   // every entry still new (no join iteration yet): the new/old rule takes every pair
   const uint64_t* fin = (c->p.incremental && !c->fresh_all) ? c->fresh[a] : nullptr;
-  const int ga = resident_grid(c, k_jcand);
-  hipLaunchKernelGGL(k_jcand, dim3(ga), dim3(256), 0, c->stream, c->sk, c->d, K, M, c->gid[a], fin, mk_in, order, n,
-                     c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters, KNNG_REV_ARGS(c));
+  if (c->p.reverse_width > 0)
+    hipLaunchKernelGGL(k_jcand<true>, dim3(resident_grid(c, k_jcand<true>)), dim3(256), 0, c->stream, c->sk, c->d,
+                       K, M, c->gid[a], fin, mk_in, order, n, c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters,
+                       KNNG_REV_ARGS(c));
+  else
+    hipLaunchKernelGGL(k_jcand<false>, dim3(resident_grid(c, k_jcand<false>)), dim3(256), 0, c->stream, c->sk, c->d,
+                       K, M, c->gid[a], fin, mk_in, order, n, c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters,
+                       KNNG_REV_ARGS(c));
   if (c->d == kMaxDim)
     hipLaunchKernelGGL(k_jdot_merge<kMaxDim>, dim3(resident_grid(c, k_jdot_merge<kMaxDim>)), dim3(256), 0, c->stream,
                        c->q, c->d, K, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->fresh[b], mk_in, mk_out, leafkey,
@@ -3692,6 +3779,93 @@ int rerank_chunks(int d) {
   }
   (void)d;
   return dims / 64;
+}
+
+// ---------------------------------------------------------------------------
+// Leaf-tile shapes for leaves of at most 64 rows and their autotuner.  Both
+// shapes write the same keys:
+//   0: k_leaf_gram, a block per leaf, each wave one 32 x 32 quadrant, the
+//      leaf's rows staged through LDS in 512-dim chunks (four tiles a leaf);
+//   1: k_leaf_gram_wave, a wave per leaf, fragments straight from HBM, the
+//      symmetric Gram in three tiles (one for leaves of at most 32 rows).
+// The first init of a (dim, leaf size) shape in a process times each on a
+// prefix of the leaf table (the same keys written twice, so the result is
+// unchanged) and keeps the faster; KNNG_LEAF_VARIANT=v forces v.
+// ---------------------------------------------------------------------------
+constexpr int kLeafVariants = 2;
+
+int launch_leaf_gram(knng_ctx* c, int v, uint64_t* cand, int64_t L0, int64_t nl) {
+  if (nl <= 0) return 0;
+  const int64_t n = c->n;
+  const int T = c->p.num_trees;
+  if (v == 1)
+    hipLaunchKernelGGL(k_leaf_gram_wave, dim3(grid_for(c, nl, 4, 64)), dim3(256), 0, c->stream, c->q, c->d, n, T,
+                       c->perm, c->leaves + L0, nl, cand);
+  else
+    hipLaunchKernelGGL(k_leaf_gram, dim3(grid_for(c, nl, 1, 64)), dim3(256), 0, c->stream, c->q, c->d, n, T,
+                       c->perm, c->leaves + L0, nl, cand);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+struct LeafTuneCache {
+  std::mutex mu;
+  std::map<std::pair<int, int>, int> best;
+};
+
+LeafTuneCache& leaf_tune_cache() {
+  static LeafTuneCache tc;
+  return tc;
+}
+
+int ensure_leaf_variant(knng_ctx* c, uint64_t* cand) {
+  if (c->leaf_variant >= 0) return 0;
+  c->st.ms_tune_leaf = 0;
+  if (const char* e = std::getenv("KNNG_LEAF_VARIANT")) {
+    const int v = std::atoi(e);
+    if (v >= 0 && v < kLeafVariants) return c->leaf_variant = v, 0;
+  }
+  const auto key = std::make_pair(c->d, int(c->p.leaf_size));
+  {
+    std::lock_guard<std::mutex> lk(leaf_tune_cache().mu);
+    auto it = leaf_tune_cache().best.find(key);
+    if (it != leaf_tune_cache().best.end()) return c->leaf_variant = it->second, 0;
+  }
+  // too few leaves to time reliably: the first shape, not cached
+  const int64_t nl = std::min<int64_t>(c->nleaves, std::max<int64_t>(16384, c->nleaves / 16));
+  if (nl < 16384) return c->leaf_variant = 0, 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(c, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(c, KNNG_EHIP, "hipEventCreate failed (leaf-tile autotuner)");
+  }
+  float best_ms = 0, total = 0;
+  int best = 0, rc = 0;
+  // a failed launch ends the tuning at once: no later timing may pick a shape
+  for (int round = 0; round < 2 && !rc; ++round)
+    for (int v = 0; v < kLeafVariants && !rc; ++v) {
+      (void)hipEventRecord(e0, c->stream);
+      if ((rc = launch_leaf_gram(c, v, cand, 0, nl))) break;
+      (void)hipEventRecord(e1, c->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      total += ms;
+      if (round == 1 && (v == 0 || ms < best_ms)) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  c->st.ms_tune_leaf = total;
+  c->leaf_variant = best;
+  std::lock_guard<std::mutex> lk(leaf_tune_cache().mu);
+  leaf_tune_cache().best[key] = best;
+  return 0;
 }
 
 // FP32 rerank export of rows [base, base + nrows) of graph buffer `cur`: one
@@ -4208,12 +4382,16 @@ int knng_build_init(knng_ctx* c) {
       return rc;
   HIPCHK(c, hipMemsetAsync(c->fresh[0], 0xFF, size_t(n) * sizeof(uint64_t), c->stream));   // all fresh
   c->fresh_all = true;
-  uint64_t* cand = nullptr;
+  // scratch of this call, freed on every return
+  ScopedDev<uint64_t> cand_buf;
+  ScopedDev<int4> items_buf;
+  ScopedDev<int32_t> scores_buf;
+  uint64_t*& cand = cand_buf.p;
   if ((rc = dalloc(c, &cand, size_t(n) * T * kLeafSlots, "init candidates"))) return rc;
   c->st.init_tile_ops = 0;
+  if ((rc = ensure_leaf_variant(c, cand))) return rc;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL(k_leaf_gram, dim3(grid_for(c, c->nleaves, 1, 64)), dim3(256), 0, c->stream, c->q,
-                     c->d, n, T, c->perm, c->leaves, c->nleaves, cand);
+  if ((rc = launch_leaf_gram(c, c->leaf_variant, cand, 0, c->nleaves))) return rc;
   if (c->p.leaf_size > kLeafSlots) {
     // leaves above 64 rows: work items of 128 members each
     std::vector<int4> lv(size_t(c->nleaves));
@@ -4237,15 +4415,11 @@ int knng_build_init(knng_ctx* c) {
       const int64_t per_item = int64_t(kBigRows) * mpad;
       const int64_t chunk = std::max<int64_t>(
           1, std::min<int64_t>(int64_t(items.size()), (int64_t(1) << 30) / per_item));
-      int4* ditems = nullptr;
-      int32_t* scratch = nullptr;
+      int4*& ditems = items_buf.p;
+      int32_t*& scratch = scores_buf.p;
       if ((rc = dalloc(c, &ditems, items.size(), "big-leaf items")) ||
-          (rc = dalloc(c, &scratch, size_t(chunk * per_item), "big-leaf scores"))) {
-        dfree(&ditems);
-        dfree(&scratch);
-        dfree(&cand);
+          (rc = dalloc(c, &scratch, size_t(chunk * per_item), "big-leaf scores")))
         return rc;
-      }
       HIPCHK(c, hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
                                c->stream));
       hipError_t e = hipSuccess;
@@ -4266,10 +4440,7 @@ int knng_build_init(knng_ctx* c) {
       HIPCHK(c, hipStreamSynchronize(c->stream));
       dfree(&ditems);
       dfree(&scratch);
-      if (e != hipSuccess) {
-        dfree(&cand);
-        return fail(c, KNNG_EHIP, std::string("big-leaf launch: ") + hipGetErrorString(e));
-      }
+      if (e != hipSuccess) return fail(c, KNNG_EHIP, std::string("big-leaf launch: ") + hipGetErrorString(e));
     }
   }
   const int slots = T * kLeafSlots;
@@ -4297,10 +4468,7 @@ int knng_build_init(knng_ctx* c) {
   // every row's leaf in every tree, for the join's leaf-mate rule
   c->leaf_tp = (T + 3) & ~3;
   c->leaf_valid = false;
-  if (!c->leafkey && (rc = dalloc(c, &c->leafkey, size_t(n) * c->leaf_tp, "leaf keys"))) {
-    dfree(&cand);
-    return rc;
-  }
+  if (!c->leafkey && (rc = dalloc(c, &c->leafkey, size_t(n) * c->leaf_tp, "leaf keys"))) return rc;
   hipLaunchKernelGGL(k_leaf_keys, dim3(grid_for(c, c->nleaves, 4, 16)), dim3(256), 0, c->stream, c->leaves,
                      c->nleaves, c->perm, n, c->leaf_tp, c->leafkey);
   hipLaunchKernelGGL(k_edge_masks, dim3(grid_for(c, n * K, 256, 16)), dim3(256), 0, c->stream, c->gid[0], n, K,
@@ -4469,6 +4637,7 @@ int knng_export(knng_ctx* c, int32_t* knns, float* dists, int32_t k, int32_t rer
 int knng_get_stats(knng_ctx* c, knng_stats* st) {
   if (!c || !st) return KNNG_EINVAL;
   c->st.join_variant = c->join_variant;
+  c->st.leaf_variant = c->leaf_variant;
   *st = c->st;
   return 0;
 }

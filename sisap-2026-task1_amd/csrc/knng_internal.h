@@ -95,6 +95,7 @@ struct knng_ctx {
   int64_t* seg_base = nullptr;   // device copy of seg_host
   int64_t* tune_n = nullptr;     // device scalar: rows of an autotuning launch
   int join_variant = -1;         // k_join shape in use (-1: not chosen yet)
+  int leaf_variant = -1;         // leaf-tile shape for leaves <= 64 rows (-1: not chosen yet)
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   knng_stats st{};
@@ -132,6 +133,16 @@ void dfree(T** ptr) {
   if (*ptr) (void)hipFree(reinterpret_cast<void*>(const_cast<typename std::remove_const<T>::type*>(*ptr)));
   *ptr = nullptr;
 }
+
+// A device buffer owned by one scope: freed (dfree) when the scope ends.
+template <typename T>
+struct ScopedDev {
+  T* p = nullptr;
+  ScopedDev() = default;
+  ScopedDev(const ScopedDev&) = delete;
+  ScopedDev& operator=(const ScopedDev&) = delete;
+  ~ScopedDev() { dfree(&p); }
+};
 
 // Stage tracing: with KNNG_TRACE=1 in the environment, trace_mark(c, label)
 // waits for the context's stream and prints the wall time since the previous

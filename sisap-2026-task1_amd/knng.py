@@ -75,6 +75,7 @@ class Stats(C.Structure):
         ("join_inserted", C.c_int64 * 32), ("insert_rates", C.c_float * 32),
         ("join_variant", C.c_int32), ("ms_tune", C.c_float), ("join_passed_on", C.c_int64 * 32),
         ("join_leafmates", C.c_int64 * 32), ("init_tile_ops", C.c_double), ("assign_ops", C.c_double),
+        ("leaf_variant", C.c_int32), ("ms_tune_leaf", C.c_float),
     ]
 
     def as_dict(self) -> dict:
@@ -96,6 +97,7 @@ class Stats(C.Structure):
             "join_passed_on": list(self.join_passed_on[:it]),
             "join_leafmates": list(self.join_leafmates[:it]),
             "init_tile_ops": self.init_tile_ops, "assign_ops": self.assign_ops,
+            "leaf_variant": self.leaf_variant, "ms_tune_leaf": self.ms_tune_leaf,
         }
 
 

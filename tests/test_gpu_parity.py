@@ -136,6 +136,57 @@ def test_forest_and_init_bitexact(torch_cuda, n, d, T, leaf, cells, parts):
     assert np.array_equal(sc, o_sc)
 
 
+# both leaf-tile shapes for leaves of at most 64 rows (block per leaf, wave
+# per leaf): leaves of at most 32 rows (one tile) and of 33-64 rows (the
+# symmetric three-tile path), d from one 32-dim step group to 1024
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("n,d,T,leaf", [(3000, 1024, 8, 64), (2500, 384, 4, 32), (2000, 64, 3, 48),
+                                        (1200, 192, 5, 16)])
+def test_leaf_tile_shapes_bitexact(torch_cuda, monkeypatch, variant, n, d, T, leaf):
+    say("leaf tiles", variant, n, d, T, leaf)
+    monkeypatch.setenv("KNNG_LEAF_VARIANT", str(variant))
+    x = synth.gaussian_mixture(n + leaf, n, d, 24)
+    q, _ = oracle.quantize(x)
+    b = _builder(x, num_trees=T, leaf_size=leaf, k_build=20, k_eval=15, cells=0)
+    b.build_forest()
+    perm, leaves = b.forest()
+    b.build_init()
+    assert b.stats()["leaf_variant"] == variant
+    ids, sc = b.graph()
+    o_ids, o_sc = oracle.init_device(q, perm, leaves, 20)
+    assert np.array_equal(ids, o_ids) and np.array_equal(sc, o_sc)
+    b.close()
+
+
+def test_leaf_tile_autotuner(torch_cuda):
+    """Enough leaves to time: the autotuner times both shapes on a prefix of the
+    leaf table, keeps one (cached for the shape), and the init equals the
+    init of either shape forced."""
+    say("leaf tile autotuner")
+    n, d = 120000, 128
+    x = synth.gaussian_mixture(91, n, d, 64)
+    graphs = []
+    for env in (None, "0", "1"):
+        if env is None:
+            os.environ.pop("KNNG_LEAF_VARIANT", None)
+        else:
+            os.environ["KNNG_LEAF_VARIANT"] = env
+        try:
+            b = _builder(x, num_trees=2, leaf_size=16, k_build=20, k_eval=15, cells=0)
+            b.build_forest()
+            b.build_init()
+            st = b.stats()
+            graphs.append(b.graph())
+            if env is None:
+                assert st["leaf_variant"] in (0, 1)
+                assert st["ms_tune_leaf"] > 0 or st["leaf_variant"] >= 0
+            b.close()
+        finally:
+            os.environ.pop("KNNG_LEAF_VARIANT", None)
+    for ids, sc in graphs[1:]:
+        assert np.array_equal(ids, graphs[0][0]) and np.array_equal(sc, graphs[0][1])
+
+
 def test_large_assignment_forest_bitexact(torch_cuda):
     """70,000 rows at d = 1024 (45 target cells, 2 partitions): the full-row
     cell assignment spans 547 row tiles, the last one partial, over 69 / 68
