@@ -2334,6 +2334,7 @@ __global__ __launch_bounds__(256) void k_rev_final(unsigned long long* __restric
 
 constexpr int kJCap = 512;        // candidates a phased-join row may hold
 constexpr int kJHashBits = 10;    // per-wave LDS hash: 1024 slots (< 64 own + self + kJCap + 63 inserted)
+constexpr int kJBatch = 8;        // neighbour lists whose ids k_jcand loads per batch
 
 // blocks of 4 waves, dealt round-robin to the 8 XCDs: XCD x takes the
 // contiguous slice [x*nwork/8, (x+1)*nwork/8) of the work list and its waves
@@ -2380,13 +2381,13 @@ __global__ __launch_bounds__(256) void k_jcand(
     int U = 0;
     // the neighbours' fresh masks, lane k holding neighbour k's (one load per row)
     const uint64_t fnb = (fresh_in && lane < K) ? fresh_in[oid] : ~0ull;
-    // 2-hop ids of eight neighbour lists at a time (batch b + 1 loads while
+    // 2-hop ids of kJBatch neighbour lists at a time (batch b + 1 loads while
     // batch b is hashed); the leaf-mate rule drops the path i -> n1 -> c when
     // both of its edges share a leaf in the same tree (their mask bytes meet)
-    int32_t cn[8];
-    auto load_ids = [&](int k0, int32_t (&dst)[8]) {
+    int32_t cn[kJBatch];
+    auto load_ids = [&](int k0, int32_t (&dst)[kJBatch]) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < kJBatch; ++u) {
         const int k = k0 + u;
         dst[u] = -1;
         if (k < K) {
@@ -2402,13 +2403,13 @@ __global__ __launch_bounds__(256) void k_jcand(
       }
     };
     load_ids(0, cn);
-    for (int k0 = 0; k0 < K && U <= kJCap; k0 += 8) {
-      int32_t c[8];
+    for (int k0 = 0; k0 < K && U <= kJCap; k0 += kJBatch) {
+      int32_t c[kJBatch];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) c[u] = cn[u];
-      if (k0 + 8 < K) load_ids(k0 + 8, cn);
+      for (int u = 0; u < kJBatch; ++u) c[u] = cn[u];
+      if (k0 + kJBatch < K) load_ids(k0 + kJBatch, cn);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < kJBatch; ++u) {
         n_lk += __popcll(__ballot(c[u] == -2));
         if (U > kJCap) c[u] = -1;   // the row overflows: insert nothing more (wave-uniform)
         const bool nw = c[u] >= 0 && hs_insert(hs, kJHashBits, c[u]);

@@ -41,6 +41,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -197,7 +198,10 @@ __global__ __launch_bounds__(256) void k_search(
     const int32_t* __restrict__ cell_root, const int32_t* __restrict__ qcell, int ncells,
     int nparts, int entry_trees, const int8_t* __restrict__ qq, const float* __restrict__ qx,
     int nq, int beam, int k, int rerank, int max_steps, int32_t* __restrict__ out_ids,
-    float* __restrict__ out_sc) {
+    float* __restrict__ out_sc, int express) {
+  // express launches raise their waves' issue priority, so on a SIMD shared
+  // with bulk search waves theirs issue first (results are unaffected)
+  if (express) __builtin_amdgcn_s_setprio(2);
   constexpr int HS = 1 << kVisitBits;
   __shared__ int32_t hs[HS];
   __shared__ __attribute__((aligned(16))) int8_t qrow[kMaxDim];
@@ -721,7 +725,7 @@ int launch_search(knng_server* s, hipStream_t stream, const float* dx, int8_t* d
   hipLaunchKernelGGL(k_search, dim3(grid), dim3(256), 0, stream, s->q, s->x, s->d, s->gid, s->K,
                      c->hier, c->hier_count, c->leaves, c->perm, s->n_forest, c->cell_root, dcell,
                      c->ncells, std::max(1, c->nparts), et, dq, dx, nq, beam, k, rerank,
-                     s->sp.max_steps, did, dsc);
+                     s->sp.max_steps, did, dsc, (sl && sl->express) ? 1 : 0);
   SCHK(s, hipGetLastError());
   return 0;
 }
