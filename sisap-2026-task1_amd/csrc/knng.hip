@@ -24,6 +24,7 @@
 #include <limits>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <cstdio>
 #include <cstdlib>
@@ -3304,41 +3305,102 @@ int seed_from_cells(knng_ctx* c, int Ct, int* nnodes) {
   int rc;
   dfree(&c->cassign);
   if ((rc = dalloc(c, &c->cassign, size_t(G) * n, "cell assignment"))) return rc;
-  KmeansBufs B;
-  B.pcap = assign_part_entries(cap, c->num_cu);
-  int8_t* merge_rows = nullptr;
-  if ((rc = dalloc(c, &B.z, size_t(cap) * d, "centroids")) ||
-      (rc = dalloc(c, &B.w, size_t(cap), "centroid norms")) ||
-      (rc = dalloc(c, &B.a, size_t(std::max<int64_t>(cap, (n + stride - 1) / stride)), "assignment")) ||
-      (rc = dalloc(c, &B.sc, size_t(cap), "merge values")) ||
-      (rc = dalloc(c, &B.members, size_t(n), "cell members")) ||
-      (rc = dalloc(c, &B.offs, size_t(cap) + 1, "cell offsets")) ||
-      (rc = dalloc(c, &B.sums, size_t(cap) * d, "cell sums")) ||
-      (rc = dalloc(c, &B.ids, size_t(cap), "seed ids")) ||
-      (rc = dalloc(c, &B.pbest, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
-      (rc = dalloc(c, &B.pidx, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
-      (rc = dalloc(c, &merge_rows, size_t(cap) * d, "merge rows")) ||
-      (rc = dalloc(c, &B.ccount, size_t(cap), "cell counts")) ||
-      (rc = dalloc(c, &B.ccur, size_t(cap), "cell cursors")) ||
-      (rc = dalloc(c, &B.partner, size_t(cap), "merge partners")) ||
-      (rc = dalloc(c, &B.dcnt, size_t(cap), "merge counts")) ||
-      (rc = dalloc(c, &B.dpairs, 1, "merge pair count"))) {
-    free_kmeans(B, &merge_rows);
-    return rc;
-  }
-  if (hipHostMalloc(reinterpret_cast<void**>(&B.hpairs), sizeof(unsigned long long), hipHostMallocDefault) !=
-      hipSuccess) {
-    free_kmeans(B, &merge_rows);
-    return fail(c, KNNG_ENOMEM, "hipHostMalloc failed (k-means merge pair count)");
-  }
+  // one buffer set per partition: the partitions' k-means run concurrently,
+  // each on a host thread with a copy of the context that owns its own
+  // stream (see below)
+  auto alloc_bufs = [&](KmeansBufs& B, int8_t** merge_rows) -> int {
+    B.pcap = assign_part_entries(cap, c->num_cu);
+    int e;
+    if ((e = dalloc(c, &B.z, size_t(cap) * d, "centroids")) ||
+        (e = dalloc(c, &B.w, size_t(cap), "centroid norms")) ||
+        (e = dalloc(c, &B.a, size_t(std::max<int64_t>(cap, (n + stride - 1) / stride)), "assignment")) ||
+        (e = dalloc(c, &B.sc, size_t(cap), "merge values")) ||
+        (e = dalloc(c, &B.members, size_t(n), "cell members")) ||
+        (e = dalloc(c, &B.offs, size_t(cap) + 1, "cell offsets")) ||
+        (e = dalloc(c, &B.sums, size_t(cap) * d, "cell sums")) ||
+        (e = dalloc(c, &B.ids, size_t(cap), "seed ids")) ||
+        (e = dalloc(c, &B.pbest, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
+        (e = dalloc(c, &B.pidx, size_t(assign_part_entries(cap, c->num_cu)), "assign partials")) ||
+        (e = dalloc(c, merge_rows, size_t(cap) * d, "merge rows")) ||
+        (e = dalloc(c, &B.ccount, size_t(cap), "cell counts")) ||
+        (e = dalloc(c, &B.ccur, size_t(cap), "cell cursors")) ||
+        (e = dalloc(c, &B.partner, size_t(cap), "merge partners")) ||
+        (e = dalloc(c, &B.dcnt, size_t(cap), "merge counts")) ||
+        (e = dalloc(c, &B.dpairs, 1, "merge pair count")))
+      return e;
+    if (hipHostMalloc(reinterpret_cast<void**>(&B.hpairs), sizeof(unsigned long long), hipHostMallocDefault) !=
+        hipSuccess)
+      return fail(c, KNNG_ENOMEM, "hipHostMalloc failed (k-means merge pair count)");
+    return 0;
+  };
   std::vector<std::vector<int8_t>> zs(G);
   std::vector<std::vector<float>> ws(G);
   c->cell_count.assign(size_t(G), 0);
-  for (int g = 0; g < G && !rc; ++g) {
-    rc = run_kmeans(c, g, Ct, B, merge_rows, &zs[size_t(g)], &ws[size_t(g)]);
-    if (!rc) c->cell_count[size_t(g)] = int32_t(ws[size_t(g)].size());
+  // The partitions are independent k-means runs (own seeds, buffers and
+  // slice of c->cassign; the int8 rows are only read), so each runs on a host
+  // thread of its own with a copy of the context whose stream is its own:
+  // the many small launches of one partition's merging rounds and its host
+  // checks overlap the other partitions' work.  Every partition computes
+  // exactly what it computes alone.  KNNG_KMEANS_SERIAL=1 runs them in turn
+  // on the context's stream.
+  const char* ser = std::getenv("KNNG_KMEANS_SERIAL");
+  const bool serial = G < 2 || (ser && ser[0] == '1');
+  const int L = serial ? 1 : G;
+  std::vector<KmeansBufs> Bs(static_cast<size_t>(L));
+  std::vector<int8_t*> merge_rows(size_t(L), nullptr);
+  auto free_all = [&]() {
+    for (int l = 0; l < L; ++l) free_kmeans(Bs[size_t(l)], &merge_rows[size_t(l)]);
+  };
+  for (int l = 0; l < L && !rc; ++l) rc = alloc_bufs(Bs[size_t(l)], &merge_rows[size_t(l)]);
+  if (rc) {
+    free_all();
+    return rc;
   }
-  free_kmeans(B, &merge_rows);
+  if (serial) {
+    for (int g = 0; g < G && !rc; ++g) {
+      rc = run_kmeans(c, g, Ct, Bs[0], merge_rows[0], &zs[size_t(g)], &ws[size_t(g)]);
+      if (!rc) c->cell_count[size_t(g)] = int32_t(ws[size_t(g)].size());
+    }
+  } else {
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {   // the int8 rows are in place
+      free_all();
+      return fail(c, KNNG_EHIP, "hipStreamSynchronize failed before the cell k-means");
+    }
+    const double ops0 = c->st.assign_ops;   // each lane adds its own to this count
+    std::vector<knng_ctx> lane(size_t(G), *c);
+    std::vector<int> rcs(size_t(G), 0);
+    for (int g = 0; g < G && !rc; ++g)
+      if (hipStreamCreateWithFlags(&lane[size_t(g)].stream, hipStreamNonBlocking) != hipSuccess) {
+        lane[size_t(g)].stream = nullptr;
+        rc = fail(c, KNNG_EHIP, "hipStreamCreate failed (cell k-means)");
+      }
+    if (!rc) {
+      std::vector<std::thread> th;
+      for (int g = 0; g < G; ++g)
+        th.emplace_back([&, g] {
+          knng_ctx* lc = &lane[size_t(g)];
+          rcs[size_t(g)] = hipSetDevice(lc->device) != hipSuccess
+                               ? fail(lc, KNNG_EHIP, "hipSetDevice failed (cell k-means)")
+                               : run_kmeans(lc, g, Ct, Bs[size_t(g)], merge_rows[size_t(g)], &zs[size_t(g)],
+                                            &ws[size_t(g)]);
+        });
+      for (std::thread& t : th) t.join();
+      for (int g = 0; g < G; ++g) {
+        c->st.assign_ops += lane[size_t(g)].st.assign_ops - ops0;
+        if (rcs[size_t(g)] && !rc) {
+          rc = rcs[size_t(g)];
+          c->err = lane[size_t(g)].err;
+        }
+        if (!rcs[size_t(g)]) c->cell_count[size_t(g)] = int32_t(ws[size_t(g)].size());
+      }
+    }
+    for (int g = 0; g < G; ++g)
+      if (lane[size_t(g)].stream) {
+        (void)hipStreamSynchronize(lane[size_t(g)].stream);
+        (void)hipStreamDestroy(lane[size_t(g)].stream);
+      }
+  }
+  free_all();
   if (rc) return rc;
   const int Cm = *std::max_element(c->cell_count.begin(), c->cell_count.end());
   c->ncells = Cm;

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -149,17 +150,19 @@ struct ScopedDev {
 // mark to stderr.  Off (and free) otherwise.
 struct TraceClock {
   bool on = false;
+  std::mutex mu;   // marks may come from several host threads (the k-means partitions)
   std::chrono::steady_clock::time_point last;
 };
 
 inline TraceClock& trace_clock() {
-  static TraceClock tc = [] {
-    TraceClock t;
+  static TraceClock tc;
+  static const bool init = [] {
     const char* e = std::getenv("KNNG_TRACE");
-    t.on = e && e[0] == '1';
-    t.last = std::chrono::steady_clock::now();
-    return t;
+    tc.on = e && e[0] == '1';
+    tc.last = std::chrono::steady_clock::now();
+    return true;
   }();
+  (void)init;
   return tc;
 }
 
@@ -167,6 +170,7 @@ inline void trace_mark(knng_ctx* c, const char* label) {
   TraceClock& tc = trace_clock();
   if (!tc.on) return;
   (void)hipStreamSynchronize(c->stream);
+  std::lock_guard<std::mutex> lk(tc.mu);
   const auto now = std::chrono::steady_clock::now();
   std::fprintf(stderr, "[knng] %-28s %9.2f ms\n", label,
                std::chrono::duration<double, std::milli>(now - tc.last).count());
