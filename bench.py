@@ -7,7 +7,7 @@ sketch prefilter x8 of the reference, with the build form chosen for this
 workload: 4 trees whose first level is an over-seeded spherical k-means
 partition each (4 partitions), leaves of up to 2048 rows brute-forced on
 int8 MFMA, join iterations until fewer than 1% of the slots get a new id,
-and an FP32 rerank of each row's 22 best list entries.  One step is the
+and an FP32 rerank of each row's 28 best list entries.  One step is the
 whole hot path from resident float32 rows to the final lists: quantise +
 sketch, cells + forest, leaf-tile init, join iterations, plain export and
 the FP32-rerank export.  The value is seconds per step
@@ -182,7 +182,7 @@ def main():
     ap.add_argument("--cell-sample", type=int, default=8, help="k-means sample: every n-th row")
     ap.add_argument("--incremental", type=int, default=1,
                     help="1: NN-Descent's new/old rule in the join (a 2-hop pair needs a fresh edge)")
-    ap.add_argument("--rerank-width", type=int, default=22,
+    ap.add_argument("--rerank-width", type=int, default=28,
                     help="list entries the FP32 rerank re-scores (0: all k_build)")
     ap.add_argument("--reverse-width", type=int, default=0,
                     help="reverse-list entries per row in the join (0: pull-only 2-hop join)")

@@ -2823,96 +2823,136 @@ __global__ void k_export_plain(const int32_t* __restrict__ gid, const int32_t* _
   }
 }
 
-// FP32 rerank in dimension slices, one launch per slice.  A whole float32 row
-// is 4 KB, and the candidate rows of the ~900 rows in flight on one XCD (about
-// 6,000 distinct rows at 6.35M x 1024, k_build 50, width 22: a row's list
-// reaches well beyond its tree-0 leaf) overflow its 4 MB L2, so a one-pass
-// kernel fetched 0.96 of its gathered bytes from HBM; a slice of 64 CH dims
-// needs 1/(16/CH) of that per pass, so a pass's window of candidate slices
-// can stay in the L2 while its rows are in flight.
-// Pass s adds the FP32 dot products over dims [64 CH s, 64 CH (s + 1)) of the
-// row's first R list entries to part[i * R + r]; the last pass ranks the sums
+__global__ void k_zero_u64(unsigned long long* __restrict__ v, int m) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x) v[j] = 0ull;
+}
+
+// waves per rerank block: a block takes this many consecutive rows of its
+// XCD's slice per step (one queue atomic per step).  4 and 8 time alike; 16
+// (one 1024-thread block per CU at 7 waves per SIMD) is slower.
+constexpr int kRerankWaves = 4;
+
+// FP32 rerank in dimension slices, one launch per slice.  Pass s adds the
+// FP32 dot products over dims [64 CH s, 64 CH (s + 1)) of the row's first R
+// list entries to part[i * R + r]; the last pass ranks the sums
 // (((p0 + p1) + p2) + ...) by (score desc, id asc, slot asc) and writes the
-// row's best k as 1-based ids, best last.  Rows are visited in `order` (tree
-// 0's leaf order; null: index order), each XCD (blockIdx.x & 7; the grid is a
-// multiple of 8 blocks) taking a contiguous eighth of it, so the rows in
-// flight on an XCD are cluster-mates whose candidate rows largely coincide.
+// row's best k as 1-based ids, best last.
+// Locality: rows are visited in `order` (tree 0's leaf order; null: index
+// order), each XCD (blockIdx.x & 7; the grid is a multiple of 8 blocks)
+// taking a contiguous eighth of it, and a block takes kRerankWaves
+// consecutive rows at a time from its XCD's eighth through the queue counter
+// rowq[64 x] (one atomic per step, claimed two steps ahead), so the rows in
+// flight on an XCD stay one window of consecutive cluster-mates.  At 6.35M x
+// 1024, width 22, a window of 1024 rows touches 0.34 distinct rows per list
+// entry.  With a fixed stride per wave instead, the waves drift apart over a
+// pass, the window widens beyond the L2, and 64-dim slices fetched 0.74 of
+// their gathered bytes from HBM against 0.35 with the queue.  Slicing keeps a
+// window's candidate slices within the 4 MB L2 (a one-pass kernel over whole
+// 4 KB rows fetched 0.96).
 // row_base: a packed build's segment offset (rows, list ids and x are in the
 // packed numbering; part, knns and dists in the segment's own, ids written
 // relative to the segment); 0 for a single build.
 // One wave per row in four groups of 16 lanes: group g scores candidates
-// g, g + 4, ... (U = 8 / CH per group in flight), lane t of a group holds
-// float4 chunks t, t + 16, ... (CH of them) of the slice.  d % 64 == 0, so a
-// chunk column is either inside the row or past its end.
+// g, g + 4, ... (U per group in flight), lane t of a group holds float4
+// chunks t, t + 16, ... (CH of them) of the slice.  d % 64 == 0, so a chunk
+// column is either inside the row or past its end.
 template <int CH, bool kFirst, bool kLast>
-__global__ __launch_bounds__(256) void k_rerank_slice(const float* __restrict__ x, int d, int64_t n, int K,
-                                                      int R, int k, int s, const int32_t* __restrict__ gid,
-                                                      float* __restrict__ part, int32_t* __restrict__ knns,
-                                                      float* __restrict__ dists,
-                                                      const int32_t* __restrict__ order, int64_t row_base) {
-  static_assert(CH == 1 || CH == 2 || CH == 4, "slice of 64, 128 or 256 dims");
-  constexpr int U = 8 / CH;   // candidates in flight per 16-lane group
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+__global__ __launch_bounds__(64 * kRerankWaves) void k_rerank_slice(
+    const float* __restrict__ x, int d, int64_t n, int K, int R, int k, int s, const int32_t* __restrict__ gid,
+    float* __restrict__ part, int32_t* __restrict__ knns, float* __restrict__ dists,
+    const int32_t* __restrict__ order, int64_t row_base, unsigned int* __restrict__ rowq) {
+  static_assert(CH == 1 || CH == 2 || CH == 4 || CH == 8, "slice of 64, 128, 256 or 512 dims");
+  constexpr int U = CH < 8 ? 8 / CH : 1;   // candidates in flight per 16-lane group
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = lane >> 4, t = lane & 15;
   const int xcd = blockIdx.x & 7;
-  const int64_t nbx = gridDim.x >> 3, bx = blockIdx.x >> 3;
-  const int64_t jhi = n * (xcd + 1) / 8;
+  const int64_t jlo = n * xcd / 8, jhi = n * (xcd + 1) / 8;
+  unsigned int* const qctr = rowq + 64 * xcd;
   const int base = s * 64 * CH;
   bool on[CH];
 #pragma unroll
   for (int c = 0; c < CH; ++c) on[c] = base + 64 * c < d;
-  for (int64_t j = n * xcd / 8 + bx * 4 + w; j < jhi; j += nbx * 4) {
-    const int64_t i = order ? order[j] : j;
-    const float4* po = reinterpret_cast<const float4*>(x + (row_base + i) * d + base);
-    float4 own[CH];
-#pragma unroll
-    for (int c = 0; c < CH; ++c) own[c] = on[c] ? po[t + 16 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const int32_t my_id = lane < R ? gid[(row_base + i) * K + lane] : int32_t(row_base);
+  // Claims run two steps ahead in a ring of three: at step t, sq[t % 3] and
+  // sq[(t + 1) % 3] hold steps t and t + 1; thread 0 claims step t + 2 and
+  // writes it into sq[(t + 2) % 3] after its row of step t, a slot last read
+  // in step t - 1, whose closing barrier every wave has passed.  Knowing step
+  // t + 1's row, a wave loads its index and list ids while step t's slices
+  // are in flight, so a row costs one dependent round trip, not three.
+  __shared__ unsigned int sq[3];
+  if (threadIdx.x == 0) {
+    sq[0] = atomicAdd(qctr, 1u);
+    sq[1] = atomicAdd(qctr, 1u);
+  }
+  __syncthreads();
+  auto row_at = [&](int64_t j) -> int64_t { return j < jhi ? (order ? order[j] : j) : int64_t(0); };
+  int64_t j = jlo + int64_t(sq[0]) * kRerankWaves + w;
+  int64_t i = row_at(j);
+  int32_t my_id = (j < jhi && lane < R) ? gid[(row_base + i) * K + lane] : int32_t(row_base);
+  for (int step = 0;; ++step) {
+    if (jlo + int64_t(sq[step % 3]) * kRerankWaves >= jhi) break;   // block-uniform
+    const unsigned int nxt = threadIdx.x == 0 ? atomicAdd(qctr, 1u) : 0u;
+    const int64_t j_next = jlo + int64_t(sq[(step + 1) % 3]) * kRerankWaves + w;
+    const int64_t i_next = row_at(j_next);
     float my_s = 0.f;
-    for (int r0 = 0; r0 < R; r0 += 4 * U) {
-      float4 v[U][CH];
+    if (j < jhi) {
+      const float4* po = reinterpret_cast<const float4*>(x + (row_base + i) * d + base);
+      float4 own[CH];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int r = r0 + 4 * u + g;
-        const int32_t id = __shfl(my_id, r < R ? r : 0);
-        const float4* pc = reinterpret_cast<const float4*>(x + int64_t(id) * d + base);
+      for (int c = 0; c < CH; ++c) own[c] = on[c] ? po[t + 16 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int r0 = 0; r0 < R; r0 += 4 * U) {
+        float4 v[U][CH];
 #pragma unroll
-        for (int c = 0; c < CH; ++c)
-          v[u][c] = (on[c] && r < R) ? pc[t + 16 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
+        for (int u = 0; u < U; ++u) {
+          const int r = r0 + 4 * u + g;
+          const int32_t id = __shfl(my_id, r < R ? r : 0);
+          const float4* pc = reinterpret_cast<const float4*>(x + int64_t(id) * d + base);
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        float sa = 0.f;
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          sa = fmaf(own[c].x, v[u][c].x, sa);
-          sa = fmaf(own[c].y, v[u][c].y, sa);
-          sa = fmaf(own[c].z, v[u][c].z, sa);
-          sa = fmaf(own[c].w, v[u][c].w, sa);
+          for (int c = 0; c < CH; ++c)
+            v[u][c] = (on[c] && r < R) ? pc[t + 16 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
-        for (int m = 8; m >= 1; m >>= 1) sa += __shfl_xor(sa, m, 16);
-        // lane r0 + 4u + g' takes group g''s sum (candidate r0 + 4u + g')
-        const int lo = r0 + 4 * u;
-        const float got = __shfl(sa, ((lane - lo) & 3) * 16);
-        if (lane >= lo && lane < lo + 4) my_s = got;
+        for (int u = 0; u < U; ++u) {
+          float sa = 0.f;
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            sa = fmaf(own[c].x, v[u][c].x, sa);
+            sa = fmaf(own[c].y, v[u][c].y, sa);
+            sa = fmaf(own[c].z, v[u][c].z, sa);
+            sa = fmaf(own[c].w, v[u][c].w, sa);
+          }
+#pragma unroll
+          for (int m = 8; m >= 1; m >>= 1) sa += __shfl_xor(sa, m, 16);
+          // lane r0 + 4u + g' takes group g''s sum (candidate r0 + 4u + g')
+          const int lo = r0 + 4 * u;
+          const float got = __shfl(sa, ((lane - lo) & 3) * 16);
+          if (lane >= lo && lane < lo + 4) my_s = got;
+        }
       }
     }
-    if (!kFirst && lane < R) my_s = part[i * R + lane] + my_s;
-    if (!kLast) {
-      if (lane < R) part[i * R + lane] = my_s;
-    } else {
-      int rank = 0;
-      for (int r = 0; r < R; ++r) {
-        const float tv = __shfl(my_s, r);
-        const int32_t tid = __shfl(my_id, r);
-        rank += (tv > my_s || (tv == my_s && (tid < my_id || (tid == my_id && r < lane)))) ? 1 : 0;
-      }
-      if (lane < R && rank < k) {
-        knns[i * k + (k - 1 - rank)] = int32_t(my_id - row_base) + 1;
-        dists[i * k + (k - 1 - rank)] = my_s;
+    const int32_t id_next = (j_next < jhi && lane < R) ? gid[(row_base + i_next) * K + lane] : int32_t(row_base);
+    if (j < jhi) {
+      if (!kFirst && lane < R) my_s = part[i * R + lane] + my_s;
+      if (!kLast) {
+        if (lane < R) part[i * R + lane] = my_s;
+      } else {
+        int rank = 0;
+        for (int r = 0; r < R; ++r) {
+          const float tv = __shfl(my_s, r);
+          const int32_t tid = __shfl(my_id, r);
+          rank += (tv > my_s || (tv == my_s && (tid < my_id || (tid == my_id && r < lane)))) ? 1 : 0;
+        }
+        if (lane < R && rank < k) {
+          knns[i * k + (k - 1 - rank)] = int32_t(my_id - row_base) + 1;
+          dists[i * k + (k - 1 - rank)] = my_s;
+        }
       }
     }
+    j = j_next;
+    i = i_next;
+    my_id = id_next;
+    if (threadIdx.x == 0) sq[(step + 2) % 3] = nxt;
+    __syncthreads();
   }
 }
 
@@ -3830,15 +3870,18 @@ int ensure_join_variant(knng_ctx* c, int a, int b, const int32_t* order, int64_t
   return 0;
 }
 
+// slices per export at most (64-dim slices of at most kMaxDim dims)
+constexpr int kRerankMaxPasses = kMaxDim / 64;
+
 // float4 chunks per lane of one rerank slice (slice = 64 * chunks dims).
-// 64-dim slices by default: at 6.35M x 1024 with width 22 one export takes
-// 72.5 ms against 73.0 (128 dims) and 80.8 (256); at width 28, 87.2 / 92.2 /
-// 98.4 ms.  KNNG_RERANK_SLICE = 64, 128 or 256 overrides it.
+// 256-dim slices by default: at 6.35M x 1024 with width 22 one export takes
+// 49 ms against 66 (64 dims), 52 (128) and 50 (512) with the row queue.
+// KNNG_RERANK_SLICE = 64, 128, 256 or 512 overrides it.
 int rerank_chunks(int d) {
-  int dims = 64;
+  int dims = 256;
   if (const char* e = std::getenv("KNNG_RERANK_SLICE")) {
     const int v = std::atoi(e);
-    if (v == 64 || v == 128 || v == 256) dims = v;
+    if (v == 64 || v == 128 || v == 256 || v == 512) dims = v;
   }
   (void)d;
   return dims / 64;
@@ -3947,12 +3990,25 @@ int launch_rerank(knng_ctx* c, int cur, int64_t nrows, int64_t base, int k, int3
   const int S = (c->d + 64 * ch - 1) / (64 * ch);
   int rc;
   if (S > 1 && (rc = dalloc(c, &c->rr_part, size_t(c->n) * K, "rerank partial sums"))) return rc;
+  // each pass's 8 row-queue counters, 256 B apart, cleared by a kernel (no
+  // memset node if a caller captures the export)
+  if ((rc = dalloc(c, &c->rr_q, size_t(kRerankMaxPasses) * 8 * 64, "rerank row queues"))) return rc;
+  if (S > kRerankMaxPasses) return fail(c, KNNG_EINVAL, "knng_export: too many rerank slices");
+  hipLaunchKernelGGL(k_zero_u64, dim3(16), dim3(256), 0, c->stream,
+                     reinterpret_cast<unsigned long long*>(c->rr_q), S * 8 * 32);
+  const int64_t steps = (nrows + kRerankWaves - 1) / kRerankWaves;
   auto run = [&](auto first, auto mid, auto last, auto only) {
     for (int s = 0; s < S; ++s) {
       auto launch = [&](auto kern) {
-        const int grid = std::min(rerank_grid(c, nrows), resident_grid(c, kern));
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, x, c->d, nrows, K, R, k, s,
-                           c->gid[cur], c->rr_part, dk, dd, order, base);
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(kern),
+                                                         64 * kRerankWaves, 0) != hipSuccess || nb < 1)
+          nb = 1;
+        // every block resident; no more blocks than steps (rounded up to the 8 XCDs)
+        const int64_t want = std::min<int64_t>(int64_t(c->num_cu) * nb, (steps + 7) / 8 * 8);
+        const int grid = int(std::max<int64_t>(8, want / 8 * 8));
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kRerankWaves), 0, c->stream, x, c->d, nrows, K, R, k,
+                           s, c->gid[cur], c->rr_part, dk, dd, order, base, c->rr_q + size_t(s) * 8 * 64);
       };
       if (S == 1)
         launch(only);
@@ -3970,9 +4026,12 @@ int launch_rerank(knng_ctx* c, int cur, int64_t nrows, int64_t base, int k, int3
   else if (ch == 2)
     run(k_rerank_slice<2, true, false>, k_rerank_slice<2, false, false>, k_rerank_slice<2, false, true>,
         k_rerank_slice<2, true, true>);
-  else
+  else if (ch == 4)
     run(k_rerank_slice<4, true, false>, k_rerank_slice<4, false, false>, k_rerank_slice<4, false, true>,
         k_rerank_slice<4, true, true>);
+  else
+    run(k_rerank_slice<8, true, false>, k_rerank_slice<8, false, false>, k_rerank_slice<8, false, true>,
+        k_rerank_slice<8, true, true>);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -4022,10 +4081,6 @@ int scratch_for_device(int dev, unsigned long long** ptr, int* num_cu) {
 }
 
 // v[0..m) = 0 (a kernel, not hipMemsetAsync, so a captured graph replays it correctly)
-__global__ void k_zero_u64(unsigned long long* __restrict__ v, int m) {
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x) v[j] = 0ull;
-}
-
 }  // namespace
 
 extern "C" {
@@ -4153,6 +4208,7 @@ int knng_destroy(knng_ctx* c) {
   dfree(&c->ex_knns);
   dfree(&c->ex_dists);
   dfree(&c->rr_part);
+  dfree(&c->rr_q);
   dfree(&c->rv_slots);
   dfree(&c->rv_id);
   dfree(&c->rv_mask);

@@ -86,6 +86,7 @@ struct knng_ctx {
   float* ex_dists = nullptr;
   int64_t ex_cap = 0;
   float* rr_part = nullptr;   // sliced FP32 rerank: running sums [n][R], R <= k_build
+  unsigned int* rr_q = nullptr;   // FP32 rerank: per-pass, per-XCD row-queue counters
 
   // independent builds packed into one context (knng_build_batch): segment s
   // is rows [seg_host[s], seg_host[s+1]); trees, leaves and edges never

@@ -34,9 +34,9 @@ def say(*a):
 
 # The form bench.py times at 6.35M rows: 4 trees seeded by 4 k-means cell
 # partitions with 2 seeds per target cell, leaves of up to 2048 rows, one full
-# join iteration and one under the new/old rule, FP32 rerank of the 22 best.
+# join iteration and one under the new/old rule, FP32 rerank of the 28 best.
 BENCH_FORM = dict(num_trees=4, leaf_size=2048, cells=-1, cell_partitions=4, cell_oversample=2,
-                  incremental=1, max_iters=2, rerank_width=22)
+                  incremental=1, max_iters=2, rerank_width=28)
 
 
 def _fp64_rel_err(torch, x, rows, knns_1b, dists):
@@ -86,8 +86,8 @@ def bench_form_case():
 @pytest.mark.timeout(600)
 def test_bench_form_bitexact(torch_cuda, bench_form_case):
     """The exact form of the headline measurement, whole build bit-exact against
-    the oracle's device mode; plain export bit-exact; the width-22 FP32 rerank
-    within 1e-5 of FP64 and selecting the oracle's FP64 top 15 of the 22."""
+    the oracle's device mode; plain export bit-exact; the width-28 FP32 rerank
+    within 1e-5 of FP64 and selecting the oracle's FP64 top 15 of the 28."""
     torch = torch_cuda
     x, q, sk, o_ids, o_sc, o_rates = bench_form_case
     n = x.shape[0]
@@ -104,7 +104,7 @@ def test_bench_form_bitexact(torch_cuda, bench_form_case):
     e_knns, e_dists = oracle.export(o_ids, o_sc)
     assert np.array_equal(knns, e_knns) and np.array_equal(dists, e_dists)
     rk, rd = b.export(15, rerank=True)
-    o_rk, o_rd = oracle.rerank_device(x, o_ids, 22, 15)
+    o_rk, o_rd = oracle.rerank_device(x, o_ids, 28, 15)
     X = x.astype(np.float64)
     i = np.repeat(np.arange(n), 15)
     exact = (X[i] * X[rk.reshape(-1) - 1]).sum(1).reshape(n, 15)
@@ -113,8 +113,8 @@ def test_bench_form_bitexact(torch_cuda, bench_form_case):
     for r in np.where(~same)[0]:   # only where FP32 cannot tell FP64 neighbours apart
         assert np.allclose(np.sort(rd[r]), np.sort(o_rd[r]), rtol=REL_TOL, atol=0), r
     assert same.mean() > 0.99
-    # the width-22 rerank only ever picks from the list's first 22 entries
-    first = np.sort(o_ids[:, :22], axis=1)
+    # the width-28 rerank only ever picks from the list's first 28 entries
+    first = np.sort(o_ids[:, :28], axis=1)
     for r in range(0, n, 97):
         assert np.isin(rk[r] - 1, first[r]).all()
     b.close()

@@ -385,10 +385,12 @@ def test_full_build_bitexact(torch_cuda, n, rule):
     assert np.array_equal(knns, e_knns) and np.array_equal(dists, e_dists)
 
 
-# slices of 64 dims (the default: 16 / 12 / 5 / 1 passes at d = 1024 / 768 /
-# 320 / 64), 128 and 256 dims (320: a partial last slice; 64: a single pass)
+# slices of 64 dims (16 / 12 / 5 / 1 passes at d = 1024 / 768 / 320 / 64),
+# 128, 256 (the default) and 512 dims (320: a partial last slice; 64, and 320
+# in 512-dim slices: a single pass)
 @pytest.mark.parametrize("d,slice_dims", [(1024, 64), (768, 64), (320, 64), (64, 64), (1024, 128),
-                                          (320, 128), (1024, 256), (768, 256), (320, 256)])
+                                          (320, 128), (1024, 256), (768, 256), (320, 256), (64, 256),
+                                          (1024, 512), (768, 512), (320, 512)])
 def test_rerank_within_fp64_tolerance(torch_cuda, monkeypatch, d, slice_dims):
     say("rerank", d, slice_dims)
     monkeypatch.setenv("KNNG_RERANK_SLICE", str(slice_dims))
