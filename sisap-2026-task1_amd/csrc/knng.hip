@@ -2340,16 +2340,29 @@ constexpr int kJBatch = 8;        // neighbour lists whose ids k_jcand loads per
 // blocks of 4 waves, dealt round-robin to the 8 XCDs: XCD x takes the
 // contiguous slice [x*nwork/8, (x+1)*nwork/8) of the work list and its waves
 // stride through it together (cluster-mates in flight share that XCD's L2)
-struct WaveSlice {
-  int64_t j0, hi, step;
+// Rows of the wave-per-row join phases: XCD x = block % 8 takes the
+// contiguous eighth [x n / 8, (x + 1) n / 8) of the work list (tree 0's leaf
+// order), and each wave takes its next row from that eighth's queue counter
+// rowq[32 x + slot] (lane 0's atomic, issued a row ahead), so the rows in
+// flight on an XCD stay one window of consecutive cluster-mates whose
+// neighbour lists, sketches and rows share the XCD's L2.  (With a fixed
+// stride per wave the waves drift apart over a launch and the window widens.)
+// Slots 0 and 1 belong to k_join; k_sum_counters clears them all after each
+// iteration.
+constexpr int kQJcand = 2, kQJdot = 3;
+struct WaveQueue {
+  unsigned long long* ctr;
+  int64_t lo, hi;
 };
-__device__ __forceinline__ WaveSlice wave_slice(int64_t nwork) {
-  const int xcd = blockIdx.x & 7, w = threadIdx.x >> 6;
-  WaveSlice ws;
-  ws.j0 = nwork * xcd / 8 + int64_t(blockIdx.x >> 3) * 4 + w;
-  ws.hi = nwork * (xcd + 1) / 8;
-  ws.step = int64_t(gridDim.x >> 3) * 4;
-  return ws;
+__device__ __forceinline__ WaveQueue wave_queue(int64_t nwork, unsigned long long* rowq, int slot) {
+  const int xcd = blockIdx.x & 7;
+  return {rowq + 32 * xcd + slot, nwork * xcd / 8, nwork * (xcd + 1) / 8};
+}
+__device__ __forceinline__ unsigned int wq_claim(const WaveQueue& q) {
+  return (threadIdx.x & 63) == 0 ? unsigned(atomicAdd(q.ctr, 1ull)) : 0u;
+}
+__device__ __forceinline__ int64_t wq_row(const WaveQueue& q, unsigned int t) {
+  return q.lo + int64_t(unsigned(__builtin_amdgcn_readfirstlane(int(t))));
 }
 
 template <bool REV>   // the reverse-candidate stage compiled in (used when R > 0)
@@ -2358,8 +2371,8 @@ __global__ __launch_bounds__(256) void k_jcand(
     const uint64_t* __restrict__ fresh_in, const uint8_t* __restrict__ masks_in,
     const int32_t* __restrict__ order, int64_t nwork, int32_t* __restrict__ cbuf,
     int32_t* __restrict__ ccount, int32_t* __restrict__ ovf, unsigned long long* __restrict__ ovf_n,
-    unsigned long long* __restrict__ counters, const int32_t* __restrict__ ids_rv,
-    const uint8_t* __restrict__ rv_mask, int R) {
+    unsigned long long* __restrict__ counters, unsigned long long* __restrict__ rowq,
+    const int32_t* __restrict__ ids_rv, const uint8_t* __restrict__ rv_mask, int R) {
   __shared__ int32_t hs_w[4][1 << kJHashBits];   // dedup hash, then (low half) sketch similarities
   __shared__ int32_t cid_w[4][kJCap];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -2368,8 +2381,10 @@ __global__ __launch_bounds__(256) void k_jcand(
   const int words = d >> 6;
   const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long n_u = 0, n_sk = 0, n_lk = 0;   // candidates, sketched candidates, paths dropped
-  const WaveSlice ws = wave_slice(nwork);
-  for (int64_t j = ws.j0; j < ws.hi; j += ws.step) {
+  const WaveQueue wq = wave_queue(nwork, rowq, kQJcand);
+  unsigned int tk = wq_claim(wq);
+  for (int64_t j = wq_row(wq, tk); j < wq.hi; j = wq_row(wq, tk)) {
+    tk = wq_claim(wq);   // the wave's next row, in flight during this one
     const int64_t i = order ? int64_t(order[j]) : j;
     for (int s = lane; s < (1 << kJHashBits); s += 64) hs[s] = -1;
     const int32_t oid = lane < K ? ids_in[i * K + lane] : -1;
@@ -2561,7 +2576,7 @@ __global__ __launch_bounds__(256, 4) void k_jdot_merge(
     uint64_t* __restrict__ fresh_out, const uint8_t* __restrict__ masks_in, uint8_t* __restrict__ masks_out,
     const int32_t* __restrict__ leafkey, int T, int TP, const int32_t* __restrict__ order, int64_t nwork,
     const int32_t* __restrict__ cbuf, const int32_t* __restrict__ ccount,
-    unsigned long long* __restrict__ counters) {
+    unsigned long long* __restrict__ counters, unsigned long long* __restrict__ rowq) {
   __shared__ uint64_t key_w[4][kJCap];   // candidate keys, then the survivors' keys
   __shared__ uint64_t okey_w[4][kMaxK];
   __shared__ int32_t nid_w[4][kMaxK], nsc_w[4][kMaxK], nfr_w[4][kMaxK];
@@ -2580,8 +2595,10 @@ __global__ __launch_bounds__(256, 4) void k_jdot_merge(
   const int g = lane >> 4, s16 = lane & 15;
   const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long n_chg = 0, n_ns = 0, n_ins = 0;
-  const WaveSlice ws = wave_slice(nwork);
-  for (int64_t j = ws.j0; j < ws.hi; j += ws.step) {
+  const WaveQueue wq = wave_queue(nwork, rowq, kQJdot);
+  unsigned int tk = wq_claim(wq);
+  for (int64_t j = wq_row(wq, tk); j < wq.hi; j = wq_row(wq, tk)) {
+    tk = wq_claim(wq);   // the wave's next row, in flight during this one
     const int S = ccount[j];
     if (S < 0) continue;   // k_join runs this row
     const int64_t i = order ? int64_t(order[j]) : j;
@@ -3776,22 +3793,24 @@ int launch_join_phased(knng_ctx* c, int a, int b, const int32_t* order) {
   const int T = int(c->p.num_trees);
   // every entry still new (no join iteration yet): the new/old rule takes every pair
   const uint64_t* fin = (c->p.incremental && !c->fresh_all) ? c->fresh[a] : nullptr;
+  // row-queue counters (zero: k_sum_counters clears them after each iteration)
+  unsigned long long* rowq = c->counters + kCounterFields * kNumCounters;
   if (c->p.reverse_width > 0)
     hipLaunchKernelGGL(k_jcand<true>, dim3(resident_grid(c, k_jcand<true>)), dim3(256), 0, c->stream, c->sk, c->d,
                        K, M, c->gid[a], fin, mk_in, order, n, c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters,
-                       KNNG_REV_ARGS(c));
+                       rowq, KNNG_REV_ARGS(c));
   else
     hipLaunchKernelGGL(k_jcand<false>, dim3(resident_grid(c, k_jcand<false>)), dim3(256), 0, c->stream, c->sk, c->d,
                        K, M, c->gid[a], fin, mk_in, order, n, c->jcbuf, c->jccount, c->ovf, c->ovf_n, c->counters,
-                       KNNG_REV_ARGS(c));
+                       rowq, KNNG_REV_ARGS(c));
   if (c->d == kMaxDim)
     hipLaunchKernelGGL(k_jdot_merge<kMaxDim>, dim3(resident_grid(c, k_jdot_merge<kMaxDim>)), dim3(256), 0, c->stream,
                        c->q, c->d, K, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->fresh[b], mk_in, mk_out, leafkey,
-                       T, c->leaf_tp, order, n, c->jcbuf, c->jccount, c->counters);
+                       T, c->leaf_tp, order, n, c->jcbuf, c->jccount, c->counters, rowq);
   else
     hipLaunchKernelGGL(k_jdot_merge<0>, dim3(resident_grid(c, k_jdot_merge<0>)), dim3(256), 0, c->stream, c->q, c->d,
                        K, c->gid[a], c->gsc[a], c->gid[b], c->gsc[b], c->fresh[b], mk_in, mk_out, leafkey, T,
-                       c->leaf_tp, order, n, c->jcbuf, c->jccount, c->counters);
+                       c->leaf_tp, order, n, c->jcbuf, c->jccount, c->counters, rowq);
   HIPCHK(c, hipGetLastError());
   return launch_join(c, 0, a, b, c->ovf, reinterpret_cast<const int64_t*>(c->ovf_n), nullptr, 0, nullptr);
 }
