@@ -359,14 +359,23 @@ __global__ __launch_bounds__(256) void k_leaf_keys(const int4* __restrict__ leav
   }
 }
 
-// the mask byte of every list entry of the graph gid (see edge_mask)
+// the mask byte of every list entry of the graph gid (see edge_mask): one
+// wave per row, lane k for entry k (K <= 64), rows in `order` (tree 0's leaf
+// order; null: index order) with XCD x = block % 8 taking the contiguous
+// eighth x of it, so the rows in flight on an XCD are cluster-mates whose
+// neighbours' leaf keys share its L2
 __global__ __launch_bounds__(256) void k_edge_masks(const int32_t* __restrict__ gid, int64_t n, int K,
                                                     const int32_t* __restrict__ leafkey, int T, int TP,
-                                                    uint8_t* __restrict__ gmask) {
-  const int64_t total = n * K;
-  for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
-    const int64_t r = e / K;
-    gmask[e] = uint8_t(edge_mask(leafkey + r * TP, leafkey + int64_t(gid[e]) * TP, T));
+                                                    uint8_t* __restrict__ gmask,
+                                                    const int32_t* __restrict__ order) {
+  const int lane = threadIdx.x & 63;
+  const int xcd = blockIdx.x & 7;
+  const int64_t hi = n * (xcd + 1) / 8, step = int64_t(gridDim.x >> 3) * 4;
+  for (int64_t j = n * xcd / 8 + int64_t(blockIdx.x >> 3) * 4 + (threadIdx.x >> 6); j < hi; j += step) {
+    const int64_t r = order ? int64_t(order[j]) : j;
+    if (lane < K)
+      gmask[r * K + lane] =
+          uint8_t(edge_mask(leafkey + r * TP, leafkey + int64_t(gid[r * K + lane]) * TP, T));
   }
 }
 
@@ -1362,15 +1371,21 @@ constexpr int kSelCap = 512;   // compacted candidates per wave in k_leaf_select
 // Top-R mates (R = min(size - 1, 64)) of one member row of a big leaf by
 // (score desc, id asc), self excluded, from its scores row; one wave per row,
 // rows (bi, rr) of the chunk's items taken in turn.  The member's 64 slots of
-// this tree get the keys in column order (0 beyond R).  The entries at or
-// above the smallest lane maximum (at least 64 of them when every lane holds
-// one) are compacted, in column order, into LDS (typically a few hundred of
-// the 2048), and the selection runs on those, 8 per lane instead of 32; a row
-// with more than kSelCap such entries selects from all of them.
+// this tree get the keys (0 beyond R; the merge sorts them).  Each lane loads
+// four consecutive columns per 16-byte load.  The entries at or above the
+// smallest lane maximum (at least 64 of them when every lane holds one) are
+// compacted into LDS (typically a few hundred of the 2048), and the selection
+// runs on those, 8 per lane instead of 32.  When more than kSelCap entries
+// reach it, the cut rises to the largest score that at least R entries reach
+// (the R-th largest score: every selected entry reaches it); a row that still
+// holds more than kSelCap entries at that cut (ties) is appended to `slow` for
+// k_leaf_select_slow.  Keeping that case out of this kernel keeps its
+// registers at 8 waves per SIMD.
 __global__ __launch_bounds__(256) void k_leaf_select(int64_t n, int T, const int32_t* __restrict__ perm,
                                                      const int4* __restrict__ items, int64_t item0,
                                                      int64_t nitems, const int32_t* __restrict__ scratch,
-                                                     int mpad, uint64_t* __restrict__ cand) {
+                                                     int mpad, uint64_t* __restrict__ cand,
+                                                     int32_t* __restrict__ slow, unsigned int* __restrict__ nslow) {
   constexpr int kPer = kMaxLeaf / 64, kC = kSelCap / 64;
   __shared__ int2 cbuf[4][kSelCap];   // (score, column) of this wave's compacted entries
   const int lane = threadIdx.x & 63;
@@ -1387,13 +1402,26 @@ __global__ __launch_bounds__(256) void k_leaf_select(int64_t n, int T, const int
     const int32_t* mid = perm + int64_t(t) * n + start;
     const int R = (size - 1) < kLeafSlots ? (size - 1) : kLeafSlots;
     const int32_t* row = scratch + (bi * kBigRows + rr) * mpad;
+    // v[4 e + k] holds column 4 lane + 256 e + k (mpad is a multiple of 64).
+    // lane4 is opaque to the compiler, so the 32 per-element column values
+    // are not hoisted out of the row loop into registers of their own.
+    int lane4 = 4 * lane;
+    asm volatile("" : "+v"(lane4));
+    const int lim = size - lane4, self = m - lane4;   // column c: c - lane4 < lim, != self
     int32_t v[kPer];
     int lmax = INT32_MIN;
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      const int col = lane + 64 * e;
-      v[e] = (col < size && col != m) ? row[col] : INT32_MIN;   // scores are > -2^25
-      lmax = max(lmax, v[e]);
+    for (int e = 0; e < kPer / 4; ++e) {
+      // columns past mpad read the next row (or the kMaxLeaf pad behind the
+      // last one) and are masked below
+      const int4 x4 = *reinterpret_cast<const int4*>(row + lane4 + 256 * e);
+      const int32_t xs[4] = {x4.x, x4.y, x4.z, x4.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int o = 256 * e + k;
+        v[4 * e + k] = (o < lim && o != self) ? xs[k] : INT32_MIN;   // scores are > -2^25
+        lmax = max(lmax, v[4 * e + k]);
+      }
     }
     // When every lane holds a valid entry, the 64 lane maxima are distinct
     // entries, so their minimum has at least 64 >= R entries at or above it;
@@ -1404,36 +1432,91 @@ __global__ __launch_bounds__(256) void k_leaf_select(int64_t n, int T, const int
     int c0 = 0;
 #pragma unroll
     for (int e = 0; e < kPer; ++e) c0 += ballot_count(v[e] >= lo);
-    uint64_t* dst = cand + int64_t(mid[m]) * (T * kLeafSlots) + t * kLeafSlots;
-    if (c0 <= kSelCap) {
-      int base = 0;
+    if (c0 > kSelCap) {
+      // largest cut with at least R entries at or above it
+      int a = lo, b = hi;
+      while (a < b) {
+        const int md = a + (b - a + 1) / 2;
+        int cnt = 0;
 #pragma unroll
-      for (int e = 0; e < kPer; ++e) {
-        const bool sel = v[e] >= lo;
-        const uint64_t bal = __ballot(sel);
-        if (sel) cb[base + __popcll(bal & ((1ull << lane) - 1ull))] = make_int2(v[e], lane + 64 * e);
-        base += __popcll(bal);
+        for (int e = 0; e < kPer; ++e) cnt += ballot_count(v[e] >= md);
+        if (cnt >= R)
+          a = md;
+        else
+          b = md - 1;
       }
-      __builtin_amdgcn_wave_barrier();
-      int32_t cv[kC], ids[kC];
+      lo = a;
+      c0 = 0;
 #pragma unroll
-      for (int k = 0; k < kC; ++k) {
-        const int i = lane + 64 * k;
-        const int2 p = i < c0 ? cb[i] : make_int2(INT32_MIN, 0);
-        cv[k] = p.x;
-        ids[k] = i < c0 ? mid[p.y] : 0;
+      for (int e = 0; e < kPer; ++e) c0 += ballot_count(v[e] >= lo);
+      if (c0 > kSelCap) {
+        if (lane == 0) slow[atomicAdd(nslow, 1u)] = int32_t(wr);
+        continue;
       }
-      __builtin_amdgcn_wave_barrier();   // cbuf is free for the next row once read
-      leaf_top_r<kC>(cv, ids, lo, hi, R, n, dst, lane);
-    } else {
-      int32_t ids[kPer];
-#pragma unroll
-      for (int e = 0; e < kPer; ++e) {
-        const int col = lane + 64 * e;
-        ids[e] = col < size ? mid[col] : 0;
-      }
-      leaf_top_r<kPer>(v, ids, lo, hi, R, n, dst, lane);
     }
+    int base = 0;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const bool sel = v[e] >= lo;
+      const uint64_t bal = __ballot(sel);
+      if (sel) cb[base + __popcll(bal & ((1ull << lane) - 1ull))] = make_int2(v[e], lane4 + 256 * (e >> 2) + (e & 3));
+      base += __popcll(bal);
+    }
+    __builtin_amdgcn_wave_barrier();
+    int32_t cv[kC], ids[kC];
+#pragma unroll
+    for (int k = 0; k < kC; ++k) {
+      const int i = lane + 64 * k;
+      const int2 p = i < c0 ? cb[i] : make_int2(INT32_MIN, 0);
+      cv[k] = p.x;
+      ids[k] = i < c0 ? mid[p.y] : 0;
+    }
+    __builtin_amdgcn_wave_barrier();   // cbuf is free for the next row once read
+    uint64_t* dst = cand + int64_t(mid[m]) * (T * kLeafSlots) + t * kLeafSlots;
+    leaf_top_r<kC>(cv, ids, lo, hi, R, n, dst, lane);
+  }
+}
+
+// The rows k_leaf_select passed on (more than kSelCap entries tie at the
+// R-th largest score): the same selection over all of the row's entries.
+// One wave per listed row.
+__global__ __launch_bounds__(256) void k_leaf_select_slow(int64_t n, int T, const int32_t* __restrict__ perm,
+                                                          const int4* __restrict__ items, int64_t item0,
+                                                          const int32_t* __restrict__ scratch, int mpad,
+                                                          uint64_t* __restrict__ cand,
+                                                          const int32_t* __restrict__ slow,
+                                                          const unsigned int* __restrict__ nslow) {
+  constexpr int kPer = kMaxLeaf / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  const int64_t cnt = *nslow;
+  for (int64_t s = wave; s < cnt; s += nwaves) {
+    const int64_t wr = slow[s];
+    const int64_t bi = wr / kBigRows;
+    const int rr = int(wr - bi * kBigRows);
+    const int4 item = items[item0 + bi];
+    const int t = item.x, start = item.y, size = item.z;
+    const int m = item.w + rr;
+    const int32_t* mid = perm + int64_t(t) * n + start;
+    const int R = (size - 1) < kLeafSlots ? (size - 1) : kLeafSlots;
+    const int32_t* row = scratch + (bi * kBigRows + rr) * mpad;
+    int32_t v[kPer], ids[kPer];
+    int lo = INT32_MAX, hi = INT32_MIN;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const int col = lane + 64 * e;
+      v[e] = (col < size && col != m) ? row[col] : INT32_MIN;
+      ids[e] = col < size ? mid[col] : 0;
+      if (v[e] != INT32_MIN) {
+        lo = min(lo, v[e]);
+        hi = max(hi, v[e]);
+      }
+    }
+    lo = wave_min_i(lo);
+    hi = wave_max_i(hi);
+    uint64_t* dst = cand + int64_t(mid[m]) * (T * kLeafSlots) + t * kLeafSlots;
+    leaf_top_r<kPer>(v, ids, lo, hi, R, n, dst, lane);
   }
 }
 
@@ -4556,13 +4639,22 @@ int knng_build_init(knng_ctx* c) {
       int4*& ditems = items_buf.p;
       int32_t*& scratch = scores_buf.p;
       if ((rc = dalloc(c, &ditems, items.size(), "big-leaf items")) ||
-          (rc = dalloc(c, &scratch, size_t(chunk * per_item), "big-leaf scores")))
+          (rc = dalloc(c, &scratch, size_t(chunk * per_item + kMaxLeaf), "big-leaf scores")))
         return rc;
       HIPCHK(c, hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
                                c->stream));
+      // rows the selection passes on to k_leaf_select_slow, per chunk
+      ScopedDev<int32_t> slow_buf;
+      ScopedDev<unsigned int> nslow_buf;
+      int32_t*& slow = slow_buf.p;
+      unsigned int*& nslow = nslow_buf.p;
+      if ((rc = dalloc(c, &slow, size_t(chunk * kBigRows), "leaf selection slow rows")) ||
+          (rc = dalloc(c, &nslow, 1, "leaf selection slow count")))
+        return rc;
       hipError_t e = hipSuccess;
       for (int64_t i0 = 0; i0 < int64_t(items.size()) && e == hipSuccess; i0 += chunk) {
         const int64_t ni = std::min<int64_t>(chunk, int64_t(items.size()) - i0);
+        HIPCHK(c, hipMemsetAsync(nslow, 0, sizeof(unsigned int), c->stream));
         const int sgrid = int(std::min<int64_t>(ni, int64_t(c->num_cu) * 2));
         const int wgrid = int(std::min<int64_t>(ni, int64_t(c->num_cu) * 4));   // two resident per CU
         if (c->d == kMaxDim)
@@ -4572,7 +4664,9 @@ int knng_build_init(knng_ctx* c) {
           hipLaunchKernelGGL((k_leaf_scores<0>), dim3(sgrid), dim3(256), 0, c->stream, c->q, c->d, n, c->perm,
                              ditems, i0, ni, scratch, mpad);
         hipLaunchKernelGGL(k_leaf_select, dim3(grid_for(c, ni * kBigRows, 4, 32)), dim3(256), 0, c->stream, n, T,
-                           c->perm, ditems, i0, ni, scratch, mpad, cand);
+                           c->perm, ditems, i0, ni, scratch, mpad, cand, slow, nslow);
+        hipLaunchKernelGGL(k_leaf_select_slow, dim3(c->num_cu), dim3(256), 0, c->stream, n, T, c->perm, ditems,
+                           i0, scratch, mpad, cand, slow, nslow);
         e = hipGetLastError();
       }
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -4609,8 +4703,8 @@ int knng_build_init(knng_ctx* c) {
   if (!c->leafkey && (rc = dalloc(c, &c->leafkey, size_t(n) * c->leaf_tp, "leaf keys"))) return rc;
   hipLaunchKernelGGL(k_leaf_keys, dim3(grid_for(c, c->nleaves, 4, 16)), dim3(256), 0, c->stream, c->leaves,
                      c->nleaves, c->perm, n, c->leaf_tp, c->leafkey);
-  hipLaunchKernelGGL(k_edge_masks, dim3(grid_for(c, n * K, 256, 16)), dim3(256), 0, c->stream, c->gid[0], n, K,
-                     c->leafkey, T, c->leaf_tp, c->gmask[0]);
+  hipLaunchKernelGGL(k_edge_masks, dim3(rerank_grid(c, n)), dim3(256), 0, c->stream, c->gid[0], n, K, c->leafkey,
+                     T, c->leaf_tp, c->gmask[0], c->perm);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));

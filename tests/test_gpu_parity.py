@@ -136,6 +136,27 @@ def test_forest_and_init_bitexact(torch_cuda, n, d, T, leaf, cells, parts):
     assert np.array_equal(sc, o_sc)
 
 
+def test_leaf_select_ties_bitexact(torch_cuda):
+    """1100 copies of one row in leaves of about 2000 rows: at least 550 of
+    them share a leaf, where each copy's scores tie by the hundred at its top,
+    so the selection's cut keeps more than 512 entries and those rows go to
+    the slow kernel (the tie is broken by id)."""
+    say("leaf select ties")
+    n, d = 4000, 256
+    x = synth.gaussian_mixture(77, n, d, 8)
+    x[1000:2100] = x[1000]
+    q, _ = oracle.quantize(x)
+    b = _builder(x, num_trees=2, leaf_size=2048, k_build=50, k_eval=15, cells=0)
+    b.build_forest()
+    perm, leaves = b.forest()
+    assert leaves[:, 2].max() > 512
+    b.build_init()
+    ids, sc = b.graph()
+    o_ids, o_sc = oracle.init_device(q, perm, leaves, 50)
+    assert np.array_equal(ids, o_ids) and np.array_equal(sc, o_sc)
+    b.close()
+
+
 # both leaf-tile shapes for leaves of at most 64 rows (block per leaf, wave
 # per leaf): leaves of at most 32 rows (one tile) and of 33-64 rows (the
 # symmetric three-tile path), d from one 32-dim step group to 1024
