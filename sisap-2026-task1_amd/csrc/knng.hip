@@ -48,9 +48,12 @@ namespace {
 // sketch words 4g..4g+3, interleaved by 4 (bit 4i+j <- lane i, element j).
 // ---------------------------------------------------------------------------
 
+// xcopy (null: none) also receives the float32 rows: a server's small launch
+// quantises straight from pinned host staging and keeps a device copy of the
+// rows for the search's FP32 rerank, in place of a separate upload
 __global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ x, int64_t nrows,
                                                   int d, int8_t* __restrict__ q,
-                                                  uint64_t* __restrict__ sk) {
+                                                  uint64_t* __restrict__ sk, float* __restrict__ xcopy) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
@@ -65,6 +68,13 @@ __global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ x, i
     for (int g = 0; g < kMaxDim / 256; ++g) {
       const int col = g * 256 + lane * 4;
       vq[g] = col < d ? *reinterpret_cast<const float4*>(xr + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (xcopy) {
+#pragma unroll
+      for (int g = 0; g < kMaxDim / 256; ++g) {
+        const int col = g * 256 + lane * 4;
+        if (col < d) *reinterpret_cast<float4*>(xcopy + r * d + col) = vq[g];
+      }
     }
 #pragma unroll
     for (int g = 0; g < kMaxDim / 256; ++g) {
@@ -3091,12 +3101,12 @@ __global__ __launch_bounds__(256) void k_score_pairs(const int8_t* __restrict__ 
 namespace knng_host {
 
 int launch_quantize(hipStream_t stream, const float* dx, int64_t nrows, int d, int8_t* q,
-                    uint64_t* sk, int num_cu) {
+                    uint64_t* sk, int num_cu, float* xcopy) {
   if (nrows <= 0) return 0;
   int64_t g = (nrows + 3) / 4;
   const int64_t cap = int64_t(num_cu) * 64;
   if (g > cap) g = cap;
-  hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, stream, dx, nrows, d, q, sk);
+  hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, stream, dx, nrows, d, q, sk, xcopy);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -4976,7 +4986,7 @@ int knng_quantize_device(const float* dx, int64_t n, int32_t d, int8_t* dq, uint
   if (n == 0) return 0;
   int64_t g = (n + 3) / 4;
   if (g > 16384) g = 16384;
-  hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, 0, dx, n, d, dq, dsk);
+  hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, 0, dx, n, d, dq, dsk, static_cast<float*>(nullptr));
   if (hipGetLastError() != hipSuccess) return KNNG_EHIP;
   return hipDeviceSynchronize() == hipSuccess ? 0 : KNNG_EHIP;
 }

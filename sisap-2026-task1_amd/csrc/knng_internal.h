@@ -210,9 +210,10 @@ int build_packed(knng_ctx* c, std::vector<int32_t>* iters, std::vector<int32_t>*
                  bool* graph_used);
 int export_segment(knng_ctx* c, int s, int cur, int k, int rerank, int32_t* dk, float* dd);
 
-// K1 on an arbitrary stream (defined in knng.hip)
+// K1 on an arbitrary stream (defined in knng.hip); xcopy (optional) also
+// receives the float32 rows
 int launch_quantize(hipStream_t stream, const float* dx, int64_t nrows, int d, int8_t* q,
-                    uint64_t* sk, int num_cu);
+                    uint64_t* sk, int num_cu, float* xcopy = nullptr);
 // route int8 rows to cells: out[j] = argmax_c float(<q_r, Z_c>) * w[c] for the rows
 // r = row_off + j * row_stride (knng.hip); exclude_self skips c == j, score gets the best value.
 // With partial arrays pbest/pidx of part_cap entries each, a launch with few rows splits the

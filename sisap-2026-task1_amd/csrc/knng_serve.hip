@@ -93,8 +93,11 @@ __global__ __launch_bounds__(64) void k_route_small(const int8_t* __restrict__ q
                                                     const int8_t* __restrict__ Z, const float* __restrict__ W,
                                                     int64_t ncells, const int32_t* __restrict__ cells_of,
                                                     float* __restrict__ pbest, int32_t* __restrict__ pidx,
-                                                    int32_t* __restrict__ ctr, int32_t* __restrict__ out) {
+                                                    int32_t* __restrict__ ctr, int32_t* __restrict__ out,
+                                                    int express) {
   __shared__ __attribute__((aligned(16))) int8_t qs[kRouteQ * kMaxDim];
+  // an express launch's waves take issue priority over resident bulk waves
+  if (express) __builtin_amdgcn_s_setprio(2);
   const int lane = threadIdx.x;
   const int s = blockIdx.x, qg = blockIdx.y, g = blockIdx.z, smax = gridDim.x;
   const int C = cells_of[g];
@@ -221,12 +224,12 @@ __global__ __launch_bounds__(256) void k_search(
     for (int c = tid; c < nch; c += 256)
       *reinterpret_cast<v4i*>(qrow + 16 * c) = *reinterpret_cast<const v4i*>(qq + int64_t(qi) * d + 16 * c);
     __syncthreads();
-    v4i own[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    // the query's own chunks are read from LDS at each use: held in registers
+    // they would cost 16 VGPRs for the whole query
+    auto own = [&](int c) -> v4i {
       const int chn = s16 + 16 * c;
-      own[c] = chn < nch ? *reinterpret_cast<const v4i*>(qrow + 16 * chn) : v4i{0, 0, 0, 0};
-    }
+      return chn < nch ? *reinterpret_cast<const v4i*>(qrow + 16 * chn) : v4i{0, 0, 0, 0};
+    };
 
     // ---- entry points: descend trees w, w+4, ... to a leaf; with cells, each
     // tree starts at the subtree of the query's cell in that tree's partition
@@ -291,8 +294,9 @@ __global__ __launch_bounds__(256) void k_search(
         for (int c = 0; c < 4; ++c) {
           const int chn = s16 + 16 * c;
           if (chn < nch) {
-            if (ra) da = dot16_i8(*reinterpret_cast<const v4i*>(ra + 16 * chn), own[c], da);
-            if (rb) db = dot16_i8(*reinterpret_cast<const v4i*>(rb + 16 * chn), own[c], db);
+            const v4i oc = own(c);
+            if (ra) da = dot16_i8(*reinterpret_cast<const v4i*>(ra + 16 * chn), oc, da);
+            if (rb) db = dot16_i8(*reinterpret_cast<const v4i*>(rb + 16 * chn), oc, db);
           }
         }
 #pragma unroll
@@ -699,9 +703,12 @@ Slot* new_slot(knng_server* s, bool express) {
 int launch_search(knng_server* s, hipStream_t stream, const float* dx, int8_t* dq, uint64_t* dsk,
                   int32_t* dcell, float* pbest, int32_t* pidx, int64_t pcap, int32_t nq, int beam,
                   int k, int rerank, int32_t* did, float* dsc, int max_blocks, int route_cu,
-                  const Slot* sl) {
+                  const Slot* sl, const float* hsrc = nullptr) {
   knng_ctx* c = s->ctx;
-  if (knng_host::launch_quantize(stream, dx, nq, s->d, dq, dsk, s->num_cu))
+  // hsrc: the queries in pinned host memory, quantised from there and copied
+  // to dx by the same kernel (no separate upload)
+  if (knng_host::launch_quantize(stream, hsrc ? hsrc : dx, nq, s->d, dq, dsk, s->num_cu,
+                                 hsrc ? const_cast<float*>(dx) : nullptr))
     return sfail(s, KNNG_EHIP, "quantise launch failed");
   // route the queries to cells: small batches (a slot's launch of at most
   // kRouteMaxQ queries) in one k_route_small launch over all partitions, the
@@ -709,7 +716,7 @@ int launch_search(knng_server* s, hipStream_t stream, const float* dx, int8_t* d
   if (c->nparts > 0 && sl && nq <= kRouteMaxQ) {
     hipLaunchKernelGGL(k_route_small, dim3(s->route_smax, (nq + kRouteQ - 1) / kRouteQ, c->nparts), dim3(64),
                        0, stream, dq, nq, s->d, c->cz, c->cw, int64_t(c->ncells), s->route_cells, sl->rbest,
-                       sl->ridx, sl->rctr, dcell);
+                       sl->ridx, sl->rctr, dcell, sl->express ? 1 : 0);
     SCHK(s, hipGetLastError());
   } else {
     for (int g = 0; g < c->nparts; ++g)
@@ -918,17 +925,27 @@ void launch_parts(knng_server* s, Slot* sl, std::unique_lock<std::mutex>& lk) {
   for (size_t j = 0; j < sl->parts.size(); ++j)
     memcpy(sl->hx + size_t(sl->parts[j].off) * s->d, src[j],
            size_t(sl->parts[j].nrows) * s->d * sizeof(float));
-  bool ok = hipMemcpyAsync(sl->dx, sl->hx, size_t(m) * s->d * sizeof(float), hipMemcpyHostToDevice,
-                           sl->stream) == hipSuccess;
   const int blocks = sl->express ? s->num_cu * 8 : s->bulk_blocks;
   const int route_cu = sl->express ? kExpressRouteCUs : s->num_cu;
-  ok = ok && !launch_search(s, sl->stream, sl->dx, sl->dq, sl->dsk, sl->dcell, sl->pbest, sl->pidx,
-                            sl->pcap, m, s->sp.beam, s->sp.k, s->sp.rerank, sl->did, sl->dsc, blocks,
-                            route_cu, sl);
-  ok = ok && hipMemcpyAsync(sl->hid, sl->did, size_t(m) * s->sp.k * sizeof(int32_t),
-                            hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
-  ok = ok && hipMemcpyAsync(sl->hsc, sl->dsc, size_t(m) * s->sp.k * sizeof(float),
-                            hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
+  bool ok;
+  if (sl->express) {
+    // zero-copy: the quantise kernel reads the pinned staging rows and the
+    // search writes ids and scores straight into the pinned result buffers,
+    // so the chain holds no copy commands (each one a dispatch that waits for
+    // free CUs behind the resident bulk searches)
+    ok = !launch_search(s, sl->stream, sl->dx, sl->dq, sl->dsk, sl->dcell, sl->pbest, sl->pidx, sl->pcap, m,
+                        s->sp.beam, s->sp.k, s->sp.rerank, sl->hid, sl->hsc, blocks, route_cu, sl, sl->hx);
+  } else {
+    ok = hipMemcpyAsync(sl->dx, sl->hx, size_t(m) * s->d * sizeof(float), hipMemcpyHostToDevice, sl->stream) ==
+         hipSuccess;
+    ok = ok && !launch_search(s, sl->stream, sl->dx, sl->dq, sl->dsk, sl->dcell, sl->pbest, sl->pidx,
+                              sl->pcap, m, s->sp.beam, s->sp.k, s->sp.rerank, sl->did, sl->dsc, blocks,
+                              route_cu, sl);
+    ok = ok && hipMemcpyAsync(sl->hid, sl->did, size_t(m) * s->sp.k * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(sl->hsc, sl->dsc, size_t(m) * s->sp.k * sizeof(float),
+                              hipMemcpyDeviceToHost, sl->stream) == hipSuccess;
+  }
   ok = ok && hipEventRecord(sl->done, sl->stream) == hipSuccess;
   lk.lock();
   if (!ok)
@@ -1132,8 +1149,8 @@ int knng_server_create(knng_server** out, knng_ctx* c, const knng_server_params*
     return bail(KNNG_ENOMEM);
   s->sp.express_rows = std::min(s->sp.express_rows, s->sp.max_batch);
   // bulk launches are not capped below the chip's block slots: a cap that left
-  // room for express blocks (~6 of the ~7 search blocks per CU) cost more
-  // bulk throughput than it saved express latency at 880K queries/s
+  // room for express blocks cost more bulk throughput than it saved express
+  // latency at 880K queries/s
   s->bulk_blocks = s->num_cu * 8;
   if (s->sp.express_slots == 0) s->sp.express_rows = 0;
   s->cu_masked = s->sp.express_rows > 0 && s->sp.express_cus > 0 && s->sp.express_cus < s->num_cu;
