@@ -1422,9 +1422,11 @@ __global__ __launch_bounds__(256) void k_leaf_select(int64_t n, int T, const int
     int lmax = INT32_MIN;
 #pragma unroll
     for (int e = 0; e < kPer / 4; ++e) {
-      // columns past mpad read the next row (or the kMaxLeaf pad behind the
-      // last one) and are masked below
-      const int4 x4 = *reinterpret_cast<const int4*>(row + lane4 + 256 * e);
+      // chunks wholly past the leaf are skipped (a wave-uniform test); columns
+      // past mpad in the last chunk read the next row (or the kMaxLeaf pad
+      // behind the last one) and are masked below
+      const int4 x4 = 256 * e < size ? *reinterpret_cast<const int4*>(row + lane4 + 256 * e)
+                                     : make_int4(INT32_MIN, INT32_MIN, INT32_MIN, INT32_MIN);
       const int32_t xs[4] = {x4.x, x4.y, x4.z, x4.w};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
