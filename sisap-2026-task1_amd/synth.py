@@ -98,6 +98,12 @@ def fixture_rows(seed: int, n: int, d: int, clusters: int) -> np.ndarray:
     return out
 
 
+def strided_sha256_torch(x, stride: int = 997) -> str:
+    """SHA-256 of rows 0, stride, 2 stride, ... of a device tensor (float32
+    bytes): a fingerprint of a generated workload without copying it whole."""
+    return sha256(x[::stride].cpu().numpy())
+
+
 def sha256(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 

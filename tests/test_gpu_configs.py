@@ -216,6 +216,28 @@ def test_small_row_join_other_dims_bitexact(torch_cuda, monkeypatch, fused):
 # ------------------------------------------------------------ c1: 100k rows
 
 @pytest.mark.timeout(600)
+# Fingerprints of the generated workloads (rows 0, 997, 1994, ... of the float32
+# rows, generated on the device by torch's seeded generator): c1, c2 and c3 as
+# bench.py and these tests draw them.
+WORKLOAD_SHA = {
+    (1, 100_000, 256): "9801fe668de6cc4b929676ef1b81384498befaa70f8b6e4cd36e938406d84ffd",
+    (2, 1_000_000, 1024): "b762e79a4e65da34acd3aa15018a3d3e46f7b06bebfc7e3c72b9959194881873",
+    (3, 6_350_000, 4096): "69a1e1e681ff787482d5224b546c7198cc67953d74cbb763a4dae33e1b5f9eeb",
+}
+
+
+@pytest.mark.parametrize("seed,n,clusters", sorted(WORKLOAD_SHA))
+def test_workload_inputs_pinned(torch_cuda, seed, n, clusters):
+    """The c1-c3 inputs are the same bytes on every run (strided SHA-256)."""
+    torch = torch_cuda
+    say("workload sha", seed, n, clusters)
+    x = synth.gaussian_mixture_torch(seed, n, 1024, clusters)
+    torch.cuda.synchronize()
+    assert synth.strided_sha256_torch(x) == WORKLOAD_SHA[(seed, n, clusters)]
+    del x
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("trees,plain_min,rerank_min", [(8, 0.80, 0.95), (16, 0.82, 0.97)])
 def test_c1_plain_forest(torch_cuda, trees, plain_min, rerank_min):
     """c1: 100,000 x 1024 float32, 256 clusters (seed 1), plain RP forest (no
