@@ -215,7 +215,6 @@ def test_small_row_join_other_dims_bitexact(torch_cuda, monkeypatch, fused):
 
 # ------------------------------------------------------------ c1: 100k rows
 
-@pytest.mark.timeout(600)
 # Fingerprints of the generated workloads (rows 0, 997, 1994, ... of the float32
 # rows, generated on the device by torch's seeded generator): c1, c2 and c3 as
 # bench.py and these tests draw them.
@@ -238,6 +237,7 @@ def test_workload_inputs_pinned(torch_cuda, seed, n, clusters):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.timeout(600)
 @pytest.mark.parametrize("trees,plain_min,rerank_min", [(8, 0.80, 0.95), (16, 0.82, 0.97)])
 def test_c1_plain_forest(torch_cuda, trees, plain_min, rerank_min):
     """c1: 100,000 x 1024 float32, 256 clusters (seed 1), plain RP forest (no
