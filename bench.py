@@ -202,6 +202,9 @@ def main():
                     help="cli workload: rows per hyperslab read (src/main.cpp:26 reads 2.2M)")
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--batch-incremental", type=int, default=1,
+                    help="batch jobs' incremental parameter (1: NN-Descent's new/old join rule; "
+                         "0: every pair in every iteration)")
     ap.add_argument("--batch-cells", type=int, default=0,
                     help="batch jobs' cells parameter (0: plain RP trees, packed; -1: n/1536 cells)")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
@@ -465,7 +468,7 @@ def batch_main(args):
         x = synth.gaussian_mixture_torch(4 * (1 << 20) + j, n, d, cl, device="cuda")
         data.append(x)
         p = knng.default_params(dim=d, k_eval=k, k_build=max(50, k), num_trees=8, keep_fp32=0,
-                                cells=args.batch_cells)
+                                cells=args.batch_cells, incremental=args.batch_incremental)
         jobs.append({"rows": x.data_ptr(), "n": n, "params": p, "rerank": 1, "k": k, "d": d})
     torch.cuda.synchronize()
     total_rows = sum(j["n"] for j in jobs)
@@ -489,6 +492,7 @@ def batch_main(args):
         "config": {"workload": f"{len(jobs)} independent builds, n log-uniform [2000,200000], d in "
                                f"{{384,768,1024}}, 16-512 clusters, k in {{10,15,32}}, 8 trees, FP32 "
                                f"rerank export", "total_rows": total_rows,
+                   "incremental": args.batch_incremental,
                    "form": "packed per dimension (one context each: forest, init, graph-replayed "
                            "join iterations over a device work list)" if packed else
                            f"whole builds on {args.workers} worker threads",
