@@ -2461,7 +2461,7 @@ __device__ __forceinline__ int64_t wq_row(const WaveQueue& q, unsigned int t) {
 }
 
 template <bool REV>   // the reverse-candidate stage compiled in (used when R > 0)
-__global__ __launch_bounds__(256) void k_jcand(
+__global__ __launch_bounds__(256, 6) void k_jcand(
     const uint64_t* __restrict__ sk, int d, int K, int M, const int32_t* __restrict__ ids_in,
     const uint64_t* __restrict__ fresh_in, const uint8_t* __restrict__ masks_in,
     const int32_t* __restrict__ order, int64_t nwork, int32_t* __restrict__ cbuf,
