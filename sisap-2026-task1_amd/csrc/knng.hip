@@ -1391,7 +1391,7 @@ constexpr int kSelCap = 512;   // compacted candidates per wave in k_leaf_select
 // holds more than kSelCap entries at that cut (ties) is appended to `slow` for
 // k_leaf_select_slow.  Keeping that case out of this kernel keeps its
 // registers at 8 waves per SIMD.
-__global__ __launch_bounds__(256) void k_leaf_select(int64_t n, int T, const int32_t* __restrict__ perm,
+__global__ __launch_bounds__(256, 5) void k_leaf_select(int64_t n, int T, const int32_t* __restrict__ perm,
                                                      const int4* __restrict__ items, int64_t item0,
                                                      int64_t nitems, const int32_t* __restrict__ scratch,
                                                      int mpad, uint64_t* __restrict__ cand,
