@@ -61,6 +61,31 @@ def test_h5_train_roundtrip_and_hyperslab():
         knng.h5().knng_h5_close(r)
 
 
+@pytest.mark.parametrize("chunk_rows", [0, 300, 1000, 4096])
+def test_h5_streamed_rows(chunk_rows):
+    """The CLI's hyperslab reads of a contiguous (chunk_rows 0) or row-chunked
+    /train give the rows exactly, across chunk boundaries, a partial last
+    chunk and single rows (src/main.cpp:171-177 reads such hyperslabs)."""
+    n, d = 3500, 192
+    x = np.random.default_rng(chunk_rows + 7).standard_normal((n, d)).astype(np.float32)
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "t.h5")
+        knng.h5_write_train(p, x, chunk_rows=chunk_rows)
+        r = ctypes.c_void_p()
+        assert knng.h5().knng_h5_open(p.encode(), b"/train", ctypes.byref(r)) == 0
+        try:
+            for row0, nrows in [(0, n), (0, 1), (299, 2), (450, 1100), (2999, 501), (3499, 1), (1000, 0)]:
+                part = np.full((nrows, d), np.nan, np.float32)
+                assert knng.h5().knng_h5_read_rows(r, row0, nrows, d, part.ctypes.data) == 0
+                assert np.array_equal(part, x[row0:row0 + nrows]), (row0, nrows)
+            part = np.empty((10, 64), np.float32)   # a column prefix: the library path
+            assert knng.h5().knng_h5_read_rows(r, 17, 10, 64, part.ctypes.data) == 0
+            assert np.array_equal(part, x[17:27, :64])
+            assert knng.h5().knng_h5_read_rows(r, 3495, 10, d, part.ctypes.data) < 0   # past the end
+        finally:
+            knng.h5().knng_h5_close(r)
+
+
 def test_h5_result_file_layout():
     n, k = 7, 15
     knns = np.arange(n * k, dtype=np.int32).reshape(n, k) + 1
