@@ -88,17 +88,23 @@ __global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ x, i
                               (uint32_t(uint8_t(int8_t(int(c2)))) << 16) |
                               (uint32_t(uint8_t(int8_t(int(c3)))) << 24);
       if (ok) *reinterpret_cast<uint32_t*>(qr + col) = packed;
-      const uint64_t b0 = __ballot(ok && c0 > 0.0f);
-      const uint64_t b1 = __ballot(ok && c1 > 0.0f);
-      const uint64_t b2 = __ballot(ok && c2 > 0.0f);
-      const uint64_t b3 = __ballot(ok && c3 > 0.0f);
+      // sketch word 4g + qd covers dims 256g + 64qd .. + 63, held by lanes
+      // 16qd .. 16qd + 15: each lane's four sign bits at bit 4 (lane % 16),
+      // OR-reduced over its 16-lane group on the vector ALU (the scalar unit
+      // a CU's four SIMDs share would otherwise bound the kernel)
+      const uint32_t nib = ok ? uint32_t(c0 > 0.0f) | (uint32_t(c1 > 0.0f) << 1) | (uint32_t(c2 > 0.0f) << 2) |
+                                    (uint32_t(c3 > 0.0f) << 3)
+                              : 0u;
+      const int sh = 4 * (lane & 15);
+      uint32_t lo = sh < 32 ? nib << sh : 0u, hi = sh >= 32 ? nib << (sh - 32) : 0u;
 #pragma unroll
-      for (int qd = 0; qd < 4; ++qd) {
-        const int sh = 16 * qd;
-        const uint64_t w = spread4(b0 >> sh) | (spread4(b1 >> sh) << 1) |
-                           (spread4(b2 >> sh) << 2) | (spread4(b3 >> sh) << 3);
-        if (lane == 4 * g + qd) mine = w;
+      for (int m = 8; m >= 1; m >>= 1) {
+        lo |= __shfl_xor(lo, m, 16);
+        hi |= __shfl_xor(hi, m, 16);
       }
+      // lane 4g + qd takes group qd's word
+      const uint32_t wlo = __shfl(lo, 16 * (lane & 3)), whi = __shfl(hi, 16 * (lane & 3));
+      if ((lane >> 2) == g) mine = (uint64_t(whi) << 32) | wlo;
     }
     if (lane < words) sk[r * words + lane] = mine;
   }
