@@ -154,33 +154,34 @@ __global__ void k_forest_seed_segs(int64_t n, int T, const int64_t* __restrict__
   }
 }
 
-// side flag of every position in an active node: 1 = left.  One wave per
-// position (a 1 KB row per step); pivot rows stay in registers while
-// consecutive positions share a node.
-template <int CPL>   // 16-byte chunks per lane (d <= 1024*CPL)
+// side flag of every position in an active node: 1 = left.  16 lanes per
+// position (a 1 KB row each, four rows per wave in flight); pivot rows stay
+// in registers while consecutive positions share a node.
+template <int CPL>   // CPL 16-byte chunks per lane of a 64-lane row (d <= 1024 CPL)
 __global__ __launch_bounds__(256) void k_forest_flags(const int8_t* __restrict__ q, int d,
                                                       int64_t n, int64_t total,
                                                       const int32_t* __restrict__ perm,
                                                       const int32_t* __restrict__ nodepos,
                                                       const Node* __restrict__ nodes,
                                                       int32_t* __restrict__ flags, int run) {
-  const int lane = threadIdx.x & 63;
+  // A wave takes `run` consecutive positions; its four 16-lane groups take
+  // every fourth of them, so four rows load at once, each group keeping its
+  // node's two pivot rows while consecutive positions stay in that node.
+  constexpr int CH = 4 * CPL;   // 16-byte chunks per lane of a 16-lane group
+  const int lane = threadIdx.x & 63, g = lane >> 4, s16 = lane & 15;
   const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
   const int nch = d >> 4;
-  v4i pa[CPL], pb[CPL];
+  v4i pa[CH], pb[CH];
   for (int64_t base = wave * run; base < total; base += nwaves * run) {
     int cur = -1;
     int64_t seg0 = 0;
     int32_t p1 = 0, p2 = 0;
     const int64_t end = base + run < total ? base + run : total;
-    for (int64_t gp = base; gp < end; ++gp) {
-      const int nd = nodepos[gp];
-      if (nd < 0) {
-        if (lane == 0) flags[gp] = 0;
-        continue;
-      }
-      if (nd != cur) {
+    for (int64_t g0 = base; g0 < end; g0 += 4) {
+      const int64_t gp = g0 + g;
+      const int nd = gp < end ? nodepos[gp] : -1;
+      if (nd >= 0 && nd != cur) {
         const Node node = nodes[nd];
         seg0 = int64_t(node.tree) * n + node.start;
         p1 = node.p1;
@@ -188,34 +189,33 @@ __global__ __launch_bounds__(256) void k_forest_flags(const int8_t* __restrict__
         const int8_t* ra = q + int64_t(perm[seg0 + p1]) * d;
         const int8_t* rb = q + int64_t(perm[seg0 + p2]) * d;
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-          const int ch = lane + 64 * c;
+        for (int c = 0; c < CH; ++c) {
+          const int ch = s16 + 16 * c;
           pa[c] = ch < nch ? *reinterpret_cast<const v4i*>(ra + 16 * ch) : v4i{0, 0, 0, 0};
           pb[c] = ch < nch ? *reinterpret_cast<const v4i*>(rb + 16 * ch) : v4i{0, 0, 0, 0};
         }
         cur = nd;
       }
       const int64_t rel = gp - seg0;
-      int f;
-      if (rel == p1) {
-        f = 1;
-      } else if (rel == p2) {
-        f = 0;
-      } else {
+      const bool dot = nd >= 0 && rel != p1 && rel != p2;
+      int s = 0;
+      if (dot) {
         const int8_t* rx = q + int64_t(perm[gp]) * d;
-        int s = 0;
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-          const int ch = lane + 64 * c;
+        for (int c = 0; c < CH; ++c) {
+          const int ch = s16 + 16 * c;
           if (ch < nch) {
             const v4i v = *reinterpret_cast<const v4i*>(rx + 16 * ch);
             s = dot16_i8(v, pa[c], s);
             s -= dot16_i8(v, pb[c], 0);
           }
         }
-        f = wave_sum_i(s) > 0 ? 1 : 0;   // <x,p1> > <x,p2>, ties go right
       }
-      if (lane == 0) flags[gp] = f;
+#pragma unroll
+      for (int m = 8; m >= 1; m >>= 1) s += __shfl_xor(s, m, 16);
+      // <x,p1> > <x,p2> goes left (1), ties go right; the pivots themselves
+      // go left (p1) and right (p2); positions outside active nodes get 0
+      if (s16 == 0 && gp < end) flags[gp] = nd < 0 ? 0 : (rel == p1 ? 1 : (rel == p2 ? 0 : (s > 0 ? 1 : 0)));
     }
   }
 }
