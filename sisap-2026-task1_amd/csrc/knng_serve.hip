@@ -194,7 +194,7 @@ __global__ __launch_bounds__(64) void k_route_small(const int8_t* __restrict__ q
   if (lane == 0) ctr[g * kRouteGroups + qg] = 0;   // re-armed for the slot's next launch
 }
 
-__global__ __launch_bounds__(256) void k_search(
+__global__ __launch_bounds__(256, 6) void k_search(
     const int8_t* __restrict__ q, const float* __restrict__ x, int d,
     const int32_t* __restrict__ gid, int K, const Node* __restrict__ hier, int64_t hier_count,
     const int4* __restrict__ leaves, const int32_t* __restrict__ perm, int64_t n_forest,
