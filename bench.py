@@ -201,11 +201,14 @@ def main():
     ap.add_argument("--h5-chunk-rows", type=int, default=2_200_000,
                     help="cli workload: rows per hyperslab read (src/main.cpp:26 reads 2.2M)")
     ap.add_argument("--jobs", type=int, default=512)
-    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--workers", type=int, default=16, help="batch workload: worker threads")
+    ap.add_argument("--batch-form", choices=["trees", "chosen"], default="trees",
+                    help="batch jobs' build form: trees (8 trees, --batch-cells / --batch-incremental) or "
+                         "chosen (the headline's form per job)")
     ap.add_argument("--batch-incremental", type=int, default=1,
                     help="batch jobs' incremental parameter (1: NN-Descent's new/old join rule; "
                          "0: every pair in every iteration)")
-    ap.add_argument("--batch-cells", type=int, default=0,
+    ap.add_argument("--batch-cells", type=int, default=-1,
                     help="batch jobs' cells parameter (0: plain RP trees, packed; -1: n/1536 cells)")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--rate", type=float, default=2000.0, help="Poisson request rate (req/s)")
@@ -468,8 +471,16 @@ def batch_main(args):
         k = int(rng.choice([10, 15, 32]))
         x = synth.gaussian_mixture_torch(4 * (1 << 20) + j, n, d, cl, device="cuda")
         data.append(x)
-        p = knng.default_params(dim=d, k_eval=k, k_build=max(50, k), num_trees=8, keep_fp32=0,
-                                cells=args.batch_cells, incremental=args.batch_incremental)
+        if args.batch_form == "chosen":
+            # the headline's form per job: 4 trees over 4 k-means partitions,
+            # leaves of up to 2048 rows, a full join iteration then one under
+            # the new/old rule
+            p = knng.default_params(dim=d, k_eval=k, k_build=max(50, k), keep_fp32=0, num_trees=4,
+                                    cells=-1, cell_partitions=4, cell_oversample=2, leaf_size=2048,
+                                    incremental=1, max_iters=2)
+        else:
+            p = knng.default_params(dim=d, k_eval=k, k_build=max(50, k), num_trees=8, keep_fp32=0,
+                                    cells=args.batch_cells, incremental=args.batch_incremental)
         jobs.append({"rows": x.data_ptr(), "n": n, "params": p, "rerank": 1, "k": k, "d": d})
     torch.cuda.synchronize()
     total_rows = sum(j["n"] for j in jobs)
@@ -477,7 +488,8 @@ def batch_main(args):
     t0 = time.time()
     rc = knng.build_batch(jobs, 0, args.workers)
     wall = time.time() - t0
-    packed = os.environ.get("KNNG_BATCH_NO_PACK", "0") != "1" and args.batch_cells == 0
+    packed = (os.environ.get("KNNG_BATCH_NO_PACK", "0") != "1" and args.batch_cells == 0
+              and args.batch_form == "trees")
     assert rc == 0, [j["status"] for j in jobs if j["status"]][:5]
     recs = []
     for j, job in zip(range(len(jobs)), jobs):
@@ -493,7 +505,7 @@ def batch_main(args):
         "config": {"workload": f"{len(jobs)} independent builds, n log-uniform [2000,200000], d in "
                                f"{{384,768,1024}}, 16-512 clusters, k in {{10,15,32}}, 8 trees, FP32 "
                                f"rerank export", "total_rows": total_rows,
-                   "incremental": args.batch_incremental,
+                   "incremental": args.batch_incremental, "batch_form": args.batch_form,
                    "form": "packed per dimension (one context each: forest, init, graph-replayed "
                            "join iterations over a device work list)" if packed else
                            f"whole builds on {args.workers} worker threads",
