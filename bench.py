@@ -222,7 +222,7 @@ def main():
     ap.add_argument("--bulk-yield", type=int, default=0,
                     help="bulk slots kept busy while express requests wait or run (0: no limit)")
     ap.add_argument("--hw-queues", type=int, default=0,
-                    help="GPU_MAX_HW_QUEUES for the serve workload (0: leave the environment's)")
+                    help="GPU_MAX_HW_QUEUES for the serve and batch workloads (0: leave the environment's)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -455,6 +455,10 @@ def batch_main(args):
     """Batched uneven builds: `jobs` independent graphs, n log-uniform in
     [2000, 200000], d from {384, 768, 1024}, 16-512 clusters, k from
     {10, 15, 32}; rows generated in HBM before the timed region."""
+    # HIP's hardware queues per process (set before HIP starts): each worker's
+    # stream on a queue of its own lets the builds' small launches interleave
+    if args.hw_queues > 0:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     import numpy as np
     import torch
 

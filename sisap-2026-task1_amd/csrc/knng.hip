@@ -4270,10 +4270,12 @@ int knng_create(knng_ctx** out, int32_t device, const knng_params* p, int64_t n)
   if ((rc = dalloc(c, &c->sk, size_t(n) * c->words, "sketches"))) return bail(rc);
   if (p->keep_fp32 && (rc = dalloc(c, &c->x, size_t(n) * c->d, "float32 rows"))) return bail(rc);
   if ((rc = dalloc(c, &c->counters, kCounterFields * kNumCounters + kRowQueueWords, "counters"))) return bail(rc);
-  // zero once; k_sum_counters re-zeroes after every join launch
-  if (hipMemset(c->counters, 0, (kCounterFields * kNumCounters + kRowQueueWords) * sizeof(unsigned long long)) !=
-      hipSuccess)
-    return bail(fail(c, KNNG_EHIP, "hipMemset of the join counters failed"));
+  // zero once, on the context's stream (a null-stream memset would wait for
+  // every blocking stream of the process, e.g. other contexts' builds);
+  // k_sum_counters re-zeroes after every join launch
+  if (hipMemsetAsync(c->counters, 0, (kCounterFields * kNumCounters + kRowQueueWords) * sizeof(unsigned long long),
+                     c->stream) != hipSuccess)
+    return bail(fail(c, KNNG_EHIP, "hipMemsetAsync of the join counters failed"));
   if ((rc = dalloc(c, &c->csum, kCounterFields, "counter sums"))) return bail(rc);
   c->st.n = n;
   c->st.dim = c->d;
