@@ -121,15 +121,6 @@ __device__ __host__ __forceinline__ void node_pivots(uint64_t seed, int tree, in
 // quantiser (src/main.cpp:179-192)
 // ---------------------------------------------------------------------------
 
-// bit i of v (16 bits) -> bit 4i
-__device__ __forceinline__ uint64_t spread4(uint64_t x) {
-  x &= 0xFFFFull;
-  x = (x | (x << 24)) & 0x000000FF000000FFull;
-  x = (x | (x << 12)) & 0x000F000F000F000Full;
-  x = (x | (x << 6)) & 0x0303030303030303ull;
-  x = (x | (x << 3)) & 0x1111111111111111ull;
-  return x;
-}
 
 // src/main.cpp:184-186: multiply, min(127), max(-128); x86 DAZ (the reference
 // is built with -ffast-math) makes a denormal input read as zero.
