@@ -68,7 +68,8 @@ def build_params(knng, args, d):
                                cell_partitions=args.partitions, rerank_width=args.rerank_width,
                                incremental=args.incremental, cell_oversample=args.oversample,
                                max_iters=args.max_iters, kmeans_iters=args.kmeans_iters,
-                               cell_sample=args.cell_sample, reverse_width=args.reverse_width)
+                               cell_sample=args.cell_sample, reverse_width=args.reverse_width,
+                               rerank_mult=args.rerank_mult)
 
 
 def exact_topk(torch, x, rows, k, chunk=262144):
@@ -184,6 +185,9 @@ def main():
                     help="1: NN-Descent's new/old rule in the join (a 2-hop pair needs a fresh edge)")
     ap.add_argument("--rerank-width", type=int, default=28,
                     help="list entries the FP32 rerank re-scores (0: all k_build)")
+    ap.add_argument("--rerank-mult", type=int, default=3,
+                    help="join: candidates re-scored exactly per row at most k_build * this (the rest "
+                         "are cut by sketch similarity, src/main.cpp:372-382)")
     ap.add_argument("--reverse-width", type=int, default=0,
                     help="reverse-list entries per row in the join (0: pull-only 2-hop join)")
     ap.add_argument("--seed", type=int, default=3)

@@ -17,8 +17,9 @@
 //   --form chosen       the measured build form for large clustered inputs: 4 trees
 //                       seeded by 4 k-means cell partitions (2 seeds per target
 //                       cell), leaves of up to 2048 rows, a full join iteration then
-//                       one under NN-Descent's new/old rule, FP32 rerank of each
-//                       list's 28 best entries (implies --rerank)
+//                       one under NN-Descent's new/old rule, at most 3 * k_build
+//                       candidates per row re-scored in the join, FP32 rerank of
+//                       each list's 28 best entries (implies --rerank)
 //   --form reference    the reference's own shape: 16 plain RP trees (no cells),
 //                       leaf 64, the reference stop rule
 //   --trees N  --k-build N  --leaf N  --iters N  --rerank-mult N  --device N
@@ -119,6 +120,7 @@ int main(int argc, char** argv) {
         p.incremental = 1;
         p.max_iters = 2;
         p.rerank_width = 28;
+        p.rerank_mult = 3;
         p.stop_rule = KNNG_STOP_INSERTED;
         rerank = 1;
       } else if (f == "reference") {

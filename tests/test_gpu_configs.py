@@ -34,9 +34,10 @@ def say(*a):
 
 # The form bench.py times at 6.35M rows: 4 trees seeded by 4 k-means cell
 # partitions with 2 seeds per target cell, leaves of up to 2048 rows, one full
-# join iteration and one under the new/old rule, FP32 rerank of the 28 best.
+# join iteration and one under the new/old rule with at most 3 * k_build
+# candidates re-scored per row, FP32 rerank of the 28 best.
 BENCH_FORM = dict(num_trees=4, leaf_size=2048, cells=-1, cell_partitions=4, cell_oversample=2,
-                  incremental=1, max_iters=2, rerank_width=28)
+                  incremental=1, max_iters=2, rerank_width=28, rerank_mult=3)
 
 
 def _fp64_rel_err(torch, x, rows, knns_1b, dists):
@@ -79,7 +80,7 @@ def bench_form_case():
     x = synth.gaussian_mixture(77, n, 1024, 2000)
     q, sk = oracle.quantize(x)
     o_ids, o_sc, o_rates = oracle.build_device_semantics(q, sk, trees=4, leaf=2048, partitions=4,
-                                                         over=2, incremental=True, max_iters=2)
+                                                         over=2, incremental=True, max_iters=2, mult=3)
     return x, q, sk, o_ids, o_sc, o_rates
 
 
